@@ -1,0 +1,495 @@
+// gev_api.hip -- the C ABI (include/gev_pm.h): argument checking, context and
+// particle-container ownership, dispatch on the precision form.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "gev_common.hpp"
+#include "gev_kernels.hpp"
+
+namespace gev {
+template <typename T> int pcls_sort_impl(gev_pcls *p);
+template <typename T> int pcls_lattice_impl(gev_pcls *p);
+template <typename T> int pcls_load_impl(gev_pcls *p, const double *pos, const double *q, const int64_t *id);
+template <typename T> int pcls_store_impl(gev_pcls *p, double *pos, double *q, int64_t *id);
+template <typename T> int dep_T00_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc);
+template <typename T> int dep_T0i_impl(gev_pcls *p, T *out, const T *phi, double mass, int acc);
+template <typename T> int dep_Tij_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc);
+template <typename T>
+int update_q_impl(gev_pcls *p, double dtau, const T *phi, const T *chi, const T *Bi, int nfield, double a,
+                  double bcoeff, double *max_ret);
+template <typename T>
+int update_q_newton_impl(gev_pcls *p, double dtau, const T *psi, const T *chi, int nfield, double a, double *max_ret);
+template <typename T>
+int update_pos_impl(gev_pcls *p, double dtau, const T *phi, const T *chi, const T *Bi, int nfield, double a,
+                    double bcoeff);
+template <typename T> int update_pos_newton_impl(gev_pcls *p, double dtau, double a);
+template <typename T> int displace_impl(gev_pcls *p, const T *xi, double coeff, double *max_disp);
+template <typename T> int initialize_q_impl(gev_pcls *p, const T *pot, double coeff, double *max_ret);
+}  // namespace gev
+
+using namespace gev;
+
+static thread_local std::string g_err;
+
+void gev_set_error(const std::string &msg) { g_err = msg; }
+
+#define CHECK(cond, msg)                      \
+    do {                                      \
+        if (!(cond)) {                        \
+            gev_set_error(msg);               \
+            return GEV_ERR_ARG;               \
+        }                                     \
+    } while (0)
+
+#define PREC_DISPATCH(ctx, CALL_D, CALL_F) ((ctx)->prec == GEV_FP64 ? (CALL_D) : (CALL_F))
+
+extern "C" {
+
+const char *gev_last_error(void) { return g_err.c_str(); }
+
+int gev_version(void) { return 1; }
+
+int gev_ctx_create(int N, int precision, void *hip_stream, gev_ctx **out)
+{
+    CHECK(out != nullptr, "gev_ctx_create: out is NULL");
+    *out = nullptr;
+    CHECK(N >= 4 && N <= 1024 && (N & (N - 1)) == 0, "gev_ctx_create: N must be a power of two in [4,1024]");
+    CHECK(precision == GEV_FP64 || precision == GEV_MIXED, "gev_ctx_create: unknown precision");
+    gev_ctx *c = new gev_ctx();
+    c->N = N;
+    c->prec = precision;
+    const int TX = (precision == GEV_FP64) ? 8 : 16;
+    c->P = ((N / 2 + 1 + TX - 1) / TX) * TX;
+    c->M = (int64_t)N * N * N;
+    c->NK = (int64_t)c->P * N * N;
+    GEV_HIP(hipGetDevice(&c->device));
+    if (hip_stream) {
+        c->stream = (hipStream_t)hip_stream;
+    } else {
+        GEV_HIP(hipStreamCreate(&c->stream));
+        c->own_stream = true;
+    }
+    // twiddles exp(-2 pi i m/N) and k-space tables (gevolution.hpp:269-274)
+    std::vector<cx<double>> tw(N), ks(N);
+    std::vector<cx<float>> twf(N);
+    std::vector<double> gk2(N);
+    for (int m = 0; m < N; m++) {
+        long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)N;
+        tw[m] = mk((double)cosl(ang), (double)-sinl(ang));
+        twf[m] = mk((float)cosl(ang), (float)-sinl(ang));
+        double s = 2.0 * (double)N * sin(M_PI * (double)m / (double)N);
+        ks[m] = mk(s * cos(M_PI * (double)m / (double)N), -s * sin(M_PI * (double)m / (double)N));
+        gk2[m] = s * s;
+    }
+    GEV_HIP(hipMalloc(&c->tw64, sizeof(cx<double>) * N));
+    GEV_HIP(hipMalloc(&c->tw32, sizeof(cx<float>) * N));
+    GEV_HIP(hipMalloc(&c->gk2, sizeof(double) * N));
+    GEV_HIP(hipMalloc(&c->kshift, sizeof(cx<double>) * N));
+    GEV_HIP(hipMemcpy(c->tw64, tw.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice));
+    GEV_HIP(hipMemcpy(c->tw32, twf.data(), sizeof(cx<float>) * N, hipMemcpyHostToDevice));
+    GEV_HIP(hipMemcpy(c->gk2, gk2.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    GEV_HIP(hipMemcpy(c->kshift, ks.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice));
+    GEV_HIP(hipMalloc(&c->red, sizeof(double) * (64 + 2048 * 8)));
+    GEV_HIP(hipHostMalloc(&c->red_host, sizeof(double) * 64, hipHostMallocDefault));
+    c->scan_blocks_n = (c->M + 2047) / 2048 + 1;
+    GEV_HIP(hipMalloc(&c->scan_blocks, sizeof(int32_t) * c->scan_blocks_n));
+    *out = c;
+    return GEV_OK;
+}
+
+int gev_ctx_destroy(gev_ctx *c)
+{
+    if (!c) return GEV_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->tw64);
+    (void)hipFree(c->tw32);
+    (void)hipFree(c->gk2);
+    (void)hipFree(c->kshift);
+    (void)hipFree(c->red);
+    (void)hipHostFree(c->red_host);
+    (void)hipFree(c->scan_blocks);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GEV_OK;
+}
+
+int gev_ctx_sync(gev_ctx *c)
+{
+    CHECK(c, "gev_ctx_sync: NULL context");
+    GEV_HIP(hipStreamSynchronize(c->stream));
+    return GEV_OK;
+}
+
+int gev_ctx_kpitch(const gev_ctx *c) { return c ? c->P : 0; }
+
+// ------------------------------------------------------------ particles ---
+int gev_pcls_create(gev_ctx *c, int64_t capacity, double mass, gev_pcls **out)
+{
+    CHECK(c && out, "gev_pcls_create: NULL argument");
+    CHECK(capacity >= 0 && capacity < (int64_t)INT32_MAX, "gev_pcls_create: capacity out of range");
+    *out = nullptr;
+    gev_pcls *p = new gev_pcls();
+    p->ctx = c;
+    p->cap = capacity;
+    p->mass = mass;
+    const size_t es = (c->prec == GEV_FP64) ? 8 : 4;
+    const size_t cap = (size_t)std::max<int64_t>(capacity, 1);
+    for (int d = 0; d < 3; d++) {
+        GEV_HIP(hipMalloc(&p->pos[d], es * cap));
+        GEV_HIP(hipMalloc(&p->q[d], es * cap));
+        GEV_HIP(hipMalloc(&p->pos2[d], es * cap));
+        GEV_HIP(hipMalloc(&p->q2[d], es * cap));
+    }
+    GEV_HIP(hipMalloc(&p->id, 8 * cap));
+    GEV_HIP(hipMalloc(&p->id2, 8 * cap));
+    GEV_HIP(hipMalloc(&p->cell, 4 * cap));
+    GEV_HIP(hipMalloc(&p->cell2, 4 * cap));
+    GEV_HIP(hipMalloc(&p->rank, 4 * cap));
+    GEV_HIP(hipMalloc(&p->cell_start, 4 * (size_t)(c->M + 1)));
+    GEV_HIP(hipMalloc(&p->cell_count, 4 * (size_t)c->M));
+    *out = p;
+    return GEV_OK;
+}
+
+int gev_pcls_destroy(gev_pcls *p)
+{
+    if (!p) return GEV_OK;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    for (int d = 0; d < 3; d++) {
+        (void)hipFree(p->pos[d]); (void)hipFree(p->q[d]);
+        (void)hipFree(p->pos2[d]); (void)hipFree(p->q2[d]);
+    }
+    (void)hipFree(p->id); (void)hipFree(p->id2);
+    (void)hipFree(p->cell); (void)hipFree(p->cell2);
+    (void)hipFree(p->rank);
+    (void)hipFree(p->cell_start);
+    (void)hipFree(p->cell_count);
+    delete p;
+    return GEV_OK;
+}
+
+int gev_pcls_load(gev_pcls *p, int64_t n, const double *pos, const double *q, const int64_t *id, int on_device)
+{
+    CHECK(p && pos && q && id, "gev_pcls_load: NULL argument");
+    CHECK(n >= 0 && n <= p->cap, "gev_pcls_load: n exceeds the container capacity");
+    gev_ctx *c = p->ctx;
+    p->n = n;
+    p->sorted = false;
+    if (n == 0) return PREC_DISPATCH(c, pcls_sort_impl<double>(p), pcls_sort_impl<float>(p));
+    const double *dpos = pos, *dq = q;
+    const int64_t *did = id;
+    double *tmp = nullptr;
+    if (!on_device) {
+        GEV_HIP(hipMalloc(&tmp, sizeof(double) * 7 * n));
+        GEV_HIP(hipMemcpyAsync(tmp, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+        GEV_HIP(hipMemcpyAsync(tmp + 3 * n, q, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+        GEV_HIP(hipMemcpyAsync(tmp + 6 * n, id, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream));
+        dpos = tmp; dq = tmp + 3 * n; did = (const int64_t *)(tmp + 6 * n);
+    }
+    int rc = PREC_DISPATCH(c, pcls_load_impl<double>(p, dpos, dq, did), pcls_load_impl<float>(p, dpos, dq, did));
+    if (tmp) {
+        GEV_HIP(hipStreamSynchronize(c->stream));
+        GEV_HIP(hipFree(tmp));
+    }
+    return rc;
+}
+
+int gev_pcls_store(gev_pcls *p, double *pos, double *q, int64_t *id, int on_device)
+{
+    CHECK(p && pos && q && id, "gev_pcls_store: NULL argument");
+    gev_ctx *c = p->ctx;
+    const int64_t n = p->n;
+    if (n == 0) return GEV_OK;
+    double *dpos = pos, *dq = q;
+    int64_t *did = id;
+    double *tmp = nullptr;
+    if (!on_device) {
+        GEV_HIP(hipMalloc(&tmp, sizeof(double) * 7 * n));
+        dpos = tmp; dq = tmp + 3 * n; did = (int64_t *)(tmp + 6 * n);
+    }
+    int rc = PREC_DISPATCH(c, pcls_store_impl<double>(p, dpos, dq, did), pcls_store_impl<float>(p, dpos, dq, did));
+    if (rc) return rc;
+    if (tmp) {
+        GEV_HIP(hipMemcpyAsync(pos, dpos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+        GEV_HIP(hipMemcpyAsync(q, dq, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+        GEV_HIP(hipMemcpyAsync(id, did, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+        GEV_HIP(hipStreamSynchronize(c->stream));
+        GEV_HIP(hipFree(tmp));
+    }
+    return GEV_OK;
+}
+
+int gev_pcls_view_get(gev_pcls *p, gev_pcls_view *v)
+{
+    CHECK(p && v, "gev_pcls_view_get: NULL argument");
+    v->n = p->n;
+    v->precision = p->ctx->prec;
+    for (int d = 0; d < 3; d++) { v->pos[d] = p->pos[d]; v->q[d] = p->q[d]; }
+    v->id = p->id;
+    v->cell = p->cell;
+    v->cell_start = p->cell_start;
+    return GEV_OK;
+}
+
+int gev_pcls_copy_index(gev_pcls *p, int32_t *cell, int32_t *cell_start)
+{
+    CHECK(p, "gev_pcls_copy_index: NULL argument");
+    if (cell && p->n > 0)
+        GEV_HIP(hipMemcpyAsync(cell, p->cell, sizeof(int32_t) * p->n, hipMemcpyDeviceToDevice, p->ctx->stream));
+    if (cell_start)
+        GEV_HIP(hipMemcpyAsync(cell_start, p->cell_start, sizeof(int32_t) * (p->ctx->M + 1), hipMemcpyDeviceToDevice,
+                               p->ctx->stream));
+    return GEV_OK;
+}
+
+int gev_pcls_sort(gev_pcls *p)
+{
+    CHECK(p, "gev_pcls_sort: NULL argument");
+    return PREC_DISPATCH(p->ctx, pcls_sort_impl<double>(p), pcls_sort_impl<float>(p));
+}
+
+int gev_pcls_init_lattice(gev_pcls *p)
+{
+    CHECK(p, "gev_pcls_init_lattice: NULL argument");
+    CHECK(p->cap >= p->ctx->M, "gev_pcls_init_lattice: capacity below N^3");
+    p->n = p->ctx->M;
+    return PREC_DISPATCH(p->ctx, pcls_lattice_impl<double>(p), pcls_lattice_impl<float>(p));
+}
+
+#define NEED_SORTED(p) CHECK((p)->sorted, "particles are not binned (call gev_pcls_sort)")
+
+// ------------------------------------------------------------ deposits ----
+int gev_project_T00(gev_pcls *p, void *T00, double a, const void *phi, double coeff, int accumulate)
+{
+    CHECK(p && T00, "gev_project_T00: NULL argument");
+    NEED_SORTED(p);
+    const double dx = 1.0 / (double)p->ctx->N;
+    double mass = p->mass * coeff / (dx * dx * dx);
+    if (phi) mass /= a;
+    return PREC_DISPATCH(p->ctx, dep_T00_impl<double>(p, (double *)T00, (const double *)phi, mass, a, accumulate),
+                         dep_T00_impl<float>(p, (float *)T00, (const float *)phi, mass, a, accumulate));
+}
+
+int gev_project_mass(gev_pcls *p, void *rho, double coeff, int accumulate)
+{
+    return gev_project_T00(p, rho, 1.0, nullptr, coeff, accumulate);
+}
+
+int gev_project_T0i(gev_pcls *p, void *T0i, const void *phi, double coeff, int accumulate)
+{
+    CHECK(p && T0i, "gev_project_T0i: NULL argument");
+    NEED_SORTED(p);
+    const double dx = 1.0 / (double)p->ctx->N;
+    double mass = p->mass * coeff / (dx * dx * dx);
+    return PREC_DISPATCH(p->ctx, dep_T0i_impl<double>(p, (double *)T0i, (const double *)phi, mass, accumulate),
+                         dep_T0i_impl<float>(p, (float *)T0i, (const float *)phi, mass, accumulate));
+}
+
+int gev_project_Tij(gev_pcls *p, void *Tij, double a, const void *phi, double coeff, int accumulate)
+{
+    CHECK(p && Tij, "gev_project_Tij: NULL argument");
+    NEED_SORTED(p);
+    const double dx = 1.0 / (double)p->ctx->N;
+    double mass = p->mass * coeff / (dx * dx * dx) / a;
+    return PREC_DISPATCH(p->ctx, dep_Tij_impl<double>(p, (double *)Tij, (const double *)phi, mass, a, accumulate),
+                         dep_Tij_impl<float>(p, (float *)Tij, (const float *)phi, mass, a, accumulate));
+}
+
+// -------------------------------------------------------------- sources ---
+int gev_prepare_source_T00(gev_ctx *c, const void *phi, const void *chi, const void *source, double bg, void *result,
+                           double c1, double c2, double c3, double *sum_source, double *sum_result)
+{
+    CHECK(c && phi && chi && source && result, "gev_prepare_source_T00: NULL argument");
+    double sums[2] = {0, 0};
+    int rc = PREC_DISPATCH(c,
+                           prepare_T00_impl<double>(c, (const double *)phi, (const double *)chi,
+                                                    (const double *)source, bg, (double *)result, c1, c2, c3, sums),
+                           prepare_T00_impl<float>(c, (const float *)phi, (const float *)chi, (const float *)source,
+                                                   bg, (float *)result, c1, c2, c3, sums));
+    if (sum_source) *sum_source = sums[0];
+    if (sum_result) *sum_result = sums[1];
+    return rc;
+}
+
+int gev_prepare_source_Tij(gev_ctx *c, const void *phi, const void *Tij, void *Sij, double coeff)
+{
+    CHECK(c && phi && Tij && Sij, "gev_prepare_source_Tij: NULL argument");
+    return PREC_DISPATCH(c, prepare_Tij_impl<double>(c, (const double *)phi, (const double *)Tij, (double *)Sij, coeff),
+                         prepare_Tij_impl<float>(c, (const float *)phi, (const float *)Tij, (float *)Sij, coeff));
+}
+
+// ------------------------------------------------------------------ FFT ---
+int gev_fft_r2c(gev_ctx *c, const void *in, void *out, int ncomp)
+{
+    CHECK(c && in && out && ncomp >= 1, "gev_fft_r2c: bad argument");
+    return PREC_DISPATCH(c, fft_r2c_impl<double>(c, (const double *)in, (cx<double> *)out, ncomp, c->tw64),
+                         fft_r2c_impl<float>(c, (const float *)in, (cx<float> *)out, ncomp, c->tw32));
+}
+
+int gev_fft_c2r(gev_ctx *c, void *in, void *out, int ncomp)
+{
+    CHECK(c && in && out && ncomp >= 1, "gev_fft_c2r: bad argument");
+    return PREC_DISPATCH(c, fft_c2r_impl<double>(c, (cx<double> *)in, (double *)out, ncomp, c->tw64),
+                         fft_c2r_impl<float>(c, (cx<float> *)in, (float *)out, ncomp, c->tw32));
+}
+
+// -------------------------------------------------------------- k-space ---
+int gev_poisson(gev_ctx *c, const void *src, void *pot, double coeff, double modif)
+{
+    CHECK(c && src && pot, "gev_poisson: NULL argument");
+    return PREC_DISPATCH(c, poisson_impl<double>(c, (const cx<double> *)src, (cx<double> *)pot, coeff, modif),
+                         poisson_impl<float>(c, (const cx<float> *)src, (cx<float> *)pot, coeff, modif));
+}
+
+int gev_project_ft_scalar(gev_ctx *c, const void *S, void *chi, int add)
+{
+    CHECK(c && S && chi, "gev_project_ft_scalar: NULL argument");
+    return PREC_DISPATCH(c, project_ft_scalar_impl<double>(c, (const cx<double> *)S, (cx<double> *)chi, add),
+                         project_ft_scalar_impl<float>(c, (const cx<float> *)S, (cx<float> *)chi, add));
+}
+
+int gev_project_ft_vector(gev_ctx *c, const void *S, void *B, double coeff, double modif)
+{
+    CHECK(c && S && B, "gev_project_ft_vector: NULL argument");
+    return PREC_DISPATCH(c, project_ft_vector_impl<double>(c, (const cx<double> *)S, (cx<double> *)B, coeff, modif),
+                         project_ft_vector_impl<float>(c, (const cx<float> *)S, (cx<float> *)B, coeff, modif));
+}
+
+int gev_evolve_ft_vector(gev_ctx *c, const void *S, void *B, double a2dtau)
+{
+    CHECK(c && S && B, "gev_evolve_ft_vector: NULL argument");
+    return PREC_DISPATCH(c, evolve_ft_vector_impl<double>(c, (const cx<double> *)S, (cx<double> *)B, a2dtau),
+                         evolve_ft_vector_impl<float>(c, (const cx<float> *)S, (cx<float> *)B, a2dtau));
+}
+
+int gev_set_k0(gev_ctx *c, void *ft, double re, double im)
+{
+    CHECK(c && ft, "gev_set_k0: NULL argument");
+    return PREC_DISPATCH(c, set_k0_impl<double>(c, (cx<double> *)ft, re, im),
+                         set_k0_impl<float>(c, (cx<float> *)ft, re, im));
+}
+
+// ---------------------------------------------------------- kick / drift --
+int gev_update_q(gev_pcls *p, double dtau, const void *phi, const void *chi, const void *Bi, int nfield, double a,
+                 double bcoeff, double *max_ret)
+{
+    CHECK(p && phi, "gev_update_q: NULL argument");
+    CHECK(nfield < 2 || chi, "gev_update_q: nfield >= 2 needs chi");
+    CHECK(nfield < 3 || Bi, "gev_update_q: nfield >= 3 needs Bi");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx,
+                         update_q_impl<double>(p, dtau, (const double *)phi, (const double *)chi, (const double *)Bi,
+                                               nfield, a, bcoeff, max_ret),
+                         update_q_impl<float>(p, dtau, (const float *)phi, (const float *)chi, (const float *)Bi,
+                                              nfield, a, bcoeff, max_ret));
+}
+
+int gev_update_q_newton(gev_pcls *p, double dtau, const void *psi, const void *chi, int nfield, double a,
+                        double *max_ret)
+{
+    CHECK(p && psi, "gev_update_q_newton: NULL argument");
+    CHECK(nfield < 2 || chi, "gev_update_q_newton: nfield >= 2 needs chi");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(
+        p->ctx, update_q_newton_impl<double>(p, dtau, (const double *)psi, (const double *)chi, nfield, a, max_ret),
+        update_q_newton_impl<float>(p, dtau, (const float *)psi, (const float *)chi, nfield, a, max_ret));
+}
+
+int gev_update_pos(gev_pcls *p, double dtau, const void *phi, const void *chi, const void *Bi, int nfield, double a,
+                   double bcoeff)
+{
+    CHECK(p, "gev_update_pos: NULL argument");
+    CHECK(nfield < 1 || phi, "gev_update_pos: nfield >= 1 needs phi");
+    CHECK(nfield < 2 || chi, "gev_update_pos: nfield >= 2 needs chi");
+    CHECK(nfield < 3 || Bi, "gev_update_pos: nfield >= 3 needs Bi");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx,
+                         update_pos_impl<double>(p, dtau, (const double *)phi, (const double *)chi,
+                                                 (const double *)Bi, nfield, a, bcoeff),
+                         update_pos_impl<float>(p, dtau, (const float *)phi, (const float *)chi, (const float *)Bi,
+                                                nfield, a, bcoeff));
+}
+
+int gev_update_pos_newton(gev_pcls *p, double dtau, double a)
+{
+    CHECK(p, "gev_update_pos_newton: NULL argument");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx, update_pos_newton_impl<double>(p, dtau, a), update_pos_newton_impl<float>(p, dtau, a));
+}
+
+// ------------------------------------------------------------------- IC ---
+int gev_ic_displacement_ft(gev_ctx *c, uint32_t seed, int ksphere, double coeff, int npk, const double *kL,
+                           const double *amp, const void *kerFT, void *potFT)
+{
+    CHECK(c && kL && amp && kerFT && potFT, "gev_ic_displacement_ft: NULL argument");
+    CHECK(npk >= 2, "gev_ic_displacement_ft: need at least two table points");
+    for (int i = 1; i < npk; i++) CHECK(kL[i] > kL[i - 1], "gev_ic_displacement_ft: k not strictly increasing");
+    // natural cubic spline second derivatives (gsl cspline semantics)
+    std::vector<double> m(npk, 0.0);
+    if (npk >= 3) {
+        int k = npk - 2;
+        std::vector<double> A(k), B(k), Cc(k), D(k);
+        for (int i = 1; i <= k; i++) {
+            double h0 = kL[i] - kL[i - 1], h1 = kL[i + 1] - kL[i];
+            A[i - 1] = h0; B[i - 1] = 2.0 * (h0 + h1); Cc[i - 1] = h1;
+            D[i - 1] = 6.0 * ((amp[i + 1] - amp[i]) / h1 - (amp[i] - amp[i - 1]) / h0);
+        }
+        for (int i = 1; i < k; i++) {
+            double w = A[i] / B[i - 1];
+            B[i] -= w * Cc[i - 1];
+            D[i] -= w * D[i - 1];
+        }
+        m[k] = D[k - 1] / B[k - 1];
+        for (int i = k - 2; i >= 0; i--) m[i + 1] = (D[i] - Cc[i] * m[i + 2]) / B[i];
+    }
+    double *tab = nullptr;
+    GEV_HIP(hipMalloc(&tab, sizeof(double) * 3 * npk));
+    GEV_HIP(hipMemcpyAsync(tab, kL, sizeof(double) * npk, hipMemcpyHostToDevice, c->stream));
+    GEV_HIP(hipMemcpyAsync(tab + npk, amp, sizeof(double) * npk, hipMemcpyHostToDevice, c->stream));
+    GEV_HIP(hipMemcpyAsync(tab + 2 * npk, m.data(), sizeof(double) * npk, hipMemcpyHostToDevice, c->stream));
+    int rc = PREC_DISPATCH(c,
+                           ic_displacement_ft_impl<double>(c, seed, ksphere, coeff, npk, tab, tab + npk, tab + 2 * npk,
+                                                           (const cx<double> *)kerFT, (cx<double> *)potFT),
+                           ic_displacement_ft_impl<float>(c, seed, ksphere, coeff, npk, tab, tab + npk, tab + 2 * npk,
+                                                          (const cx<float> *)kerFT, (cx<float> *)potFT));
+    GEV_HIP(hipStreamSynchronize(c->stream));
+    GEV_HIP(hipFree(tab));
+    return rc;
+}
+
+int gev_displace_particles(gev_pcls *p, const void *xi, double coeff, double *max_disp)
+{
+    CHECK(p && xi, "gev_displace_particles: NULL argument");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx, displace_impl<double>(p, (const double *)xi, coeff, max_disp),
+                         displace_impl<float>(p, (const float *)xi, coeff, max_disp));
+}
+
+int gev_initialize_q(gev_pcls *p, const void *pot, double coeff, double *max_ret)
+{
+    CHECK(p && pot, "gev_initialize_q: NULL argument");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx, initialize_q_impl<double>(p, (const double *)pot, coeff, max_ret),
+                         initialize_q_impl<float>(p, (const float *)pot, coeff, max_ret));
+}
+
+// ------------------------------------------------------------ utilities ---
+int gev_field_sum(gev_ctx *c, const void *f, int ncomp, double *sum)
+{
+    CHECK(c && f && sum && ncomp >= 1, "gev_field_sum: bad argument");
+    return PREC_DISPATCH(c, field_sum_impl<double>(c, (const double *)f, c->M * ncomp, sum),
+                         field_sum_impl<float>(c, (const float *)f, c->M * ncomp, sum));
+}
+
+int gev_field_fill(gev_ctx *c, void *f, int64_t count, double value)
+{
+    CHECK(c && f && count >= 0, "gev_field_fill: bad argument");
+    return PREC_DISPATCH(c, field_fill_impl<double>(c, (double *)f, count, value),
+                         field_fill_impl<float>(c, (float *)f, count, (float)value));
+}
+
+}  // extern "C"

@@ -1,0 +1,31 @@
+// gev_kernels.hpp -- internal launchers shared between the HIP translation
+// units of the library (not part of the C ABI).
+#pragma once
+#include "gev_common.hpp"
+
+namespace gev {
+
+template <typename T> int fft_r2c_impl(gev_ctx *ctx, const T *in, cx<T> *out, int ncomp, const cx<T> *tw);
+template <typename T> int fft_c2r_impl(gev_ctx *ctx, cx<T> *in, T *out, int ncomp, const cx<T> *tw);
+
+template <typename T> int poisson_impl(gev_ctx *ctx, const cx<T> *src, cx<T> *pot, double coeff, double modif);
+template <typename T> int project_ft_scalar_impl(gev_ctx *ctx, const cx<T> *S, cx<T> *chi, int add);
+template <typename T> int project_ft_vector_impl(gev_ctx *ctx, const cx<T> *S, cx<T> *B, double coeff, double modif);
+template <typename T> int evolve_ft_vector_impl(gev_ctx *ctx, const cx<T> *S, cx<T> *B, double a2dtau);
+template <typename T>
+int ic_displacement_ft_impl(gev_ctx *ctx, uint32_t seed, int ksphere, double coeff, int npk, const double *dx,
+                            const double *dy, const double *dm, const cx<T> *ker, cx<T> *pot);
+
+template <typename T> int set_k0_impl(gev_ctx *ctx, cx<T> *ft, double re, double im);
+
+template <typename T>
+int prepare_T00_impl(gev_ctx *ctx, const T *phi, const T *chi, const T *src, double bg, T *res, double c1, double c2,
+                     double c3, double *sums);
+template <typename T> int prepare_Tij_impl(gev_ctx *ctx, const T *phi, const T *Tij, T *Sij, double coeff);
+template <typename T> int field_sum_impl(gev_ctx *ctx, const T *f, int64_t count, double *sum);
+template <typename T> int field_fill_impl(gev_ctx *ctx, T *f, int64_t count, T v);
+
+// deterministic two-level FP64 sum: per-block partials then one block
+int reduce_partials(gev_ctx *ctx, const double *partials, int nparts, int nvals, double *out_dev);
+
+}  // namespace gev

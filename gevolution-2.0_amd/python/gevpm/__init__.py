@@ -1,0 +1,383 @@
+"""gevpm -- Python host layer of the MI355X-native gevolution particle-mesh step.
+
+Thin, allocation-owning wrappers over the C ABI (include/gev_pm.h): torch
+tensors provide device memory and the stream, every computation runs in the
+HIP kernels of libgevpm.so.  Names follow the reference interfaces
+(gevolution.hpp projection_* / prepareFTsource / solveModifiedPoissonFT /
+projectFT*, perfParticles updateVel / moveParticles).
+
+Tensor layouts:
+  real field     (ncomp, N, N, N) indexed [c, z, y, x]
+  Fourier field  (ncomp, N, N, P) complex, indexed [c, kz, ky, kx], P = ctx.P
+"""
+import ctypes as C
+import math
+
+import torch
+
+from ._lib import (GEV_FP64, GEV_MIXED, GevError, check, gev_fields, gev_pcls_view, gev_step_state, lib)
+
+__all__ = ["Context", "Particles", "Fields", "StepState", "GEV_FP64", "GEV_MIXED", "GevError"]
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise GevError("gevpm: tensors must live on the GPU")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One N^3 periodic lattice in one precision form (LATfield2 Lattice + PlanFFT)."""
+
+    def __init__(self, N, precision=GEV_FP64, device="cuda"):
+        if not torch.cuda.is_available():
+            raise GevError("gevpm needs a GPU (HIP device) -- there is no CPU path")
+        self.N = int(N)
+        self.precision = precision
+        self.device = torch.device(device)
+        self.rdtype = torch.float64 if precision == GEV_FP64 else torch.float32
+        self.cdtype = torch.complex128 if precision == GEV_FP64 else torch.complex64
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            self.stream = torch.cuda.current_stream(self.device)
+            check(lib.gev_ctx_create(self.N, precision, C.c_void_p(self.stream.cuda_stream), C.byref(h)),
+                  "gev_ctx_create")
+        self._h = h
+        self.P = lib.gev_ctx_kpitch(h)
+        self.M = self.N ** 3
+
+    @property
+    def handle(self):
+        return self._h
+
+    def bind_stream(self):
+        """Make the library follow torch's current stream."""
+        s = torch.cuda.current_stream(self.device)
+        if s.cuda_stream != self.stream.cuda_stream:
+            self.stream = s
+            check(lib.gev_ctx_set_stream(self._h, C.c_void_p(s.cuda_stream)), "gev_ctx_set_stream")
+
+    def sync(self):
+        check(lib.gev_ctx_sync(self._h), "gev_ctx_sync")
+
+    def close(self):
+        if self._h:
+            lib.gev_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------- allocation
+    def real(self, ncomp=1):
+        return torch.zeros((ncomp, self.N, self.N, self.N), dtype=self.rdtype, device=self.device)
+
+    def fourier(self, ncomp=1):
+        return torch.zeros((ncomp, self.N, self.N, self.P), dtype=self.cdtype, device=self.device)
+
+    def unpad(self, F):
+        """(c, N, N, P) -> (c, N, N, N//2+1): the numpy rfftn layout."""
+        return F[..., : self.N // 2 + 1]
+
+    # -------------------------------------------------------------- FFT
+    def fft_r2c(self, f, out=None):
+        f = f.contiguous()
+        nc = f.numel() // self.M
+        out = self.fourier(nc) if out is None else out
+        self.bind_stream()
+        check(lib.gev_fft_r2c(self._h, _ptr(f), _ptr(out), nc), "gev_fft_r2c")
+        return out
+
+    def fft_c2r(self, F, out=None, preserve=True):
+        nc = F.numel() // (self.N * self.N * self.P)
+        src = F.clone() if preserve else F
+        out = self.real(nc) if out is None else out
+        self.bind_stream()
+        check(lib.gev_fft_c2r(self._h, _ptr(src), _ptr(out), nc), "gev_fft_c2r")
+        return out
+
+    # ----------------------------------------------------------- k-space
+    def poisson(self, src, coeff, modif=0.0, out=None):
+        out = torch.empty_like(src) if out is None else out
+        self.bind_stream()
+        check(lib.gev_poisson(self._h, _ptr(src), _ptr(out), coeff, modif), "gev_poisson")
+        return out
+
+    def project_ft_scalar(self, S, chi=None):
+        add = chi is not None
+        out = self.fourier(1) if chi is None else chi
+        self.bind_stream()
+        check(lib.gev_project_ft_scalar(self._h, _ptr(S), _ptr(out), int(add)), "gev_project_ft_scalar")
+        return out
+
+    def project_ft_vector(self, S, coeff=1.0, modif=0.0, out=None):
+        out = torch.empty_like(S) if out is None else out
+        self.bind_stream()
+        check(lib.gev_project_ft_vector(self._h, _ptr(S), _ptr(out), coeff, modif), "gev_project_ft_vector")
+        return out
+
+    def evolve_ft_vector(self, S, B, a2dtau):
+        self.bind_stream()
+        check(lib.gev_evolve_ft_vector(self._h, _ptr(S), _ptr(B), a2dtau), "gev_evolve_ft_vector")
+        return B
+
+    def set_k0(self, F, re, im=0.0):
+        self.bind_stream()
+        check(lib.gev_set_k0(self._h, _ptr(F), re, im), "gev_set_k0")
+
+    # ----------------------------------------------------------- sources
+    def prepare_source_T00(self, phi, chi, source, bg, c1, c2, c3, result=None):
+        result = torch.empty_like(source) if result is None else result
+        s1, s2 = C.c_double(), C.c_double()
+        self.bind_stream()
+        check(lib.gev_prepare_source_T00(self._h, _ptr(phi), _ptr(chi), _ptr(source), bg, _ptr(result), c1, c2, c3,
+                                         C.byref(s1), C.byref(s2)), "gev_prepare_source_T00")
+        return result, s1.value, s2.value
+
+    def prepare_source_Tij(self, phi, Tij, coeff, Sij=None):
+        Sij = torch.empty_like(Tij) if Sij is None else Sij
+        self.bind_stream()
+        check(lib.gev_prepare_source_Tij(self._h, _ptr(phi), _ptr(Tij), _ptr(Sij), coeff), "gev_prepare_source_Tij")
+        return Sij
+
+    def field_sum(self, f):
+        s = C.c_double()
+        self.bind_stream()
+        check(lib.gev_field_sum(self._h, _ptr(f.contiguous()), f.numel() // self.M, C.byref(s)), "gev_field_sum")
+        return s.value
+
+    def ic_displacement_ft(self, seed, kL, amp, kerFT, coeff=0.0, ksphere=1, out=None):
+        import numpy as np
+        kL = np.ascontiguousarray(kL, np.float64)
+        amp = np.ascontiguousarray(amp, np.float64)
+        out = self.fourier(1) if out is None else out
+        self.bind_stream()
+        check(lib.gev_ic_displacement_ft(self._h, seed, ksphere, coeff, len(kL),
+                                         kL.ctypes.data_as(C.POINTER(C.c_double)),
+                                         amp.ctypes.data_as(C.POINTER(C.c_double)), _ptr(kerFT), _ptr(out)),
+              "gev_ic_displacement_ft")
+        return out
+
+
+class Particles:
+    """Cell-sorted particle container (perfParticles / perfParticles_gevolution)."""
+
+    def __init__(self, ctx, capacity, mass):
+        self.ctx = ctx
+        self.mass = float(mass)
+        h = C.c_void_p()
+        check(lib.gev_pcls_create(ctx.handle, int(capacity), self.mass, C.byref(h)), "gev_pcls_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib.gev_pcls_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def view(self):
+        v = gev_pcls_view()
+        check(lib.gev_pcls_view_get(self._h, C.byref(v)), "gev_pcls_view_get")
+        return v
+
+    @property
+    def n(self):
+        return self.view().n
+
+    def load(self, pos, q, pid):
+        """pos, q: (3, n) float64; pid: (n,) int64 -- host (numpy/CPU torch) or device tensors."""
+        if isinstance(pos, torch.Tensor) and pos.is_cuda:
+            pos = pos.to(torch.float64).contiguous(); q = q.to(torch.float64).contiguous()
+            pid = pid.to(torch.int64).contiguous()
+            self.ctx.bind_stream()
+            check(lib.gev_pcls_load(self._h, pos.shape[1], _ptr(pos), _ptr(q), _ptr(pid), 1), "gev_pcls_load")
+        else:
+            import numpy as np
+            pos = np.ascontiguousarray(pos, np.float64); q = np.ascontiguousarray(q, np.float64)
+            pid = np.ascontiguousarray(pid, np.int64)
+            self.ctx.bind_stream()
+            check(lib.gev_pcls_load(self._h, pos.shape[1], C.c_void_p(pos.ctypes.data), C.c_void_p(q.ctypes.data),
+                                    C.c_void_p(pid.ctypes.data), 0), "gev_pcls_load")
+
+    def store(self):
+        """-> (pos (3,n) float64, q (3,n) float64, id (n,) int64) device tensors, in cell order."""
+        n = self.n
+        dev = self.ctx.device
+        pos = torch.empty((3, n), dtype=torch.float64, device=dev)
+        q = torch.empty((3, n), dtype=torch.float64, device=dev)
+        pid = torch.empty((n,), dtype=torch.int64, device=dev)
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_store(self._h, _ptr(pos), _ptr(q), _ptr(pid), 1), "gev_pcls_store")
+        return pos, q, pid
+
+    def cells(self):
+        """cell index (x + N(y + N z)) of every particle, in container order"""
+        out = torch.empty((max(self.n, 1),), dtype=torch.int32, device=self.ctx.device)
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_copy_index(self._h, _ptr(out), None), "gev_pcls_copy_index")
+        return out[: self.n]
+
+    def cell_start(self):
+        out = torch.empty((self.ctx.M + 1,), dtype=torch.int32, device=self.ctx.device)
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_copy_index(self._h, None, _ptr(out)), "gev_pcls_copy_index")
+        return out
+
+    def sort(self):
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_sort(self._h), "gev_pcls_sort")
+
+    def init_lattice(self):
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_init_lattice(self._h), "gev_pcls_init_lattice")
+
+    # ---------------------------------------------------------- deposits
+    def project_T00(self, out, a=1.0, phi=None, coeff=1.0, accumulate=False):
+        self.ctx.bind_stream()
+        check(lib.gev_project_T00(self._h, _ptr(out), a, _ptr(phi), coeff, int(accumulate)), "gev_project_T00")
+        return out
+
+    def project_mass(self, out, coeff=1.0, accumulate=False):
+        self.ctx.bind_stream()
+        check(lib.gev_project_mass(self._h, _ptr(out), coeff, int(accumulate)), "gev_project_mass")
+        return out
+
+    def project_T0i(self, out, phi=None, coeff=1.0, accumulate=False):
+        self.ctx.bind_stream()
+        check(lib.gev_project_T0i(self._h, _ptr(out), _ptr(phi), coeff, int(accumulate)), "gev_project_T0i")
+        return out
+
+    def project_Tij(self, out, a=1.0, phi=None, coeff=1.0, accumulate=False):
+        self.ctx.bind_stream()
+        check(lib.gev_project_Tij(self._h, _ptr(out), a, _ptr(phi), coeff, int(accumulate)), "gev_project_Tij")
+        return out
+
+    # ------------------------------------------------------------ updates
+    def update_q(self, dtau, phi, chi, Bi, nfield, a, bcoeff):
+        r = C.c_double()
+        self.ctx.bind_stream()
+        check(lib.gev_update_q(self._h, dtau, _ptr(phi), _ptr(chi), _ptr(Bi), nfield, a, bcoeff, C.byref(r)),
+              "gev_update_q")
+        return r.value
+
+    def update_q_newton(self, dtau, psi, a, chi=None, nfield=1):
+        r = C.c_double()
+        self.ctx.bind_stream()
+        check(lib.gev_update_q_newton(self._h, dtau, _ptr(psi), _ptr(chi), nfield, a, C.byref(r)),
+              "gev_update_q_newton")
+        return r.value
+
+    def update_pos(self, dtau, phi, chi, Bi, nfield, a, bcoeff):
+        self.ctx.bind_stream()
+        check(lib.gev_update_pos(self._h, dtau, _ptr(phi), _ptr(chi), _ptr(Bi), nfield, a, bcoeff), "gev_update_pos")
+
+    def update_pos_newton(self, dtau, a):
+        self.ctx.bind_stream()
+        check(lib.gev_update_pos_newton(self._h, dtau, a), "gev_update_pos_newton")
+
+    def displace(self, xi, coeff=1.0):
+        r = C.c_double()
+        self.ctx.bind_stream()
+        check(lib.gev_displace_particles(self._h, _ptr(xi), coeff, C.byref(r)), "gev_displace_particles")
+        return r.value
+
+    def initialize_q(self, pot, coeff):
+        r = C.c_double()
+        self.ctx.bind_stream()
+        check(lib.gev_initialize_q(self._h, _ptr(pot), coeff, C.byref(r)), "gev_initialize_q")
+        return r.value
+
+
+class Fields:
+    """The metric / source / work fields of main.cu (phi, chi, Bi, source, Sij and their FT work)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.phi, self.chi, self.Bi = ctx.real(1), ctx.real(1), ctx.real(3)
+        self.source, self.Sij = ctx.real(1), ctx.real(6)
+        self.scalarFT, self.SijFT, self.BiFT = ctx.fourier(1), ctx.fourier(6), ctx.fourier(3)
+
+    def c_struct(self):
+        f = gev_fields()
+        for name in ("phi", "chi", "Bi", "source", "Sij", "scalarFT", "SijFT", "BiFT"):
+            setattr(f, name, getattr(self, name).data_ptr())
+        return f
+
+    def copy_from(self, other):
+        for name in ("phi", "chi", "Bi"):
+            getattr(self, name).copy_(getattr(other, name).to(getattr(self, name).dtype))
+
+
+class StepState:
+    """Scalars carried by the main loop (a, tau, dtau, dtau_old, maxvel ...)."""
+
+    def __init__(self, gr, fourpiG, Omega_m, Omega_Lambda, Omega_rad=0.0, z_relax=None, elliptic=True):
+        s = gev_step_state()
+        s.gr = int(gr); s.vector_elliptic = int(elliptic)
+        s.fourpiG = fourpiG; s.Omega_m = Omega_m; s.Omega_Lambda = Omega_Lambda; s.Omega_rad = Omega_rad
+        s.z_relax = -2.0 if z_relax is None else z_relax
+        self.s = s
+
+    def __getattr__(self, k):
+        if k == "s":
+            raise AttributeError(k)
+        return getattr(self.s, k)
+
+    def __setattr__(self, k, v):
+        if k == "s":
+            object.__setattr__(self, k, v)
+        else:
+            setattr(self.s, k, v)
+
+
+def fourpiG_of(boxsize):
+    """4 pi G in box units (main.cu:418, C_SPEED_OF_LIGHT of metadata.hpp:97)."""
+    return 1.5 * boxsize * boxsize / 2997.92458 / 2997.92458
+
+
+def hconf(a, fourpiG, om, ol, orad=0.0):
+    return lib.gev_hconf(a, fourpiG, om, ol, orad)
+
+
+def rk4bg(a, fourpiG, om, ol, dtau, orad=0.0):
+    return lib.gev_rk4bg(a, fourpiG, om, ol, orad, dtau)
+
+
+def particle_horizon(a, fourpiG, om, ol, orad=0.0):
+    return lib.gev_particle_horizon(a, fourpiG, om, ol, orad)
+
+
+def ic_generate(pcls, fields, state, seed, boxsize, z_in, k_hmpc, P_mpch3, ksphere=1):
+    import numpy as np
+    k = np.ascontiguousarray(k_hmpc, np.float64)
+    P = np.ascontiguousarray(P_mpch3, np.float64)
+    if state.z_relax < -1.0:
+        state.z_relax = z_in  # parser.hpp:1468 default
+    md = C.c_double()
+    pcls.ctx.bind_stream()
+    f = fields.c_struct()
+    check(lib.gev_ic_generate(pcls.handle, C.byref(f), C.byref(state.s), seed, ksphere, boxsize, z_in, len(k),
+                              k.ctypes.data_as(C.POINTER(C.c_double)), P.ctypes.data_as(C.POINTER(C.c_double)),
+                              C.byref(md)), "gev_ic_generate")
+    return md.value
+
+
+def step(pcls, fields, state):
+    pcls.ctx.bind_stream()
+    f = fields.c_struct()
+    check(lib.gev_step(pcls.handle, C.byref(f), C.byref(state.s)), "gev_step")
+    return state

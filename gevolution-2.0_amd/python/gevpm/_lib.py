@@ -1,0 +1,106 @@
+"""ctypes binding of libgevpm.so (the C ABI declared in include/gev_pm.h).
+
+The library is loaded from the in-tree build (gevolution-2.0_amd/lib); there is
+no CPU fallback: if the shared object is missing or fails to load, importing
+this module raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "lib", "libgevpm.so"))
+
+GEV_OK, GEV_ERR_ARG, GEV_ERR_HIP, GEV_ERR_STATE, GEV_ERR_NOMEM = 0, -1, -2, -3, -4
+GEV_FP64, GEV_MIXED = 0, 1
+
+
+class GevError(RuntimeError):
+    pass
+
+
+class gev_pcls_view(C.Structure):
+    _fields_ = [("n", C.c_int64), ("precision", C.c_int), ("pos", C.c_void_p * 3), ("q", C.c_void_p * 3),
+                ("id", C.c_void_p), ("cell", C.c_void_p), ("cell_start", C.c_void_p)]
+
+
+class gev_fields(C.Structure):
+    _fields_ = [("phi", C.c_void_p), ("chi", C.c_void_p), ("Bi", C.c_void_p), ("source", C.c_void_p),
+                ("Sij", C.c_void_p), ("scalarFT", C.c_void_p), ("SijFT", C.c_void_p), ("BiFT", C.c_void_p)]
+
+
+class gev_step_state(C.Structure):
+    _fields_ = [("gr", C.c_int), ("vector_elliptic", C.c_int), ("fourpiG", C.c_double), ("Omega_m", C.c_double),
+                ("Omega_Lambda", C.c_double), ("Omega_rad", C.c_double), ("z_relax", C.c_double),
+                ("a", C.c_double), ("tau", C.c_double), ("dtau", C.c_double), ("dtau_old", C.c_double),
+                ("maxvel", C.c_double), ("T00hom", C.c_double), ("phi_hom", C.c_double)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/gev_pm.h
+_vp, _i, _d, _i64, _u32 = C.c_void_p, C.c_int, C.c_double, C.c_int64, C.c_uint32
+_dp = C.POINTER(C.c_double)
+SIGNATURES = {
+    "gev_last_error": (C.c_char_p, []),
+    "gev_version": (_i, []),
+    "gev_ctx_create": (_i, [_i, _i, _vp, C.POINTER(_vp)]),
+    "gev_ctx_destroy": (_i, [_vp]),
+    "gev_ctx_sync": (_i, [_vp]),
+    "gev_ctx_kpitch": (_i, [_vp]),
+    "gev_ctx_set_stream": (_i, [_vp, _vp]),
+    "gev_pcls_create": (_i, [_vp, _i64, _d, C.POINTER(_vp)]),
+    "gev_pcls_destroy": (_i, [_vp]),
+    "gev_pcls_load": (_i, [_vp, _i64, _vp, _vp, _vp, _i]),
+    "gev_pcls_store": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "gev_pcls_view_get": (_i, [_vp, C.POINTER(gev_pcls_view)]),
+    "gev_pcls_sort": (_i, [_vp]),
+    "gev_pcls_init_lattice": (_i, [_vp]),
+    "gev_pcls_copy_index": (_i, [_vp, _vp, _vp]),
+    "gev_project_T00": (_i, [_vp, _vp, _d, _vp, _d, _i]),
+    "gev_project_mass": (_i, [_vp, _vp, _d, _i]),
+    "gev_project_T0i": (_i, [_vp, _vp, _vp, _d, _i]),
+    "gev_project_Tij": (_i, [_vp, _vp, _d, _vp, _d, _i]),
+    "gev_prepare_source_T00": (_i, [_vp, _vp, _vp, _vp, _d, _vp, _d, _d, _d, _dp, _dp]),
+    "gev_prepare_source_Tij": (_i, [_vp, _vp, _vp, _vp, _d]),
+    "gev_fft_r2c": (_i, [_vp, _vp, _vp, _i]),
+    "gev_fft_c2r": (_i, [_vp, _vp, _vp, _i]),
+    "gev_poisson": (_i, [_vp, _vp, _vp, _d, _d]),
+    "gev_project_ft_scalar": (_i, [_vp, _vp, _vp, _i]),
+    "gev_project_ft_vector": (_i, [_vp, _vp, _vp, _d, _d]),
+    "gev_evolve_ft_vector": (_i, [_vp, _vp, _vp, _d]),
+    "gev_set_k0": (_i, [_vp, _vp, _d, _d]),
+    "gev_update_q": (_i, [_vp, _d, _vp, _vp, _vp, _i, _d, _d, _dp]),
+    "gev_update_q_newton": (_i, [_vp, _d, _vp, _vp, _i, _d, _dp]),
+    "gev_update_pos": (_i, [_vp, _d, _vp, _vp, _vp, _i, _d, _d]),
+    "gev_update_pos_newton": (_i, [_vp, _d, _d]),
+    "gev_ic_displacement_ft": (_i, [_vp, _u32, _i, _d, _i, _dp, _dp, _vp, _vp]),
+    "gev_displace_particles": (_i, [_vp, _vp, _d, _dp]),
+    "gev_initialize_q": (_i, [_vp, _vp, _d, _dp]),
+    "gev_hconf": (_d, [_d, _d, _d, _d, _d]),
+    "gev_rk4bg": (_d, [_d, _d, _d, _d, _d, _d]),
+    "gev_particle_horizon": (_d, [_d, _d, _d, _d, _d]),
+    "gev_step": (_i, [_vp, C.POINTER(gev_fields), C.POINTER(gev_step_state)]),
+    "gev_ic_generate": (_i, [_vp, C.POINTER(gev_fields), C.POINTER(gev_step_state), _u32, _i, _d, _d, _i, _dp, _dp,
+                             _dp]),
+    "gev_field_sum": (_i, [_vp, _vp, _i, _dp]),
+    "gev_field_fill": (_i, [_vp, _vp, _i64, _d]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libgevpm.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc, what=""):
+    if rc != GEV_OK:
+        msg = lib.gev_last_error().decode(errors="replace")
+        raise GevError(f"{what} failed ({rc}): {msg}")
+    return rc

@@ -1,0 +1,176 @@
+/*
+ * gev_pm.h -- C ABI of the MI355X-native gevolution particle-mesh step.
+ *
+ * One context (gev_ctx) = one periodic lattice of N^3 sites (N a power of two,
+ * 4 <= N <= 1024) in one precision form on one GPU, with one HIP stream.
+ *
+ * Memory layout of every buffer passed in (device pointers, caller-owned):
+ *   real field     T[ncomp][N^3], site (x,y,z) at x + N*(y + N*z); no halo,
+ *                  periodic indexing is done inside the kernels.  Symmetric
+ *                  tensors (Tij, Sij) store components 00,01,02,11,12,22.
+ *   Fourier field  complex T2[ncomp][N][N][P] (re,im interleaved); mode
+ *                  (kx,ky,kz) at kx + P*(ky + N*kz), kx in [0,N/2], ky,kz in
+ *                  [0,N); P = gev_ctx_kpitch() >= N/2+1 (padding columns are
+ *                  zero after a forward transform and ignored otherwise).
+ *   T = double for GEV_FP64 contexts, float for GEV_MIXED contexts.
+ *   Forward transforms use exp(-2 pi i k.x/N); both directions are
+ *   unnormalised, as LATfield2's PlanFFT (gevolution.hpp:817 divides by N^3).
+ *
+ * Particles live in a library-owned container (gev_pcls), the analogue of
+ * LATfield2's perfParticles used by perfParticles_gevolution
+ * (Particles_gevolution.hpp:176-211).  They are kept sorted by cell
+ * (cell = x + N*(y + N*z) with x = floor(pos*N)).  GEV_FP64 stores absolute
+ * positions in box units (double); GEV_MIXED stores FP32 cell-relative
+ * offsets in [0,1) next to the int32 cell.  Load/store always exchange FP64
+ * absolute positions.
+ *
+ * Every entry point returns GEV_OK (0) or a negative code; gev_last_error()
+ * gives the message.  Calls are asynchronous on the context stream unless
+ * they return a host value (reductions), which synchronises.  A context is
+ * not re-entrant: one host thread at a time.
+ */
+#ifndef GEV_PM_H
+#define GEV_PM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GEV_OK = 0, GEV_ERR_ARG = -1, GEV_ERR_HIP = -2, GEV_ERR_STATE = -3, GEV_ERR_NOMEM = -4 };
+enum { GEV_FP64 = 0, GEV_MIXED = 1 };
+
+typedef struct gev_ctx gev_ctx;
+typedef struct gev_pcls gev_pcls;
+
+/* raw device view of a particle container (valid until the next re-binning) */
+typedef struct {
+    int64_t n;
+    int precision;
+    void *pos[3];        /* double* (FP64, box units) or float* (MIXED, cell offsets) */
+    void *q[3];          /* double* or float*: q/m ("v a", gevolution.hpp:860-863) */
+    int64_t *id;
+    int32_t *cell;
+    int32_t *cell_start; /* [N^3 + 1] */
+} gev_pcls_view;
+
+const char *gev_last_error(void);
+int gev_version(void);
+
+/* ---- context (replaces LATfield2 Lattice + PlanFFT setup, main.cu:300-357) */
+int gev_ctx_create(int N, int precision, void *hip_stream, gev_ctx **out);
+int gev_ctx_destroy(gev_ctx *ctx);
+int gev_ctx_sync(gev_ctx *ctx);
+int gev_ctx_kpitch(const gev_ctx *ctx);
+/* launch subsequent work on this stream (NULL = the legacy default stream) */
+int gev_ctx_set_stream(gev_ctx *ctx, void *hip_stream);
+
+/* ---- particles (perfParticles: initialize / addParticle_global /
+ *      updateRowBuffers, ic_basic.hpp:1639,2206-2211; sampleParticles,
+ *      unit_tests.cu:529,537) */
+int gev_pcls_create(gev_ctx *ctx, int64_t capacity, double mass, gev_pcls **out);
+int gev_pcls_destroy(gev_pcls *p);
+int gev_pcls_load(gev_pcls *p, int64_t n, const double *pos, const double *q, const int64_t *id, int on_device);
+int gev_pcls_store(gev_pcls *p, double *pos, double *q, int64_t *id, int on_device);
+int gev_pcls_view_get(gev_pcls *p, gev_pcls_view *v);
+int gev_pcls_sort(gev_pcls *p);
+/* copy the cell index of every particle (n int32) and/or the cell offsets
+ * (N^3+1 int32) into caller device buffers; either may be NULL */
+int gev_pcls_copy_index(gev_pcls *p, int32_t *cell, int32_t *cell_start);
+/* sc1-type lattice: one particle at every cell centre, ID = cell index
+ * (initializeParticlePositions, ic_basic.hpp:1612-1644, tiling N/4 of the
+ * 64-particle cell-centred template) */
+int gev_pcls_init_lattice(gev_pcls *p);
+
+/* ---- CIC projections (out = deposit, or out += deposit if accumulate) */
+/* projection_T00_project, gevolution.hpp:1887-1899 (functor :1729-1854);
+ * phi == NULL gives the plain mass density (nfield = 1 branch) */
+int gev_project_T00(gev_pcls *p, void *T00, double a, const void *phi, double coeff, int accumulate);
+/* scalarProjectionCIC_project / meshprojection_project, gevolution.hpp:1902-1905 */
+int gev_project_mass(gev_pcls *p, void *rho, double coeff, int accumulate);
+/* projection_T0i_project, gevolution.hpp:2232-2245 (functor :2031-2201) */
+int gev_project_T0i(gev_pcls *p, void *T0i, const void *phi, double coeff, int accumulate);
+/* projection_Tij_project, gevolution.hpp:2999-3033 (functor :2582-2965), no CIC_PROJECT_TIJ */
+int gev_project_Tij(gev_pcls *p, void *Tij, double a, const void *phi, double coeff, int accumulate);
+
+/* ---- real-space sources */
+/* prepareFTsource (T00 form, PHINONLINEAR), gevolution.hpp:160-218.
+ * *sum_source = sum of source before the update (the reference's return
+ * value); *sum_result = FP64 sum of the result (used for the k=0 mode). */
+int gev_prepare_source_T00(gev_ctx *ctx, const void *phi, const void *chi, const void *source, double bgmodel,
+                           void *result, double c1, double c2, double c3, double *sum_source, double *sum_result);
+/* prepareFTsource (Tij form, PHINONLINEAR), gevolution.hpp:64-138 */
+int gev_prepare_source_Tij(gev_ctx *ctx, const void *phi, const void *Tij, void *Sij, double coeff);
+
+/* ---- FFT (PlanFFT<Cplx>::execute FFT_FORWARD / FFT_BACKWARD, main.cu:742-948) */
+int gev_fft_r2c(gev_ctx *ctx, const void *in, void *out, int ncomp);
+int gev_fft_c2r(gev_ctx *ctx, void *in, void *out, int ncomp); /* clobbers in */
+
+/* ---- k-space solvers */
+int gev_poisson(gev_ctx *ctx, const void *src, void *pot, double coeff, double modif);  /* gevolution.hpp:806-852 */
+int gev_project_ft_scalar(gev_ctx *ctx, const void *Sij, void *chi, int add);           /* gevolution.hpp:251-340 */
+int gev_project_ft_vector(gev_ctx *ctx, const void *Si, void *Bi, double coeff, double modif); /* :449-515 */
+int gev_evolve_ft_vector(gev_ctx *ctx, const void *Sij, void *Bi, double a2dtau);       /* gevolution.hpp:359-429 */
+/* overwrite the k=0 mode (used to restore the FP64 mean in the mixed form) */
+int gev_set_k0(gev_ctx *ctx, void *ft, double re, double im);
+
+/* ---- particle updates (perfParticles::updateVel / moveParticles with the
+ *      gevolution functors, main.cu:1227-1266).  *max_ret receives the
+ *      maximum over particles of the functor's return value (v^2 / a^2). */
+int gev_update_q(gev_pcls *p, double dtau, const void *phi, const void *chi, const void *Bi, int nfield, double a,
+                 double bcoeff, double *max_ret);                            /* update_q, gevolution.hpp:888-1011 */
+int gev_update_q_newton(gev_pcls *p, double dtau, const void *psi, const void *chi, int nfield, double a,
+                        double *max_ret);                                    /* gevolution.hpp:1051-1118 */
+int gev_update_pos(gev_pcls *p, double dtau, const void *phi, const void *chi, const void *Bi, int nfield, double a,
+                   double bcoeff);                                           /* gevolution.hpp:1162-1232 + re-bin */
+int gev_update_pos_newton(gev_pcls *p, double dtau, double a);               /* gevolution.hpp:1270-1273 + re-bin */
+
+/* ---- initial conditions (generateIC_basic mPk path, ic_basic.hpp:1834-2505) */
+/* displacement potential in k-space (generateDisplacementField, ic_basic.hpp:1164-1270);
+ * kL/amp: power-spectrum table converted as loadPowerSpectrum (ic_basic.hpp:445-447) */
+int gev_ic_displacement_ft(gev_ctx *ctx, uint32_t seed, int ksphere, double coeff, int npk, const double *kL,
+                           const double *amp, const void *kerFT, void *potFT);
+/* displace_pcls_ic_basic (ic_basic.hpp:65-96) + re-bin; *max_disp = max |displacement| */
+int gev_displace_particles(gev_pcls *p, const void *xi, double coeff, double *max_disp);
+/* initialize_q_ic_basic (ic_basic.hpp:133-168); *max_ret = max q^2 */
+int gev_initialize_q(gev_pcls *p, const void *pot, double coeff, double *max_ret);
+
+/* ---- background (background.hpp:200-290), Omega_rad/ncdm/fld as given */
+double gev_hconf(double a, double fourpiG, double Omega_m, double Omega_Lambda, double Omega_rad);
+double gev_rk4bg(double a, double fourpiG, double Omega_m, double Omega_Lambda, double Omega_rad, double dtau);
+double gev_particle_horizon(double a, double fourpiG, double Omega_m, double Omega_Lambda, double Omega_rad);
+
+/* ---- the main-loop cycle (main.cu:608-1317) on one GPU.  Fields are
+ *      caller-owned device buffers in the context precision. */
+typedef struct {
+    void *phi, *chi, *Bi, *source, *Sij; /* real: 1,1,3,1,6 components */
+    void *scalarFT, *SijFT, *BiFT;       /* Fourier work: 1,6,3 components */
+} gev_fields;
+
+typedef struct {
+    int gr;              /* gravity theory: 1 = GR (weak field), 0 = Newton */
+    int vector_elliptic; /* vector method: 1 = elliptic, 0 = parabolic */
+    double fourpiG, Omega_m, Omega_Lambda, Omega_rad, z_relax;
+    double a, tau, dtau, dtau_old; /* in/out; dtau is the step to take */
+    double maxvel, T00hom, phi_hom; /* out */
+} gev_step_state;
+
+int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st);
+
+/* basic IC generator, mPk path (generateIC_basic, ic_basic.hpp:1834-2505,
+ * standard CIC kernel).  Requires a GEV_FP64 context; particles are placed
+ * on the cell-centred lattice (N^3 of them).  k_hmpc/P_mpch3: the mPk table
+ * (k in h/Mpc, P in (Mpc/h)^3).  On return st->a = 1/(1+z_in), st->tau =
+ * particleHorizon(a), dtau_old = 0, maxvel set; *max_disp in box units. */
+int gev_ic_generate(gev_pcls *p, const gev_fields *f, gev_step_state *st, uint32_t seed, int ksphere, double boxsize,
+                    double z_in, int npk, const double *k_hmpc, const double *P_mpch3, double *max_disp);
+
+/* ---- reductions / utilities */
+int gev_field_sum(gev_ctx *ctx, const void *f, int ncomp, double *sum);
+int gev_field_fill(gev_ctx *ctx, void *f, int64_t count, double value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

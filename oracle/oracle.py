@@ -1,0 +1,247 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/libgevoracle).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU baseline,
+never as part of the product path.
+
+Array conventions (numpy):
+  real field     float64 [N^3]  (x fastest), components stacked [c, N^3]
+  Fourier field  complex128 [N, N, N//2+1] indexed [kz, ky, kx] (numpy rfftn
+                 layout of an array indexed [z, y, x]); components stacked first
+  particles      pos, q float64 [3, np]; id int64 [np]
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libgevoracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _opt(a):
+    return None if a is None else np.ascontiguousarray(a)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    I, D, Lg, V = C.c_int, C.c_double, C.c_long, None
+    vp = C.c_void_p
+    sig = {
+        "or_version": (I, []),
+        "or_set_threads": (V, [I]),
+        "or_get_threads": (I, []),
+        "or_fft_r2c": (V, [I, _dp, _cp]),
+        "or_fft_c2r": (V, [I, _cp, _dp]),
+        "or_poisson": (V, [I, _cp, _cp, D, D]),
+        "or_project_ft_scalar": (V, [I, _cp, _cp, I]),
+        "or_project_ft_vector": (V, [I, _cp, _cp, D, D]),
+        "or_evolve_ft_vector": (V, [I, _cp, _cp, D]),
+        "or_prepare_source_T00": (D, [I, _dp, _dp, _dp, D, _dp, D, D, D]),
+        "or_prepare_source_Tij": (V, [I, _dp, _dp, _dp, D]),
+        "or_project_T00": (V, [I, Lg, _dp, _dp, D, _dp, D, vp, D]),
+        "or_project_T0i": (V, [I, Lg, _dp, _dp, D, _dp, vp, D]),
+        "or_project_Tij": (V, [I, Lg, _dp, _dp, D, _dp, D, vp, D]),
+        "or_update_q": (D, [I, Lg, _dp, _dp, D, _dp, vp, vp, I, D, D]),
+        "or_update_q_newton": (D, [I, Lg, _dp, _dp, D, _dp, vp, I, D]),
+        "or_update_pos": (V, [I, Lg, _dp, _dp, D, vp, vp, vp, I, D, D]),
+        "or_update_pos_newton": (V, [I, Lg, _dp, _dp, D, D]),
+        "or_displace_particles": (D, [I, Lg, _dp, _dp, D]),
+        "or_initialize_q": (D, [I, Lg, _dp, _dp, _dp, D]),
+        "or_hconf": (D, [D, D, D, D, D]),
+        "or_rk4bg": (D, [D, D, D, D, D, D]),
+        "or_particle_horizon": (D, [D, D, D, D, D]),
+        "or_philox4x32_10": (V, [_u32p, _u32p, _u32p]),
+        "or_spline_eval": (D, [I, _dp, _dp, D]),
+        "or_ic_displacement_ft": (V, [I, C.c_uint32, I, D, I, _dp, _dp, _cp, _cp]),
+        "or_pk_table": (V, [I, _dp, _dp, D, _dp, _dp]),
+        "or_ic_generate": (D, [I, C.c_uint32, I, I, D, D, D, D, D, I, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp,
+                               _dp, _dp, C.POINTER(C.c_double)]),
+        "or_step": (V, [I, Lg, I, D, D, D, D, D, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "or_power_spectrum": (V, [I, _cp, I, I, _dp, _dp, _dp, _dp, _i32p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def set_threads(n):
+    lib().or_set_threads(int(n))
+
+
+def get_threads():
+    return lib().or_get_threads()
+
+
+def fft_r2c(f):
+    N = round(len(f) ** (1 / 3))
+    out = np.zeros((N, N, N // 2 + 1), np.complex128)
+    lib().or_fft_r2c(N, np.ascontiguousarray(f, np.float64), out)
+    return out
+
+
+def fft_c2r(F):
+    N = F.shape[0]
+    out = np.zeros(N ** 3, np.float64)
+    lib().or_fft_c2r(N, np.ascontiguousarray(F, np.complex128), out)
+    return out
+
+
+def poisson(F, coeff, modif=0.0):
+    out = np.empty_like(F)
+    lib().or_poisson(F.shape[0], np.ascontiguousarray(F), out, coeff, modif)
+    return out
+
+
+def project_ft_scalar(S, chi=None):
+    N = S.shape[1]
+    add = chi is not None
+    out = np.ascontiguousarray(chi.copy()) if add else np.zeros(S.shape[1:], np.complex128)
+    lib().or_project_ft_scalar(N, np.ascontiguousarray(S), out, int(add))
+    return out
+
+
+def project_ft_vector(S, coeff=1.0, modif=0.0):
+    out = np.empty_like(S)
+    lib().or_project_ft_vector(S.shape[1], np.ascontiguousarray(S), out, coeff, modif)
+    return out
+
+
+def evolve_ft_vector(S, B, a2dtau):
+    out = np.ascontiguousarray(B.copy())
+    lib().or_evolve_ft_vector(S.shape[1], np.ascontiguousarray(S), out, a2dtau)
+    return out
+
+
+def prepare_source_T00(phi, chi, source, bg, c1, c2, c3):
+    N = round(len(phi) ** (1 / 3))
+    out = np.empty_like(source)
+    s = lib().or_prepare_source_T00(N, phi, chi, source, bg, out, c1, c2, c3)
+    return out, s
+
+
+def prepare_source_Tij(phi, Tij, coeff):
+    N = round(len(phi) ** (1 / 3))
+    out = np.empty_like(Tij)
+    lib().or_prepare_source_Tij(N, phi, np.ascontiguousarray(Tij), out, coeff)
+    return out
+
+
+def project_T00(N, pos, q, pmass, a=1.0, phi=None, coeff=1.0, out=None):
+    out = np.zeros(N ** 3) if out is None else out
+    lib().or_project_T00(N, pos.shape[1], pos, q, pmass, out, a, _ptr(_opt(phi)), coeff)
+    return out
+
+
+def project_T0i(N, pos, q, pmass, phi=None, coeff=1.0, out=None):
+    out = np.zeros((3, N ** 3)) if out is None else out
+    lib().or_project_T0i(N, pos.shape[1], pos, q, pmass, out, _ptr(_opt(phi)), coeff)
+    return out
+
+
+def project_Tij(N, pos, q, pmass, a=1.0, phi=None, coeff=1.0, out=None):
+    out = np.zeros((6, N ** 3)) if out is None else out
+    lib().or_project_Tij(N, pos.shape[1], pos, q, pmass, out, a, _ptr(_opt(phi)), coeff)
+    return out
+
+
+def update_q(N, pos, q, dtau, phi, chi, Bi, nfield, a, bcoeff):
+    phi, chi, Bi = _opt(phi), _opt(chi), _opt(Bi)
+    return lib().or_update_q(N, pos.shape[1], pos, q, dtau, phi, _ptr(chi), _ptr(Bi), nfield, a, bcoeff)
+
+
+def update_q_newton(N, pos, q, dtau, psi, a, chi=None, nfield=1):
+    chi = _opt(chi)
+    return lib().or_update_q_newton(N, pos.shape[1], pos, q, dtau, psi, _ptr(chi), nfield, a)
+
+
+def update_pos(N, pos, q, dtau, phi, chi, Bi, nfield, a, bcoeff):
+    phi, chi, Bi = _opt(phi), _opt(chi), _opt(Bi)
+    lib().or_update_pos(N, pos.shape[1], pos, q, dtau, _ptr(phi), _ptr(chi), _ptr(Bi), nfield, a, bcoeff)
+
+
+def update_pos_newton(N, pos, q, dtau, a):
+    lib().or_update_pos_newton(N, pos.shape[1], pos, q, dtau, a)
+
+
+def displace_particles(N, pos, xi, coeff=1.0):
+    return lib().or_displace_particles(N, pos.shape[1], pos, xi, coeff)
+
+
+def initialize_q(N, pos, q, pot, coeff):
+    return lib().or_initialize_q(N, pos.shape[1], pos, q, pot, coeff)
+
+
+def hconf(a, fourpiG, om, ol, orad=0.0):
+    return lib().or_hconf(a, fourpiG, om, ol, orad)
+
+
+def rk4bg(a, fourpiG, om, ol, dtau, orad=0.0):
+    return lib().or_rk4bg(a, fourpiG, om, ol, orad, dtau)
+
+
+def particle_horizon(a, fourpiG, om, ol, orad=0.0):
+    return lib().or_particle_horizon(a, fourpiG, om, ol, orad)
+
+
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def spline_eval(x, y, xv):
+    return lib().or_spline_eval(len(x), np.asarray(x, np.float64), np.asarray(y, np.float64), float(xv))
+
+
+def ic_generate(N, seed, k_hmpc, P_mpch3, boxsize, fourpiG, om, ol, a, gr=0, ksphere=1):
+    M = N ** 3
+    pos = np.zeros((3, M)); q = np.zeros((3, M)); pid = np.zeros(M, np.int64)
+    phi = np.zeros(M); chi = np.zeros(M); Bi = np.zeros((3, M)); src = np.zeros(M); Sij = np.zeros((6, M))
+    md = C.c_double(0.0)
+    mv = lib().or_ic_generate(N, seed, ksphere, gr, boxsize, fourpiG, om, ol, a, len(k_hmpc),
+                              np.ascontiguousarray(k_hmpc, np.float64), np.ascontiguousarray(P_mpch3, np.float64),
+                              pos, q, pid, phi, chi, Bi, src, Sij, C.byref(md))
+    return dict(pos=pos, q=q, id=pid, phi=phi, chi=chi, Bi=Bi, source=src, Sij=Sij, maxvel=mv, max_disp=md.value)
+
+
+def step(N, st, gr, pmass, fourpiG, om, ol, z_relax):
+    """One main-loop cycle; st is a dict with arrays pos,q,phi,chi,Bi,source,Sij and
+    scalars a,tau,dtau,dtau_old (updated in place)."""
+    sc = np.array([st["a"], st["tau"], st["dtau"], st["dtau_old"], st.get("maxvel", 0.0), 0.0, 0.0])
+    lib().or_step(N, st["pos"].shape[1], gr, pmass, fourpiG, om, ol, z_relax, st["pos"], st["q"], st["phi"],
+                  st["chi"], st["Bi"], st["source"], st["Sij"], sc)
+    st["a"], st["tau"], st["dtau_old"], st["maxvel"], st["T00hom"], st["phi_hom"] = sc[0], sc[1], sc[3], sc[4], sc[5], sc[6]
+    return st
+
+
+def power_spectrum(FT, numbins, deconvolve=True):
+    N = FT.shape[0]
+    kb, pw, ks, ps = (np.zeros(numbins) for _ in range(4))
+    occ = np.zeros(numbins, np.int32)
+    lib().or_power_spectrum(N, np.ascontiguousarray(FT), numbins, int(deconvolve), kb, pw, ks, ps, occ)
+    return kb, pw, ks, ps, occ

@@ -1,0 +1,389 @@
+"""GPU parity: every HIP entry point against the FP64 oracle on identical
+seeded inputs, called through the C ABI.  Tolerances: FP64 paths are held to
+relative max-norm errors of 1e-13..1e-11 (reordered sums); FP32 (mixed form)
+paths to the single-precision tolerances written beside each check.  Particle
+results are compared per particle ID (the GPU keeps particles cell-sorted)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def gp():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gevpm
+    return gevpm
+
+
+def relmax(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+def real_np(t):
+    """(c,N,N,N) tensor -> (c, N^3) float64 array"""
+    a = to_np(t).astype(np.float64)
+    return a.reshape(a.shape[0], -1)
+
+
+def ft_np(ctx, T):
+    return to_np(ctx.unpad(T)).astype(np.complex128)
+
+
+def ft_gpu(ctx, F):
+    """numpy (c,N,N,N/2+1) complex -> padded device tensor"""
+    F = np.asarray(F)
+    if F.ndim == 3:
+        F = F[None]
+    out = ctx.fourier(F.shape[0])
+    out[..., : ctx.N // 2 + 1] = torch.from_numpy(F).to(ctx.cdtype).to(ctx.device)
+    return out
+
+
+def field_gpu(ctx, f, ncomp=1):
+    return torch.from_numpy(np.ascontiguousarray(f).reshape(ncomp, ctx.N, ctx.N, ctx.N)).to(ctx.rdtype).to(ctx.device)
+
+
+# --------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("N", [4, 8, 16, 32, 64, 128])
+def test_fft_fp64_vs_oracle_and_numpy(gp, O, N):
+    ctx = gp.Context(N, gp.GEV_FP64)
+    rng = np.random.default_rng(N)
+    f = rng.standard_normal((2, N ** 3))
+    F = ctx.fft_r2c(field_gpu(ctx, f, 2))
+    Fn = ft_np(ctx, F)
+    for c in range(2):
+        ref = np.fft.rfftn(f[c].reshape(N, N, N))
+        assert relmax(Fn[c], ref) < 1e-13
+        if N <= 64:
+            assert relmax(Fn[c], O.fft_r2c(f[c])) < 1e-13
+    # padding columns are zero
+    assert torch.all(F[..., N // 2 + 1:] == 0)
+    g = ctx.fft_c2r(F)
+    assert relmax(real_np(g) / N ** 3, f) < 1e-13
+    # non-Hermitian input: imaginary DC / Nyquist ignored like the oracle
+    G = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    g2 = ctx.fft_c2r(ft_gpu(ctx, G))
+    assert relmax(real_np(g2)[0], O.fft_c2r(G)) < 1e-12
+
+
+@pytest.mark.parametrize("N", [16, 64, 256])
+def test_fft_fp32(gp, N):
+    ctx = gp.Context(N, gp.GEV_MIXED)
+    rng = np.random.default_rng(N + 1)
+    f = rng.standard_normal(N ** 3).astype(np.float32)
+    F = ft_np(ctx, ctx.fft_r2c(field_gpu(ctx, f)))[0]
+    ref = np.fft.rfftn(f.astype(np.float64).reshape(N, N, N))
+    assert relmax(F, ref) < 3e-6  # FP32: ~eps * log2(N^3)
+    g = real_np(ctx.fft_c2r(ctx.fft_r2c(field_gpu(ctx, f))))[0] / N ** 3
+    assert relmax(g, f) < 3e-6
+
+
+def test_fft_512_roundtrip(gp):
+    N = 512
+    for prec, tol in ((gp.GEV_FP64, 1e-13), (gp.GEV_MIXED, 5e-6)):
+        ctx = gp.Context(N, prec)
+        g = torch.Generator(device="cuda").manual_seed(5)
+        f = torch.randn((1, N, N, N), generator=g, device="cuda", dtype=ctx.rdtype)
+        F = ctx.fft_r2c(f)
+        # Parseval
+        w = torch.full((N // 2 + 1,), 2.0, device="cuda", dtype=torch.float64)
+        w[0] = w[N // 2] = 1.0
+        e_k = (w * (ctx.unpad(F).abs().double() ** 2)).sum().item() / N ** 3
+        e_x = (f.double() ** 2).sum().item()
+        assert abs(e_k / e_x - 1) < tol * 10
+        h = ctx.fft_c2r(F, preserve=False)
+        err = ((h / N ** 3 - f).abs().max() / f.abs().max()).item()
+        assert err < tol
+        del F, h, f, ctx
+        torch.cuda.empty_cache()
+
+
+# ----------------------------------------------------------------- k-space
+def hermitian_ft(O, rng, N, ncomp):
+    return np.stack([O.fft_r2c(rng.standard_normal(N ** 3)) for _ in range(ncomp)])
+
+
+@pytest.mark.parametrize("N", [8, 32])
+def test_kspace_ops_fp64(gp, O, N):
+    ctx = gp.Context(N, gp.GEV_FP64)
+    rng = np.random.default_rng(100 + N)
+    S1 = hermitian_ft(O, rng, N, 1)[0]
+    for modif in (0.0, 12.5):
+        got = ft_np(ctx, ctx.poisson(ft_gpu(ctx, S1), 0.37, modif))[0]
+        assert relmax(got, O.poisson(S1, 0.37, modif)) < 1e-14
+    S6 = hermitian_ft(O, rng, N, 6)
+    got = ft_np(ctx, ctx.project_ft_scalar(ft_gpu(ctx, S6)))[0]
+    assert relmax(got, O.project_ft_scalar(S6)) < 1e-13
+    base = ft_gpu(ctx, S1)
+    got = ft_np(ctx, ctx.project_ft_scalar(ft_gpu(ctx, S6), base))[0]
+    assert relmax(got, O.project_ft_scalar(S6, S1)) < 1e-13
+    S3 = hermitian_ft(O, rng, N, 3)
+    for modif in (0.0, 3.0):
+        got = ft_np(ctx, ctx.project_ft_vector(ft_gpu(ctx, S3), 0.7, modif))
+        assert relmax(got, O.project_ft_vector(S3, 0.7, modif)) < 1e-13
+    B = ft_gpu(ctx, S3)
+    got = ft_np(ctx, ctx.evolve_ft_vector(ft_gpu(ctx, S6), B, 0.21))
+    assert relmax(got, O.evolve_ft_vector(S6, S3, 0.21)) < 1e-13
+
+
+def test_kspace_ops_mixed(gp, O):
+    N = 32
+    ctx = gp.Context(N, gp.GEV_MIXED)
+    rng = np.random.default_rng(7)
+    S6 = hermitian_ft(O, rng, N, 6).astype(np.complex64).astype(np.complex128)
+    got = ft_np(ctx, ctx.project_ft_scalar(ft_gpu(ctx, S6)))[0]
+    assert relmax(got, O.project_ft_scalar(S6)) < 1e-6  # FP64 combination, FP32 storage
+    S1 = S6[0]
+    got = ft_np(ctx, ctx.poisson(ft_gpu(ctx, S1), 0.37, 2.0))[0]
+    assert relmax(got, O.poisson(S1, 0.37, 2.0)) < 1e-6
+
+
+# ----------------------------------------------------------------- sources
+@pytest.mark.parametrize("N", [8, 32])
+def test_sources_fp64(gp, O, N):
+    ctx = gp.Context(N, gp.GEV_FP64)
+    rng = np.random.default_rng(200 + N)
+    phi = 1e-3 * rng.standard_normal(N ** 3)
+    chi = 1e-4 * rng.standard_normal(N ** 3)
+    src = 0.31 + rng.standard_normal(N ** 3)
+    args = (0.31, 0.2, 1.3, 0.07)
+    res, s1, s2 = ctx.prepare_source_T00(field_gpu(ctx, phi), field_gpu(ctx, chi), field_gpu(ctx, src), *args)
+    ref, sref = O.prepare_source_T00(phi, chi, src, *args)
+    assert relmax(real_np(res)[0], ref) < 1e-14
+    assert np.isclose(s1, sref, rtol=1e-12)
+    assert np.isclose(s2, ref.sum(), rtol=1e-12, atol=1e-12 * np.abs(ref).sum())
+    T = rng.standard_normal((6, N ** 3))
+    got = real_np(ctx.prepare_source_Tij(field_gpu(ctx, phi), field_gpu(ctx, T, 6), 0.9))
+    assert relmax(got, O.prepare_source_Tij(phi, T, 0.9)) < 1e-14
+    # in place
+    Tg = field_gpu(ctx, T, 6)
+    ctx.prepare_source_Tij(field_gpu(ctx, phi), Tg, 0.9, Sij=Tg)
+    assert relmax(real_np(Tg), O.prepare_source_Tij(phi, T, 0.9)) < 1e-14
+
+
+# --------------------------------------------------------------- particles
+def make_particles(N, n, seed, vmax=0.05, cluster=False):
+    rng = np.random.default_rng(seed)
+    if cluster:
+        # half the particles crammed into a handful of cells, the rest uniform
+        c = rng.integers(0, N, (3, 1)) / N
+        pos = np.concatenate([c + rng.uniform(0, 2.0 / N, (3, n // 2)), rng.uniform(0, 1, (3, n - n // 2))], 1) % 1.0
+    else:
+        pos = rng.uniform(0, 1, (3, n))
+    q = rng.uniform(-vmax, vmax, (3, n))
+    pid = rng.permutation(n).astype(np.int64) * 3 + 7
+    return pos, q, pid
+
+
+def gpu_pcls(gp, ctx, pos, q, pid, mass, cap=None):
+    p = gp.Particles(ctx, cap or max(pos.shape[1], 1), mass)
+    p.load(pos, q, pid)
+    return p
+
+
+def by_id(p):
+    pos, q, pid = p.store()
+    pos, q, pid = to_np(pos), to_np(q), to_np(pid)
+    o = np.argsort(pid)
+    return pos[:, o], q[:, o], pid[o]
+
+
+def test_sort_invariants(gp):
+    N = 32
+    ctx = gp.Context(N, gp.GEV_FP64)
+    pos, q, pid = make_particles(N, 50000, 1, cluster=True)
+    p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+    gpos, gq, gid = (to_np(t) for t in p.store())
+    assert np.array_equal(np.sort(gid), np.sort(pid))
+    cells = to_np(p.cells())
+    c = np.floor(gpos * N).astype(np.int64)
+    assert np.array_equal(cells, c[0] + N * (c[1] + N * c[2]))          # bit-exact keys
+    assert np.all(np.diff(cells) >= 0)                                    # sorted by cell
+    start = to_np(p.cell_start())
+    assert start[0] == 0 and start[-1] == 50000
+    assert np.array_equal(np.diff(start), np.bincount(cells, minlength=N ** 3))
+    o = np.argsort(gid); oi = np.argsort(pid)
+    assert np.array_equal(gpos[:, o], pos[:, oi]) and np.array_equal(gq[:, o], q[:, oi])
+
+
+@pytest.mark.parametrize("N,n,cluster", [(4, 300, False), (16, 20000, False), (32, 60000, True), (64, 100000, False)])
+def test_deposits_fp64(gp, O, N, n, cluster):
+    ctx = gp.Context(N, gp.GEV_FP64)
+    pos, q, pid = make_particles(N, n, 300 + N, vmax=0.2, cluster=cluster)
+    m, a = 0.31 / n, 0.05
+    p = gpu_pcls(gp, ctx, pos, q, pid, m)
+    phi = 1e-3 * np.random.default_rng(N).standard_normal(N ** 3)
+    gphi = field_gpu(ctx, phi)
+    out = ctx.real(1)
+    p.project_mass(out)
+    assert relmax(real_np(out)[0], O.project_T00(N, pos, q, m)) < 1e-13
+    p.project_T00(out, a, gphi, 1.3)
+    assert relmax(real_np(out)[0], O.project_T00(N, pos, q, m, a=a, phi=phi, coeff=1.3)) < 1e-13
+    p.project_T00(out, a, gphi, 1.3, accumulate=True)
+    assert relmax(real_np(out)[0], 2 * O.project_T00(N, pos, q, m, a=a, phi=phi, coeff=1.3)) < 1e-13
+    o3 = ctx.real(3)
+    for ph in (None, phi):
+        p.project_T0i(o3, None if ph is None else gphi, 0.8)
+        assert relmax(real_np(o3), O.project_T0i(N, pos, q, m, phi=ph, coeff=0.8)) < 1e-13
+    o6 = ctx.real(6)
+    for ph in (None, phi):
+        p.project_Tij(o6, a, None if ph is None else gphi, 0.9)
+        assert relmax(real_np(o6), O.project_Tij(N, pos, q, m, a=a, phi=ph, coeff=0.9)) < 1e-13
+
+
+def test_deposits_empty_and_single(gp, O):
+    N = 8
+    ctx = gp.Context(N, gp.GEV_FP64)
+    p = gp.Particles(ctx, 4, 1.0)
+    p.load(np.zeros((3, 0)), np.zeros((3, 0)), np.zeros(0, np.int64))
+    out = ctx.real(1)
+    out.fill_(5.0)
+    p.project_mass(out)
+    assert torch.all(out == 0)
+    pos = np.array([[0.999], [0.0], [0.5]])
+    p.load(pos, np.array([[0.1], [0.2], [0.3]]), np.array([1], np.int64))
+    p.project_Tij(ctx.real(6), 0.3)
+    p.project_mass(out)
+    assert relmax(real_np(out)[0], O.project_T00(N, pos, np.zeros((3, 1)), 1.0)) < 1e-14
+
+
+@pytest.mark.parametrize("N", [16, 64])
+def test_deposits_mixed(gp, O, N):
+    ctx = gp.Context(N, gp.GEV_MIXED)
+    n = 4 * N ** 3 // 8
+    pos, q, pid = make_particles(N, n, 400 + N, vmax=0.2)
+    m, a = 0.31 / n, 0.05
+    p = gpu_pcls(gp, ctx, pos, q, pid, m)
+    # the mixed form rounds offsets and momenta to FP32: compare with the oracle on those inputs
+    gpos, gq, gid = by_id(p)
+    o = np.argsort(pid)
+    assert np.abs(((gpos - pos[:, o]) + 0.5) % 1.0 - 0.5).max() < 1.0 / N * 1.2e-7
+    phi = 1e-3 * np.random.default_rng(N).standard_normal(N ** 3).astype(np.float32).astype(np.float64)
+    out = ctx.real(1)
+    p.project_T00(out, a, field_gpu(ctx, phi))
+    assert relmax(real_np(out)[0], O.project_T00(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
+    o6 = ctx.real(6)
+    p.project_Tij(o6, a, field_gpu(ctx, phi))
+    assert relmax(real_np(o6), O.project_Tij(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
+
+
+# ---------------------------------------------------------- kick / drift --
+@pytest.mark.parametrize("N", [8, 32])
+def test_kick_drift_fp64(gp, O, N):
+    ctx = gp.Context(N, gp.GEV_FP64)
+    n = 20000
+    pos, q, pid = make_particles(N, n, 500 + N, vmax=0.3)
+    o = np.argsort(pid)
+    rng = np.random.default_rng(N)
+    phi, chi = 1e-3 * rng.standard_normal(N ** 3), 1e-4 * rng.standard_normal(N ** 3)
+    B = 1e-2 * rng.standard_normal((3, N ** 3))
+    gphi, gchi, gB = field_gpu(ctx, phi), field_gpu(ctx, chi), field_gpu(ctx, B, 3)
+    a, bc = 0.2, 0.04 * N
+    for nf in (1, 2, 3):
+        p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+        qo = q.copy()
+        ret = p.update_q(0.013, gphi, gchi if nf >= 2 else None, gB if nf >= 3 else None, nf, a, bc)
+        rref = O.update_q(N, pos, qo, 0.013, phi, chi, B, nf, a, bc)
+        _, gq, _ = by_id(p)
+        assert relmax(gq, qo[:, o]) < 1e-13
+        assert np.isclose(ret, rref, rtol=1e-13)
+    p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+    qo = q.copy()
+    ret = p.update_q_newton(0.013, gphi, a)
+    rref = O.update_q_newton(N, pos, qo, 0.013, phi, a)
+    assert relmax(by_id(p)[1], qo[:, o]) < 1e-13 and np.isclose(ret, rref, rtol=1e-13)
+    for nf in (0, 1, 2, 3):
+        p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+        po = pos.copy()
+        p.update_pos(0.02, gphi, gchi, gB, nf, a, bc)
+        O.update_pos(N, po, q, 0.02, phi, chi, B, nf, a, bc)
+        gpos = by_id(p)[0]
+        assert np.abs(((gpos - po[:, o]) + 0.5) % 1.0 - 0.5).max() < 1e-15
+        cells = to_np(p.cells())
+        assert np.all(np.diff(cells) >= 0)
+    p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+    po = pos.copy()
+    p.update_pos_newton(0.02, a)
+    O.update_pos_newton(N, po, q, 0.02, a)
+    assert np.abs(((by_id(p)[0] - po[:, o]) + 0.5) % 1.0 - 0.5).max() < 1e-15
+    p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+    po = pos.copy()
+    d = p.displace(gphi, 0.5)
+    dref = O.displace_particles(N, po, phi, 0.5)
+    assert np.abs(((by_id(p)[0] - po[:, o]) + 0.5) % 1.0 - 0.5).max() < 1e-15 and np.isclose(d, dref, rtol=1e-13)
+    p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+    qo = np.zeros_like(q)
+    r = p.initialize_q(gphi, 3.0)
+    rref = O.initialize_q(N, pos, qo, phi, 3.0)
+    assert relmax(by_id(p)[1], qo[:, o]) < 1e-13 and np.isclose(r, rref, rtol=1e-13)
+
+
+# -------------------------------------------------------------- IC / step --
+C_LIGHT = 2997.92458
+
+
+def gpu_ic(gp, N, L, gr, seed=42, z_in=100.0, om=0.31):
+    import gevpm.cosmology as cosmo
+    k, P = cosmo.bbks_table(omega_m=om, z=z_in)
+    ctx = gp.Context(N, gp.GEV_FP64)
+    p = gp.Particles(ctx, N ** 3, om / N ** 3)
+    f = gp.Fields(ctx)
+    st = gp.StepState(gr, gp.fourpiG_of(L), om, 1 - om, z_relax=z_in)
+    md = gp.ic_generate(p, f, st, seed, L, z_in, k, P)
+    return ctx, p, f, st, md, (k, P)
+
+
+@pytest.mark.parametrize("gr", [0, 1])
+def test_ic_generate_vs_oracle(gp, O, gr):
+    N, L = 32, 320.0
+    ctx, p, f, st, md, (k, P) = gpu_ic(gp, N, L, gr)
+    ic = O.ic_generate(N, 42, k, P, L, gp.fourpiG_of(L), 0.31, 0.69, 1 / 101.0, gr=gr)
+    gpos, gq, gid = by_id(p)
+    assert np.array_equal(gid, np.arange(N ** 3))
+    assert np.abs(((gpos - ic["pos"]) + 0.5) % 1.0 - 0.5).max() < 1e-12 / N
+    assert relmax(gq, ic["q"]) < 1e-10
+    assert relmax(real_np(f.phi)[0], ic["phi"]) < 1e-10
+    assert relmax(real_np(f.chi)[0], ic["chi"]) < 1e-8
+    assert relmax(real_np(f.Bi), ic["Bi"]) < 1e-8
+    assert np.isclose(st.maxvel, ic["maxvel"], rtol=1e-10)
+    assert np.isclose(md, ic["max_disp"], rtol=1e-10)
+
+
+@pytest.mark.parametrize("gr,N,steps", [(0, 64, 10), (1, 32, 6)])
+def test_steps_vs_oracle(gp, O, gr, N, steps):
+    """C1-type Newton cycles and GR weak-field cycles from the same IC."""
+    L = 320.0 if gr == 0 else 640.0
+    ctx, p, f, st, md, _ = gpu_ic(gp, N, L, gr)
+    pos0, q0, pid = by_id(p)
+    M = N ** 3
+    ost = dict(pos=pos0.copy(), q=q0.copy(), phi=real_np(f.phi)[0].copy(), chi=real_np(f.chi)[0].copy(),
+               Bi=real_np(f.Bi).copy(), source=np.zeros(M), Sij=np.zeros((6, M)), a=st.a, tau=st.tau,
+               dtau_old=0.0, maxvel=st.maxvel)
+    fpg = gp.fourpiG_of(L)
+    a_end = 1 / 51.0 if gr == 0 else 1 / 80.0
+    t1 = gp.particle_horizon(a_end, fpg, 0.31, 0.69)
+    dtau = (t1 - st.tau) / steps
+    for _ in range(steps):
+        st.dtau = dtau
+        gp.step(p, f, st)
+        ost["dtau"] = dtau
+        O.step(N, ost, gr, 0.31 / M, fpg, 0.31, 0.69, 100.0)
+    gpos, gq, _ = by_id(p)
+    assert np.isclose(st.a, ost["a"], rtol=1e-14)
+    dpos = np.abs(((gpos - ost["pos"]) + 0.5) % 1.0 - 0.5).max()
+    assert dpos < 1e-11 / N, dpos
+    assert relmax(gq, ost["q"]) < 1e-9
+    assert relmax(real_np(f.phi)[0], ost["phi"]) < 1e-9
+    if gr:
+        assert relmax(real_np(f.chi)[0], ost["chi"]) < 1e-7
+        assert relmax(real_np(f.Bi), ost["Bi"]) < 1e-7
+    assert np.isclose(st.maxvel, ost["maxvel"], rtol=1e-9)

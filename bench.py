@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Throughput of the gevolution particle-mesh cycle on one MI355X.
+
+Workload (default): the weak-field GR cycle of main.cu (T00/Tij/T0i CIC
+projections, phi/chi/B solves, kick, drift + re-binning) with 512^3
+particles on a 512^3 mesh, L = 1280 Mpc/h, Zel'dovich ICs from a BBKS LCDM
+spectrum (Omega_m=0.31, h=0.67, n_s=0.965, sigma_8=0.81) at z=100, 100
+cycles uniform in conformal time to z=0, in the mixed-precision form.  The
+same run is repeated in FP64 from the same IC to report the FP64 rate and
+the P(k) / field errors of the mixed form.
+
+Prints one JSON line.  --gpus N > 1 is rejected: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gevolution-2.0_amd", "python"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s at 512^3 (FP64 vs mixed, P(k) err); adjoint/forward time ratio"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=98, help="timed cycles")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed cycles before the timed ones")
+    ap.add_argument("--mesh", type=int, default=512)
+    ap.add_argument("--boxsize", type=float, default=1280.0)
+    ap.add_argument("--z-in", type=float, default=100.0)
+    ap.add_argument("--z-end", type=float, default=0.0)
+    ap.add_argument("--total-steps", type=int, default=100, help="cycles spanning z_in -> z_end")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (no P(k) error)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-mesh", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the PM step on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is designed but not built.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import gevpm
+    import gevpm.cosmology as cosmo
+    from gevpm.spectra import density_power
+
+    N, L = args.mesh, args.boxsize
+    om, ol = 0.31, 0.69
+    fpg = gevpm.fourpiG_of(L)
+    k, P = cosmo.bbks_table(omega_m=om, h=0.67, n_s=0.965, sigma8=0.81, z=args.z_in)
+    total = args.total_steps
+    assert args.warmup + args.steps <= total, "warmup + steps must not exceed --total-steps"
+
+    # ---------------------------------------------------------------- IC (FP64)
+    t0 = time.time()
+    c64 = gevpm.Context(N, gevpm.GEV_FP64)
+    p64 = gevpm.Particles(c64, N ** 3, om / N ** 3)
+    f64 = gevpm.Fields(c64)
+    st64 = gevpm.StepState(1, fpg, om, ol, z_relax=args.z_in)
+    gevpm.ic_generate(p64, f64, st64, 42, L, args.z_in, k, P)
+    a_end = 1.0 / (1.0 + args.z_end)
+    dtau = (gevpm.particle_horizon(a_end, fpg, om, ol) - st64.tau) / total
+    torch.cuda.synchronize()
+    log(f"IC {N}^3 generated in {time.time() - t0:.1f}s, a={st64.a:.5f} tau={st64.tau:.4f} dtau={dtau:.5f}")
+
+    # ------------------------------------------------------- mixed-form copy
+    cm = gevpm.Context(N, gevpm.GEV_MIXED)
+    pm = gevpm.Particles(cm, N ** 3, om / N ** 3)
+    pos, q, pid = p64.store()
+    pm.load(pos, q, pid)
+    del pos, q, pid
+    fm = gevpm.Fields(cm)
+    fm.copy_from(f64)
+    stm = gevpm.StepState(1, fpg, om, ol, z_relax=args.z_in)
+    for key in ("a", "tau", "dtau_old", "maxvel"):
+        setattr(stm, key, getattr(st64, key))
+
+    def run(ctx, pcls, fields, st, label):
+        gevpm.step_timing(ctx, True)
+        for _ in range(args.warmup):
+            st.dtau = dtau
+            gevpm.step(pcls, fields, st)
+        phases = {}
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        w0 = time.time()
+        for _ in range(args.steps):
+            st.dtau = dtau
+            gevpm.step(pcls, fields, st)
+            for kname, v in gevpm.step_timings(ctx).items():
+                phases[kname] = phases.get(kname, 0.0) + v
+        e1.record()
+        torch.cuda.synchronize()
+        wall = time.time() - w0
+        ms = e0.elapsed_time(e1) / args.steps
+        for _ in range(total - args.warmup - args.steps):
+            st.dtau = dtau
+            gevpm.step(pcls, fields, st)
+        torch.cuda.synchronize()
+        log(f"{label}: {ms:.2f} ms/cycle (wall {1e3 * wall / args.steps:.2f}), z_final={1 / st.a - 1:.4f}")
+        return ms, {kk: v / args.steps for kk, v in phases.items()}
+
+    ms_m, ph_m = run(cm, pm, fm, stm, "mixed")
+    result = {}
+    if not args.no_fp64:
+        ms_64, ph_64 = run(c64, p64, f64, st64, "fp64")
+        # ---------------------------------------------- errors of the mixed form
+        nb = 64
+        kb, pw64, _, _, occ = density_power(c64, p64, nb, work=f64.source, F=f64.scalarFT)
+        _, pwm, _, _, _ = density_power(cm, pm, nb, work=fm.source, F=fm.scalarFT)
+        knyq = math.pi * N  # in units of 1/box (k L)
+        sel = (occ > 0) & (kb < 0.5 * knyq) & (kb > 0)
+        pk_err = float(((pwm[sel] - pw64[sel]).abs() / pw64[sel].abs()).max())
+
+        def rel_l2(a, b):
+            a, b = a.double(), b.double()
+            return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+        def rel_max(a, b):
+            a, b = a.double(), b.double()
+            return float((a - b).abs().max() / b.abs().max())
+
+        fields_err = {nm: {"rel_l2": rel_l2(getattr(fm, nm), getattr(f64, nm)),
+                           "rel_max": rel_max(getattr(fm, nm), getattr(f64, nm))} for nm in ("phi", "chi", "Bi")}
+        pa, _, ia = p64.store()
+        pb, _, ib = pm.store()
+        oa, ob = torch.argsort(ia), torch.argsort(ib)
+        d = torch.remainder(pb[:, ob] - pa[:, oa] + 0.5, 1.0) - 0.5
+        pos_err = {"rms_cells": float(d.pow(2).sum(0).mean().sqrt() * N), "max_cells": float(d.abs().max() * N)}
+        del pa, pb, ia, ib, d
+        result.update(fp64={"value": N ** 3 * 1e3 / ms_64, "ms_per_step": ms_64, "phases_ms": ph_64},
+                      mixed_over_fp64_speedup=ms_64 / ms_m, pk_rel_err_max=pk_err,
+                      pk_bins_compared=int(sel.sum()), field_errors=fields_err, position_errors=pos_err)
+    torch.cuda.empty_cache()
+
+    # ----------------------------------------------------- dominant kernel
+    roof = kernel_roofline(cm, pm, fm, args.kernel_reps, ph_m)
+    cpu = None if args.no_cpu else cpu_baseline(args, om, ol, k, P)
+    value = N ** 3 * 1e3 / ms_m
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_m,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"GR weak-field PM cycle (main.cu order), {N}^3 particles on {N}^3 mesh, "
+                               f"L={L:g} Mpc/h, BBKS Zel'dovich IC z={args.z_in:g}, {total} cycles to "
+                               f"z={args.z_end:g}, mixed precision (FP32 fields/FFT/deposit, FP64 k=0, "
+                               "chi combination, sums)",
+                   "mesh": N, "particles": N ** 3, "boxsize_mpc_h": L, "cycles": total},
+        "phases_ms": ph_m,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+def kernel_roofline(ctx, pcls, fields, reps, phases):
+    """Time the dominant kernel alone with HIP events on the library stream and
+    relate its algorithmic bytes per launch to the HBM peak."""
+    import torch
+    import gevpm
+    N, M, n = ctx.N, ctx.M, pcls.n
+    R = 4  # bytes per real in the mixed form
+    cands = {
+        "deposit_Tij": (lambda: pcls.project_Tij(fields.Sij, 0.5, fields.phi),
+                        n * 6 * R + M * (4 + R + 6 * R),
+                        "k_dep_Tij: per particle 3 offsets + 3 momenta read; per site cell offset + phi read, "
+                        "6 components written"),
+        "kick": (lambda: pcls.update_q(1e-9, fields.phi, fields.chi, fields.Bi, 3, 0.5, 0.25 * N),
+                 n * (6 * R + 4 + 3 * R) + M * 5 * R,
+                 "k_update_q: per particle 3 offsets + 3 momenta + cell read, 3 momenta written; phi, chi, B read"),
+        "deposit_T00": (lambda: pcls.project_T00(fields.source, 0.5, fields.phi),
+                        n * 6 * R + M * (4 + R + R),
+                        "k_dep_T00: per particle 3 offsets + 3 momenta read; per site cell offset + phi read, "
+                        "1 component written"),
+    }
+    name = max(cands, key=lambda kk: phases.get(kk, 0.0))
+    fn, bytes_per_launch, desc = cands[name]
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = bytes_per_launch / (ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": ms,
+            "algorithmic_bytes_per_launch": bytes_per_launch, "bytes_model": desc}
+
+
+def cpu_baseline(args, om, ol, k, P):
+    """The FP64 oracle (oracle/, a C + OpenMP restatement of the reference
+    step) timed on this host: GR cycles on a cpu-mesh^3 sample from the same
+    IC recipe.  The reference CPU build (LATfield2 + FFTW + MPI) cannot be
+    built here."""
+    import numpy as np
+    import torch
+    import gevpm
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    O.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    N, L = args.cpu_mesh, args.boxsize * args.cpu_mesh / args.mesh
+    fpg = gevpm.fourpiG_of(L)
+    c = gevpm.Context(N, gevpm.GEV_FP64)
+    p = gevpm.Particles(c, N ** 3, om / N ** 3)
+    f = gevpm.Fields(c)
+    st = gevpm.StepState(1, fpg, om, ol, z_relax=args.z_in)
+    gevpm.ic_generate(p, f, st, 42, L, args.z_in, k, P)
+    pos, q, _ = (t.cpu().numpy() for t in p.store())
+    M = N ** 3
+    ost = dict(pos=np.ascontiguousarray(pos), q=np.ascontiguousarray(q), phi=f.phi.cpu().numpy().ravel().copy(),
+               chi=f.chi.cpu().numpy().ravel().copy(), Bi=f.Bi.cpu().numpy().reshape(3, -1).copy(),
+               source=np.zeros(M), Sij=np.zeros((6, M)), a=st.a, tau=st.tau)
+    del p, f, c
+    torch.cuda.empty_cache()
+    dtau = (gevpm.particle_horizon(1.0, fpg, om, ol) - st.tau) / args.total_steps
+    ost["dtau_old"] = dtau  # a regular cycle (the first cycle of a run skips the phi solve)
+    t0 = time.time()
+    for _ in range(args.cpu_steps):
+        ost["dtau"] = dtau
+        O.step(N, ost, 1, om / M, fpg, om, ol, args.z_in)
+    dt = time.time() - t0
+    return {"value": M * args.cpu_steps / dt, "unit": "particle-steps/s", "cores": O.get_threads(),
+            "kind": "port", "sample": f"{args.cpu_steps} GR cycles, {N}^3 particles on {N}^3 mesh "
+                                      f"(L scaled to {L:g} Mpc/h, same mass resolution), FP64 oracle, "
+                                      f"{dt:.1f}s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -112,6 +112,8 @@ int gev_ctx_destroy(gev_ctx *c)
     (void)hipFree(c->red);
     (void)hipHostFree(c->red_host);
     (void)hipFree(c->scan_blocks);
+    for (int i = 0; i < gev_ctx::MAX_MARKS; i++)
+        if (c->marks[i]) (void)hipEventDestroy(c->marks[i]);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GEV_OK;

@@ -93,6 +93,12 @@ struct gev_ctx {
     // scan scratch
     int32_t *scan_blocks = nullptr;
     int64_t scan_blocks_n = 0;
+    // optional per-phase timing of gev_step (events recorded on the stream)
+    static constexpr int MAX_MARKS = 24;
+    bool timing = false;
+    int nmarks = 0;
+    hipEvent_t marks[MAX_MARKS] = {};
+    int mark_id[MAX_MARKS] = {};
     std::string err;
 };
 

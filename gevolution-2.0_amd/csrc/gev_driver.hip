@@ -14,7 +14,38 @@
         if (rc_ != GEV_OK) return rc_; \
     } while (0)
 
+static int mark(gev_ctx *c, int id)
+{
+    if (!c->timing || c->nmarks >= gev_ctx::MAX_MARKS) return GEV_OK;
+    hipEvent_t &e = c->marks[c->nmarks];
+    if (!e) GEV_HIP(hipEventCreate(&e));
+    GEV_HIP(hipEventRecord(e, c->stream));
+    c->mark_id[c->nmarks++] = id;
+    return GEV_OK;
+}
+
 extern "C" {
+
+int gev_step_timing(gev_ctx *c, int enable)
+{
+    if (!c) { gev_set_error("gev_step_timing: NULL context"); return GEV_ERR_ARG; }
+    c->timing = enable != 0;
+    c->nmarks = 0;
+    return GEV_OK;
+}
+
+int gev_step_timings(gev_ctx *c, int *ids, float *ms, int max)
+{
+    if (!c || !ids || !ms) { gev_set_error("gev_step_timings: NULL argument"); return GEV_ERR_ARG; }
+    if (c->nmarks < 2) return 0;
+    GEV_HIP(hipEventSynchronize(c->marks[c->nmarks - 1]));
+    int n = 0;
+    for (int i = 1; i < c->nmarks && n < max; i++, n++) {
+        GEV_HIP(hipEventElapsedTime(&ms[n], c->marks[i - 1], c->marks[i]));
+        ids[n] = c->mark_id[i];
+    }
+    return n;
+}
 
 int gev_ctx_set_stream(gev_ctx *c, void *s)
 {
@@ -79,10 +110,13 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
     const double om = st->Omega_m, ol = st->Omega_Lambda, orad = st->Omega_rad, fpg = st->fourpiG;
     double a = st->a, dtau = st->dtau, dtau_old = st->dtau_old;
     double maxv2 = 0.0;
+    c->nmarks = 0;
+    RC(mark(c, -1));
 
     if (st->gr) {
         // T00 with the old phi (main.cu:622-651), Sij zeroed by the Tij deposit itself
         RC(gev_project_T00(p, f->source, a, f->phi, 1.0, 0));
+        RC(mark(c, GEV_PHASE_DEPOSIT_T00));
         const double H = gev_hconf(a, fpg, om, ol, orad);
         if (dtau_old > 0.) {
             double s_src = 0, s_res = 0;
@@ -91,19 +125,27 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
             st->T00hom = s_src / (double)M;
             RC(gev_fft_r2c(c, f->source, f->scalarFT, 1));
             if (mixed) RC(gev_set_k0(c, f->scalarFT, s_res, 0.0)); // FP64 k=0 mode
+            RC(mark(c, GEV_PHASE_SOURCE_T00));
         }
         // Tij with the old phi (main.cu:750-758)
         RC(gev_project_Tij(p, f->Sij, a, f->phi, 1.0, 0));
+        RC(mark(c, GEV_PHASE_DEPOSIT_TIJ));
         if (dtau_old > 0.) {
             RC(gev_poisson(c, f->scalarFT, f->scalarFT, 1. / (dx * dx), 3. * H / dtau_old)); // main.cu:769
             RC(gev_fft_c2r(c, f->scalarFT, f->phi, 1));
+            RC(mark(c, GEV_PHASE_SOLVE_PHI));
         }
         // chi (main.cu:822-902) and B (main.cu:908-955)
         RC(gev_prepare_source_Tij(c, f->phi, f->Sij, f->Sij, 2. * fpg * dx * dx / a));
         RC(gev_fft_r2c(c, f->Sij, f->SijFT, 6));
-        if (st->vector_elliptic) RC(gev_project_T0i(p, f->Bi, f->phi, 1.0, 0));
+        RC(mark(c, GEV_PHASE_SOURCE_TIJ));
+        if (st->vector_elliptic) {
+            RC(gev_project_T0i(p, f->Bi, f->phi, 1.0, 0));
+            RC(mark(c, GEV_PHASE_DEPOSIT_T0I));
+        }
         RC(gev_project_ft_scalar(c, f->SijFT, f->scalarFT, 0));
         RC(gev_fft_c2r(c, f->scalarFT, f->chi, 1));
+        RC(mark(c, GEV_PHASE_SOLVE_CHI));
         if (st->vector_elliptic) {
             RC(gev_fft_r2c(c, f->Bi, f->BiFT, 3));
             RC(gev_project_ft_vector(c, f->BiFT, f->BiFT, fpg * dx * dx, 0.0));
@@ -115,23 +157,30 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
             // c2r clobbers its input; parabolic B must persist in k-space
             RC(gev_fft_r2c(c, f->Bi, f->BiFT, 3));
         }
+        RC(mark(c, GEV_PHASE_SOLVE_B));
         // kick (main.cu:1227-1240), half background step, drift (:1253-1266)
         int nf = (1. / a < st->z_relax + 1.) ? 3 : 2;
         RC(gev_update_q(p, (dtau + dtau_old) / 2., f->phi, f->chi, f->Bi, nf, a, a * a * (double)N, &maxv2));
+        RC(mark(c, GEV_PHASE_KICK));
         a = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
         nf = (1. / a < st->z_relax + 1.) ? 3 : 0;
         RC(gev_update_pos(p, dtau, f->phi, f->chi, f->Bi, nf, a, a * a * (double)N));
+        RC(mark(c, GEV_PHASE_DRIFT));
         a = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
         double mv = sqrt(maxv2);
         st->maxvel = mv / sqrt(mv * mv + 1.0); // main.cu:1279-1283
     } else {
         RC(gev_project_mass(p, f->source, 1.0, 0));
+        RC(mark(c, GEV_PHASE_DEPOSIT_T00));
         RC(gev_fft_r2c(c, f->source, f->scalarFT, 1));
         RC(gev_poisson(c, f->scalarFT, f->scalarFT, fpg / a, 0.0)); // main.cu:765
         RC(gev_fft_c2r(c, f->scalarFT, f->phi, 1));
+        RC(mark(c, GEV_PHASE_SOLVE_PHI));
         RC(gev_update_q_newton(p, (dtau + dtau_old) / 2., f->phi, nullptr, 1, a, &maxv2));
+        RC(mark(c, GEV_PHASE_KICK));
         a = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
         RC(gev_update_pos_newton(p, dtau, a));
+        RC(mark(c, GEV_PHASE_DRIFT));
         a = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
         st->maxvel = sqrt(maxv2);
     }

@@ -45,6 +45,8 @@ class Context:
             check(lib.gev_ctx_create(self.N, precision, C.c_void_p(self.stream.cuda_stream), C.byref(h)),
                   "gev_ctx_create")
         self._h = h
+        # follow torch's stream exactly, including the legacy default (NULL) stream
+        check(lib.gev_ctx_set_stream(h, C.c_void_p(self.stream.cuda_stream)), "gev_ctx_set_stream")
         self.P = lib.gev_ctx_kpitch(h)
         self.M = self.N ** 3
 
@@ -381,3 +383,24 @@ def step(pcls, fields, state):
     f = fields.c_struct()
     check(lib.gev_step(pcls.handle, C.byref(f), C.byref(state.s)), "gev_step")
     return state
+
+
+PHASES = ["deposit_T00", "source_T00", "deposit_Tij", "solve_phi", "source_Tij", "deposit_T0i", "solve_chi",
+          "solve_B", "kick", "drift_rebin"]
+
+
+def step_timing(ctx, enable=True):
+    check(lib.gev_step_timing(ctx.handle, int(enable)), "gev_step_timing")
+
+
+def step_timings(ctx):
+    """{phase name: ms} of the last gev_step (with timing enabled)"""
+    ids = (C.c_int * 24)()
+    ms = (C.c_float * 24)()
+    n = lib.gev_step_timings(ctx.handle, ids, ms, 24)
+    if n < 0:
+        check(n, "gev_step_timings")
+    out = {}
+    for i in range(n):
+        out[PHASES[ids[i]]] = out.get(PHASES[ids[i]], 0.0) + ms[i]
+    return out

@@ -78,6 +78,8 @@ SIGNATURES = {
     "gev_rk4bg": (_d, [_d, _d, _d, _d, _d, _d]),
     "gev_particle_horizon": (_d, [_d, _d, _d, _d, _d]),
     "gev_step": (_i, [_vp, C.POINTER(gev_fields), C.POINTER(gev_step_state)]),
+    "gev_step_timing": (_i, [_vp, _i]),
+    "gev_step_timings": (_i, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_float), _i]),
     "gev_ic_generate": (_i, [_vp, C.POINTER(gev_fields), C.POINTER(gev_step_state), _u32, _i, _d, _d, _i, _dp, _dp,
                              _dp]),
     "gev_field_sum": (_i, [_vp, _vp, _i, _dp]),

@@ -158,6 +158,20 @@ typedef struct {
 
 int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st);
 
+/* per-phase timing of gev_step (measurement support, no reference
+ * counterpart; the reference accumulates MPI_Wtime per phase under
+ * -DBENCHMARK, main.cu:1340-1371).  When enabled, gev_step records HIP
+ * events at its phase boundaries; gev_step_timings returns, for the last
+ * step, ids[i] (a GEV_PHASE_* value) and the milliseconds ms[i] spent in
+ * that phase.  Returns the number of phases written (<= max). */
+enum {
+    GEV_PHASE_DEPOSIT_T00 = 0, GEV_PHASE_SOURCE_T00 = 1, GEV_PHASE_DEPOSIT_TIJ = 2, GEV_PHASE_SOLVE_PHI = 3,
+    GEV_PHASE_SOURCE_TIJ = 4, GEV_PHASE_DEPOSIT_T0I = 5, GEV_PHASE_SOLVE_CHI = 6, GEV_PHASE_SOLVE_B = 7,
+    GEV_PHASE_KICK = 8, GEV_PHASE_DRIFT = 9, GEV_PHASE_COUNT = 10
+};
+int gev_step_timing(gev_ctx *ctx, int enable);
+int gev_step_timings(gev_ctx *ctx, int *ids, float *ms, int max);
+
 /* basic IC generator, mPk path (generateIC_basic, ic_basic.hpp:1834-2505,
  * standard CIC kernel).  Requires a GEV_FP64 context; particles are placed
  * on the cell-centred lattice (N^3 of them).  k_hmpc/P_mpch3: the mPk table

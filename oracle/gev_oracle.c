@@ -70,7 +70,19 @@ OR_API int or_get_threads(void)
  * Pinned by: brute-force DFT at N=4,8 and numpy.fft.rfftn fixtures at N=8,16
  * (tests/golden), Parseval, c2r(r2c(x)) = N^3 x.
  * ------------------------------------------------------------------------ */
-static void fft_line(int n, cplx *v, int sign)
+/* tw[k] = exp(-2 pi i k / n), k < n/2 */
+static cplx *twiddles(int n)
+{
+    cplx *tw = (cplx *)malloc(sizeof(cplx) * (n / 2 + 1));
+    for (int k = 0; k < n / 2; k++) {
+        double ang = 2.0 * M_PI * (double)k / (double)n;
+        tw[k].re = cos(ang);
+        tw[k].im = -sin(ang);
+    }
+    return tw;
+}
+
+static void fft_line(int n, cplx *v, int sign, const cplx *tw)
 {
     /* bit reversal */
     for (int i = 1, j = 0; i < n; i++) {
@@ -80,10 +92,10 @@ static void fft_line(int n, cplx *v, int sign)
         if (i < j) { cplx t = v[i]; v[i] = v[j]; v[j] = t; }
     }
     for (int len = 2; len <= n; len <<= 1) {
-        int half = len >> 1;
+        int half = len >> 1, step = n / len;
         for (int k = 0; k < half; k++) {
-            double ang = 2.0 * M_PI * (double)k / (double)len;
-            cplx w = {cos(ang), sign * sin(ang)};
+            cplx w = tw[k * step];
+            if (sign > 0) w.im = -w.im;
             for (int i = 0; i < n; i += len) {
                 cplx u = v[i + k];
                 cplx t = cmul(w, v[i + k + half]);
@@ -100,6 +112,7 @@ static void fft_pass_yz(int N, cplx *a, int axis, int sign)
     const long P = N / 2 + 1;
     const long stride = (axis == 1) ? P : P * N;
     const long nouter = P * N; /* number of lines */
+    cplx *tw = twiddles(N);
 #pragma omp parallel
     {
         cplx *buf = (cplx *)malloc(sizeof(cplx) * N);
@@ -108,28 +121,31 @@ static void fft_pass_yz(int N, cplx *a, int axis, int sign)
             long kx = l % P, o = l / P; /* o = the other axis index */
             long base = (axis == 1) ? kx + P * N * o : kx + P * o;
             for (int j = 0; j < N; j++) buf[j] = a[base + j * stride];
-            fft_line(N, buf, sign);
+            fft_line(N, buf, sign, tw);
             for (int j = 0; j < N; j++) a[base + j * stride] = buf[j];
         }
         free(buf);
     }
+    free(tw);
 }
 
 OR_API void or_fft_r2c(int N, const double *in, double *outd)
 {
     cplx *out = (cplx *)outd;
     const long P = N / 2 + 1;
+    cplx *tw = twiddles(N);
 #pragma omp parallel
     {
         cplx *buf = (cplx *)malloc(sizeof(cplx) * N);
 #pragma omp for schedule(static)
         for (long l = 0; l < (long)N * N; l++) {
             for (int x = 0; x < N; x++) { buf[x].re = in[l * N + x]; buf[x].im = 0.0; }
-            fft_line(N, buf, -1);
+            fft_line(N, buf, -1, tw);
             for (int k = 0; k < P; k++) out[l * P + k] = buf[k];
         }
         free(buf);
     }
+    free(tw);
     fft_pass_yz(N, out, 1, -1);
     fft_pass_yz(N, out, 2, -1);
 }
@@ -144,6 +160,7 @@ OR_API void or_fft_c2r(int N, const double *ind, double *out)
     memcpy(t, ind, sizeof(cplx) * nk);
     fft_pass_yz(N, t, 2, +1);
     fft_pass_yz(N, t, 1, +1);
+    cplx *tw = twiddles(N);
 #pragma omp parallel
     {
         cplx *buf = (cplx *)malloc(sizeof(cplx) * N);
@@ -153,11 +170,12 @@ OR_API void or_fft_c2r(int N, const double *ind, double *out)
             buf[0].re = h[0].re; buf[0].im = 0.0;
             buf[N / 2].re = h[N / 2].re; buf[N / 2].im = 0.0;
             for (int k = 1; k < N / 2; k++) { buf[k] = h[k]; buf[N - k] = cconj(h[k]); }
-            fft_line(N, buf, +1);
+            fft_line(N, buf, +1, tw);
             for (int x = 0; x < N; x++) out[l * N + x] = buf[x].re;
         }
         free(buf);
     }
+    free(tw);
     free(t);
 }
 

@@ -32,7 +32,11 @@ def build():
 
 
 def _opt(a):
-    return None if a is None else np.ascontiguousarray(a)
+    return None if a is None else np.ascontiguousarray(a, np.float64)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, np.float64)
 
 
 def lib():
@@ -153,19 +157,22 @@ def prepare_source_Tij(phi, Tij, coeff):
 
 def project_T00(N, pos, q, pmass, a=1.0, phi=None, coeff=1.0, out=None):
     out = np.zeros(N ** 3) if out is None else out
-    lib().or_project_T00(N, pos.shape[1], pos, q, pmass, out, a, _ptr(_opt(phi)), coeff)
+    phi = _opt(phi)
+    lib().or_project_T00(N, pos.shape[1], _c(pos), _c(q), pmass, out, a, _ptr(phi), coeff)
     return out
 
 
 def project_T0i(N, pos, q, pmass, phi=None, coeff=1.0, out=None):
     out = np.zeros((3, N ** 3)) if out is None else out
-    lib().or_project_T0i(N, pos.shape[1], pos, q, pmass, out, _ptr(_opt(phi)), coeff)
+    phi = _opt(phi)
+    lib().or_project_T0i(N, pos.shape[1], _c(pos), _c(q), pmass, out, _ptr(phi), coeff)
     return out
 
 
 def project_Tij(N, pos, q, pmass, a=1.0, phi=None, coeff=1.0, out=None):
     out = np.zeros((6, N ** 3)) if out is None else out
-    lib().or_project_Tij(N, pos.shape[1], pos, q, pmass, out, a, _ptr(_opt(phi)), coeff)
+    phi = _opt(phi)
+    lib().or_project_Tij(N, pos.shape[1], _c(pos), _c(q), pmass, out, a, _ptr(phi), coeff)
     return out
 
 

@@ -194,7 +194,7 @@ def by_id(p):
     pos, q, pid = p.store()
     pos, q, pid = to_np(pos), to_np(q), to_np(pid)
     o = np.argsort(pid)
-    return pos[:, o], q[:, o], pid[o]
+    return np.ascontiguousarray(pos[:, o]), np.ascontiguousarray(q[:, o]), pid[o]
 
 
 def test_sort_invariants(gp):

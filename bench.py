@@ -193,9 +193,10 @@ def kernel_roofline(ctx, pcls, fields, reps, phases):
                         n * 6 * R + M * (4 + R + 6 * R),
                         "k_dep_Tij: per particle 3 offsets + 3 momenta read; per site cell offset + phi read, "
                         "6 components written"),
-        "kick": (lambda: pcls.update_q(1e-9, fields.phi, fields.chi, fields.Bi, 3, 0.5, 0.25 * N),
-                 n * (6 * R + 4 + 3 * R) + M * 5 * R,
-                 "k_update_q: per particle 3 offsets + 3 momenta + cell read, 3 momenta written; phi, chi, B read"),
+        "kick_drift": (lambda: pcls.update_q(1e-9, fields.phi, fields.chi, fields.Bi, 3, 0.5, 0.25 * N),
+                       n * (6 * R + 4 + 3 * R) + M * 5 * R,
+                       "k_update_tiled (kick form): per particle 3 offsets + 3 momenta + cell read, 3 momenta "
+                       "written; phi, chi, B read once"),
         "deposit_T00": (lambda: pcls.project_T00(fields.source, 0.5, fields.phi),
                         n * 6 * R + M * (4 + R + R),
                         "k_dep_T00: per particle 3 offsets + 3 momenta read; per site cell offset + phi read, "

@@ -27,6 +27,10 @@ template <typename T>
 int update_pos_impl(gev_pcls *p, double dtau, const T *phi, const T *chi, const T *Bi, int nfield, double a,
                     double bcoeff);
 template <typename T> int update_pos_newton_impl(gev_pcls *p, double dtau, double a);
+template <typename T>
+int kick_drift_impl(gev_pcls *p, int gr, double dtau_k, double dtau_d, const T *phi, const T *chi, const T *Bi,
+                    int nf_k, int nf_d, double a_k, double a_d, double bc_k, double bc_d, double *max_ret, int rebin);
+template <typename T> int pcls_sort_counted_impl(gev_pcls *p);
 template <typename T> int displace_impl(gev_pcls *p, const T *xi, double coeff, double *max_disp);
 template <typename T> int initialize_q_impl(gev_pcls *p, const T *pot, double coeff, double *max_ret);
 }  // namespace gev
@@ -421,6 +425,30 @@ int gev_update_pos_newton(gev_pcls *p, double dtau, double a)
     CHECK(p, "gev_update_pos_newton: NULL argument");
     NEED_SORTED(p);
     return PREC_DISPATCH(p->ctx, update_pos_newton_impl<double>(p, dtau, a), update_pos_newton_impl<float>(p, dtau, a));
+}
+
+int gev_kick_drift(gev_pcls *p, int gr, double dtau_kick, double dtau_drift, const void *phi, const void *chi,
+                   const void *Bi, int nfield_kick, int nfield_drift, double a_kick, double a_drift,
+                   double bcoeff_kick, double bcoeff_drift, double *max_ret, int rebin)
+{
+    CHECK(p && phi, "gev_kick_drift: NULL argument");
+    const int nmax = nfield_kick > nfield_drift ? nfield_kick : nfield_drift;
+    CHECK(nmax < 2 || chi, "gev_kick_drift: nfield >= 2 needs chi");
+    CHECK(nmax < 3 || Bi, "gev_kick_drift: nfield >= 3 needs Bi");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx,
+                         kick_drift_impl<double>(p, gr, dtau_kick, dtau_drift, (const double *)phi, (const double *)chi,
+                                                 (const double *)Bi, nfield_kick, nfield_drift, a_kick, a_drift,
+                                                 bcoeff_kick, bcoeff_drift, max_ret, rebin),
+                         kick_drift_impl<float>(p, gr, dtau_kick, dtau_drift, (const float *)phi, (const float *)chi,
+                                                (const float *)Bi, nfield_kick, nfield_drift, a_kick, a_drift,
+                                                bcoeff_kick, bcoeff_drift, max_ret, rebin));
+}
+
+int gev_pcls_rebin_counted(gev_pcls *p)
+{
+    CHECK(p, "gev_pcls_rebin_counted: NULL argument");
+    return PREC_DISPATCH(p->ctx, pcls_sort_counted_impl<double>(p), pcls_sort_counted_impl<float>(p));
 }
 
 // ------------------------------------------------------------------- IC ---

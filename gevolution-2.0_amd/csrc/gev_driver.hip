@@ -158,15 +158,17 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
             RC(gev_fft_r2c(c, f->Bi, f->BiFT, 3));
         }
         RC(mark(c, GEV_PHASE_SOLVE_B));
-        // kick (main.cu:1227-1240), half background step, drift (:1253-1266)
-        int nf = (1. / a < st->z_relax + 1.) ? 3 : 2;
-        RC(gev_update_q(p, (dtau + dtau_old) / 2., f->phi, f->chi, f->Bi, nf, a, a * a * (double)N, &maxv2));
+        // kick (main.cu:1227-1240), half background step, drift (:1253-1266):
+        // one fused pass over the particles, then the re-binning scatter
+        const int nf_k = (1. / a < st->z_relax + 1.) ? 3 : 2;
+        const double a_d = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
+        const int nf_d = (1. / a_d < st->z_relax + 1.) ? 3 : 0;
+        RC(gev_kick_drift(p, 1, (dtau + dtau_old) / 2., dtau, f->phi, f->chi, f->Bi, nf_k, nf_d, a, a_d,
+                          a * a * (double)N, a_d * a_d * (double)N, &maxv2, 0));
         RC(mark(c, GEV_PHASE_KICK));
-        a = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
-        nf = (1. / a < st->z_relax + 1.) ? 3 : 0;
-        RC(gev_update_pos(p, dtau, f->phi, f->chi, f->Bi, nf, a, a * a * (double)N));
+        RC(gev_pcls_rebin_counted(p));
         RC(mark(c, GEV_PHASE_DRIFT));
-        a = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
+        a = gev_rk4bg(a_d, fpg, om, ol, orad, 0.5 * dtau);
         double mv = sqrt(maxv2);
         st->maxvel = mv / sqrt(mv * mv + 1.0); // main.cu:1279-1283
     } else {
@@ -176,12 +178,13 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
         RC(gev_poisson(c, f->scalarFT, f->scalarFT, fpg / a, 0.0)); // main.cu:765
         RC(gev_fft_c2r(c, f->scalarFT, f->phi, 1));
         RC(mark(c, GEV_PHASE_SOLVE_PHI));
-        RC(gev_update_q_newton(p, (dtau + dtau_old) / 2., f->phi, nullptr, 1, a, &maxv2));
+        const double a_d = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
+        RC(gev_kick_drift(p, 0, (dtau + dtau_old) / 2., dtau, f->phi, nullptr, nullptr, 1, 0, a, a_d, 1.0, 1.0,
+                          &maxv2, 0));
         RC(mark(c, GEV_PHASE_KICK));
-        a = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
-        RC(gev_update_pos_newton(p, dtau, a));
+        RC(gev_pcls_rebin_counted(p));
         RC(mark(c, GEV_PHASE_DRIFT));
-        a = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
+        a = gev_rk4bg(a_d, fpg, om, ol, orad, 0.5 * dtau);
         st->maxvel = sqrt(maxv2);
     }
     st->a = a;

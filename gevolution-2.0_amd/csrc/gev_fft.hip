@@ -3,51 +3,10 @@
 // strided columns read as TX-wide coalesced row segments).  Radix-8/4/2
 // Stockham stages in LDS, twiddles from a per-size table in global memory.
 #include "gev_common.hpp"
+#include "gev_fft_engine.hpp"
 #include "gev_kernels.hpp"
 
 namespace gev {
-
-// ------------------------------------------------------------ butterflies --
-template <typename T> __device__ __forceinline__ void dft2(cx<T> *v)
-{
-    cx<T> t = v[0] - v[1];
-    v[0] = v[0] + v[1];
-    v[1] = t;
-}
-
-template <typename T> __device__ __forceinline__ void dft4(cx<T> *v, int dir)
-{
-    cx<T> a0 = v[0] + v[2], a1 = v[0] - v[2], a2 = v[1] + v[3], a3 = mul_i(v[1] - v[3], dir);
-    v[0] = a0 + a2;
-    v[2] = a0 - a2;
-    v[1] = a1 + a3;
-    v[3] = a1 - a3;
-}
-
-template <typename T> __device__ __forceinline__ void dft8(cx<T> *v, int dir)
-{
-    cx<T> e[4] = {v[0], v[2], v[4], v[6]};
-    cx<T> o[4] = {v[1], v[3], v[5], v[7]};
-    dft4(e, dir);
-    dft4(o, dir);
-    const T r = (T)0.707106781186547524400844362104849039;
-    const T d = (T)dir;
-    cx<T> t0 = o[0];
-    cx<T> t1 = mk((o[1].x - d * o[1].y) * r, (o[1].y + d * o[1].x) * r);
-    cx<T> t2 = mul_i(o[2], dir);
-    cx<T> t3 = mk((-o[3].x - d * o[3].y) * r, (-o[3].y + d * o[3].x) * r);
-    v[0] = e[0] + t0; v[4] = e[0] - t0;
-    v[1] = e[1] + t1; v[5] = e[1] - t1;
-    v[2] = e[2] + t2; v[6] = e[2] - t2;
-    v[3] = e[3] + t3; v[7] = e[3] - t3;
-}
-
-template <typename T, int R> __device__ __forceinline__ void dftR(cx<T> *v, int dir)
-{
-    if constexpr (R == 8) dft8(v, dir);
-    else if constexpr (R == 4) dft4(v, dir);
-    else dft2(v);
-}
 
 // Stockham autosort stages over NLINES lines of length L stored in LDS with
 // line pitch LDL.  tw = exp(-2 pi i m / (L*TWM)), m in [0, L*TWM).
@@ -200,31 +159,141 @@ __global__ __launch_bounds__(FFT_NT) void k_c2c_col(cx<T> *__restrict__ data, in
     }
 }
 
-// ---------------------------------------------------------- dispatchers ---
+// ============================================ register-staged passes ====
 template <typename T> constexpr int tx_of() { return sizeof(T) == 8 ? 8 : 16; }
+template <typename T, int L> constexpr int col_tx() { return (tx_of<T>() * L / 8 > 1024) ? 8192 / L : tx_of<T>(); }
+template <typename T, int N> constexpr int xlines() { return (sizeof(T) == 8 ? 4096 : 8192) / N; }
+
+// y / z column pass, in place: TX adjacent kx columns x one full line.
+template <typename T, int L, int TX>
+__global__ __launch_bounds__(TX * L / 8) void k_col_reg(cx<T> *__restrict__ data, int P, int axis, int dir,
+                                                        const cx<T> *__restrict__ tw)
+{
+    constexpr int NT = TX * L / 8;
+    __shared__ cx<T> lds[TX * (L + 1)];
+    __shared__ cx<T> twl[L];
+    for (int m = threadIdx.x; m < L; m += NT) twl[m] = tw[m];
+    const int ntiles = P / TX;
+    const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int ct = (int)(b % ntiles);
+    const int64_t o = b / ntiles;
+    const int64_t stride = (axis == 1) ? (int64_t)P : (int64_t)P * L;
+    cx<T> *base = data + (int64_t)ct * TX + ((axis == 1) ? (int64_t)P * L * o : (int64_t)P * o);
+    auto ld = [&](int line, int row) { return base[(int64_t)row * stride + line]; };
+    auto st = [&](int line, int row, cx<T> v) { base[(int64_t)row * stride + line] = v; };
+    __syncthreads();
+    reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, dir, ld, st);
+}
+
+// x pass r2c: NL consecutive lines of N reals -> N/2+1 complex (pitch P)
+template <typename T, int N, int NL>
+__global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(const T *__restrict__ in, cx<T> *__restrict__ out, int P,
+                                                           int64_t nlines, const cx<T> *__restrict__ tw)
+{
+    constexpr int H = N / 2, NT = NL * H / 8, LDL = H + 1;
+    __shared__ cx<T> lds[NL * LDL];
+    __shared__ cx<T> twl[N];
+    for (int m = threadIdx.x; m < N; m += NT) twl[m] = tw[m];
+    const int64_t line0 = (int64_t)blockIdx.x * NL;
+    const cx<T> *src = reinterpret_cast<const cx<T> *>(in) + line0 * H;
+    auto ld = [&](int line, int n) { return (line0 + line < nlines) ? src[line * H + n] : mk((T)0, (T)0); };
+    auto st = [&](int line, int k, cx<T> v) { lds[line * LDL + k] = v; };
+    __syncthreads();
+    reg_fft<T, H, NL, false, 2, true, 1>(lds, twl, -1, ld, st);
+    __syncthreads();
+    cx<T> *dst = out + line0 * P;
+    for (int e = threadIdx.x; e < NL * P; e += NT) {
+        int l = e / P, k = e % P;
+        if (line0 + l >= nlines) continue;
+        cx<T> X = mk((T)0, (T)0);
+        if (k <= H) {
+            cx<T> Zk = lds[l * LDL + (k % H)];
+            cx<T> Zm = conj(lds[l * LDL + ((H - k) % H)]);
+            cx<T> Fe = (Zk + Zm) * (T)0.5;
+            cx<T> Fo = mul_i(Zk - Zm, -1) * (T)0.5;
+            X = Fe + twl[k] * Fo;
+        }
+        dst[e] = X;
+    }
+}
+
+// x pass c2r: N/2+1 complex (pitch P) -> N reals; DC and Nyquist taken as real
+template <typename T, int N, int NL>
+__global__ __launch_bounds__(NL * N / 16) void k_c2r_x_reg(const cx<T> *__restrict__ in, T *__restrict__ out, int P,
+                                                           int64_t nlines, const cx<T> *__restrict__ tw)
+{
+    constexpr int H = N / 2, NT = NL * H / 8, LDL = H + 1;
+    __shared__ cx<T> xin[NL * LDL];
+    __shared__ cx<T> lds[NL * LDL];
+    __shared__ cx<T> twl[N];
+    for (int m = threadIdx.x; m < N; m += NT) twl[m] = tw[m];
+    const int64_t line0 = (int64_t)blockIdx.x * NL;
+    const cx<T> *src = in + line0 * P;
+    for (int e = threadIdx.x; e < NL * P; e += NT) {
+        int l = e / P, k = e % P;
+        if (k <= H) xin[l * LDL + k] = (line0 + l < nlines) ? src[e] : mk((T)0, (T)0);
+    }
+    __syncthreads();
+    auto ld = [&](int line, int k) {
+        cx<T> Xk = xin[line * LDL + k];
+        cx<T> Xm = xin[line * LDL + (H - k)];
+        if (k == 0) { Xk.y = 0; Xm.y = 0; }
+        cx<T> cm = conj(Xm);
+        cx<T> E = Xk + cm;
+        cx<T> O = (Xk - cm) * conj(twl[k]);
+        return E + mul_i(O, +1);
+    };
+    cx<T> *dst = reinterpret_cast<cx<T> *>(out) + line0 * H;
+    auto st = [&](int line, int n, cx<T> v) {
+        if (line0 + line < nlines) dst[line * H + n] = v;
+    };
+    reg_fft<T, H, NL, false, 2, false, 1>(lds, twl, +1, ld, st);
+}
+
+// ---------------------------------------------------------- dispatchers ---
 
 template <typename T, int N>
 static void launch_r2c(const T *in, cx<T> *out, int P, int64_t nlines, const cx<T> *tw, hipStream_t s)
 {
-    constexpr int NLINES = (4096 / N > 1024) ? 1024 : 4096 / N;
-    int64_t nb = (nlines + NLINES - 1) / NLINES;
-    hipLaunchKernelGGL((k_r2c_x<T, N, NLINES>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, in, out, P, nlines, tw);
+    if constexpr (N >= 128) {
+        constexpr int NL = xlines<T, N>();
+        int64_t nb = (nlines + NL - 1) / NL;
+        hipLaunchKernelGGL((k_r2c_x_reg<T, N, NL>), dim3((unsigned)nb), dim3(NL * N / 16), 0, s, in, out, P, nlines,
+                           tw);
+    } else {
+        constexpr int NLINES = (4096 / N > 1024) ? 1024 : 4096 / N;
+        int64_t nb = (nlines + NLINES - 1) / NLINES;
+        hipLaunchKernelGGL((k_r2c_x<T, N, NLINES>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, in, out, P, nlines, tw);
+    }
 }
 
 template <typename T, int N>
 static void launch_c2r(const cx<T> *in, T *out, int P, int64_t nlines, const cx<T> *tw, hipStream_t s)
 {
-    constexpr int NLINES = (4096 / N > 1024) ? 1024 : 4096 / N;
-    int64_t nb = (nlines + NLINES - 1) / NLINES;
-    hipLaunchKernelGGL((k_c2r_x<T, N, NLINES>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, in, out, P, nlines, tw);
+    if constexpr (N >= 128) {
+        constexpr int NL = xlines<T, N>();
+        int64_t nb = (nlines + NL - 1) / NL;
+        hipLaunchKernelGGL((k_c2r_x_reg<T, N, NL>), dim3((unsigned)nb), dim3(NL * N / 16), 0, s, in, out, P, nlines,
+                           tw);
+    } else {
+        constexpr int NLINES = (4096 / N > 1024) ? 1024 : 4096 / N;
+        int64_t nb = (nlines + NLINES - 1) / NLINES;
+        hipLaunchKernelGGL((k_c2r_x<T, N, NLINES>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, in, out, P, nlines, tw);
+    }
 }
 
 template <typename T, int N>
 static void launch_col(cx<T> *data, int P, int axis, int dir, const cx<T> *tw, hipStream_t s)
 {
-    constexpr int TX = tx_of<T>();
-    int64_t nb = (int64_t)(P / TX) * N;
-    hipLaunchKernelGGL((k_c2c_col<T, N, TX>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, data, P, axis, dir, tw);
+    if constexpr (N >= 64) {
+        constexpr int TX = col_tx<T, N>();
+        int64_t nb = (int64_t)(P / TX) * N;
+        hipLaunchKernelGGL((k_col_reg<T, N, TX>), dim3((unsigned)nb), dim3(TX * N / 8), 0, s, data, P, axis, dir, tw);
+    } else {
+        constexpr int TX = tx_of<T>();
+        int64_t nb = (int64_t)(P / TX) * N;
+        hipLaunchKernelGGL((k_c2c_col<T, N, TX>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, data, P, axis, dir, tw);
+    }
 }
 
 #define GEV_FOR_EACH_N(X) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)

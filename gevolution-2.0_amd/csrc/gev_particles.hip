@@ -11,110 +11,10 @@
 //     wavefront's stencil reads hit the same cache lines.
 #include "gev_common.hpp"
 #include "gev_kernels.hpp"
+#include "gev_pcommon.hpp"
 
 namespace gev {
 
-constexpr int PT_NT = 256;
-
-// ---------------------------------------------------------------- views ---
-template <typename T> struct PV {
-    T *p0, *p1, *p2;  // FP64: absolute box coords; FP32: cell offsets
-    T *q0, *q1, *q2;
-    int64_t *id;
-    int32_t *cell;
-    const int32_t *start;
-    int64_t n;
-    int N, lg;
-};
-
-template <typename T> static PV<T> make_pv(gev_pcls *p)
-{
-    PV<T> v;
-    v.p0 = (T *)p->pos[0]; v.p1 = (T *)p->pos[1]; v.p2 = (T *)p->pos[2];
-    v.q0 = (T *)p->q[0]; v.q1 = (T *)p->q[1]; v.q2 = (T *)p->q[2];
-    v.id = p->id;
-    v.cell = p->cell;
-    v.start = p->cell_start;
-    v.n = p->n;
-    v.N = p->ctx->N;
-    v.lg = __builtin_ctz(p->ctx->N);
-    return v;
-}
-
-__device__ __forceinline__ void cell_coords(int32_t c, int N, int lg, int &x, int &y, int &z)
-{
-    x = c & (N - 1);
-    y = (c >> lg) & (N - 1);
-    z = c >> (2 * lg);
-}
-
-// ref_dist of particle i in cell (x,y,z): FP64 form (p - c dx)/dx, exact for
-// N a power of two; FP32 form stores it directly.
-template <typename T> __device__ __forceinline__ void ref_dist(const PV<T> &v, int64_t i, int x, int y, int z, T r[3]);
-template <> __device__ __forceinline__ void ref_dist<double>(const PV<double> &v, int64_t i, int x, int y, int z,
-                                                             double r[3])
-{
-    const double dx = 1.0 / (double)v.N;
-    r[0] = (v.p0[i] - (double)x * dx) / dx;
-    r[1] = (v.p1[i] - (double)y * dx) / dx;
-    r[2] = (v.p2[i] - (double)z * dx) / dx;
-}
-template <> __device__ __forceinline__ void ref_dist<float>(const PV<float> &v, int64_t i, int x, int y, int z,
-                                                            float r[3])
-{
-    r[0] = v.p0[i];
-    r[1] = v.p1[i];
-    r[2] = v.p2[i];
-}
-
-__device__ __forceinline__ double wrap_pos(double p)
-{
-    if (p >= 1.0) p -= 1.0;
-    else if (p < 0.0) p += 1.0;
-    if (p >= 1.0) p = 0.0;
-    if (p < 0.0) p = 0.0;
-    return p;
-}
-
-__device__ __forceinline__ int cell1(double p, int N)
-{
-    int c = (int)floor(p * (double)N);
-    return c < 0 ? 0 : (c >= N ? N - 1 : c);
-}
-
-// move particle i by d (box units) and recompute its cell
-template <typename T> __device__ __forceinline__ void move_by(const PV<T> &v, int64_t i, int x, int y, int z,
-                                                              const double d[3]);
-template <> __device__ __forceinline__ void move_by<double>(const PV<double> &v, int64_t i, int x, int y, int z,
-                                                            const double d[3])
-{
-    double p0 = wrap_pos(v.p0[i] + d[0]), p1 = wrap_pos(v.p1[i] + d[1]), p2 = wrap_pos(v.p2[i] + d[2]);
-    v.p0[i] = p0; v.p1[i] = p1; v.p2[i] = p2;
-    const int N = v.N;
-    v.cell[i] = (int32_t)((int64_t)cell1(p0, N) + (int64_t)N * ((int64_t)cell1(p1, N) + (int64_t)N * cell1(p2, N)));
-}
-__device__ __forceinline__ void off_step(float &r, int &c, double dcells, int N)
-{
-    double rn = (double)r + dcells;
-    double sh = floor(rn);
-    rn -= sh;
-    float rf = (float)rn;
-    int s = (int)sh;
-    if (rf >= 1.0f) { rf = 0.0f; s += 1; }
-    r = rf;
-    c = (c + s) & (N - 1);
-}
-template <> __device__ __forceinline__ void move_by<float>(const PV<float> &v, int64_t i, int x, int y, int z,
-                                                           const double d[3])
-{
-    const int N = v.N;
-    float r0 = v.p0[i], r1 = v.p1[i], r2 = v.p2[i];
-    off_step(r0, x, d[0] * N, N);
-    off_step(r1, y, d[1] * N, N);
-    off_step(r2, z, d[2] * N, N);
-    v.p0[i] = r0; v.p1[i] = r1; v.p2[i] = r2;
-    v.cell[i] = (int32_t)((int64_t)x + (int64_t)N * ((int64_t)y + (int64_t)N * z));
-}
 
 // ------------------------------------------------------------- lattice ----
 template <typename T> __global__ void k_lattice(PV<T> v)
@@ -287,16 +187,6 @@ template <typename T> __global__ void k_scatter(PV<T> s, PV<T> d, const int32_t 
 }
 
 // ------------------------------------------------------------ deposits ----
-// tile -> (x0,y0,z0), XCD-aware
-template <int TX, int TY, int TZ>
-__device__ __forceinline__ void tile_origin(int N, int &x0, int &y0, int &z0)
-{
-    const int ntx = N / TX, nty = N / TY;
-    int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-    x0 = (int)(b % ntx) * TX;
-    y0 = (int)((b / ntx) % nty) * TY;
-    z0 = (int)(b / ((int64_t)ntx * nty)) * TZ;
-}
 
 // projection_T00 (functor gevolution.hpp:1729-1854): 8 corner values per cell
 template <typename T, int TX, int TY, int TZ>
@@ -535,247 +425,6 @@ __global__ __launch_bounds__(PT_NT) void k_dep_Tij(PV<T> v, T *out, const T *__r
     }
 }
 
-// ---------------------------------------------------------- kick / drift --
-template <typename T> struct FieldAt {
-    const T *f;
-    int N, x, y, z;
-    __device__ __forceinline__ T operator()(int dx, int dy, int dz) const { return f[site(N, x + dx, y + dy, z + dz)]; }
-};
-
-template <typename T> __device__ __forceinline__ T cic(const FieldAt<T> &F, const T r[3])
-{
-    T s = F(0, 0, 0) * ((T)1 - r[0]) * ((T)1 - r[1]) * ((T)1 - r[2]);
-    s += F(1, 0, 0) * r[0] * ((T)1 - r[1]) * ((T)1 - r[2]);
-    s += F(0, 1, 0) * ((T)1 - r[0]) * r[1] * ((T)1 - r[2]);
-    s += F(1, 1, 0) * r[0] * r[1] * ((T)1 - r[2]);
-    s += F(0, 0, 1) * ((T)1 - r[0]) * ((T)1 - r[1]) * r[2];
-    s += F(1, 0, 1) * r[0] * ((T)1 - r[1]) * r[2];
-    s += F(0, 1, 1) * ((T)1 - r[0]) * r[1] * r[2];
-    s += F(1, 1, 1) * r[0] * r[1] * r[2];
-    return s;
-}
-
-// GRADIENT_ORDER = 2 (gevolution.hpp:930-941)
-template <typename T> __device__ __forceinline__ void grad2(const FieldAt<T> &F, const T r[3], T g[3])
-{
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const int ex = (i == 0), ey = (i == 1), ez = (i == 2);
-        T s;
-        s = (T)0.5 * ((T)1 - r[0]) * ((T)1 - r[1]) * ((T)1 - r[2]) * (F(ex, ey, ez) - F(-ex, -ey, -ez));
-        s += (T)0.5 * r[0] * ((T)1 - r[1]) * ((T)1 - r[2]) * (F(ex + 1, ey, ez) - F(-ex + 1, -ey, -ez));
-        s += (T)0.5 * ((T)1 - r[0]) * r[1] * ((T)1 - r[2]) * (F(ex, ey + 1, ez) - F(-ex, -ey + 1, -ez));
-        s += (T)0.5 * r[0] * r[1] * ((T)1 - r[2]) * (F(ex + 1, ey + 1, ez) - F(-ex + 1, -ey + 1, -ez));
-        s += (T)0.5 * ((T)1 - r[0]) * ((T)1 - r[1]) * r[2] * (F(ex, ey, ez + 1) - F(-ex, -ey, -ez + 1));
-        s += (T)0.5 * r[0] * ((T)1 - r[1]) * r[2] * (F(ex + 1, ey, ez + 1) - F(-ex + 1, -ey, -ez + 1));
-        s += (T)0.5 * ((T)1 - r[0]) * r[1] * r[2] * (F(ex, ey + 1, ez + 1) - F(-ex, -ey + 1, -ez + 1));
-        s += (T)0.5 * r[0] * r[1] * r[2] * (F(ex + 1, ey + 1, ez + 1) - F(-ex + 1, -ey + 1, -ez + 1));
-        g[i] = s;
-    }
-}
-
-// CIC edge gradient (gevolution.hpp:950-964; ic_basic.hpp:73-84)
-template <typename T> __device__ __forceinline__ void grad1(const FieldAt<T> &F, const T r[3], T g[3])
-{
-    g[0] = ((T)1 - r[1]) * ((T)1 - r[2]) * (F(1, 0, 0) - F(0, 0, 0));
-    g[1] = ((T)1 - r[0]) * ((T)1 - r[2]) * (F(0, 1, 0) - F(0, 0, 0));
-    g[2] = ((T)1 - r[0]) * ((T)1 - r[1]) * (F(0, 0, 1) - F(0, 0, 0));
-    g[0] += r[1] * ((T)1 - r[2]) * (F(1, 1, 0) - F(0, 1, 0));
-    g[1] += r[0] * ((T)1 - r[2]) * (F(1, 1, 0) - F(1, 0, 0));
-    g[2] += r[0] * ((T)1 - r[1]) * (F(1, 0, 1) - F(1, 0, 0));
-    g[0] += ((T)1 - r[1]) * r[2] * (F(1, 0, 1) - F(0, 0, 1));
-    g[1] += ((T)1 - r[0]) * r[2] * (F(0, 1, 1) - F(0, 0, 1));
-    g[2] += ((T)1 - r[0]) * r[1] * (F(0, 1, 1) - F(0, 1, 0));
-    g[0] += r[1] * r[2] * (F(1, 1, 1) - F(0, 1, 1));
-    g[1] += r[0] * r[2] * (F(1, 1, 1) - F(1, 0, 1));
-    g[2] += r[0] * r[1] * (F(1, 1, 1) - F(1, 1, 0));
-}
-
-__device__ __forceinline__ void block_max_to(double v, unsigned long long *slot)
-{
-    __shared__ double sh[PT_NT / 64];
-    v = wave_max(v);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) sh[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = sh[0];
-        for (int j = 1; j < PT_NT / 64; j++) m = fmax(m, sh[j]);
-        atomic_max_nonneg(slot, m);
-    }
-}
-
-// update_q, gevolution.hpp:888-1011
-template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_update_q(PV<T> v, double dtau_d, const T *__restrict__ phi,
-                                                    const T *__restrict__ chi, const T *__restrict__ Bi, int nfield,
-                                                    double a_d, double bcoeff_d, int64_t M, unsigned long long *maxslot)
-{
-    int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
-    double ret = 0.0;
-    if (i < v.n) {
-        const int N = v.N;
-        int x, y, z;
-        cell_coords(v.cell[i], N, v.lg, x, y, z);
-        T r[3];
-        ref_dist(v, i, x, y, z, r);
-        const T a = (T)a_d, dtau = (T)dtau_d, bcoeff = (T)bcoeff_d;
-        T vv[3] = {v.q0[i], v.q1[i], v.q2[i]};
-        T v2 = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
-        T e2 = v2 + a * a;
-        FieldAt<T> Fp{phi, N, x, y, z};
-        T phiint = cic(Fp, r);
-        phiint /= e2;
-        phiint = (T)4 * phiint * v2 + ((T)1 + a * a * phiint) * (v2 + e2) / e2;
-        T g[3];
-        grad2(Fp, r, g);
-        g[0] *= phiint; g[1] *= phiint; g[2] *= phiint;
-        if (nfield >= 2 && chi) {
-            FieldAt<T> Fc{chi, N, x, y, z};
-            T gc[3];
-            grad1(Fc, r, gc);
-            g[0] -= gc[0]; g[1] -= gc[1]; g[2] -= gc[2];
-        }
-        e2 = sqrt(e2);
-        if (nfield >= 3 && Bi) {
-            FieldAt<T> B0{Bi, N, x, y, z}, B1{Bi + M, N, x, y, z}, B2{Bi + 2 * M, N, x, y, z};
-            const T o = (T)1;
-            T pg[3];
-            pg[0] = ((o - r[2]) * (B1(1, 0, 0) - B1(0, 0, 0)) + r[2] * (B1(1, 0, 1) - B1(0, 0, 1))) * vv[1];
-            pg[0] += ((o - r[1]) * (B2(1, 0, 0) - B2(0, 0, 0)) + r[1] * (B2(1, 1, 0) - B2(0, 1, 0))) * vv[2];
-            pg[0] += (o - r[1]) * (o - r[2]) * ((r[0] - o) * B0(-1, 0, 0) + (o - (T)2 * r[0]) * B0(0, 0, 0) + r[0] * B0(1, 0, 0)) * vv[0];
-            pg[0] += r[1] * (o - r[2]) * ((r[0] - o) * B0(-1, 1, 0) + (o - (T)2 * r[0]) * B0(0, 1, 0) + r[0] * B0(1, 1, 0)) * vv[0];
-            pg[0] += (o - r[1]) * r[2] * ((r[0] - o) * B0(-1, 0, 1) + (o - (T)2 * r[0]) * B0(0, 0, 1) + r[0] * B0(1, 0, 1)) * vv[0];
-            pg[0] += r[1] * r[2] * ((r[0] - o) * B0(-1, 1, 1) + (o - (T)2 * r[0]) * B0(0, 1, 1) + r[0] * B0(1, 1, 1)) * vv[0];
-
-            pg[1] = ((o - r[0]) * (B2(0, 1, 0) - B2(0, 0, 0)) + r[0] * (B2(1, 1, 0) - B2(1, 0, 0))) * vv[2];
-            pg[1] += ((o - r[2]) * (B0(0, 1, 0) - B0(0, 0, 0)) + r[2] * (B0(0, 1, 1) - B0(0, 0, 1))) * vv[0];
-            pg[1] += (o - r[0]) * (o - r[2]) * ((r[1] - o) * B1(0, -1, 0) + (o - (T)2 * r[1]) * B1(0, 0, 0) + r[1] * B1(0, 1, 0)) * vv[1];
-            pg[1] += r[0] * (o - r[2]) * ((r[1] - o) * B1(1, -1, 0) + (o - (T)2 * r[1]) * B1(1, 0, 0) + r[1] * B1(1, 1, 0)) * vv[1];
-            pg[1] += (o - r[0]) * r[2] * ((r[1] - o) * B1(0, -1, 1) + (o - (T)2 * r[1]) * B1(0, 0, 1) + r[1] * B1(0, 1, 1)) * vv[1];
-            pg[1] += r[0] * r[2] * ((r[1] - o) * B1(1, -1, 1) + (o - (T)2 * r[1]) * B1(1, 0, 1) + r[1] * B1(1, 1, 1)) * vv[1];
-
-            pg[2] = ((o - r[1]) * (B0(0, 0, 1) - B0(0, 0, 0)) + r[1] * (B0(0, 1, 1) - B0(0, 1, 0))) * vv[0];
-            pg[2] += ((o - r[0]) * (B1(0, 0, 1) - B1(0, 0, 0)) + r[0] * (B1(1, 0, 1) - B1(1, 0, 0))) * vv[1];
-            pg[2] += (o - r[0]) * (o - r[1]) * ((r[2] - o) * B2(0, 0, -1) + (o - (T)2 * r[2]) * B2(0, 0, 0) + r[2] * B2(0, 0, 1)) * vv[2];
-            pg[2] += r[0] * (o - r[1]) * ((r[2] - o) * B2(1, 0, -1) + (o - (T)2 * r[2]) * B2(1, 0, 0) + r[2] * B2(1, 0, 1)) * vv[2];
-            pg[2] += (o - r[0]) * r[1] * ((r[2] - o) * B2(0, 1, -1) + (o - (T)2 * r[2]) * B2(0, 1, 0) + r[2] * B2(0, 1, 1)) * vv[2];
-            pg[2] += r[0] * r[1] * ((r[2] - o) * B2(1, 1, -1) + (o - (T)2 * r[2]) * B2(1, 1, 0) + r[2] * B2(1, 1, 1)) * vv[2];
-
-            g[0] += pg[0] / bcoeff / e2;
-            g[1] += pg[1] / bcoeff / e2;
-            g[2] += pg[2] / bcoeff / e2;
-        }
-        const T Nn = (T)N;
-        v2 = (T)0;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            vv[d] -= dtau * e2 * g[d] * Nn;
-            v2 += vv[d] * vv[d];
-        }
-        v.q0[i] = vv[0]; v.q1[i] = vv[1]; v.q2[i] = vv[2];
-        ret = (double)(v2 / a / a);
-    }
-    block_max_to(ret, maxslot);
-}
-
-// update_q_Newton, gevolution.hpp:1051-1118
-template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_update_q_newton(PV<T> v, double dtau_d, const T *__restrict__ psi,
-                                                           const T *__restrict__ chi, int nfield, double a_d,
-                                                           unsigned long long *maxslot)
-{
-    int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
-    double ret = 0.0;
-    if (i < v.n) {
-        const int N = v.N;
-        int x, y, z;
-        cell_coords(v.cell[i], N, v.lg, x, y, z);
-        T r[3];
-        ref_dist(v, i, x, y, z, r);
-        const T a = (T)a_d, dtau = (T)dtau_d;
-        FieldAt<T> Fp{psi, N, x, y, z};
-        T g[3];
-        grad2(Fp, r, g);
-        if (nfield >= 2 && chi) {
-            FieldAt<T> Fc{chi, N, x, y, z};
-            T gc[3];
-            grad1(Fc, r, gc);
-            g[0] -= gc[0]; g[1] -= gc[1]; g[2] -= gc[2];
-        }
-        const T Nn = (T)N;
-        T q[3] = {v.q0[i], v.q1[i], v.q2[i]};
-        T v2 = (T)0;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            q[d] = q[d] - dtau * a * g[d] * Nn;
-            v2 += q[d] * q[d];
-        }
-        v.q0[i] = q[0]; v.q1[i] = q[1]; v.q2[i] = q[2];
-        ret = (double)(v2 / a / a);
-    }
-    block_max_to(ret, maxslot);
-}
-
-// update_pos, gevolution.hpp:1162-1232
-template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_update_pos(PV<T> v, double dtau, const T *__restrict__ phi,
-                                                      const T *__restrict__ chi, const T *__restrict__ Bi, int nfield,
-                                                      double a_d, double bcoeff_d, int64_t M)
-{
-    int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
-    if (i >= v.n) return;
-    const int N = v.N;
-    int x, y, z;
-    cell_coords(v.cell[i], N, v.lg, x, y, z);
-    T r[3];
-    ref_dist(v, i, x, y, z, r);
-    const T a = (T)a_d, bcoeff = (T)bcoeff_d;
-    T q0 = v.q0[i], q1 = v.q1[i], q2 = v.q2[i];
-    T v2 = q0 * q0 + q1 * q1 + q2 * q2;
-    T e2 = v2 + a * a;
-    T ph = (T)0, ch = (T)0;
-    if (nfield >= 1) ph = cic(FieldAt<T>{phi, N, x, y, z}, r);
-    if (nfield >= 2) ch = cic(FieldAt<T>{chi, N, x, y, z}, r);
-    v2 = ((T)1 + ((T)3 - v2 / e2) * ph - ch) / sqrt(e2);
-    T vel[3] = {q0 * v2, q1 * v2, q2 * v2};
-    double d[3];
-    if (nfield >= 3) {
-        FieldAt<T> B0{Bi, N, x, y, z}, B1{Bi + M, N, x, y, z}, B2{Bi + 2 * M, N, x, y, z};
-        const T o = (T)1;
-        T b[3];
-        b[0] = B0(0, 0, 0) * (o - r[1]) * (o - r[2]);
-        b[1] = B1(0, 0, 0) * (o - r[0]) * (o - r[2]);
-        b[2] = B2(0, 0, 0) * (o - r[0]) * (o - r[1]);
-        b[1] += B1(1, 0, 0) * r[0] * (o - r[2]);
-        b[2] += B2(1, 0, 0) * r[0] * (o - r[1]);
-        b[0] += B0(0, 1, 0) * r[1] * (o - r[2]);
-        b[2] += B2(0, 1, 0) * (o - r[0]) * r[1];
-        b[0] += B0(0, 0, 1) * (o - r[1]) * r[2];
-        b[1] += B1(0, 0, 1) * (o - r[0]) * r[2];
-        b[1] += B1(1, 0, 1) * r[0] * r[2];
-        b[0] += B0(0, 1, 1) * r[1] * r[2];
-        b[2] += B2(1, 1, 0) * r[0] * r[1];
-#pragma unroll
-        for (int l = 0; l < 3; l++) d[l] = dtau * (double)(vel[l] + b[l] / bcoeff);
-    } else {
-#pragma unroll
-        for (int l = 0; l < 3; l++) d[l] = dtau * (double)vel[l];
-    }
-    move_by(v, i, x, y, z, d);
-}
-
-// update_pos_Newton, gevolution.hpp:1270-1273
-template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_update_pos_newton(PV<T> v, double dtau, double a)
-{
-    int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
-    if (i >= v.n) return;
-    int x, y, z;
-    cell_coords(v.cell[i], v.N, v.lg, x, y, z);
-    double d[3] = {dtau * (double)v.q0[i] / a, dtau * (double)v.q1[i] / a, dtau * (double)v.q2[i] / a};
-    move_by(v, i, x, y, z, d);
-}
 
 // displace_pcls_ic_basic, ic_basic.hpp:65-96 (one field)
 template <typename T>
@@ -833,15 +482,12 @@ __global__ __launch_bounds__(PT_NT) void k_initialize_q(PV<T> v, const T *__rest
 // ============================================================ host side ===
 static inline unsigned pblocks(int64_t n) { return (unsigned)((n + PT_NT - 1) / PT_NT); }
 
-template <typename T> int pcls_sort_impl(gev_pcls *p)
+// scan + scatter, given per-cell counts and per-particle ranks
+template <typename T> int pcls_sort_counted_impl(gev_pcls *p)
 {
     gev_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const int64_t M = ctx->M, n = p->n;
-    int rc = field_fill_impl<int32_t>(ctx, p->cell_count, M, 0);
-    if (rc) return rc;
-    if (n > 0)
-        hipLaunchKernelGGL(k_count, dim3(pblocks(n)), dim3(PT_NT), 0, s, p->cell, p->rank, p->cell_count, n);
     int64_t nb = (M + SCAN_CHUNK - 1) / SCAN_CHUNK;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_NT), 0, s, p->cell_count, M, ctx->scan_blocks);
     hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(1024), 0, s, ctx->scan_blocks, nb);
@@ -865,6 +511,18 @@ template <typename T> int pcls_sort_impl(gev_pcls *p)
     GEV_HIP(hipGetLastError());
     p->sorted = true;
     return GEV_OK;
+}
+
+template <typename T> int pcls_sort_impl(gev_pcls *p)
+{
+    gev_ctx *ctx = p->ctx;
+    const int64_t M = ctx->M, n = p->n;
+    int rc = field_fill_impl<int32_t>(ctx, p->cell_count, M, 0);
+    if (rc) return rc;
+    if (n > 0)
+        hipLaunchKernelGGL(k_count, dim3(pblocks(n)), dim3(PT_NT), 0, ctx->stream, p->cell, p->rank, p->cell_count, n);
+    GEV_HIP(hipGetLastError());
+    return pcls_sort_counted_impl<T>(p);
 }
 
 template <typename T> int pcls_lattice_impl(gev_pcls *p)
@@ -949,55 +607,6 @@ static int clear_max(gev_ctx *c)
     return field_fill_impl<double>(c, c->red + 8, 1, 0.0);
 }
 
-template <typename T>
-int update_q_impl(gev_pcls *p, double dtau, const T *phi, const T *chi, const T *Bi, int nfield, double a,
-                  double bcoeff, double *max_ret)
-{
-    gev_ctx *c = p->ctx;
-    int rc = clear_max(c);
-    if (rc) return rc;
-    if (p->n > 0)
-        hipLaunchKernelGGL((k_update_q<T>), dim3(pblocks(p->n)), dim3(PT_NT), 0, c->stream, make_pv<T>(p), dtau, phi,
-                           chi, Bi, nfield, a, bcoeff, c->M, (unsigned long long *)(c->red + 8));
-    GEV_HIP(hipGetLastError());
-    return max_ret ? read_max(c, max_ret) : GEV_OK;
-}
-
-template <typename T>
-int update_q_newton_impl(gev_pcls *p, double dtau, const T *psi, const T *chi, int nfield, double a, double *max_ret)
-{
-    gev_ctx *c = p->ctx;
-    int rc = clear_max(c);
-    if (rc) return rc;
-    if (p->n > 0)
-        hipLaunchKernelGGL((k_update_q_newton<T>), dim3(pblocks(p->n)), dim3(PT_NT), 0, c->stream, make_pv<T>(p), dtau,
-                           psi, chi, nfield, a, (unsigned long long *)(c->red + 8));
-    GEV_HIP(hipGetLastError());
-    return max_ret ? read_max(c, max_ret) : GEV_OK;
-}
-
-template <typename T>
-int update_pos_impl(gev_pcls *p, double dtau, const T *phi, const T *chi, const T *Bi, int nfield, double a,
-                    double bcoeff)
-{
-    gev_ctx *c = p->ctx;
-    if (p->n > 0)
-        hipLaunchKernelGGL((k_update_pos<T>), dim3(pblocks(p->n)), dim3(PT_NT), 0, c->stream, make_pv<T>(p), dtau, phi,
-                           chi, Bi, nfield, a, bcoeff, c->M);
-    GEV_HIP(hipGetLastError());
-    return pcls_sort_impl<T>(p);
-}
-
-template <typename T> int update_pos_newton_impl(gev_pcls *p, double dtau, double a)
-{
-    gev_ctx *c = p->ctx;
-    if (p->n > 0)
-        hipLaunchKernelGGL((k_update_pos_newton<T>), dim3(pblocks(p->n)), dim3(PT_NT), 0, c->stream, make_pv<T>(p),
-                           dtau, a);
-    GEV_HIP(hipGetLastError());
-    return pcls_sort_impl<T>(p);
-}
-
 template <typename T> int displace_impl(gev_pcls *p, const T *xi, double coeff, double *max_disp)
 {
     gev_ctx *c = p->ctx;
@@ -1028,17 +637,13 @@ template <typename T> int initialize_q_impl(gev_pcls *p, const T *pot, double co
 
 #define INST(T)                                                                                                   \
     template int pcls_sort_impl<T>(gev_pcls *);                                                                   \
+    template int pcls_sort_counted_impl<T>(gev_pcls *);                                                           \
     template int pcls_lattice_impl<T>(gev_pcls *);                                                                \
     template int pcls_load_impl<T>(gev_pcls *, const double *, const double *, const int64_t *);                  \
     template int pcls_store_impl<T>(gev_pcls *, double *, double *, int64_t *);                                   \
     template int dep_T00_impl<T>(gev_pcls *, T *, const T *, double, double, int);                                \
     template int dep_T0i_impl<T>(gev_pcls *, T *, const T *, double, int);                                        \
     template int dep_Tij_impl<T>(gev_pcls *, T *, const T *, double, double, int);                                \
-    template int update_q_impl<T>(gev_pcls *, double, const T *, const T *, const T *, int, double, double,       \
-                                  double *);                                                                      \
-    template int update_q_newton_impl<T>(gev_pcls *, double, const T *, const T *, int, double, double *);       \
-    template int update_pos_impl<T>(gev_pcls *, double, const T *, const T *, const T *, int, double, double);    \
-    template int update_pos_newton_impl<T>(gev_pcls *, double, double);                                          \
     template int displace_impl<T>(gev_pcls *, const T *, double, double *);                                       \
     template int initialize_q_impl<T>(gev_pcls *, const T *, double, double *);
 INST(double)

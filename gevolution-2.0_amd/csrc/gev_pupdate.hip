@@ -1,0 +1,214 @@
+// gev_pupdate.hip -- tiled particle updates (kick, drift, and both fused).
+//
+// One workgroup owns a tile of TX x TY x TZ cells.  Every field value the
+// update_q / update_pos stencils can touch (offsets -1..+2 around the tile)
+// is staged in LDS once; the tile's particles -- TY*TZ contiguous row
+// segments of the cell-sorted container -- are then spread over the threads
+// and read their ~60 stencil values from LDS instead of global memory.  The
+// fused form also counts the re-binning ranks of the drifted particles, so a
+// full cycle touches each particle's data once before the scatter.
+#include "gev_common.hpp"
+#include "gev_kernels.hpp"
+#include "gev_pcommon.hpp"
+
+namespace gev {
+
+enum { UPD_KICK_GR = 1, UPD_KICK_NEWTON = 2, UPD_DRIFT_GR = 4, UPD_DRIFT_NEWTON = 8, UPD_COUNT = 16 };
+
+struct UpdParams {
+    double dtau_k, a_k, bc_k;  // kick: dtau, a, a^2 N
+    double dtau_d, a_d, bc_d;  // drift
+    int nf_k, nf_d, flags;
+};
+
+template <typename T, int TX, int TY, int TZ>
+__global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, const T *__restrict__ phi,
+                                                        const T *__restrict__ chi, const T *__restrict__ Bi, int64_t M,
+                                                        unsigned long long *maxslot, int32_t *count, int32_t *rank)
+{
+    constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3, NS = SX * SY * SZ, NROW = TY * TZ;
+    __shared__ T fld[5][NS];
+    __shared__ int32_t rs[NROW + 1];
+    __shared__ int32_t rbeg[NROW];
+    const int N = v.N;
+    const int tid = threadIdx.x;
+    int x0, y0, z0;
+    tile_origin<TX, TY, TZ>(N, x0, y0, z0);
+    const bool kick = u.flags & (UPD_KICK_GR | UPD_KICK_NEWTON);
+    const bool drift = u.flags & (UPD_DRIFT_GR | UPD_DRIFT_NEWTON);
+    const int nf_k = (u.flags & UPD_KICK_NEWTON) ? u.nf_k : u.nf_k;
+    const int nf_d = (u.flags & UPD_DRIFT_GR) ? u.nf_d : 0;
+    const int nfmax = kick ? (nf_k > nf_d ? nf_k : nf_d) : nf_d;
+    const bool use_phi = kick || nf_d >= 1, use_chi = nfmax >= 2, use_B = nfmax >= 3;
+    for (int e = tid; e < NS; e += PT_NT) {
+        int lx = e % SX, ly = (e / SX) % SY, lz = e / (SX * SY);
+        int64_t g = site(N, x0 - 1 + lx, y0 - 1 + ly, z0 - 1 + lz);
+        if (use_phi) fld[0][e] = phi[g];
+        if (use_chi) fld[1][e] = chi[g];
+        if (use_B) {
+            fld[2][e] = Bi[g];
+            fld[3][e] = Bi[g + M];
+            fld[4][e] = Bi[g + 2 * M];
+        }
+    }
+    if (tid < NROW) {
+        int ry = tid % TY, rz = tid / TY;
+        int64_t rb = site(N, x0, y0 + ry, z0 + rz);
+        int32_t s0 = v.start[rb], s1 = v.start[rb + TX];
+        rbeg[tid] = s0;
+        rs[tid + 1] = s1 - s0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        rs[0] = 0;
+        for (int i = 1; i <= NROW; i++) rs[i] += rs[i - 1];
+    }
+    __syncthreads();
+    const int total = rs[NROW];
+    const T Nn = (T)N;
+    double ret = 0.0;
+    for (int f = tid; f < total; f += PT_NT) {
+        int lo = 0, hi = NROW;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (rs[mid] <= f) lo = mid; else hi = mid;
+        }
+        const int64_t p = (int64_t)rbeg[lo] + (f - rs[lo]);
+        int x, y, z;
+        cell_coords(v.cell[p], N, v.lg, x, y, z);
+        const int o = (x - x0 + 1) + SX * ((y - y0 + 1) + SY * (z - z0 + 1));
+        T r[3];
+        ref_dist(v, p, x, y, z, r);
+        T q[3] = {v.q0[p], v.q1[p], v.q2[p]};
+        FieldLds<T> Fp{fld[0], o, SX, SX * SY}, Fc{fld[1], o, SX, SX * SY};
+        FieldLds<T> B0{fld[2], o, SX, SX * SY}, B1{fld[3], o, SX, SX * SY}, B2{fld[4], o, SX, SX * SY};
+        if (u.flags & UPD_KICK_GR) {
+            T rv = phys_update_q<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn);
+            ret = fmax(ret, (double)rv);
+        } else if (u.flags & UPD_KICK_NEWTON) {
+            T rv = phys_update_q_newton<T>(r, q, Fp, Fc, nf_k, (T)u.a_k, (T)u.dtau_k, Nn);
+            ret = fmax(ret, (double)rv);
+        }
+        if (kick) { v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2]; }
+        if (drift) {
+            double d[3];
+            if (u.flags & UPD_DRIFT_GR) {
+                phys_update_pos<T>(r, q, Fp, Fc, B0, B1, B2, nf_d, (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
+            } else {
+                d[0] = u.dtau_d * (double)q[0] / u.a_d;
+                d[1] = u.dtau_d * (double)q[1] / u.a_d;
+                d[2] = u.dtau_d * (double)q[2] / u.a_d;
+            }
+            move_by(v, p, x, y, z, d);
+            if (u.flags & UPD_COUNT) rank[p] = atomicAdd(&count[v.cell[p]], 1);
+        }
+    }
+    if (kick) block_max_to(ret, maxslot);
+}
+
+template <typename T> int pcls_sort_impl(gev_pcls *p);
+template <typename T> int pcls_sort_counted_impl(gev_pcls *p);
+
+template <typename T> static int launch_update(gev_pcls *p, const UpdParams &u, const T *phi, const T *chi, const T *Bi)
+{
+    gev_ctx *c = p->ctx;
+    const int N = c->N;
+    if (u.flags & (UPD_KICK_GR | UPD_KICK_NEWTON)) {
+        int rc = field_fill_impl<double>(c, c->red + 8, 1, 0.0);
+        if (rc) return rc;
+    }
+    if (u.flags & UPD_COUNT) {
+        int rc = field_fill_impl<int32_t>(c, p->cell_count, c->M, 0);
+        if (rc) return rc;
+    }
+    unsigned long long *slot = (unsigned long long *)(c->red + 8);
+    if (N >= 8) {
+        unsigned nb = (unsigned)((int64_t)(N / 8) * (N / 8) * (N / 4));
+        hipLaunchKernelGGL((k_update_tiled<T, 8, 8, 4>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), u, phi, chi,
+                           Bi, c->M, slot, p->cell_count, p->rank);
+    } else {
+        unsigned nb = (unsigned)((int64_t)(N / 4) * (N / 4) * (N / 4));
+        hipLaunchKernelGGL((k_update_tiled<T, 4, 4, 4>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), u, phi, chi,
+                           Bi, c->M, slot, p->cell_count, p->rank);
+    }
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+static int read_max_slot(gev_ctx *c, double *out)
+{
+    GEV_HIP(hipMemcpyAsync(c->red_host, c->red + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GEV_HIP(hipStreamSynchronize(c->stream));
+    *out = c->red_host[0];
+    return GEV_OK;
+}
+
+template <typename T>
+int update_q_impl(gev_pcls *p, double dtau, const T *phi, const T *chi, const T *Bi, int nfield, double a,
+                  double bcoeff, double *max_ret)
+{
+    UpdParams u = {dtau, a, bcoeff, 0, 1, 1, nfield, 0, UPD_KICK_GR};
+    int rc = launch_update<T>(p, u, phi, chi, Bi);
+    if (rc) return rc;
+    return max_ret ? read_max_slot(p->ctx, max_ret) : GEV_OK;
+}
+
+template <typename T>
+int update_q_newton_impl(gev_pcls *p, double dtau, const T *psi, const T *chi, int nfield, double a, double *max_ret)
+{
+    UpdParams u = {dtau, a, 1, 0, 1, 1, nfield, 0, UPD_KICK_NEWTON};
+    int rc = launch_update<T>(p, u, psi, chi, nullptr);
+    if (rc) return rc;
+    return max_ret ? read_max_slot(p->ctx, max_ret) : GEV_OK;
+}
+
+template <typename T>
+int update_pos_impl(gev_pcls *p, double dtau, const T *phi, const T *chi, const T *Bi, int nfield, double a,
+                    double bcoeff)
+{
+    UpdParams u = {0, 1, 1, dtau, a, bcoeff, 0, nfield, UPD_DRIFT_GR | UPD_COUNT};
+    int rc = launch_update<T>(p, u, phi, chi, Bi);
+    if (rc) return rc;
+    return pcls_sort_counted_impl<T>(p);
+}
+
+template <typename T> int update_pos_newton_impl(gev_pcls *p, double dtau, double a)
+{
+    UpdParams u = {0, 1, 1, dtau, a, 1, 0, 0, UPD_DRIFT_NEWTON | UPD_COUNT};
+    int rc = launch_update<T>(p, u, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    return pcls_sort_counted_impl<T>(p);
+}
+
+template <typename T>
+int kick_drift_impl(gev_pcls *p, int gr, double dtau_k, double dtau_d, const T *phi, const T *chi, const T *Bi,
+                    int nf_k, int nf_d, double a_k, double a_d, double bc_k, double bc_d, double *max_ret, int rebin)
+{
+    UpdParams u = {dtau_k, a_k, bc_k, dtau_d, a_d, bc_d, nf_k, nf_d,
+                   (gr ? (UPD_KICK_GR | UPD_DRIFT_GR) : (UPD_KICK_NEWTON | UPD_DRIFT_NEWTON)) | UPD_COUNT};
+    int rc = launch_update<T>(p, u, phi, chi, Bi);
+    if (rc) return rc;
+    if (max_ret) {
+        rc = read_max_slot(p->ctx, max_ret);
+        if (rc) return rc;
+    }
+    if (!rebin) {
+        p->sorted = false;  // counts and ranks are ready; the scatter is pending
+        return GEV_OK;
+    }
+    return pcls_sort_counted_impl<T>(p);
+}
+
+#define INST(T)                                                                                                   \
+    template int update_q_impl<T>(gev_pcls *, double, const T *, const T *, const T *, int, double, double,       \
+                                  double *);                                                                      \
+    template int update_q_newton_impl<T>(gev_pcls *, double, const T *, const T *, int, double, double *);       \
+    template int update_pos_impl<T>(gev_pcls *, double, const T *, const T *, const T *, int, double, double);    \
+    template int update_pos_newton_impl<T>(gev_pcls *, double, double);                                          \
+    template int kick_drift_impl<T>(gev_pcls *, int, double, double, const T *, const T *, const T *, int, int,   \
+                                    double, double, double, double, double *, int);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace gev

@@ -291,6 +291,14 @@ class Particles:
         self.ctx.bind_stream()
         check(lib.gev_update_pos_newton(self._h, dtau, a), "gev_update_pos_newton")
 
+    def kick_drift(self, gr, dtau_kick, dtau_drift, phi, chi, Bi, nf_kick, nf_drift, a_kick, a_drift, bc_kick,
+                   bc_drift):
+        r = C.c_double()
+        self.ctx.bind_stream()
+        check(lib.gev_kick_drift(self._h, int(gr), dtau_kick, dtau_drift, _ptr(phi), _ptr(chi), _ptr(Bi), nf_kick,
+                                 nf_drift, a_kick, a_drift, bc_kick, bc_drift, C.byref(r), 1), "gev_kick_drift")
+        return r.value
+
     def displace(self, xi, coeff=1.0):
         r = C.c_double()
         self.ctx.bind_stream()
@@ -386,7 +394,7 @@ def step(pcls, fields, state):
 
 
 PHASES = ["deposit_T00", "source_T00", "deposit_Tij", "solve_phi", "source_Tij", "deposit_T0i", "solve_chi",
-          "solve_B", "kick", "drift_rebin"]
+          "solve_B", "kick_drift", "rebin"]
 
 
 def step_timing(ctx, enable=True):

@@ -71,6 +71,8 @@ SIGNATURES = {
     "gev_update_q_newton": (_i, [_vp, _d, _vp, _vp, _i, _d, _dp]),
     "gev_update_pos": (_i, [_vp, _d, _vp, _vp, _vp, _i, _d, _d]),
     "gev_update_pos_newton": (_i, [_vp, _d, _d]),
+    "gev_kick_drift": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _dp, _i]),
+    "gev_pcls_rebin_counted": (_i, [_vp]),
     "gev_ic_displacement_ft": (_i, [_vp, _u32, _i, _d, _i, _dp, _dp, _vp, _vp]),
     "gev_displace_particles": (_i, [_vp, _vp, _d, _dp]),
     "gev_initialize_q": (_i, [_vp, _vp, _d, _dp]),

@@ -126,6 +126,19 @@ int gev_update_pos(gev_pcls *p, double dtau, const void *phi, const void *chi, c
                    double bcoeff);                                           /* gevolution.hpp:1162-1232 + re-bin */
 int gev_update_pos_newton(gev_pcls *p, double dtau, double a);               /* gevolution.hpp:1270-1273 + re-bin */
 
+/* kick and drift fused (main.cu:1227-1266 with the background half-step in
+ * between done by the caller): one pass reads each particle once, applies
+ * update_q (gr=1) or update_q_Newton (gr=0) with (dtau_kick, a_kick,
+ * bcoeff_kick, nfield_kick), then update_pos / update_pos_Newton with the
+ * kicked momentum and (dtau_drift, a_drift, bcoeff_drift, nfield_drift), and
+ * counts the re-binning ranks.  rebin=1 finishes the re-binning; rebin=0
+ * leaves it to gev_pcls_rebin_counted (no other call may touch the container
+ * in between). */
+int gev_kick_drift(gev_pcls *p, int gr, double dtau_kick, double dtau_drift, const void *phi, const void *chi,
+                   const void *Bi, int nfield_kick, int nfield_drift, double a_kick, double a_drift,
+                   double bcoeff_kick, double bcoeff_drift, double *max_ret, int rebin);
+int gev_pcls_rebin_counted(gev_pcls *p);
+
 /* ---- initial conditions (generateIC_basic mPk path, ic_basic.hpp:1834-2505) */
 /* displacement potential in k-space (generateDisplacementField, ic_basic.hpp:1164-1270);
  * kL/amp: power-spectrum table converted as loadPowerSpectrum (ic_basic.hpp:445-447) */

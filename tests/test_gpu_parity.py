@@ -387,3 +387,37 @@ def test_steps_vs_oracle(gp, O, gr, N, steps):
         assert relmax(real_np(f.chi)[0], ost["chi"]) < 1e-7
         assert relmax(real_np(f.Bi), ost["Bi"]) < 1e-7
     assert np.isclose(st.maxvel, ost["maxvel"], rtol=1e-9)
+
+
+@pytest.mark.parametrize("gr", [0, 1])
+def test_fused_kick_drift_vs_oracle(gp, O, gr):
+    """gev_kick_drift = update_q then update_pos with the kicked momentum (main.cu:1232-1266)."""
+    N = 16
+    ctx = gp.Context(N, gp.GEV_FP64)
+    n = 30000
+    pos, q, pid = make_particles(N, n, 900 + gr, vmax=0.3)
+    o = np.argsort(pid)
+    rng = np.random.default_rng(77)
+    phi, chi = 1e-3 * rng.standard_normal(N ** 3), 1e-4 * rng.standard_normal(N ** 3)
+    B = 1e-2 * rng.standard_normal((3, N ** 3))
+    p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+    a_k, a_d = 0.2, 0.21
+    args = (field_gpu(ctx, phi), field_gpu(ctx, chi), field_gpu(ctx, B, 3))
+    if gr:
+        ret = p.kick_drift(1, 0.013, 0.02, *args, 3, 3, a_k, a_d, a_k ** 2 * N, a_d ** 2 * N)
+        qo, po = q.copy(), pos.copy()
+        rref = O.update_q(N, po, qo, 0.013, phi, chi, B, 3, a_k, a_k ** 2 * N)
+        O.update_pos(N, po, qo, 0.02, phi, chi, B, 3, a_d, a_d ** 2 * N)
+    else:
+        ret = p.kick_drift(0, 0.013, 0.02, args[0], None, None, 1, 0, a_k, a_d, 1.0, 1.0)
+        qo, po = q.copy(), pos.copy()
+        rref = O.update_q_newton(N, po, qo, 0.013, phi, a_k)
+        O.update_pos_newton(N, po, qo, 0.02, a_d)
+    gpos, gq, gid = by_id(p)
+    assert np.array_equal(gid, np.sort(pid))
+    assert relmax(gq, qo[:, o]) < 1e-13
+    assert np.abs(((gpos - po[:, o]) + 0.5) % 1.0 - 0.5).max() < 1e-15
+    assert np.isclose(ret, rref, rtol=1e-13)
+    cells = to_np(p.cells())
+    c = np.floor(gpos * N).astype(np.int64)
+    assert np.array_equal(np.sort(cells), np.sort(c[0] + N * (c[1] + N * c[2])))

@@ -1,0 +1,303 @@
+// gev_deposit.hip -- CIC projections of T00, T0i and Tij onto the sites of a
+// tile, particle-parallel.
+//
+// One workgroup owns a tile of TX x TY x TZ sites.  Its sites receive
+// contributions from the cells [x0-1, x0+TX-1] x ... (the tile plus one lower
+// halo layer), whose particles lie in 2 (TY+1)(TZ+1) contiguous runs of the
+// cell-sorted arrays (per halo row: the x0-1 cell, then the TX cells of the
+// tile; the split keeps a run contiguous when x0-1 wraps).  The runs are
+// flattened and walked one particle per thread, so the loads are coalesced and
+// every lane does useful work whatever the occupation of the cells.  Each
+// particle evaluates the reference functor and adds the corner / link values
+// that land inside the tile to per-site LDS accumulators with LDS atomics; the
+// tile then issues one plain store per site and component.  The particles of
+// the halo layer are evaluated by both neighbouring tiles, which trades
+// (TX+1)(TY+1)(TZ+1)/(TX TY TZ) particle reads for having no global atomics
+// and no zeroing pass.  The summation order inside a site follows LDS atomic
+// arbitration, as the reference's atomicAdd projections do.
+#include "gev_common.hpp"
+#include "gev_kernels.hpp"
+#include "gev_pcommon.hpp"
+
+namespace gev {
+
+enum { DEP_T00 = 0, DEP_T0I = 1, DEP_TIJ = 2 };
+template <int KIND> struct DepC { static constexpr int C = KIND == DEP_T00 ? 1 : (KIND == DEP_T0I ? 3 : 6); };
+
+// FP64 LDS accumulators in both precisions: ds_add_f64 is the fast LDS atomic
+// on this part, and the per-site sums carry the full FP64 mantissa.
+__device__ __forceinline__ void lds_add(double *a, double v)
+{
+    __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <typename T, int KIND, int TX, int TY, int TZ>
+__global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, const T *__restrict__ phi,
+                                                   double mass_d, double a_d, int accumulate)
+{
+    constexpr int C = DepC<KIND>::C;
+    constexpr int NS = TX * TY * TZ;                    // owned sites
+    constexpr int HY = TY + 1, NR = HY * (TZ + 1);      // halo rows of cells
+    constexpr int NSEG = 2 * NR;                        // runs: halo cell, tile cells
+    constexpr int PX = TX + 2, PY = TY + 2, PZ = TZ + 2;  // phi box x0-1 .. x0+TX
+    constexpr int NPH = PX * PY * PZ;
+    static_assert(NSEG <= PT_NT, "one run per thread in the scan");
+    __shared__ double acc[C][NS];
+    __shared__ T phs[NPH];
+    __shared__ int32_t rbeg[NSEG + 1];
+    __shared__ int32_t rst[NSEG];
+    __shared__ int32_t wsum[PT_NT / 64];
+    const int N = v.N;
+    const uint32_t msk = (uint32_t)N - 1;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int x0, y0, z0;
+    tile_origin<TX, TY, TZ>(N, x0, y0, z0);
+    const bool use_phi = phi != nullptr;
+
+    for (int e = t; e < NS; e += PT_NT) {
+#pragma unroll
+        for (int c = 0; c < C; c++) acc[c][e] = 0.0;
+    }
+    if (use_phi) {
+        for (int e = t; e < NPH; e += PT_NT) {
+            const int px = e % PX, py = (e / PX) % PY, pz = e / (PX * PY);
+            const uint32_t g = ((uint32_t)(x0 - 1 + px) & msk) +
+                               (uint32_t)N * (((uint32_t)(y0 - 1 + py) & msk) + (uint32_t)N * ((uint32_t)(z0 - 1 + pz) & msk));
+            phs[e] = phi[g];
+        }
+    }
+    // runs and their lengths; exclusive scan over NSEG <= PT_NT entries
+    int32_t len = 0;
+    if (t < NSEG) {
+        const int r = t >> 1, ly = r % HY, lz = r / HY;
+        const uint32_t row = (uint32_t)N * (((uint32_t)(y0 - 1 + ly) & msk) + (uint32_t)N * ((uint32_t)(z0 - 1 + lz) & msk));
+        const uint32_t c0 = (t & 1) ? row + (uint32_t)x0 : row + ((uint32_t)(x0 - 1) & msk);
+        const uint32_t c1 = (t & 1) ? c0 + TX : c0 + 1;
+        const int32_t s0 = v.start[c0];
+        rst[t] = s0;
+        len = v.start[c1] - s0;
+    }
+    int32_t incl = len;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int32_t off = 0;
+    for (int j = 0; j < wv; j++) off += wsum[j];
+    if (t < NSEG) rbeg[t] = off + incl - len;
+    if (t == NSEG - 1) rbeg[NSEG] = off + incl;
+    __syncthreads();
+    const int32_t total = rbeg[NSEG];
+    const T a = (T)a_d, o = (T)1;
+
+    // U particles per thread and pass: their loads are issued together
+    constexpr int U = KIND == DEP_TIJ ? 2 : 4;
+    const bool need_q = KIND != DEP_T00 || use_phi;
+    for (int32_t i0 = t; i0 < total; i0 += U * PT_NT) {
+        int run[U];
+        int64_t pu[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int32_t i = i0 + u * PT_NT;
+            run[u] = -1;
+            pu[u] = 0;
+            if (i < total) {
+                // run holding flattened index i
+                int lo = 0, hi = NSEG - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (rbeg[mid] <= i) lo = mid;
+                    else hi = mid - 1;
+                }
+                run[u] = lo;
+                pu[u] = (int64_t)rst[lo] + (i - rbeg[lo]);
+            }
+        }
+        T xu[U][3], qu[U][3];
+        uint32_t cu[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (run[u] < 0) continue;
+            const int64_t p = pu[u];
+            cu[u] = (run[u] & 1) ? (uint32_t)v.cell[p] : 0u;
+            xu[u][0] = v.p0[p]; xu[u][1] = v.p1[p]; xu[u][2] = v.p2[p];
+            if (need_q) { qu[u][0] = v.q0[p]; qu[u][1] = v.q1[p]; qu[u][2] = v.q2[p]; }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+        if (run[u] < 0) continue;
+        const int r = run[u] >> 1, ly = r % HY, lz = r / HY;
+        const int lx = (run[u] & 1) ? 1 + (int)((cu[u] & msk) - (uint32_t)x0) : 0;
+        const int gx = (x0 - 1 + lx) & (int)msk, gy = (y0 - 1 + ly) & (int)msk, gz = (z0 - 1 + lz) & (int)msk;
+        T rd[3];
+        cell_rel(xu[u], gx, gy, gz, N, rd);
+        // site of corner (bx,by,bz) in tile coordinates and whether the tile owns it
+        const int sx = lx - 1, sy = ly - 1, sz = lz - 1;
+        const bool inx0 = sx >= 0, inx1 = sx + 1 < TX;
+        const bool iny0 = sy >= 0, iny1 = sy + 1 < TY;
+        const bool inz0 = sz >= 0, inz1 = sz + 1 < TZ;
+        const int sb = sx + TX * (sy + TY * sz);
+        auto own = [&](int bx, int by, int bz) {
+            return (bx ? inx1 : inx0) && (by ? iny1 : iny0) && (bz ? inz1 : inz0);
+        };
+        auto sidx = [&](int bx, int by, int bz) { return sb + bx + TX * (by + TY * bz); };
+        const int pb = lx + PX * (ly + PY * lz);
+        if constexpr (KIND == DEP_T00) {
+            // projection_T00 functor, gevolution.hpp:1729-1854
+            if (use_phi) {
+                const T q0 = qu[u][0], q1 = qu[u][1], q2 = qu[u][2];
+                T f2 = q0 * q0 + q1 * q1 + q2 * q2;
+                const T e = sqrt(f2 + a * a);
+                const T f = (T)3 * e + f2 / e;
+                f2 = ((T)0.5 * f * f + f2 / (o + f2 / a / a)) / e;
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+                    if (own(bx, by, bz)) {
+                        const T ph = phs[pb + bx + PX * (by + PY * bz)];
+                        const T w = (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]);
+                        lds_add(&acc[0][sidx(bx, by, bz)], w * (e + f * ph + f2 * ph * ph));
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+                    if (own(bx, by, bz))
+                        lds_add(&acc[0][sidx(bx, by, bz)],
+                                (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
+                }
+            }
+        } else if constexpr (KIND == DEP_T0I) {
+            // projection_T0i functor, gevolution.hpp:2031-2201: 12 links per cell,
+            // each weighted by (1 + phi_a + phi_b) of its two end points
+            const T v0 = qu[u][0], v1 = qu[u][1], v2 = qu[u][2];
+            T g[12] = {(o - rd[1]) * (o - rd[2]) * v0, (o - rd[0]) * (o - rd[2]) * v1, (o - rd[0]) * (o - rd[1]) * v2,
+                       rd[0] * (o - rd[2]) * v1,       rd[0] * (o - rd[1]) * v2,       rd[1] * (o - rd[2]) * v0,
+                       rd[1] * (o - rd[0]) * v2,       rd[2] * (o - rd[1]) * v0,       rd[2] * (o - rd[0]) * v1,
+                       rd[1] * rd[2] * v0,             rd[0] * rd[2] * v1,             rd[0] * rd[1] * v2};
+            if (use_phi) {
+                T ph[8];
+#pragma unroll
+                for (int b = 0; b < 8; b++) ph[b] = phs[pb + ((b >> 2) & 1) + PX * (((b >> 1) & 1) + PY * (b & 1))];
+                g[0] *= o + ph[0] + ph[4]; g[1] *= o + ph[0] + ph[2]; g[2] *= o + ph[0] + ph[1];
+                g[3] *= o + ph[4] + ph[6]; g[4] *= o + ph[4] + ph[5]; g[5] *= o + ph[2] + ph[6];
+                g[6] *= o + ph[2] + ph[3]; g[7] *= o + ph[1] + ph[5]; g[8] *= o + ph[1] + ph[3];
+                g[9] *= o + ph[3] + ph[7]; g[10] *= o + ph[5] + ph[7]; g[11] *= o + ph[6] + ph[7];
+            }
+            // link k lands on component comp[k] of the site cell + offset[k]
+            if (own(0, 0, 0)) {
+                lds_add(&acc[0][sidx(0, 0, 0)], g[0]);
+                lds_add(&acc[1][sidx(0, 0, 0)], g[1]);
+                lds_add(&acc[2][sidx(0, 0, 0)], g[2]);
+            }
+            if (own(1, 0, 0)) { lds_add(&acc[1][sidx(1, 0, 0)], g[3]); lds_add(&acc[2][sidx(1, 0, 0)], g[4]); }
+            if (own(0, 1, 0)) { lds_add(&acc[0][sidx(0, 1, 0)], g[5]); lds_add(&acc[2][sidx(0, 1, 0)], g[6]); }
+            if (own(0, 0, 1)) { lds_add(&acc[0][sidx(0, 0, 1)], g[7]); lds_add(&acc[1][sidx(0, 0, 1)], g[8]); }
+            if (own(0, 1, 1)) lds_add(&acc[0][sidx(0, 1, 1)], g[9]);
+            if (own(1, 0, 1)) lds_add(&acc[1][sidx(1, 0, 1)], g[10]);
+            if (own(1, 1, 0)) lds_add(&acc[2][sidx(1, 1, 0)], g[11]);
+        } else {
+            // projection_Tij functor, gevolution.hpp:2582-2965 (no CIC_PROJECT_TIJ):
+            // diagonal parts CIC onto the 8 corners, off-diagonal parts onto 2 sites
+            const T vv[3] = {qu[u][0], qu[u][1], qu[u][2]};
+            T f = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
+            const T e = sqrt(f + a * a);
+            if (use_phi) {
+                const T *P = phs + pb;
+                f = (T)4 + a * a / (f + a * a);
+                f *= (o - rd[0]) * ((o - rd[1]) * ((o - rd[2]) * P[0] + rd[2] * P[PX * PY]) +
+                                    rd[1] * ((o - rd[2]) * P[PX] + rd[2] * P[PX + PX * PY])) +
+                     rd[0] * ((o - rd[1]) * ((o - rd[2]) * P[1] + rd[2] * P[1 + PX * PY]) +
+                              rd[1] * ((o - rd[2]) * P[1 + PX] + rd[2] * P[1 + PX + PX * PY]));
+                f += o;
+            } else {
+                f = o;
+            }
+            constexpr int dcomp[3] = {0, 3, 5};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const T w = f * vv[d] * vv[d] / e;
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+                    if (own(bx, by, bz))
+                        lds_add(&acc[dcomp[d]][sidx(bx, by, bz)],
+                                w * (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
+                }
+            }
+            T w = f * vv[0] * vv[1] / e;
+            if (own(0, 0, 0)) lds_add(&acc[1][sidx(0, 0, 0)], w * (o - rd[2]));
+            if (own(0, 0, 1)) lds_add(&acc[1][sidx(0, 0, 1)], w * rd[2]);
+            w = f * vv[0] * vv[2] / e;
+            if (own(0, 0, 0)) lds_add(&acc[2][sidx(0, 0, 0)], w * (o - rd[1]));
+            if (own(0, 1, 0)) lds_add(&acc[2][sidx(0, 1, 0)], w * rd[1]);
+            w = f * vv[1] * vv[2] / e;
+            if (own(0, 0, 0)) lds_add(&acc[4][sidx(0, 0, 0)], w * (o - rd[0]));
+            if (own(1, 0, 0)) lds_add(&acc[4][sidx(1, 0, 0)], w * rd[0]);
+        }
+        }
+    }
+    __syncthreads();
+
+    for (int s = t; s < NS; s += PT_NT) {
+        const int sx = s % TX, sy = (s / TX) % TY, sz = s / (TX * TY);
+        const uint32_t gs = (uint32_t)(x0 + sx) + (uint32_t)N * ((uint32_t)(y0 + sy) + (uint32_t)N * (uint32_t)(z0 + sz));
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            T *oc = out + c * M;
+            T val = (T)(acc[c][s] * mass_d);
+            if (accumulate) val += oc[gs];
+            oc[gs] = val;
+        }
+    }
+}
+
+template <typename T, int KIND, int TX, int TY, int TZ>
+static int launch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
+{
+    gev_ctx *c = p->ctx;
+    const int N = c->N;
+    const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
+    hipLaunchKernelGGL((k_deposit<T, KIND, TX, TY, TZ>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), out, c->M,
+                       phi, mass, a, acc);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+template <typename T, int KIND> static int dispatch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
+{
+    const int N = p->ctx->N;
+    // Tij keeps 6 FP64 accumulators per site: the smaller tile holds 4 blocks per CU
+    constexpr bool big = KIND != DEP_TIJ;
+    if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, phi, mass, a, acc);
+    if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, phi, mass, a, acc);
+    return launch_dep<T, KIND, 4, 4, 4>(p, out, phi, mass, a, acc);
+}
+
+template <typename T> int dep_T00_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
+{
+    return dispatch_dep<T, DEP_T00>(p, out, phi, mass, a, acc);
+}
+
+template <typename T> int dep_T0i_impl(gev_pcls *p, T *out, const T *phi, double mass, int acc)
+{
+    return dispatch_dep<T, DEP_T0I>(p, out, phi, mass, 1.0, acc);
+}
+
+template <typename T> int dep_Tij_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
+{
+    return dispatch_dep<T, DEP_TIJ>(p, out, phi, mass, a, acc);
+}
+
+#define INST(T)                                                                   \
+    template int dep_T00_impl<T>(gev_pcls *, T *, const T *, double, double, int); \
+    template int dep_T0i_impl<T>(gev_pcls *, T *, const T *, double, int);         \
+    template int dep_Tij_impl<T>(gev_pcls *, T *, const T *, double, double, int);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace gev

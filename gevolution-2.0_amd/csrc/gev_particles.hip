@@ -186,246 +186,6 @@ template <typename T> __global__ void k_scatter(PV<T> s, PV<T> d, const int32_t 
     d.cell[j] = c;
 }
 
-// ------------------------------------------------------------ deposits ----
-
-// projection_T00 (functor gevolution.hpp:1729-1854): 8 corner values per cell
-template <typename T, int TX, int TY, int TZ>
-__global__ __launch_bounds__(PT_NT) void k_dep_T00(PV<T> v, T *out, const T *__restrict__ phi, double mass_d,
-                                                   double a_d, int accumulate)
-{
-    constexpr int CX = TX + 1, CY = TY + 1, CZ = TZ + 1, NC = CX * CY * CZ;
-    __shared__ T lds[8][NC];
-    const int N = v.N;
-    int x0, y0, z0;
-    tile_origin<TX, TY, TZ>(N, x0, y0, z0);
-    const T a = (T)a_d;
-    for (int c = threadIdx.x; c < NC; c += PT_NT) {
-        int lx = c % CX, ly = (c / CX) % CY, lz = c / (CX * CY);
-        int gx = (x0 - 1 + lx) & (N - 1), gy = (y0 - 1 + ly) & (N - 1), gz = (z0 - 1 + lz) & (N - 1);
-        int64_t g = site(N, gx, gy, gz);
-        T acc[8];
-#pragma unroll
-        for (int b = 0; b < 8; b++) acc[b] = (T)0;
-        int32_t p0 = v.start[g], p1 = v.start[g + 1];
-        if (phi) {
-            T ph[8];
-#pragma unroll
-            for (int b = 0; b < 8; b++) ph[b] = phi[site(N, gx + ((b >> 2) & 1), gy + ((b >> 1) & 1), gz + (b & 1))];
-            for (int32_t p = p0; p < p1; p++) {
-                T r[3];
-                ref_dist(v, p, gx, gy, gz, r);
-                T q0 = v.q0[p], q1 = v.q1[p], q2 = v.q2[p];
-                T f2 = q0 * q0 + q1 * q1 + q2 * q2;
-                T e = sqrt(f2 + a * a);
-                T f = (T)3 * e + f2 / e;
-                f2 = ((T)0.5 * f * f + f2 / ((T)1 + f2 / a / a)) / e;
-#pragma unroll
-                for (int b = 0; b < 8; b++) {
-                    T w = (((b >> 2) & 1) ? r[0] : (T)1 - r[0]) * (((b >> 1) & 1) ? r[1] : (T)1 - r[1]) *
-                          ((b & 1) ? r[2] : (T)1 - r[2]);
-                    acc[b] += w * (e + f * ph[b] + f2 * ph[b] * ph[b]);
-                }
-            }
-        } else {
-            for (int32_t p = p0; p < p1; p++) {
-                T r[3];
-                ref_dist(v, p, gx, gy, gz, r);
-#pragma unroll
-                for (int b = 0; b < 8; b++)
-                    acc[b] += (((b >> 2) & 1) ? r[0] : (T)1 - r[0]) * (((b >> 1) & 1) ? r[1] : (T)1 - r[1]) *
-                              ((b & 1) ? r[2] : (T)1 - r[2]);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 8; b++) lds[b][c] = acc[b];
-    }
-    __syncthreads();
-    const T mass = (T)mass_d;
-    for (int s = threadIdx.x; s < TX * TY * TZ; s += PT_NT) {
-        int sx = s % TX, sy = (s / TX) % TY, sz = s / (TX * TY);
-        T val = (T)0;
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-            int lc = (sx + 1 - bx) + CX * ((sy + 1 - by) + CY * (sz + 1 - bz));
-            val += lds[b][lc];
-        }
-        val *= mass;
-        int64_t gs = site(N, x0 + sx, y0 + sy, z0 + sz);
-        out[gs] = accumulate ? out[gs] + val : val;
-    }
-}
-
-// projection_T0i (functor gevolution.hpp:2031-2201): 12 link values per cell.
-// g[k] lands on component comp[k] of the site cell + off[k].
-template <typename T, int TX, int TY, int TZ>
-__global__ __launch_bounds__(PT_NT) void k_dep_T0i(PV<T> v, T *out, const T *__restrict__ phi, double mass_d,
-                                                   int64_t M, int accumulate)
-{
-    constexpr int CX = TX + 1, CY = TY + 1, CZ = TZ + 1, NC = CX * CY * CZ;
-    __shared__ T lds[12][NC];
-    const int N = v.N;
-    int x0, y0, z0;
-    tile_origin<TX, TY, TZ>(N, x0, y0, z0);
-    for (int c = threadIdx.x; c < NC; c += PT_NT) {
-        int lx = c % CX, ly = (c / CX) % CY, lz = c / (CX * CY);
-        int gx = (x0 - 1 + lx) & (N - 1), gy = (y0 - 1 + ly) & (N - 1), gz = (z0 - 1 + lz) & (N - 1);
-        int64_t g = site(N, gx, gy, gz);
-        T acc[12];
-#pragma unroll
-        for (int k = 0; k < 12; k++) acc[k] = (T)0;
-        T fac[12];
-        if (phi) {
-            T ph[8];
-#pragma unroll
-            for (int b = 0; b < 8; b++) ph[b] = phi[site(N, gx + ((b >> 2) & 1), gy + ((b >> 1) & 1), gz + (b & 1))];
-            fac[0] = (T)1 + ph[0] + ph[4]; fac[1] = (T)1 + ph[0] + ph[2]; fac[2] = (T)1 + ph[0] + ph[1];
-            fac[3] = (T)1 + ph[4] + ph[6]; fac[4] = (T)1 + ph[4] + ph[5]; fac[5] = (T)1 + ph[2] + ph[6];
-            fac[6] = (T)1 + ph[2] + ph[3]; fac[7] = (T)1 + ph[1] + ph[5]; fac[8] = (T)1 + ph[1] + ph[3];
-            fac[9] = (T)1 + ph[3] + ph[7]; fac[10] = (T)1 + ph[5] + ph[7]; fac[11] = (T)1 + ph[6] + ph[7];
-        }
-        int32_t p0 = v.start[g], p1 = v.start[g + 1];
-        for (int32_t p = p0; p < p1; p++) {
-            T r[3];
-            ref_dist(v, p, gx, gy, gz, r);
-            T v0 = v.q0[p], v1 = v.q1[p], v2 = v.q2[p];
-            T s[12];
-            s[0] = ((T)1 - r[1]) * ((T)1 - r[2]) * v0;
-            s[1] = ((T)1 - r[0]) * ((T)1 - r[2]) * v1;
-            s[2] = ((T)1 - r[0]) * ((T)1 - r[1]) * v2;
-            s[3] = r[0] * ((T)1 - r[2]) * v1;
-            s[4] = r[0] * ((T)1 - r[1]) * v2;
-            s[5] = r[1] * ((T)1 - r[2]) * v0;
-            s[6] = r[1] * ((T)1 - r[0]) * v2;
-            s[7] = r[2] * ((T)1 - r[1]) * v0;
-            s[8] = r[2] * ((T)1 - r[0]) * v1;
-            s[9] = r[1] * r[2] * v0;
-            s[10] = r[0] * r[2] * v1;
-            s[11] = r[0] * r[1] * v2;
-#pragma unroll
-            for (int k = 0; k < 12; k++) acc[k] += s[k];
-        }
-        if (phi) {
-#pragma unroll
-            for (int k = 0; k < 12; k++) acc[k] *= fac[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 12; k++) lds[k][c] = acc[k];
-    }
-    __syncthreads();
-    const T mass = (T)mass_d;
-    for (int s = threadIdx.x; s < TX * TY * TZ; s += PT_NT) {
-        int sx = s % TX, sy = (s / TX) % TY, sz = s / (TX * TY);
-        auto L = [&](int k, int ox, int oy, int oz) {
-            return lds[k][(sx + 1 - ox) + CX * ((sy + 1 - oy) + CY * (sz + 1 - oz))];
-        };
-        // component 0: g0 (0,0,0), g5 (0,1,0), g7 (0,0,1), g9 (0,1,1)
-        T c0 = L(0, 0, 0, 0) + L(5, 0, 1, 0) + L(7, 0, 0, 1) + L(9, 0, 1, 1);
-        // component 1: g1 (0,0,0), g3 (1,0,0), g8 (0,0,1), g10 (1,0,1)
-        T c1 = L(1, 0, 0, 0) + L(3, 1, 0, 0) + L(8, 0, 0, 1) + L(10, 1, 0, 1);
-        // component 2: g2 (0,0,0), g4 (1,0,0), g6 (0,1,0), g11 (1,1,0)
-        T c2 = L(2, 0, 0, 0) + L(4, 1, 0, 0) + L(6, 0, 1, 0) + L(11, 1, 1, 0);
-        int64_t gs = site(N, x0 + sx, y0 + sy, z0 + sz);
-        c0 *= mass; c1 *= mass; c2 *= mass;
-        if (accumulate) { c0 += out[gs]; c1 += out[gs + M]; c2 += out[gs + 2 * M]; }
-        out[gs] = c0; out[gs + M] = c1; out[gs + 2 * M] = c2;
-    }
-}
-
-// projection_Tij (functor gevolution.hpp:2582-2965, no CIC_PROJECT_TIJ):
-// per cell 24 diagonal corner values + 6 off-diagonal edge values.
-template <typename T, int TX, int TY, int TZ>
-__global__ __launch_bounds__(PT_NT) void k_dep_Tij(PV<T> v, T *out, const T *__restrict__ phi, double mass_d,
-                                                   double a_d, int64_t M, int accumulate)
-{
-    constexpr int CX = TX + 1, CY = TY + 1, CZ = TZ + 1, NC = CX * CY * CZ;
-    __shared__ T lds[30][NC];
-    const int N = v.N;
-    int x0, y0, z0;
-    tile_origin<TX, TY, TZ>(N, x0, y0, z0);
-    const T a = (T)a_d;
-    for (int c = threadIdx.x; c < NC; c += PT_NT) {
-        int lx = c % CX, ly = (c / CX) % CY, lz = c / (CX * CY);
-        int gx = (x0 - 1 + lx) & (N - 1), gy = (y0 - 1 + ly) & (N - 1), gz = (z0 - 1 + lz) & (N - 1);
-        int64_t g = site(N, gx, gy, gz);
-        T acc[30];
-#pragma unroll
-        for (int k = 0; k < 30; k++) acc[k] = (T)0;
-        T ph[8];
-        if (phi) {
-#pragma unroll
-            for (int b = 0; b < 8; b++) ph[b] = phi[site(N, gx + ((b >> 2) & 1), gy + ((b >> 1) & 1), gz + (b & 1))];
-        }
-        int32_t p0 = v.start[g], p1 = v.start[g + 1];
-        for (int32_t p = p0; p < p1; p++) {
-            T r[3];
-            ref_dist(v, p, gx, gy, gz, r);
-            T vv[3] = {v.q0[p], v.q1[p], v.q2[p]};
-            T f = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
-            T e = sqrt(f + a * a);
-            if (phi) {
-                f = (T)4 + a * a / (f + a * a);
-                f *= ((T)1 - r[0]) * (((T)1 - r[1]) * (((T)1 - r[2]) * ph[0] + r[2] * ph[1]) +
-                                      r[1] * (((T)1 - r[2]) * ph[2] + r[2] * ph[3])) +
-                     r[0] * (((T)1 - r[1]) * (((T)1 - r[2]) * ph[4] + r[2] * ph[5]) +
-                             r[1] * (((T)1 - r[2]) * ph[6] + r[2] * ph[7]));
-                f += (T)1;
-            } else {
-                f = (T)1;
-            }
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                T w = f * vv[d] * vv[d] / e;
-#pragma unroll
-                for (int b = 0; b < 8; b++) {
-                    T wt = w * (((b >> 2) & 1) ? r[0] : (T)1 - r[0]) * (((b >> 1) & 1) ? r[1] : (T)1 - r[1]) *
-                           ((b & 1) ? r[2] : (T)1 - r[2]);
-                    acc[d * 8 + b] += wt;
-                }
-            }
-            T w = f * vv[0] * vv[1] / e;
-            acc[24] += w * ((T)1 - r[2]);
-            acc[25] += w * r[2];
-            w = f * vv[0] * vv[2] / e;
-            acc[26] += w * ((T)1 - r[1]);
-            acc[27] += w * r[1];
-            w = f * vv[1] * vv[2] / e;
-            acc[28] += w * ((T)1 - r[0]);
-            acc[29] += w * r[0];
-        }
-#pragma unroll
-        for (int k = 0; k < 30; k++) lds[k][c] = acc[k];
-    }
-    __syncthreads();
-    const T mass = (T)mass_d;
-    for (int s = threadIdx.x; s < TX * TY * TZ; s += PT_NT) {
-        int sx = s % TX, sy = (s / TX) % TY, sz = s / (TX * TY);
-        auto L = [&](int k, int ox, int oy, int oz) {
-            return lds[k][(sx + 1 - ox) + CX * ((sy + 1 - oy) + CY * (sz + 1 - oz))];
-        };
-        T d[3];
-#pragma unroll
-        for (int dd = 0; dd < 3; dd++) {
-            T val = (T)0;
-#pragma unroll
-            for (int b = 0; b < 8; b++) val += L(dd * 8 + b, (b >> 2) & 1, (b >> 1) & 1, b & 1);
-            d[dd] = val * mass;
-        }
-        T o01 = (L(24, 0, 0, 0) + L(25, 0, 0, 1)) * mass;
-        T o02 = (L(26, 0, 0, 0) + L(27, 0, 1, 0)) * mass;
-        T o12 = (L(28, 0, 0, 0) + L(29, 1, 0, 0)) * mass;
-        int64_t gs = site(N, x0 + sx, y0 + sy, z0 + sz);
-        T vals[6] = {d[0], o01, o02, d[1], o12, d[2]};
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-            T o = vals[k];
-            if (accumulate) o += out[gs + k * M];
-            out[gs + k * M] = o;
-        }
-    }
-}
-
-
 // displace_pcls_ic_basic, ic_basic.hpp:65-96 (one field)
 template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_displace(PV<T> v, const T *__restrict__ xi, double coeff,
@@ -546,54 +306,6 @@ template <typename T> int pcls_store_impl(gev_pcls *p, double *pos, double *q, i
     return GEV_OK;
 }
 
-template <typename T, int TX, int TY, int TZ> static unsigned tiles(int N)
-{
-    return (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
-}
-
-// tile shapes per deposit and precision (LDS = values * cells * sizeof(T))
-template <typename T> int dep_T00_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
-{
-    gev_ctx *c = p->ctx;
-    const int N = c->N;
-    if (N >= 8)
-        hipLaunchKernelGGL((k_dep_T00<T, 8, 8, 8>), dim3(tiles<T, 8, 8, 8>(N)), dim3(PT_NT), 0, c->stream,
-                           make_pv<T>(p), out, phi, mass, a, acc);
-    else
-        hipLaunchKernelGGL((k_dep_T00<T, 4, 4, 4>), dim3(tiles<T, 4, 4, 4>(N)), dim3(PT_NT), 0, c->stream,
-                           make_pv<T>(p), out, phi, mass, a, acc);
-    GEV_HIP(hipGetLastError());
-    return GEV_OK;
-}
-
-template <typename T> int dep_T0i_impl(gev_pcls *p, T *out, const T *phi, double mass, int acc)
-{
-    gev_ctx *c = p->ctx;
-    const int N = c->N;
-    if (N >= 8)
-        hipLaunchKernelGGL((k_dep_T0i<T, 8, 8, 4>), dim3(tiles<T, 8, 8, 4>(N)), dim3(PT_NT), 0, c->stream,
-                           make_pv<T>(p), out, phi, mass, c->M, acc);
-    else
-        hipLaunchKernelGGL((k_dep_T0i<T, 4, 4, 4>), dim3(tiles<T, 4, 4, 4>(N)), dim3(PT_NT), 0, c->stream,
-                           make_pv<T>(p), out, phi, mass, c->M, acc);
-    GEV_HIP(hipGetLastError());
-    return GEV_OK;
-}
-
-template <typename T> int dep_Tij_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
-{
-    gev_ctx *c = p->ctx;
-    const int N = c->N;
-    if (N >= 8)
-        hipLaunchKernelGGL((k_dep_Tij<T, 8, 4, 4>), dim3(tiles<T, 8, 4, 4>(N)), dim3(PT_NT), 0, c->stream,
-                           make_pv<T>(p), out, phi, mass, a, c->M, acc);
-    else
-        hipLaunchKernelGGL((k_dep_Tij<T, 4, 4, 4>), dim3(tiles<T, 4, 4, 4>(N)), dim3(PT_NT), 0, c->stream,
-                           make_pv<T>(p), out, phi, mass, a, c->M, acc);
-    GEV_HIP(hipGetLastError());
-    return GEV_OK;
-}
-
 static int read_max(gev_ctx *c, double *out)
 {
     GEV_HIP(hipMemcpyAsync(c->red_host, c->red + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -641,9 +353,6 @@ template <typename T> int initialize_q_impl(gev_pcls *p, const T *pot, double co
     template int pcls_lattice_impl<T>(gev_pcls *);                                                                \
     template int pcls_load_impl<T>(gev_pcls *, const double *, const double *, const int64_t *);                  \
     template int pcls_store_impl<T>(gev_pcls *, double *, double *, int64_t *);                                   \
-    template int dep_T00_impl<T>(gev_pcls *, T *, const T *, double, double, int);                                \
-    template int dep_T0i_impl<T>(gev_pcls *, T *, const T *, double, int);                                        \
-    template int dep_Tij_impl<T>(gev_pcls *, T *, const T *, double, double, int);                                \
     template int displace_impl<T>(gev_pcls *, const T *, double, double *);                                       \
     template int initialize_q_impl<T>(gev_pcls *, const T *, double, double *);
 INST(double)

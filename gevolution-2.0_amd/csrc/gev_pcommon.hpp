@@ -47,10 +47,11 @@ template <typename T> __device__ __forceinline__ void ref_dist(const PV<T> &v, i
 template <> __device__ __forceinline__ void ref_dist<double>(const PV<double> &v, int64_t i, int x, int y, int z,
                                                              double r[3])
 {
-    const double dx = 1.0 / (double)v.N;
-    r[0] = (v.p0[i] - (double)x * dx) / dx;
-    r[1] = (v.p1[i] - (double)y * dx) / dx;
-    r[2] = (v.p2[i] - (double)z * dx) / dx;
+    // dividing by dx = 1/N is exactly multiplying by N (N a power of two)
+    const double dx = 1.0 / (double)v.N, Nd = (double)v.N;
+    r[0] = (v.p0[i] - (double)x * dx) * Nd;
+    r[1] = (v.p1[i] - (double)y * dx) * Nd;
+    r[2] = (v.p2[i] - (double)z * dx) * Nd;
 }
 template <> __device__ __forceinline__ void ref_dist<float>(const PV<float> &v, int64_t i, int x, int y, int z,
                                                             float r[3])
@@ -58,6 +59,21 @@ template <> __device__ __forceinline__ void ref_dist<float>(const PV<float> &v, 
     r[0] = v.p0[i];
     r[1] = v.p1[i];
     r[2] = v.p2[i];
+}
+
+// the same from coordinates already loaded (x[] as stored in the container)
+__device__ __forceinline__ void cell_rel(const double x[3], int cx, int cy, int cz, int N, double r[3])
+{
+    const double dx = 1.0 / (double)N, Nd = (double)N;
+    r[0] = (x[0] - (double)cx * dx) * Nd;
+    r[1] = (x[1] - (double)cy * dx) * Nd;
+    r[2] = (x[2] - (double)cz * dx) * Nd;
+}
+__device__ __forceinline__ void cell_rel(const float x[3], int, int, int, int, float r[3])
+{
+    r[0] = x[0];
+    r[1] = x[1];
+    r[2] = x[2];
 }
 
 __device__ __forceinline__ double wrap_pos(double p)
@@ -84,7 +100,7 @@ template <> __device__ __forceinline__ void move_by<double>(const PV<double> &v,
     double p0 = wrap_pos(v.p0[i] + d[0]), p1 = wrap_pos(v.p1[i] + d[1]), p2 = wrap_pos(v.p2[i] + d[2]);
     v.p0[i] = p0; v.p1[i] = p1; v.p2[i] = p2;
     const int N = v.N;
-    v.cell[i] = (int32_t)((int64_t)cell1(p0, N) + (int64_t)N * ((int64_t)cell1(p1, N) + (int64_t)N * cell1(p2, N)));
+    v.cell[i] = (int32_t)((uint32_t)cell1(p0, N) + (uint32_t)N * ((uint32_t)cell1(p1, N) + (uint32_t)N * (uint32_t)cell1(p2, N)));
 }
 __device__ __forceinline__ void off_step(float &r, int &c, double dcells, int N)
 {
@@ -106,7 +122,7 @@ template <> __device__ __forceinline__ void move_by<float>(const PV<float> &v, i
     off_step(r1, y, d[1] * N, N);
     off_step(r2, z, d[2] * N, N);
     v.p0[i] = r0; v.p1[i] = r1; v.p2[i] = r2;
-    v.cell[i] = (int32_t)((int64_t)x + (int64_t)N * ((int64_t)y + (int64_t)N * z));
+    v.cell[i] = (int32_t)((uint32_t)x + (uint32_t)N * ((uint32_t)y + (uint32_t)N * (uint32_t)z));
 }
 
 // tile -> (x0,y0,z0), XCD-aware

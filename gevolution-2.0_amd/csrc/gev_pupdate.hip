@@ -23,45 +23,66 @@ struct UpdParams {
 
 template <typename T, int TX, int TY, int TZ>
 __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, const T *__restrict__ phi,
-                                                        const T *__restrict__ chi, const T *__restrict__ Bi, int64_t M,
+                                                        const T *__restrict__ chi, const T *__restrict__ B0g,
+                                                        const T *__restrict__ B1g, const T *__restrict__ B2g,
                                                         unsigned long long *maxslot, int32_t *count, int32_t *rank)
 {
     constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3, NS = SX * SY * SZ, NROW = TY * TZ;
+    static_assert(NROW <= 64 && SX <= 64, "tile too large for the wave-level staging");
     __shared__ T fld[5][NS];
     __shared__ int32_t rs[NROW + 1];
     __shared__ int32_t rbeg[NROW];
     const int N = v.N;
-    const int tid = threadIdx.x;
+    const uint32_t msk = (uint32_t)N - 1;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int x0, y0, z0;
     tile_origin<TX, TY, TZ>(N, x0, y0, z0);
     const bool kick = u.flags & (UPD_KICK_GR | UPD_KICK_NEWTON);
     const bool drift = u.flags & (UPD_DRIFT_GR | UPD_DRIFT_NEWTON);
-    const int nf_k = (u.flags & UPD_KICK_NEWTON) ? u.nf_k : u.nf_k;
+    const int nf_k = u.nf_k;
     const int nf_d = (u.flags & UPD_DRIFT_GR) ? u.nf_d : 0;
     const int nfmax = kick ? (nf_k > nf_d ? nf_k : nf_d) : nf_d;
     const bool use_phi = kick || nf_d >= 1, use_chi = nfmax >= 2, use_B = nfmax >= 3;
-    for (int e = tid; e < NS; e += PT_NT) {
-        int lx = e % SX, ly = (e / SX) % SY, lz = e / (SX * SY);
-        int64_t g = site(N, x0 - 1 + lx, y0 - 1 + ly, z0 - 1 + lz);
-        if (use_phi) fld[0][e] = phi[g];
-        if (use_chi) fld[1][e] = chi[g];
-        if (use_B) {
-            fld[2][e] = Bi[g];
-            fld[3][e] = Bi[g + M];
-            fld[4][e] = Bi[g + 2 * M];
+    // stage [x0-1, x0+TX+1] x ... : each wave covers RPW box rows per pass, lane -> (row, x)
+    {
+        constexpr int RPW = 64 / SX, NR = SY * SZ, NW = PT_NT / 64;
+        const int sub = lane / SX, lx = lane - sub * SX;
+        const uint32_t gx = (uint32_t)(x0 - 1 + lx) & msk;
+        if (sub < RPW) {
+            for (int row = wv * RPW + sub; row < NR; row += NW * RPW) {
+                const int ly = row % SY, lz = row / SY;
+                const uint32_t g = gx + (uint32_t)N * (((uint32_t)(y0 - 1 + ly) & msk) +
+                                                       (uint32_t)N * ((uint32_t)(z0 - 1 + lz) & msk));
+                const int e = lx + SX * row;
+                if (use_phi) fld[0][e] = phi[g];
+                if (use_chi) fld[1][e] = chi[g];
+                if (use_B) {
+                    fld[2][e] = B0g[g];
+                    fld[3][e] = B1g[g];
+                    fld[4][e] = B2g[g];
+                }
+            }
         }
     }
-    if (tid < NROW) {
-        int ry = tid % TY, rz = tid / TY;
-        int64_t rb = site(N, x0, y0 + ry, z0 + rz);
-        int32_t s0 = v.start[rb], s1 = v.start[rb + TX];
-        rbeg[tid] = s0;
-        rs[tid + 1] = s1 - s0;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        rs[0] = 0;
-        for (int i = 1; i <= NROW; i++) rs[i] += rs[i - 1];
+    // particle ranges of the tile's TY*TZ rows and their prefix (one wave)
+    if (wv == 0) {
+        int32_t cnt = 0, s0 = 0;
+        if (lane < NROW) {
+            const int ry = lane % TY, rz = lane / TY;
+            const uint32_t rb = (uint32_t)x0 + (uint32_t)N * ((uint32_t)(y0 + ry) + (uint32_t)N * (uint32_t)(z0 + rz));
+            s0 = v.start[rb];
+            cnt = v.start[rb + TX] - s0;
+        }
+        int32_t incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            int32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane < NROW) {
+            rbeg[lane] = s0;
+            rs[lane + 1] = incl;
+        }
+        if (lane == 0) rs[0] = 0;
     }
     __syncthreads();
     const int total = rs[NROW];
@@ -122,14 +143,15 @@ template <typename T> static int launch_update(gev_pcls *p, const UpdParams &u, 
         if (rc) return rc;
     }
     unsigned long long *slot = (unsigned long long *)(c->red + 8);
+    const T *B0 = Bi, *B1 = Bi ? Bi + c->M : nullptr, *B2 = Bi ? Bi + 2 * c->M : nullptr;
     if (N >= 8) {
         unsigned nb = (unsigned)((int64_t)(N / 8) * (N / 8) * (N / 4));
         hipLaunchKernelGGL((k_update_tiled<T, 8, 8, 4>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), u, phi, chi,
-                           Bi, c->M, slot, p->cell_count, p->rank);
+                           B0, B1, B2, slot, p->cell_count, p->rank);
     } else {
         unsigned nb = (unsigned)((int64_t)(N / 4) * (N / 4) * (N / 4));
         hipLaunchKernelGGL((k_update_tiled<T, 4, 4, 4>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), u, phi, chi,
-                           Bi, c->M, slot, p->cell_count, p->rank);
+                           B0, B1, B2, slot, p->cell_count, p->rank);
     }
     GEV_HIP(hipGetLastError());
     return GEV_OK;

@@ -121,6 +121,7 @@ struct gev_pcls {
     int32_t *cell_start = nullptr;  // M+1 offsets, valid when sorted
     int32_t *cell_count = nullptr;  // M counters
     bool sorted = false;
+    bool counts_zero = false;       // cell_count is all zero (the scan clears what it consumes)
 };
 
 #define GEV_HIP(call)                                                                        \

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
     if (t == NSEG - 1) rbeg[NSEG] = off + incl;
     __syncthreads();
     const int32_t total = rbeg[NSEG];
-    const T a = (T)a_d, o = (T)1;
+    const T a = (T)a_d, o = (T)1, ia2 = (T)(1.0 / (a_d * a_d));
 
     // U particles per thread and pass: their loads are issued together
     constexpr int U = KIND == DEP_TIJ ? 2 : 4;
@@ -149,9 +149,9 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
             if (use_phi) {
                 const T q0 = qu[u][0], q1 = qu[u][1], q2 = qu[u][2];
                 T f2 = q0 * q0 + q1 * q1 + q2 * q2;
-                const T e = sqrt(f2 + a * a);
-                const T f = (T)3 * e + f2 / e;
-                f2 = ((T)0.5 * f * f + f2 / (o + f2 / a / a)) / e;
+                const T e = fsqrt(f2 + a * a), ie = frcp(e);
+                const T f = (T)3 * e + f2 * ie;
+                f2 = ((T)0.5 * f * f + f2 * frcp(o + f2 * ia2)) * ie;
 #pragma unroll
                 for (int b = 0; b < 8; b++) {
                     const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
@@ -204,10 +204,10 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
             // diagonal parts CIC onto the 8 corners, off-diagonal parts onto 2 sites
             const T vv[3] = {qu[u][0], qu[u][1], qu[u][2]};
             T f = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
-            const T e = sqrt(f + a * a);
+            const T e2 = f + a * a, ie = frsqrt(e2);
             if (use_phi) {
                 const T *P = phs + pb;
-                f = (T)4 + a * a / (f + a * a);
+                f = (T)4 + a * a * frcp(e2);
                 f *= (o - rd[0]) * ((o - rd[1]) * ((o - rd[2]) * P[0] + rd[2] * P[PX * PY]) +
                                     rd[1] * ((o - rd[2]) * P[PX] + rd[2] * P[PX + PX * PY])) +
                      rd[0] * ((o - rd[1]) * ((o - rd[2]) * P[1] + rd[2] * P[1 + PX * PY]) +
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
             constexpr int dcomp[3] = {0, 3, 5};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const T w = f * vv[d] * vv[d] / e;
+                const T w = f * vv[d] * vv[d] * ie;
 #pragma unroll
                 for (int b = 0; b < 8; b++) {
                     const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
@@ -228,13 +228,13 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
                                 w * (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
                 }
             }
-            T w = f * vv[0] * vv[1] / e;
+            T w = f * vv[0] * vv[1] * ie;
             if (own(0, 0, 0)) lds_add(&acc[1][sidx(0, 0, 0)], w * (o - rd[2]));
             if (own(0, 0, 1)) lds_add(&acc[1][sidx(0, 0, 1)], w * rd[2]);
-            w = f * vv[0] * vv[2] / e;
+            w = f * vv[0] * vv[2] * ie;
             if (own(0, 0, 0)) lds_add(&acc[2][sidx(0, 0, 0)], w * (o - rd[1]));
             if (own(0, 1, 0)) lds_add(&acc[2][sidx(0, 1, 0)], w * rd[1]);
-            w = f * vv[1] * vv[2] / e;
+            w = f * vv[1] * vv[2] * ie;
             if (own(0, 0, 0)) lds_add(&acc[4][sidx(0, 0, 0)], w * (o - rd[0]));
             if (own(1, 0, 0)) lds_add(&acc[4][sidx(1, 0, 0)], w * rd[0]);
         }

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "gev_common.hpp"
+#include "gev_kernels.hpp"
 
 #define RC(call)                 \
     do {                         \
@@ -164,10 +165,11 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
         const double a_d = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
         const int nf_d = (1. / a_d < st->z_relax + 1.) ? 3 : 0;
         RC(gev_kick_drift(p, 1, (dtau + dtau_old) / 2., dtau, f->phi, f->chi, f->Bi, nf_k, nf_d, a, a_d,
-                          a * a * (double)N, a_d * a_d * (double)N, &maxv2, 0));
-        RC(mark(c, GEV_PHASE_KICK));
+                          a * a * (double)N, a_d * a_d * (double)N, nullptr, 0));
+        RC(mark(c, GEV_PHASE_KICK_DRIFT));
         RC(gev_pcls_rebin_counted(p));
-        RC(mark(c, GEV_PHASE_DRIFT));
+        RC(mark(c, GEV_PHASE_REBIN));
+        RC(gev::read_max_slot(c, &maxv2));  // the scatter leaves the kick's maximum alone
         a = gev_rk4bg(a_d, fpg, om, ol, orad, 0.5 * dtau);
         double mv = sqrt(maxv2);
         st->maxvel = mv / sqrt(mv * mv + 1.0); // main.cu:1279-1283
@@ -180,10 +182,11 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
         RC(mark(c, GEV_PHASE_SOLVE_PHI));
         const double a_d = gev_rk4bg(a, fpg, om, ol, orad, 0.5 * dtau);
         RC(gev_kick_drift(p, 0, (dtau + dtau_old) / 2., dtau, f->phi, nullptr, nullptr, 1, 0, a, a_d, 1.0, 1.0,
-                          &maxv2, 0));
-        RC(mark(c, GEV_PHASE_KICK));
+                          nullptr, 0));
+        RC(mark(c, GEV_PHASE_KICK_DRIFT));
         RC(gev_pcls_rebin_counted(p));
-        RC(mark(c, GEV_PHASE_DRIFT));
+        RC(mark(c, GEV_PHASE_REBIN));
+        RC(gev::read_max_slot(c, &maxv2));
         a = gev_rk4bg(a_d, fpg, om, ol, orad, 0.5 * dtau);
         st->maxvel = sqrt(maxv2);
     }

@@ -25,6 +25,12 @@ template <typename T> int prepare_Tij_impl(gev_ctx *ctx, const T *phi, const T *
 template <typename T> int field_sum_impl(gev_ctx *ctx, const T *f, int64_t count, double *sum);
 template <typename T> int field_fill_impl(gev_ctx *ctx, T *f, int64_t count, T v);
 
+// copy the last kick's maximum (FP64 slot) to the host; synchronises the stream
+int read_max_slot(gev_ctx *c, double *out);
+
+// zero the cell counters unless the last scan already left them zero
+int pcls_clear_counts(gev_pcls *p);
+
 // deterministic two-level FP64 sum: per-block partials then one block
 int reduce_partials(gev_ctx *ctx, const double *partials, int nparts, int nvals, double *out_dev);
 

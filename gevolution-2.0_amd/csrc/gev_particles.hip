@@ -152,7 +152,8 @@ __global__ __launch_bounds__(1024) void k_scan_mid(int32_t *bsum, int64_t nb)
         }
 }
 
-__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const int32_t *__restrict__ cnt, int64_t M,
+// writes the exclusive prefix of cnt to start and clears cnt for the next count
+__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(int32_t *__restrict__ cnt, int64_t M,
                                                         const int32_t *__restrict__ bsum, int32_t *__restrict__ start,
                                                         int32_t total)
 {
@@ -163,6 +164,10 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const int32_t *__restric
         int64_t i = base + j;
         v[j] = (i < M) ? cnt[i] : 0;
         s += v[j];
+    }
+    for (int j = 0; j < SCAN_PER; j++) {
+        int64_t i = base + j;
+        if (i < M) cnt[i] = 0;
     }
     int32_t tot;
     int32_t off = block_excl_scan(s, tot) + bsum[blockIdx.x];
@@ -197,15 +202,16 @@ __global__ __launch_bounds__(PT_NT) void k_displace(PV<T> v, const T *__restrict
         const int N = v.N;
         int x, y, z;
         cell_coords(v.cell[i], N, v.lg, x, y, z);
+        const T xs[3] = {v.p0[i], v.p1[i], v.p2[i]};
         T r[3];
-        ref_dist(v, i, x, y, z, r);
+        cell_rel(xs, x, y, z, N, r);
         T g[3];
         grad1(FieldAt<T>{xi, N, x, y, z}, r, g);
         const T Nn = (T)N;
         g[0] *= Nn; g[1] *= Nn; g[2] *= Nn;
         ret = coeff * sqrt((double)(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]));
         double d[3] = {coeff * (double)g[0], coeff * (double)g[1], coeff * (double)g[2]};
-        move_by(v, i, x, y, z, d);
+        move_by(v, i, xs, x, y, z, d);
     }
     block_max_to(ret, maxslot);
 }
@@ -242,6 +248,17 @@ __global__ __launch_bounds__(PT_NT) void k_initialize_q(PV<T> v, const T *__rest
 // ============================================================ host side ===
 static inline unsigned pblocks(int64_t n) { return (unsigned)((n + PT_NT - 1) / PT_NT); }
 
+// counters must start at zero: normally the last scan left them so
+int pcls_clear_counts(gev_pcls *p)
+{
+    if (!p->counts_zero) {
+        int rc = field_fill_impl<int32_t>(p->ctx, p->cell_count, p->ctx->M, 0);
+        if (rc) return rc;
+    }
+    p->counts_zero = false;  // the caller is about to count
+    return GEV_OK;
+}
+
 // scan + scatter, given per-cell counts and per-particle ranks
 template <typename T> int pcls_sort_counted_impl(gev_pcls *p)
 {
@@ -253,6 +270,7 @@ template <typename T> int pcls_sort_counted_impl(gev_pcls *p)
     hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(1024), 0, s, ctx->scan_blocks, nb);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_NT), 0, s, p->cell_count, M, ctx->scan_blocks,
                        p->cell_start, (int32_t)n);
+    p->counts_zero = true;
     if (n > 0) {
         PV<T> src = make_pv<T>(p);
         PV<T> dst = src;
@@ -277,7 +295,7 @@ template <typename T> int pcls_sort_impl(gev_pcls *p)
 {
     gev_ctx *ctx = p->ctx;
     const int64_t M = ctx->M, n = p->n;
-    int rc = field_fill_impl<int32_t>(ctx, p->cell_count, M, 0);
+    int rc = pcls_clear_counts(p);
     if (rc) return rc;
     if (n > 0)
         hipLaunchKernelGGL(k_count, dim3(pblocks(n)), dim3(PT_NT), 0, ctx->stream, p->cell, p->rank, p->cell_count, n);

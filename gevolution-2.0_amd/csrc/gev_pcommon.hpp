@@ -91,16 +91,19 @@ __device__ __forceinline__ int cell1(double p, int N)
     return c < 0 ? 0 : (c >= N ? N - 1 : c);
 }
 
-// move particle i by d (box units) and recompute its cell
-template <typename T> __device__ __forceinline__ void move_by(const PV<T> &v, int64_t i, int x, int y, int z,
-                                                              const double d[3]);
-template <> __device__ __forceinline__ void move_by<double>(const PV<double> &v, int64_t i, int x, int y, int z,
-                                                            const double d[3])
+// move particle i (stored coordinates x[]) by d (box units); writes the new
+// coordinates and cell and returns the cell
+template <typename T> __device__ __forceinline__ int32_t move_by(const PV<T> &v, int64_t i, const T x[3], int cx, int cy,
+                                                                 int cz, const double d[3]);
+template <> __device__ __forceinline__ int32_t move_by<double>(const PV<double> &v, int64_t i, const double x[3], int, int,
+                                                               int, const double d[3])
 {
-    double p0 = wrap_pos(v.p0[i] + d[0]), p1 = wrap_pos(v.p1[i] + d[1]), p2 = wrap_pos(v.p2[i] + d[2]);
+    double p0 = wrap_pos(x[0] + d[0]), p1 = wrap_pos(x[1] + d[1]), p2 = wrap_pos(x[2] + d[2]);
     v.p0[i] = p0; v.p1[i] = p1; v.p2[i] = p2;
     const int N = v.N;
-    v.cell[i] = (int32_t)((uint32_t)cell1(p0, N) + (uint32_t)N * ((uint32_t)cell1(p1, N) + (uint32_t)N * (uint32_t)cell1(p2, N)));
+    const int32_t c = (int32_t)((uint32_t)cell1(p0, N) + (uint32_t)N * ((uint32_t)cell1(p1, N) + (uint32_t)N * (uint32_t)cell1(p2, N)));
+    v.cell[i] = c;
+    return c;
 }
 __device__ __forceinline__ void off_step(float &r, int &c, double dcells, int N)
 {
@@ -113,16 +116,18 @@ __device__ __forceinline__ void off_step(float &r, int &c, double dcells, int N)
     r = rf;
     c = (c + s) & (N - 1);
 }
-template <> __device__ __forceinline__ void move_by<float>(const PV<float> &v, int64_t i, int x, int y, int z,
-                                                           const double d[3])
+template <> __device__ __forceinline__ int32_t move_by<float>(const PV<float> &v, int64_t i, const float x[3], int cx, int cy,
+                                                              int cz, const double d[3])
 {
     const int N = v.N;
-    float r0 = v.p0[i], r1 = v.p1[i], r2 = v.p2[i];
-    off_step(r0, x, d[0] * N, N);
-    off_step(r1, y, d[1] * N, N);
-    off_step(r2, z, d[2] * N, N);
+    float r0 = x[0], r1 = x[1], r2 = x[2];
+    off_step(r0, cx, d[0] * N, N);
+    off_step(r1, cy, d[1] * N, N);
+    off_step(r2, cz, d[2] * N, N);
     v.p0[i] = r0; v.p1[i] = r1; v.p2[i] = r2;
-    v.cell[i] = (int32_t)((uint32_t)x + (uint32_t)N * ((uint32_t)y + (uint32_t)N * (uint32_t)z));
+    const int32_t c = (int32_t)((uint32_t)cx + (uint32_t)N * ((uint32_t)cy + (uint32_t)N * (uint32_t)cz));
+    v.cell[i] = c;
+    return c;
 }
 
 // tile -> (x0,y0,z0), XCD-aware
@@ -150,6 +155,16 @@ template <typename T> struct FieldLds {
     int o, SX, SXY;
     __device__ __forceinline__ T operator()(int dx, int dy, int dz) const { return f[o + dx + SX * dy + SXY * dz]; }
 };
+
+// reciprocal and square roots: FP32 uses the hardware approximations (1 ulp),
+// FP64 the correctly rounded forms.  The functors' divisions by a per-particle
+// quantity become one reciprocal and multiplications.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float frsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double frsqrt(double x) { return 1.0 / sqrt(x); }
 
 template <typename T, class FA> __device__ __forceinline__ T cic(const FA &F_, const T r[3])
 {
@@ -211,9 +226,9 @@ __device__ __forceinline__ T phys_update_q(const T r[3], T vv[3], const FA &Fp, 
 {
     T v2 = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
     T e2 = v2 + a * a;
-    T phiint = cic<T>(Fp, r);
-    phiint /= e2;
-    phiint = (T)4 * phiint * v2 + ((T)1 + a * a * phiint) * (v2 + e2) / e2;
+    const T ie2 = frcp(e2);
+    T phiint = cic<T>(Fp, r) * ie2;
+    phiint = (T)4 * phiint * v2 + ((T)1 + a * a * phiint) * (v2 + e2) * ie2;
     T g[3];
     grad2<T>(Fp, r, g);
     g[0] *= phiint; g[1] *= phiint; g[2] *= phiint;
@@ -222,7 +237,7 @@ __device__ __forceinline__ T phys_update_q(const T r[3], T vv[3], const FA &Fp, 
         grad1<T>(Fc, r, gc);
         g[0] -= gc[0]; g[1] -= gc[1]; g[2] -= gc[2];
     }
-    e2 = sqrt(e2);
+    e2 = fsqrt(e2);
     if (nfield >= 3) {
         const T o = (T)1;
         T pg[3];
@@ -247,9 +262,10 @@ __device__ __forceinline__ T phys_update_q(const T r[3], T vv[3], const FA &Fp, 
         pg[2] += (o - r[0]) * r[1] * ((r[2] - o) * B2(0, 1, -1) + (o - (T)2 * r[2]) * B2(0, 1, 0) + r[2] * B2(0, 1, 1)) * vv[2];
         pg[2] += r[0] * r[1] * ((r[2] - o) * B2(1, 1, -1) + (o - (T)2 * r[2]) * B2(1, 1, 0) + r[2] * B2(1, 1, 1)) * vv[2];
 
-        g[0] += pg[0] / bcoeff / e2;
-        g[1] += pg[1] / bcoeff / e2;
-        g[2] += pg[2] / bcoeff / e2;
+        const T s = frcp(bcoeff * e2);
+        g[0] += pg[0] * s;
+        g[1] += pg[1] * s;
+        g[2] += pg[2] * s;
     }
     v2 = (T)0;
 #pragma unroll
@@ -257,7 +273,7 @@ __device__ __forceinline__ T phys_update_q(const T r[3], T vv[3], const FA &Fp, 
         vv[d] -= dtau * e2 * g[d] * Nn;
         v2 += vv[d] * vv[d];
     }
-    return v2 / a / a;
+    return v2 * frcp(a * a);
 }
 
 // update_q_Newton, gevolution.hpp:1051-1118
@@ -278,7 +294,7 @@ __device__ __forceinline__ T phys_update_q_newton(const T r[3], T q[3], const FA
         q[d] = q[d] - dtau * a * g[d] * Nn;
         v2 += q[d] * q[d];
     }
-    return v2 / a / a;
+    return v2 * frcp(a * a);
 }
 
 // update_pos, gevolution.hpp:1162-1232: displacement d (box units) of one drift
@@ -292,7 +308,8 @@ __device__ __forceinline__ void phys_update_pos(const T r[3], const T q[3], cons
     T ph = (T)0, ch = (T)0;
     if (nfield >= 1) ph = cic<T>(Fp, r);
     if (nfield >= 2) ch = cic<T>(Fc, r);
-    v2 = ((T)1 + ((T)3 - v2 / e2) * ph - ch) / sqrt(e2);
+    const T ie = frsqrt(e2);
+    v2 = ((T)1 + ((T)3 - v2 * ie * ie) * ph - ch) * ie;
     T vel[3] = {q[0] * v2, q[1] * v2, q[2] * v2};
     if (nfield >= 3) {
         const T o = (T)1;
@@ -309,12 +326,116 @@ __device__ __forceinline__ void phys_update_pos(const T r[3], const T q[3], cons
         b[1] += B1(1, 0, 1) * r[0] * r[2];
         b[0] += B0(0, 1, 1) * r[1] * r[2];
         b[2] += B2(1, 1, 0) * r[0] * r[1];
+        const T ibc = frcp(bcoeff);
 #pragma unroll
-        for (int l = 0; l < 3; l++) d[l] = dtau * (double)(vel[l] + b[l] / bcoeff);
+        for (int l = 0; l < 3; l++) d[l] = dtau * (double)(vel[l] + b[l] * ibc);
     } else {
 #pragma unroll
         for (int l = 0; l < 3; l++) d[l] = dtau * (double)vel[l];
     }
+}
+
+// update_q (gevolution.hpp:888-1011) followed by update_pos (:1162-1232) of
+// the same particle, as the fused GR cycle applies them: the field stencils
+// are read once and the CIC corner and edge weights computed once for both.
+// nf_d = 0 is the free-streaming drift of the unrelaxed phase.  Updates q,
+// returns the kick's v^2/a^2 and the drift displacement d (box units).
+template <typename T, class FA>
+__device__ __forceinline__ T phys_kick_drift(const T r[3], T q[3], const FA &Fp, const FA &Fc, const FA &B0,
+                                             const FA &B1, const FA &B2, int nf_k, T a_k, T dtau_k, T bc_k, T Nn,
+                                             int nf_d, T a_d, T bc_d, double dtau_d, double d[3])
+{
+    const T o = (T)1, h = (T)0.5;
+    const T r0 = r[0], r1 = r[1], r2 = r[2], s0 = o - r0, s1 = o - r1, s2 = o - r2;
+    // edge weights: x-edges (y,z), y-edges (x,z), z-edges (x,y)
+    const T ex00 = s1 * s2, ex10 = r1 * s2, ex01 = s1 * r2, ex11 = r1 * r2;
+    const T ey00 = s0 * s2, ey10 = r0 * s2, ey01 = s0 * r2, ey11 = r0 * r2;
+    const T ez00 = s0 * s1, ez10 = r0 * s1, ez01 = s0 * r1, ez11 = r0 * r1;
+    // corner weights w[x][y][z]
+    const T w000 = s0 * ex00, w100 = r0 * ex00, w010 = s0 * ex10, w110 = r0 * ex10;
+    const T w001 = s0 * ex01, w101 = r0 * ex01, w011 = s0 * ex11, w111 = r0 * ex11;
+    auto cic8 = [&](const FA &F) {
+        return w000 * F(0, 0, 0) + w100 * F(1, 0, 0) + w010 * F(0, 1, 0) + w110 * F(1, 1, 0) + w001 * F(0, 0, 1) +
+               w101 * F(1, 0, 1) + w011 * F(0, 1, 1) + w111 * F(1, 1, 1);
+    };
+    const T phic = cic8(Fp);
+
+    // ---- kick
+    T v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    T e2 = v2 + a_k * a_k;
+    const T ie2 = frcp(e2);
+    T phiint = phic * ie2;
+    phiint = (T)4 * phiint * v2 + (o + a_k * a_k * phiint) * (v2 + e2) * ie2;
+    T g[3];
+    g[0] = w000 * (Fp(1, 0, 0) - Fp(-1, 0, 0)) + w100 * (Fp(2, 0, 0) - Fp(0, 0, 0)) +
+           w010 * (Fp(1, 1, 0) - Fp(-1, 1, 0)) + w110 * (Fp(2, 1, 0) - Fp(0, 1, 0)) +
+           w001 * (Fp(1, 0, 1) - Fp(-1, 0, 1)) + w101 * (Fp(2, 0, 1) - Fp(0, 0, 1)) +
+           w011 * (Fp(1, 1, 1) - Fp(-1, 1, 1)) + w111 * (Fp(2, 1, 1) - Fp(0, 1, 1));
+    g[1] = w000 * (Fp(0, 1, 0) - Fp(0, -1, 0)) + w100 * (Fp(1, 1, 0) - Fp(1, -1, 0)) +
+           w010 * (Fp(0, 2, 0) - Fp(0, 0, 0)) + w110 * (Fp(1, 2, 0) - Fp(1, 0, 0)) +
+           w001 * (Fp(0, 1, 1) - Fp(0, -1, 1)) + w101 * (Fp(1, 1, 1) - Fp(1, -1, 1)) +
+           w011 * (Fp(0, 2, 1) - Fp(0, 0, 1)) + w111 * (Fp(1, 2, 1) - Fp(1, 0, 1));
+    g[2] = w000 * (Fp(0, 0, 1) - Fp(0, 0, -1)) + w100 * (Fp(1, 0, 1) - Fp(1, 0, -1)) +
+           w010 * (Fp(0, 1, 1) - Fp(0, 1, -1)) + w110 * (Fp(1, 1, 1) - Fp(1, 1, -1)) +
+           w001 * (Fp(0, 0, 2) - Fp(0, 0, 0)) + w101 * (Fp(1, 0, 2) - Fp(1, 0, 0)) +
+           w011 * (Fp(0, 1, 2) - Fp(0, 1, 0)) + w111 * (Fp(1, 1, 2) - Fp(1, 1, 0));
+    const T hp = h * phiint;
+    g[0] *= hp; g[1] *= hp; g[2] *= hp;
+    if (nf_k >= 2) {
+        g[0] -= ex00 * (Fc(1, 0, 0) - Fc(0, 0, 0)) + ex10 * (Fc(1, 1, 0) - Fc(0, 1, 0)) +
+                ex01 * (Fc(1, 0, 1) - Fc(0, 0, 1)) + ex11 * (Fc(1, 1, 1) - Fc(0, 1, 1));
+        g[1] -= ey00 * (Fc(0, 1, 0) - Fc(0, 0, 0)) + ey10 * (Fc(1, 1, 0) - Fc(1, 0, 0)) +
+                ey01 * (Fc(0, 1, 1) - Fc(0, 0, 1)) + ey11 * (Fc(1, 1, 1) - Fc(1, 0, 1));
+        g[2] -= ez00 * (Fc(0, 0, 1) - Fc(0, 0, 0)) + ez10 * (Fc(1, 0, 1) - Fc(1, 0, 0)) +
+                ez01 * (Fc(0, 1, 1) - Fc(0, 1, 0)) + ez11 * (Fc(1, 1, 1) - Fc(1, 1, 0));
+    }
+    const T e = fsqrt(e2);
+    if (nf_k >= 3) {
+        const T t0 = (T)2 * r0, t1 = (T)2 * r1, t2 = (T)2 * r2;
+        auto l0 = [&](int y, int z) { return (r0 - o) * B0(-1, y, z) + (o - t0) * B0(0, y, z) + r0 * B0(1, y, z); };
+        auto l1 = [&](int x, int z) { return (r1 - o) * B1(x, -1, z) + (o - t1) * B1(x, 0, z) + r1 * B1(x, 1, z); };
+        auto l2 = [&](int x, int y) { return (r2 - o) * B2(x, y, -1) + (o - t2) * B2(x, y, 0) + r2 * B2(x, y, 1); };
+        T pg0 = (s2 * (B1(1, 0, 0) - B1(0, 0, 0)) + r2 * (B1(1, 0, 1) - B1(0, 0, 1))) * q[1];
+        pg0 += (s1 * (B2(1, 0, 0) - B2(0, 0, 0)) + r1 * (B2(1, 1, 0) - B2(0, 1, 0))) * q[2];
+        pg0 += (ex00 * l0(0, 0) + ex10 * l0(1, 0) + ex01 * l0(0, 1) + ex11 * l0(1, 1)) * q[0];
+        T pg1 = (s0 * (B2(0, 1, 0) - B2(0, 0, 0)) + r0 * (B2(1, 1, 0) - B2(1, 0, 0))) * q[2];
+        pg1 += (s2 * (B0(0, 1, 0) - B0(0, 0, 0)) + r2 * (B0(0, 1, 1) - B0(0, 0, 1))) * q[0];
+        pg1 += (ey00 * l1(0, 0) + ey10 * l1(1, 0) + ey01 * l1(0, 1) + ey11 * l1(1, 1)) * q[1];
+        T pg2 = (s1 * (B0(0, 0, 1) - B0(0, 0, 0)) + r1 * (B0(0, 1, 1) - B0(0, 1, 0))) * q[0];
+        pg2 += (s0 * (B1(0, 0, 1) - B1(0, 0, 0)) + r0 * (B1(1, 0, 1) - B1(1, 0, 0))) * q[1];
+        pg2 += (ez00 * l2(0, 0) + ez10 * l2(1, 0) + ez01 * l2(0, 1) + ez11 * l2(1, 1)) * q[2];
+        const T sb = frcp(bc_k * e);
+        g[0] += pg0 * sb;
+        g[1] += pg1 * sb;
+        g[2] += pg2 * sb;
+    }
+    const T kf = dtau_k * e * Nn;
+    v2 = (T)0;
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        q[l] -= kf * g[l];
+        v2 += q[l] * q[l];
+    }
+    const T ret = v2 * frcp(a_k * a_k);
+
+    // ---- drift with the kicked momentum
+    const T ed2 = v2 + a_d * a_d, ie = frsqrt(ed2);
+    const T ph = nf_d >= 1 ? phic : (T)0;
+    const T ch = nf_d >= 2 ? cic8(Fc) : (T)0;
+    const T vf = (o + ((T)3 - v2 * ie * ie) * ph - ch) * ie;
+    if (nf_d >= 3) {
+        const T ibc = frcp(bc_d);
+        const T b0 = B0(0, 0, 0) * ex00 + B0(0, 1, 0) * ex10 + B0(0, 0, 1) * ex01 + B0(0, 1, 1) * ex11;
+        const T b1 = B1(0, 0, 0) * ey00 + B1(1, 0, 0) * ey10 + B1(0, 0, 1) * ey01 + B1(1, 0, 1) * ey11;
+        const T b2 = B2(0, 0, 0) * ez00 + B2(1, 0, 0) * ez10 + B2(0, 1, 0) * ez01 + B2(1, 1, 0) * ez11;
+        d[0] = dtau_d * (double)(q[0] * vf + b0 * ibc);
+        d[1] = dtau_d * (double)(q[1] * vf + b1 * ibc);
+        d[2] = dtau_d * (double)(q[2] * vf + b2 * ibc);
+    } else {
+#pragma unroll
+        for (int l = 0; l < 3; l++) d[l] = dtau_d * (double)(q[l] * vf);
+    }
+    return ret;
 }
 
 __device__ __forceinline__ void block_max_to(double v, unsigned long long *slot)

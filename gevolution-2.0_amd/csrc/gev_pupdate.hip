@@ -21,7 +21,7 @@ struct UpdParams {
     int nf_k, nf_d, flags;
 };
 
-template <typename T, int TX, int TY, int TZ>
+template <typename T, int TX, int TY, int TZ, int F>
 __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, const T *__restrict__ phi,
                                                         const T *__restrict__ chi, const T *__restrict__ B0g,
                                                         const T *__restrict__ B1g, const T *__restrict__ B2g,
@@ -37,10 +37,11 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int x0, y0, z0;
     tile_origin<TX, TY, TZ>(N, x0, y0, z0);
-    const bool kick = u.flags & (UPD_KICK_GR | UPD_KICK_NEWTON);
-    const bool drift = u.flags & (UPD_DRIFT_GR | UPD_DRIFT_NEWTON);
+    // F: the kick / drift forms (compile time); UPD_COUNT in u.flags (run time)
+    constexpr bool kick = F & (UPD_KICK_GR | UPD_KICK_NEWTON);
+    constexpr bool drift = F & (UPD_DRIFT_GR | UPD_DRIFT_NEWTON);
     const int nf_k = u.nf_k;
-    const int nf_d = (u.flags & UPD_DRIFT_GR) ? u.nf_d : 0;
+    const int nf_d = (F & UPD_DRIFT_GR) ? u.nf_d : 0;
     const int nfmax = kick ? (nf_k > nf_d ? nf_k : nf_d) : nf_d;
     const bool use_phi = kick || nf_d >= 1, use_chi = nfmax >= 2, use_B = nfmax >= 3;
     // stage [x0-1, x0+TX+1] x ... : each wave covers RPW box rows per pass, lane -> (row, x)
@@ -88,40 +89,74 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
     const int total = rs[NROW];
     const T Nn = (T)N;
     double ret = 0.0;
-    for (int f = tid; f < total; f += PT_NT) {
-        int lo = 0, hi = NROW;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (rs[mid] <= f) lo = mid; else hi = mid;
-        }
-        const int64_t p = (int64_t)rbeg[lo] + (f - rs[lo]);
-        int x, y, z;
-        cell_coords(v.cell[p], N, v.lg, x, y, z);
-        const int o = (x - x0 + 1) + SX * ((y - y0 + 1) + SY * (z - z0 + 1));
-        T r[3];
-        ref_dist(v, p, x, y, z, r);
-        T q[3] = {v.q0[p], v.q1[p], v.q2[p]};
-        FieldLds<T> Fp{fld[0], o, SX, SX * SY}, Fc{fld[1], o, SX, SX * SY};
-        FieldLds<T> B0{fld[2], o, SX, SX * SY}, B1{fld[3], o, SX, SX * SY}, B2{fld[4], o, SX, SX * SY};
-        if (u.flags & UPD_KICK_GR) {
-            T rv = phys_update_q<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn);
-            ret = fmax(ret, (double)rv);
-        } else if (u.flags & UPD_KICK_NEWTON) {
-            T rv = phys_update_q_newton<T>(r, q, Fp, Fc, nf_k, (T)u.a_k, (T)u.dtau_k, Nn);
-            ret = fmax(ret, (double)rv);
-        }
-        if (kick) { v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2]; }
-        if (drift) {
-            double d[3];
-            if (u.flags & UPD_DRIFT_GR) {
-                phys_update_pos<T>(r, q, Fp, Fc, B0, B1, B2, nf_d, (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
-            } else {
-                d[0] = u.dtau_d * (double)q[0] / u.a_d;
-                d[1] = u.dtau_d * (double)q[1] / u.a_d;
-                d[2] = u.dtau_d * (double)q[2] / u.a_d;
+    // U particles per thread and pass: their loads are issued together
+    constexpr int U = sizeof(T) == 4 ? 2 : 1;
+    for (int f0 = tid; f0 < total; f0 += U * PT_NT) {
+        int64_t pu[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int f = f0 + k * PT_NT;
+            pu[k] = -1;
+            if (f < total) {
+                int lo = 0, hi = NROW;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (rs[mid] <= f) lo = mid; else hi = mid;
+                }
+                pu[k] = (int64_t)rbeg[lo] + (f - rs[lo]);
             }
-            move_by(v, p, x, y, z, d);
-            if (u.flags & UPD_COUNT) rank[p] = atomicAdd(&count[v.cell[p]], 1);
+        }
+        int32_t cu[U];
+        T xu[U][3], qu[U][3];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            if (pu[k] < 0) continue;
+            const int64_t p = pu[k];
+            cu[k] = v.cell[p];
+            xu[k][0] = v.p0[p]; xu[k][1] = v.p1[p]; xu[k][2] = v.p2[p];
+            qu[k][0] = v.q0[p]; qu[k][1] = v.q1[p]; qu[k][2] = v.q2[p];
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            if (pu[k] < 0) continue;
+            const int64_t p = pu[k];
+            const int x = cu[k] & (N - 1), y = (cu[k] >> v.lg) & (N - 1), z = cu[k] >> (2 * v.lg);
+            const int o = (x - x0 + 1) + SX * ((y - y0 + 1) + SY * (z - z0 + 1));
+            T r[3];
+            cell_rel(xu[k], x, y, z, N, r);
+            T *q = qu[k];
+            FieldLds<T> Fp{fld[0], o, SX, SX * SY}, Fc{fld[1], o, SX, SX * SY};
+            FieldLds<T> B0{fld[2], o, SX, SX * SY}, B1{fld[3], o, SX, SX * SY}, B2{fld[4], o, SX, SX * SY};
+            if constexpr ((F & (UPD_KICK_GR | UPD_DRIFT_GR)) == (UPD_KICK_GR | UPD_DRIFT_GR)) {
+                double d[3];
+                T rv = phys_kick_drift<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn, nf_d,
+                                          (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
+                ret = fmax(ret, (double)rv);
+                v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2];
+                const int32_t c = move_by(v, p, xu[k], x, y, z, d);
+                if (u.flags & UPD_COUNT) rank[p] = atomicAdd(&count[c], 1);
+            } else {
+            if constexpr (F & UPD_KICK_GR) {
+                T rv = phys_update_q<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn);
+                ret = fmax(ret, (double)rv);
+            } else if constexpr (F & UPD_KICK_NEWTON) {
+                T rv = phys_update_q_newton<T>(r, q, Fp, Fc, nf_k, (T)u.a_k, (T)u.dtau_k, Nn);
+                ret = fmax(ret, (double)rv);
+            }
+            if (kick) { v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2]; }
+            if (drift) {
+                double d[3];
+                if constexpr (F & UPD_DRIFT_GR) {
+                    phys_update_pos<T>(r, q, Fp, Fc, B0, B1, B2, nf_d, (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
+                } else {
+                    d[0] = u.dtau_d * (double)q[0] / u.a_d;
+                    d[1] = u.dtau_d * (double)q[1] / u.a_d;
+                    d[2] = u.dtau_d * (double)q[2] / u.a_d;
+                }
+                const int32_t c = move_by(v, p, xu[k], x, y, z, d);
+                if (u.flags & UPD_COUNT) rank[p] = atomicAdd(&count[c], 1);
+            }
+            }
         }
     }
     if (kick) block_max_to(ret, maxslot);
@@ -129,6 +164,36 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
 
 template <typename T> int pcls_sort_impl(gev_pcls *p);
 template <typename T> int pcls_sort_counted_impl(gev_pcls *p);
+
+template <typename T, int TX, int TY, int TZ, int F>
+static void launch_tile(gev_pcls *p, const UpdParams &u, const T *phi, const T *chi, const T *B0, const T *B1,
+                        const T *B2, unsigned long long *slot)
+{
+    gev_ctx *c = p->ctx;
+    const int N = c->N;
+    const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
+    if constexpr (!(sizeof(T) == 8 && TX == 16))  // the large tile exists for FP32 only
+        hipLaunchKernelGGL((k_update_tiled<T, TX, TY, TZ, F>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), u, phi,
+                           chi, B0, B1, B2, slot, p->cell_count, p->rank);
+}
+
+template <typename T, int TX, int TY, int TZ>
+static void launch_mode(gev_pcls *p, const UpdParams &u, const T *phi, const T *chi, const T *B0, const T *B1,
+                        const T *B2, unsigned long long *slot)
+{
+    switch (u.flags & ~UPD_COUNT) {
+    case UPD_KICK_GR: launch_tile<T, TX, TY, TZ, UPD_KICK_GR>(p, u, phi, chi, B0, B1, B2, slot); break;
+    case UPD_KICK_NEWTON: launch_tile<T, TX, TY, TZ, UPD_KICK_NEWTON>(p, u, phi, chi, B0, B1, B2, slot); break;
+    case UPD_DRIFT_GR: launch_tile<T, TX, TY, TZ, UPD_DRIFT_GR>(p, u, phi, chi, B0, B1, B2, slot); break;
+    case UPD_DRIFT_NEWTON: launch_tile<T, TX, TY, TZ, UPD_DRIFT_NEWTON>(p, u, phi, chi, B0, B1, B2, slot); break;
+    case UPD_KICK_GR | UPD_DRIFT_GR:
+        launch_tile<T, TX, TY, TZ, UPD_KICK_GR | UPD_DRIFT_GR>(p, u, phi, chi, B0, B1, B2, slot);
+        break;
+    case UPD_KICK_NEWTON | UPD_DRIFT_NEWTON:
+        launch_tile<T, TX, TY, TZ, UPD_KICK_NEWTON | UPD_DRIFT_NEWTON>(p, u, phi, chi, B0, B1, B2, slot);
+        break;
+    }
+}
 
 template <typename T> static int launch_update(gev_pcls *p, const UpdParams &u, const T *phi, const T *chi, const T *Bi)
 {
@@ -139,25 +204,26 @@ template <typename T> static int launch_update(gev_pcls *p, const UpdParams &u, 
         if (rc) return rc;
     }
     if (u.flags & UPD_COUNT) {
-        int rc = field_fill_impl<int32_t>(c, p->cell_count, c->M, 0);
+        int rc = pcls_clear_counts(p);
         if (rc) return rc;
     }
     unsigned long long *slot = (unsigned long long *)(c->red + 8);
     const T *B0 = Bi, *B1 = Bi ? Bi + c->M : nullptr, *B2 = Bi ? Bi + 2 * c->M : nullptr;
-    if (N >= 8) {
-        unsigned nb = (unsigned)((int64_t)(N / 8) * (N / 8) * (N / 4));
-        hipLaunchKernelGGL((k_update_tiled<T, 8, 8, 4>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), u, phi, chi,
-                           B0, B1, B2, slot, p->cell_count, p->rank);
+    // FP32 fits the 16x8x8 tile (5 staged fields, 46 KB of LDS); FP64 keeps 8x8x4
+    bool big = false;
+    if constexpr (sizeof(T) == 4) big = N >= 16;
+    if (big) {
+        launch_mode<T, 16, 8, 8>(p, u, phi, chi, B0, B1, B2, slot);
+    } else if (N >= 8) {
+        launch_mode<T, 8, 8, 4>(p, u, phi, chi, B0, B1, B2, slot);
     } else {
-        unsigned nb = (unsigned)((int64_t)(N / 4) * (N / 4) * (N / 4));
-        hipLaunchKernelGGL((k_update_tiled<T, 4, 4, 4>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), u, phi, chi,
-                           B0, B1, B2, slot, p->cell_count, p->rank);
+        launch_mode<T, 4, 4, 4>(p, u, phi, chi, B0, B1, B2, slot);
     }
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }
 
-static int read_max_slot(gev_ctx *c, double *out)
+int read_max_slot(gev_ctx *c, double *out)
 {
     GEV_HIP(hipMemcpyAsync(c->red_host, c->red + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     GEV_HIP(hipStreamSynchronize(c->stream));

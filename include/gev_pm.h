@@ -180,7 +180,7 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st);
 enum {
     GEV_PHASE_DEPOSIT_T00 = 0, GEV_PHASE_SOURCE_T00 = 1, GEV_PHASE_DEPOSIT_TIJ = 2, GEV_PHASE_SOLVE_PHI = 3,
     GEV_PHASE_SOURCE_TIJ = 4, GEV_PHASE_DEPOSIT_T0I = 5, GEV_PHASE_SOLVE_CHI = 6, GEV_PHASE_SOLVE_B = 7,
-    GEV_PHASE_KICK = 8, GEV_PHASE_DRIFT = 9, GEV_PHASE_COUNT = 10
+    GEV_PHASE_KICK_DRIFT = 8, GEV_PHASE_REBIN = 9, GEV_PHASE_COUNT = 10
 };
 int gev_step_timing(gev_ctx *ctx, int enable);
 int gev_step_timings(gev_ctx *ctx, int *ids, float *ms, int max);

@@ -17,8 +17,32 @@
  *   - particles: SoA, pos[c*np+i] in box units [0,1), q[c*np+i] = q/m.
  *   - N must be a power of two.
  *
- * Parity status of each function is recorded beside it ("pinned by ...");
- * the pins live in tests/test_oracle_*.py.
+ * What pins each function (tests/test_oracle_*.py, tests/test_abi.py); every
+ * check is fixed by mathematics or data independent of this file:
+ *   or_fft_r2c / or_fft_c2r   brute-force DFT (N=4,8), numpy.fft fixtures
+ *                             (N=8,16), Parseval + round trip, c2r ignoring
+ *                             Im of the kx=0 / N/2 planes
+ *   or_poisson                7-point Laplacian of the result = source (+modif)
+ *   or_project_ft_scalar      longitudinal / pure-trace tensors, closed forms
+ *   or_project_ft_vector      inverse of (-Lap + modif) on transverse input,
+ *                             D- transversality of the output
+ *   or_evolve_ft_vector       PARTIALLY PINNED: trace/longitudinal inputs leave
+ *                             B unchanged, increment is transverse; no closed
+ *                             form fixes the transverse amplitude
+ *   or_prepare_source_T00/Tij hand-evaluated stencils, phi = 0 limits
+ *   or_project_T00/T0i/Tij    sum rules (energy, momentum, stress), single
+ *                             particle weights, staggering by symmetry, lattice
+ *   or_update_q[_newton]      linear potential (exact force), Newton limit of
+ *                             the GR kick, chi and B terms by construction
+ *   or_update_pos[_newton]    free streaming and uniform-field closed forms
+ *   or_displace_particles /
+ *   or_initialize_q           closed forms on single modes
+ *   or_hconf / or_rk4bg /
+ *   or_particle_horizon       Einstein-de Sitter closed forms
+ *   gev_oracle_step.c         Philox known-answer vectors, natural spline vs
+ *                             scipy, P(k) single-mode closed form, IC P(k) vs
+ *                             input table, linear growth D(a) (Newton and GR),
+ *                             momentum conservation of the step
  */
 #include <math.h>
 #include <stdint.h>

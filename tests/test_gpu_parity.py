@@ -421,3 +421,19 @@ def test_fused_kick_drift_vs_oracle(gp, O, gr):
     cells = to_np(p.cells())
     c = np.floor(gpos * N).astype(np.int64)
     assert np.array_equal(np.sort(cells), np.sort(c[0] + N * (c[1] + N * c[2])))
+
+
+# ------------------------------------------------------------------- P(k)
+@pytest.mark.parametrize("N", [16, 64])
+def test_power_spectrum_vs_oracle(gp, O, N):
+    """device P(k) estimator (extractPowerSpectrum semantics) against the oracle's"""
+    from gevpm import spectra
+    ctx = gp.Context(N, gp.GEV_FP64)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(N ** 3)
+    F = O.fft_r2c(f)
+    res = spectra.power_spectrum(ctx, ft_gpu(ctx, F), 24, deconvolve=True)
+    ref = O.power_spectrum(F, 24, deconvolve=True)
+    for got, want in zip(res[:4], ref[:4]):
+        assert relmax(got.numpy(), want) < 1e-12
+    assert np.array_equal(res[4].numpy(), ref[4])

@@ -171,3 +171,40 @@ def test_evolve_ft_vector_structure(O):
     divB = sum(dminus(Br[i], i, N) for i in range(3))
     assert np.abs(divB).max() <= 1e-9 * np.abs(Br).max() * N
     assert np.abs(Br).max() > 0
+
+
+@pytest.mark.parametrize("mode", [(3, 1, 2), (0, 5, 0), (8, 0, 0)])
+def test_power_spectrum_single_mode(O, mode):
+    """extractPowerSpectrum semantics on one real Fourier mode: all power in the
+    bin of |k|, equal to (weight |F|^2 |k|^3 / sinc^2) / occupation, where the
+    occupation and mean |k| of every bin are brute-force counts over the full
+    (unfolded) N^3 frequency grid."""
+    N, nb, A = 16, 12, 0.7
+    nx, ny, nz = mode
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    f = A * np.cos(2 * np.pi * (nx * x + ny * y + nz * z) / N)
+    F = O.fft_r2c(f.ravel())
+    kb, pw, ks, ps, occ = O.power_spectrum(F, nb, deconvolve=True)
+    # full-grid brute force of the bins
+    fr = np.fft.fftfreq(N, 1.0 / N)
+    KZ, KY, KX = np.meshgrid(fr, fr, fr, indexing="ij")
+    kk = 2 * np.pi * np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+    kmax = np.sqrt(3.0) * 2 * np.pi * (N // 2)
+    b = np.floor(nb * kk / kmax).astype(int)
+    b[0, 0, 0] = -1
+    for j in range(nb):
+        sel = b == j
+        assert occ[j] == sel.sum()
+        if sel.sum():
+            assert abs(kb[j] - kk[sel].mean()) <= 1e-12 * kk[sel].mean()
+    k = 2 * np.pi * np.sqrt(nx * nx + ny * ny + nz * nz)
+    jb = int(np.floor(nb * k / kmax))
+    sinc = np.prod([np.sin(np.pi * n / N) * N / (np.pi * n) if n else 1.0 for n in mode])
+    # a cosine puts |F| = A N^3 / 2 at +n and -n (both in the full grid: 2 entries,
+    # or one self-conjugate entry of amplitude A N^3 at a Nyquist/zero plane)
+    selfconj = all(n in (0, N // 2) for n in mode)
+    full = (A * N ** 3) ** 2 if selfconj else 2 * (A * N ** 3 / 2) ** 2
+    expect = full * k ** 3 / sinc ** 2 / occ[jb]
+    assert abs(pw[jb] - expect) <= 1e-10 * expect
+    others = np.delete(pw, jb)
+    assert np.abs(others).max() <= 1e-20 * expect

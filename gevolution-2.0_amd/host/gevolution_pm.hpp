@@ -1,0 +1,355 @@
+// gevolution_pm.hpp -- C++ host façade over the C ABI (include/gev_pm.h) with
+// the names, argument meaning and error behaviour of gevolution-2.0's
+// particle-mesh interfaces, so a main.cu-shaped driver compiles against it.
+//
+//   Lattice                       LATfield2 Lattice + the GPU context (main.cu:300-357)
+//   Field<Real> / Field<Cplx>     LATfield2 Field in real / Fourier layout
+//   PlanFFT<Cplx>                 LATfield2 PlanFFT (execute FFT_FORWARD / FFT_BACKWARD)
+//   perfParticles_gevolution<>    Particles_gevolution.hpp:176-211 (+ perfParticles base:
+//                                 updateVel, moveParticles, num_particles, sampleParticles)
+//   projection_* / prepareFTsource / solveModifiedPoissonFT /
+//   projectFTscalar / projectFTvector / evolveFTvector
+//                                 gevolution.hpp free functions, same defaults
+//   Hconf / rungekutta4bg / particleHorizon
+//                                 background.hpp:200-290
+//
+// Errors: any non-zero status of the library becomes std::runtime_error with
+// the library's message (the reference throws std::runtime_error on GPU
+// errors, cuda_staging.hpp:12-16).  Precision: compile with -DSINGLE for the
+// mixed form (Real = float, as the reference's -DSINGLE build), otherwise FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "gev_pm.h"
+
+namespace gevpm {
+
+#ifdef SINGLE
+using Real = float;
+constexpr int kPrecision = GEV_MIXED;
+#else
+using Real = double;
+constexpr int kPrecision = GEV_FP64;
+#endif
+using Cplx = std::complex<Real>;  // (re, im) interleaved, the library's Fourier layout
+
+inline void check(int rc, const char *what)
+{
+    if (rc != GEV_OK) throw std::runtime_error(std::string(what) + ": " + gev_last_error());
+}
+
+inline void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+enum { FFT_FORWARD = 0, FFT_BACKWARD = 1 };
+
+// ------------------------------------------------------------------ lattice
+class Lattice {
+  public:
+    explicit Lattice(int N, int precision = kPrecision, hipStream_t stream = nullptr) : N_(N), prec_(precision)
+    {
+        check(gev_ctx_create(N, precision, (void *)stream, &ctx_), "gev_ctx_create");
+        P_ = gev_ctx_kpitch(ctx_);
+    }
+    ~Lattice() { gev_ctx_destroy(ctx_); }
+    Lattice(const Lattice &) = delete;
+    Lattice &operator=(const Lattice &) = delete;
+
+    int size() const { return N_; }
+    int precision() const { return prec_; }
+    int kpitch() const { return P_; }
+    int64_t sites() const { return (int64_t)N_ * N_ * N_; }
+    int64_t ksites() const { return (int64_t)N_ * N_ * P_; }
+    gev_ctx *ctx() const { return ctx_; }
+    void sync() const { check(gev_ctx_sync(ctx_), "gev_ctx_sync"); }
+
+  private:
+    int N_, prec_, P_ = 0;
+    gev_ctx *ctx_ = nullptr;
+};
+
+// -------------------------------------------------------------------- field
+// Field<Real>: components x N^3 reals; Field<Cplx>: components x N x N x P complex.
+// Device memory owned by the field (RAII).
+template <typename T> class Field {
+  public:
+    Field(Lattice &lat, int components = 1) : lat_(&lat), ncomp_(components)
+    {
+        n_ = (int64_t)components * (is_complex() ? lat.ksites() : lat.sites());
+        hip_check(hipMalloc(&d_, sizeof(T) * (size_t)n_), "hipMalloc(Field)");
+        hip_check(hipMemset(d_, 0, sizeof(T) * (size_t)n_), "hipMemset(Field)");
+    }
+    ~Field() { (void)hipFree(d_); }
+    Field(const Field &) = delete;
+    Field &operator=(const Field &) = delete;
+
+    static constexpr bool is_complex() { return !std::is_same<T, Real>::value; }
+    T *data() { return d_; }
+    const T *data() const { return d_; }
+    int components() const { return ncomp_; }
+    int64_t count() const { return n_; }
+    Lattice &lattice() const { return *lat_; }
+
+    void to_host(std::vector<T> &h) const
+    {
+        lat_->sync();
+        h.resize((size_t)n_);
+        hip_check(hipMemcpy(h.data(), d_, sizeof(T) * (size_t)n_, hipMemcpyDeviceToHost), "Field::to_host");
+    }
+    void from_host(const std::vector<T> &h)
+    {
+        if ((int64_t)h.size() != n_) throw std::runtime_error("Field::from_host: size mismatch");
+        lat_->sync();
+        hip_check(hipMemcpy(d_, h.data(), sizeof(T) * (size_t)n_, hipMemcpyHostToDevice), "Field::from_host");
+    }
+
+  private:
+    Lattice *lat_;
+    int ncomp_;
+    int64_t n_ = 0;
+    T *d_ = nullptr;
+};
+
+// ------------------------------------------------------------------ PlanFFT
+template <typename T> class PlanFFT {
+  public:
+    PlanFFT(Field<Real> *rfield, Field<T> *kfield, int components = 1) : r_(rfield), k_(kfield), ncomp_(components)
+    {
+        static_assert(!std::is_same<T, Real>::value, "PlanFFT<Cplx> pairs a real and a Fourier field");
+    }
+    // FFT_BACKWARD overwrites the Fourier field (the library's c2r works in place)
+    void execute(int dir)
+    {
+        gev_ctx *c = r_->lattice().ctx();
+        if (dir == FFT_FORWARD) check(gev_fft_r2c(c, r_->data(), k_->data(), ncomp_), "PlanFFT::execute(FFT_FORWARD)");
+        else check(gev_fft_c2r(c, k_->data(), r_->data(), ncomp_), "PlanFFT::execute(FFT_BACKWARD)");
+    }
+
+  private:
+    Field<Real> *r_;
+    Field<T> *k_;
+    int ncomp_;
+};
+
+// ------------------------------------------------------------- cosmology
+struct cosmology {
+    double Omega_m = 0.31, Omega_Lambda = 0.69, Omega_rad = 0.0;
+};
+inline double Hconf(double a, double fourpiG, const cosmology &c)
+{
+    return gev_hconf(a, fourpiG, c.Omega_m, c.Omega_Lambda, c.Omega_rad);
+}
+inline void rungekutta4bg(double &a, double fourpiG, const cosmology &c, double dtau)
+{
+    a = gev_rk4bg(a, fourpiG, c.Omega_m, c.Omega_Lambda, c.Omega_rad, dtau);
+}
+inline double particleHorizon(double a, double fourpiG, const cosmology &c)
+{
+    return gev_particle_horizon(a, fourpiG, c.Omega_m, c.Omega_Lambda, c.Omega_rad);
+}
+
+// ---------------------------------------------------------------- particles
+struct part_simple {
+    int64_t ID;
+    double pos[3];
+    double vel[3];  // q/m
+};
+struct part_simple_info {
+    double mass = 0.0;
+    const char *type_name = "part_simple";
+};
+
+// functor tags: which gevolution functor updateVel / moveParticles apply
+struct update_q_functor {};         // gevolution.hpp:888-1011
+struct update_q_Newton_functor {};  // gevolution.hpp:1051-1118
+struct update_pos_functor {};       // gevolution.hpp:1162-1232
+struct update_pos_Newton_functor {};  // gevolution.hpp:1270-1273
+
+template <class part = part_simple, class part_info = part_simple_info> class perfParticles_gevolution {
+  public:
+    perfParticles_gevolution() = default;
+    ~perfParticles_gevolution()
+    {
+        if (p_) gev_pcls_destroy(p_);
+    }
+    perfParticles_gevolution(const perfParticles_gevolution &) = delete;
+    perfParticles_gevolution &operator=(const perfParticles_gevolution &) = delete;
+
+    // perfParticles::initialize (ic_basic.hpp:2206); the box is the unit box
+    void initialize(const part_info &info, Lattice *lat, uint64_t capacity)
+    {
+        lat_ = lat;
+        info_ = info;
+        check(gev_pcls_create(lat->ctx(), (int64_t)capacity, info.mass, &p_), "perfParticles::initialize");
+    }
+    // addParticle_global for a whole set + updateRowBuffers (ic_basic.hpp:1639,2211)
+    void addParticles_global(const std::vector<part> &ps)
+    {
+        const size_t n = ps.size();
+        std::vector<double> pos(3 * n), q(3 * n);
+        std::vector<int64_t> id(n);
+        for (size_t i = 0; i < n; i++) {
+            for (int d = 0; d < 3; d++) {
+                pos[d * n + i] = ps[i].pos[d];
+                q[d * n + i] = ps[i].vel[d];
+            }
+            id[i] = ps[i].ID;
+        }
+        check(gev_pcls_load(p_, (int64_t)n, pos.data(), q.data(), id.data(), 0), "addParticle_global");
+    }
+    void updateRowBuffers() { check(gev_pcls_sort(p_), "updateRowBuffers"); }
+
+    int64_t num_particles()
+    {
+        gev_pcls_view v;
+        check(gev_pcls_view_get(p_, &v), "num_particles");
+        return v.n;
+    }
+    // sampleParticles (unit_tests.cu:537): every particle, FP64, host memory
+    void sampleParticles(std::vector<part> &out)
+    {
+        const int64_t n = num_particles();
+        std::vector<double> pos(3 * n), q(3 * n);
+        std::vector<int64_t> id(n);
+        check(gev_pcls_store(p_, pos.data(), q.data(), id.data(), 0), "sampleParticles");
+        out.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) {
+            for (int d = 0; d < 3; d++) {
+                out[i].pos[d] = pos[d * n + i];
+                out[i].vel[d] = q[d * n + i];
+            }
+            out[i].ID = id[i];
+        }
+    }
+
+    // updateVel: applies the kick functor and returns sqrt of the maximum of
+    // the functor values (v^2/a^2), as main.cu:1232 then converts it to maxvel
+    Real updateVel(update_q_functor, double dtau, Field<Real> **fields, int nfield, const double *params)
+    {
+        double m = 0;
+        check(gev_update_q(p_, dtau, fields[0]->data(), nfield >= 2 ? fields[1]->data() : nullptr,
+                           nfield >= 3 ? fields[2]->data() : nullptr, nfield, params[0], params[1], &m),
+              "updateVel(update_q)");
+        return (Real)std::sqrt(m);
+    }
+    Real updateVel(update_q_Newton_functor, double dtau, Field<Real> **fields, int nfield, const double *params)
+    {
+        double m = 0;
+        check(gev_update_q_newton(p_, dtau, fields[0]->data(), nfield >= 2 ? fields[1]->data() : nullptr, nfield,
+                                  params[0], &m),
+              "updateVel(update_q_Newton)");
+        return (Real)std::sqrt(m);
+    }
+    void moveParticles(update_pos_functor, double dtau, Field<Real> **fields, int nfield, const double *params)
+    {
+        check(gev_update_pos(p_, dtau, nfield >= 1 ? fields[0]->data() : nullptr,
+                             nfield >= 2 ? fields[1]->data() : nullptr, nfield >= 3 ? fields[2]->data() : nullptr,
+                             nfield, params[0], params[1]),
+              "moveParticles(update_pos)");
+    }
+    void moveParticles(update_pos_Newton_functor, double dtau, Field<Real> **, int, const double *params)
+    {
+        check(gev_update_pos_newton(p_, dtau, params[0]), "moveParticles(update_pos_Newton)");
+    }
+
+    gev_pcls *handle() const { return p_; }
+    Lattice *lattice() const { return lat_; }
+    const part_info &parts_info() const { return info_; }
+
+  private:
+    gev_pcls *p_ = nullptr;
+    Lattice *lat_ = nullptr;
+    part_info info_{};
+};
+
+using Particles = perfParticles_gevolution<part_simple, part_simple_info>;
+
+// ------------------------------------------------------------ projections
+// projection_init: zero a field before additive projections
+inline void projection_init(Field<Real> *f)
+{
+    check(gev_field_fill(f->lattice().ctx(), f->data(), f->count(), 0.0), "projection_init");
+}
+// gevolution.hpp:1887 (functor :1729-1854); phi == nullptr: plain mass density
+inline void projection_T00_project(Particles *pcls, Field<Real> *T00, double a = 1., Field<Real> *phi = nullptr,
+                                   double coeff = 1.)
+{
+    check(gev_project_T00(pcls->handle(), T00->data(), a, phi ? phi->data() : nullptr, coeff, 1),
+          "projection_T00_project");
+}
+// gevolution.hpp:1902
+inline void scalarProjectionCIC_project(Particles *pcls, Field<Real> *rho)
+{
+    check(gev_project_mass(pcls->handle(), rho->data(), 1.0, 1), "scalarProjectionCIC_project");
+}
+// gevolution.hpp:2232 (functor :2031-2201)
+inline void projection_T0i_project(Particles *pcls, Field<Real> *T0i, Field<Real> *phi = nullptr, double coeff = 1.)
+{
+    check(gev_project_T0i(pcls->handle(), T0i->data(), phi ? phi->data() : nullptr, coeff, 1),
+          "projection_T0i_project");
+}
+// gevolution.hpp:2999 (functor :2582-2965, no CIC_PROJECT_TIJ)
+inline void projection_Tij_project(Particles *pcls, Field<Real> *Tij, double a = 1., Field<Real> *phi = nullptr,
+                                   double coeff = 1.)
+{
+    check(gev_project_Tij(pcls->handle(), Tij->data(), a, phi ? phi->data() : nullptr, coeff, 1),
+          "projection_Tij_project");
+}
+
+// ----------------------------------------------------------------- sources
+// gevolution.hpp:183 (T00 form); returns the sum of source, as the reference
+inline double prepareFTsource(Field<Real> &phi, Field<Real> &chi, Field<Real> &source, const double bgmodel,
+                              Field<Real> &result, const double coeff, const double coeff2, const double coeff3,
+                              double *result_sum = nullptr)
+{
+    double s = 0, r = 0;
+    check(gev_prepare_source_T00(phi.lattice().ctx(), phi.data(), chi.data(), source.data(), bgmodel, result.data(),
+                                 coeff, coeff2, coeff3, &s, &r),
+          "prepareFTsource(T00)");
+    if (result_sum) *result_sum = r;
+    return s;
+}
+// gevolution.hpp:119 (Tij form)
+inline void prepareFTsource(Field<Real> &phi, Field<Real> &Tij, Field<Real> &Sij, const double coeff)
+{
+    check(gev_prepare_source_Tij(phi.lattice().ctx(), phi.data(), Tij.data(), Sij.data(), coeff),
+          "prepareFTsource(Tij)");
+}
+
+// ----------------------------------------------------------------- k-space
+inline void solveModifiedPoissonFT(Field<Cplx> &sourceFT, Field<Cplx> &potFT, Real coeff, const Real modif = 0)
+{
+    check(gev_poisson(sourceFT.lattice().ctx(), sourceFT.data(), potFT.data(), coeff, modif),
+          "solveModifiedPoissonFT");
+}
+inline void projectFTscalar(Field<Cplx> &SijFT, Field<Cplx> &chiFT, const int add = 0)
+{
+    check(gev_project_ft_scalar(SijFT.lattice().ctx(), SijFT.data(), chiFT.data(), add), "projectFTscalar");
+}
+inline void projectFTvector(Field<Cplx> &SiFT, Field<Cplx> &BiFT, const Real coeff = 1, const Real modif = 0)
+{
+    check(gev_project_ft_vector(SiFT.lattice().ctx(), SiFT.data(), BiFT.data(), coeff, modif), "projectFTvector");
+}
+inline void evolveFTvector(Field<Cplx> &SijFT, Field<Cplx> &BiFT, const Real a2dtau)
+{
+    check(gev_evolve_ft_vector(SijFT.lattice().ctx(), SijFT.data(), BiFT.data(), a2dtau), "evolveFTvector");
+}
+// the mixed form's FP64 k=0 mode (no reference counterpart: FP64 mean of an FP32 field)
+inline void set_k0(Field<Cplx> &ft, double re, double im = 0.0)
+{
+    check(gev_set_k0(ft.lattice().ctx(), ft.data(), re, im), "set_k0");
+}
+
+}  // namespace gevpm

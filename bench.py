@@ -189,15 +189,12 @@ def _bytes_models(M, n, R=4):
                        "k_update_tiled (fused GR kick + drift): per particle 3 offsets + 3 momenta + cell read, "
                        "3 offsets + 3 momenta + cell + rank written, cell counter read + written; per cell phi, "
                        "chi, B0..B2 read"),
-        "deposit_Tij": (n * 7 * R + M * (4 + R + 6 * R),
-                        "k_deposit Tij: per particle 3 offsets + 3 momenta + cell read; per cell start + phi read, "
-                        "6 components written"),
+        "deposit_T00_Tij": (n * 7 * R + M * (4 + R + 7 * R),
+                            "k_deposit T00+Tij: per particle 3 offsets + 3 momenta + cell read; per cell start + "
+                            "phi read, 7 components written"),
         "deposit_T0i": (n * 7 * R + M * (4 + R + 3 * R),
                         "k_deposit T0i: per particle 3 offsets + 3 momenta + cell read; per cell start + phi read, "
                         "3 components written"),
-        "deposit_T00": (n * 7 * R + M * (4 + R + R),
-                        "k_deposit T00: per particle 3 offsets + 3 momenta + cell read; per cell start + phi read, "
-                        "1 component written"),
     }
 
 

@@ -16,6 +16,8 @@ template <typename T> int pcls_lattice_impl(gev_pcls *p);
 template <typename T> int pcls_load_impl(gev_pcls *p, const double *pos, const double *q, const int64_t *id);
 template <typename T> int pcls_store_impl(gev_pcls *p, double *pos, double *q, int64_t *id);
 template <typename T> int dep_T00_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc);
+template <typename T>
+int dep_T00_Tij_impl(gev_pcls *p, T *T00, T *Tij, const T *phi, double mass00, double massij, double a, int acc);
 template <typename T> int dep_T0i_impl(gev_pcls *p, T *out, const T *phi, double mass, int acc);
 template <typename T> int dep_Tij_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc);
 template <typename T>
@@ -305,6 +307,20 @@ int gev_project_Tij(gev_pcls *p, void *Tij, double a, const void *phi, double co
                          dep_Tij_impl<float>(p, (float *)Tij, (const float *)phi, mass, a, accumulate));
 }
 
+int gev_project_T00_Tij(gev_pcls *p, void *T00, void *Tij, double a, const void *phi, double coeff, int accumulate)
+{
+    CHECK(p && T00 && Tij, "gev_project_T00_Tij: NULL argument");
+    NEED_SORTED(p);
+    const double dx = 1.0 / (double)p->ctx->N;
+    const double m = p->mass * coeff / (dx * dx * dx);
+    const double m00 = phi ? m / a : m, mij = m / a;  // as gev_project_T00 / gev_project_Tij
+    return PREC_DISPATCH(p->ctx,
+                         dep_T00_Tij_impl<double>(p, (double *)T00, (double *)Tij, (const double *)phi, m00, mij, a,
+                                                  accumulate),
+                         dep_T00_Tij_impl<float>(p, (float *)T00, (float *)Tij, (const float *)phi, m00, mij, a,
+                                                 accumulate));
+}
+
 // -------------------------------------------------------------- sources ---
 int gev_prepare_source_T00(gev_ctx *c, const void *phi, const void *chi, const void *source, double bg, void *result,
                            double c1, double c2, double c3, double *sum_source, double *sum_result)
@@ -370,6 +386,51 @@ int gev_evolve_ft_vector(gev_ctx *c, const void *S, void *B, double a2dtau)
     CHECK(c && S && B, "gev_evolve_ft_vector: NULL argument");
     return PREC_DISPATCH(c, evolve_ft_vector_impl<double>(c, (const cx<double> *)S, (cx<double> *)B, a2dtau),
                          evolve_ft_vector_impl<float>(c, (const cx<float> *)S, (cx<float> *)B, a2dtau));
+}
+
+// ---- fused real -> k-space -> real solves (the operator runs inside the z pass)
+int gev_solve_poisson(gev_ctx *c, const void *src, void *pot, void *workFT, double coeff, double modif, int set_k0,
+                      double k0_re)
+{
+    CHECK(c && src && pot && workFT, "gev_solve_poisson: NULL argument");
+    if (zsolve_supported(c))
+        return PREC_DISPATCH(c,
+                             solve_poisson_fused_impl<double>(c, (const double *)src, (double *)pot,
+                                                              (cx<double> *)workFT, coeff, modif, set_k0, k0_re),
+                             solve_poisson_fused_impl<float>(c, (const float *)src, (float *)pot, (cx<float> *)workFT,
+                                                             coeff, modif, set_k0, k0_re));
+    int rc = gev_fft_r2c(c, src, workFT, 1);
+    if (!rc && set_k0) rc = gev_set_k0(c, workFT, k0_re, 0.0);
+    if (!rc) rc = gev_poisson(c, workFT, workFT, coeff, modif);
+    if (!rc) rc = gev_fft_c2r(c, workFT, pot, 1);
+    return rc;
+}
+
+int gev_solve_chi(gev_ctx *c, const void *Sij, void *chi, void *SijFT, void *chiFT)
+{
+    CHECK(c && Sij && chi && SijFT && chiFT, "gev_solve_chi: NULL argument");
+    if (zsolve_supported(c))
+        return PREC_DISPATCH(c,
+                             solve_chi_fused_impl<double>(c, (const double *)Sij, (double *)chi, (cx<double> *)SijFT,
+                                                          (cx<double> *)chiFT),
+                             solve_chi_fused_impl<float>(c, (const float *)Sij, (float *)chi, (cx<float> *)SijFT,
+                                                         (cx<float> *)chiFT));
+    int rc = gev_fft_r2c(c, Sij, SijFT, 6);
+    if (!rc) rc = gev_project_ft_scalar(c, SijFT, chiFT, 0);
+    if (!rc) rc = gev_fft_c2r(c, chiFT, chi, 1);
+    return rc;
+}
+
+int gev_solve_vector(gev_ctx *c, void *Bi, void *BiFT, double coeff, double modif)
+{
+    CHECK(c && Bi && BiFT, "gev_solve_vector: NULL argument");
+    if (zsolve_supported(c))
+        return PREC_DISPATCH(c, solve_vector_fused_impl<double>(c, (double *)Bi, (cx<double> *)BiFT, coeff, modif),
+                             solve_vector_fused_impl<float>(c, (float *)Bi, (cx<float> *)BiFT, coeff, modif));
+    int rc = gev_fft_r2c(c, Bi, BiFT, 3);
+    if (!rc) rc = gev_project_ft_vector(c, BiFT, BiFT, coeff, modif);
+    if (!rc) rc = gev_fft_c2r(c, BiFT, Bi, 3);
+    return rc;
 }
 
 int gev_set_k0(gev_ctx *c, void *ft, double re, double im)

@@ -21,8 +21,11 @@
 
 namespace gev {
 
-enum { DEP_T00 = 0, DEP_T0I = 1, DEP_TIJ = 2 };
-template <int KIND> struct DepC { static constexpr int C = KIND == DEP_T00 ? 1 : (KIND == DEP_T0I ? 3 : 6); };
+// DEP_T00TIJ: T00 and Tij of the same particles and phi in one pass (7 outputs)
+enum { DEP_T00 = 0, DEP_T0I = 1, DEP_TIJ = 2, DEP_T00TIJ = 3 };
+template <int KIND> struct DepC {
+    static constexpr int C = KIND == DEP_T00 ? 1 : (KIND == DEP_T0I ? 3 : (KIND == DEP_TIJ ? 6 : 7));
+};
 
 // FP64 LDS accumulators in both precisions: ds_add_f64 is the fast LDS atomic
 // on this part, and the per-site sums carry the full FP64 mantissa.
@@ -32,8 +35,8 @@ __device__ __forceinline__ void lds_add(double *a, double v)
 }
 
 template <typename T, int KIND, int TX, int TY, int TZ>
-__global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, const T *__restrict__ phi,
-                                                   double mass_d, double a_d, int accumulate)
+__global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int64_t M, const T *__restrict__ phi,
+                                                   double mass_d, double mass2_d, double a_d, int accumulate)
 {
     constexpr int C = DepC<KIND>::C;
     constexpr int NS = TX * TY * TZ;                    // owned sites
@@ -93,7 +96,8 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
     const T a = (T)a_d, o = (T)1, ia2 = (T)(1.0 / (a_d * a_d));
 
     // U particles per thread and pass: their loads are issued together
-    constexpr int U = KIND == DEP_TIJ ? 2 : 4;
+    constexpr int U = (KIND == DEP_TIJ || KIND == DEP_T00TIJ) ? 2 : 4;
+    constexpr int TB = KIND == DEP_T00TIJ ? 1 : 0;  // first Tij accumulator
     const bool need_q = KIND != DEP_T00 || use_phi;
     for (int32_t i0 = t; i0 < total; i0 += U * PT_NT) {
         int run[U];
@@ -144,7 +148,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
         };
         auto sidx = [&](int bx, int by, int bz) { return sb + bx + TX * (by + TY * bz); };
         const int pb = lx + PX * (ly + PY * lz);
-        if constexpr (KIND == DEP_T00) {
+        if constexpr (KIND == DEP_T00 || KIND == DEP_T00TIJ) {
             // projection_T00 functor, gevolution.hpp:1729-1854
             if (use_phi) {
                 const T q0 = qu[u][0], q1 = qu[u][1], q2 = qu[u][2];
@@ -170,7 +174,8 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
                                 (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
                 }
             }
-        } else if constexpr (KIND == DEP_T0I) {
+        }
+        if constexpr (KIND == DEP_T0I) {
             // projection_T0i functor, gevolution.hpp:2031-2201: 12 links per cell,
             // each weighted by (1 + phi_a + phi_b) of its two end points
             const T v0 = qu[u][0], v1 = qu[u][1], v2 = qu[u][2];
@@ -199,7 +204,8 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
             if (own(0, 1, 1)) lds_add(&acc[0][sidx(0, 1, 1)], g[9]);
             if (own(1, 0, 1)) lds_add(&acc[1][sidx(1, 0, 1)], g[10]);
             if (own(1, 1, 0)) lds_add(&acc[2][sidx(1, 1, 0)], g[11]);
-        } else {
+        }
+        if constexpr (KIND == DEP_TIJ || KIND == DEP_T00TIJ) {
             // projection_Tij functor, gevolution.hpp:2582-2965 (no CIC_PROJECT_TIJ):
             // diagonal parts CIC onto the 8 corners, off-diagonal parts onto 2 sites
             const T vv[3] = {qu[u][0], qu[u][1], qu[u][2]};
@@ -216,7 +222,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
             } else {
                 f = o;
             }
-            constexpr int dcomp[3] = {0, 3, 5};
+            constexpr int dcomp[3] = {TB + 0, TB + 3, TB + 5};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const T w = f * vv[d] * vv[d] * ie;
@@ -229,14 +235,14 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
                 }
             }
             T w = f * vv[0] * vv[1] * ie;
-            if (own(0, 0, 0)) lds_add(&acc[1][sidx(0, 0, 0)], w * (o - rd[2]));
-            if (own(0, 0, 1)) lds_add(&acc[1][sidx(0, 0, 1)], w * rd[2]);
+            if (own(0, 0, 0)) lds_add(&acc[TB + 1][sidx(0, 0, 0)], w * (o - rd[2]));
+            if (own(0, 0, 1)) lds_add(&acc[TB + 1][sidx(0, 0, 1)], w * rd[2]);
             w = f * vv[0] * vv[2] * ie;
-            if (own(0, 0, 0)) lds_add(&acc[2][sidx(0, 0, 0)], w * (o - rd[1]));
-            if (own(0, 1, 0)) lds_add(&acc[2][sidx(0, 1, 0)], w * rd[1]);
+            if (own(0, 0, 0)) lds_add(&acc[TB + 2][sidx(0, 0, 0)], w * (o - rd[1]));
+            if (own(0, 1, 0)) lds_add(&acc[TB + 2][sidx(0, 1, 0)], w * rd[1]);
             w = f * vv[1] * vv[2] * ie;
-            if (own(0, 0, 0)) lds_add(&acc[4][sidx(0, 0, 0)], w * (o - rd[0]));
-            if (own(1, 0, 0)) lds_add(&acc[4][sidx(1, 0, 0)], w * rd[0]);
+            if (own(0, 0, 0)) lds_add(&acc[TB + 4][sidx(0, 0, 0)], w * (o - rd[0]));
+            if (own(1, 0, 0)) lds_add(&acc[TB + 4][sidx(1, 0, 0)], w * rd[0]);
         }
         }
     }
@@ -247,8 +253,8 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
         const uint32_t gs = (uint32_t)(x0 + sx) + (uint32_t)N * ((uint32_t)(y0 + sy) + (uint32_t)N * (uint32_t)(z0 + sz));
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            T *oc = out + c * M;
-            T val = (T)(acc[c][s] * mass_d);
+            T *oc = (KIND == DEP_T00TIJ && c > 0) ? out2 + (c - 1) * M : out + c * M;
+            T val = (T)(acc[c][s] * ((KIND == DEP_T00TIJ && c > 0) ? mass2_d : mass_d));
             if (accumulate) val += oc[gs];
             oc[gs] = val;
         }
@@ -256,25 +262,27 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, int64_t M, c
 }
 
 template <typename T, int KIND, int TX, int TY, int TZ>
-static int launch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
+static int launch_dep(gev_pcls *p, T *out, T *out2, const T *phi, double mass, double mass2, double a, int acc)
 {
     gev_ctx *c = p->ctx;
     const int N = c->N;
     const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
-    hipLaunchKernelGGL((k_deposit<T, KIND, TX, TY, TZ>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), out, c->M,
-                       phi, mass, a, acc);
+    hipLaunchKernelGGL((k_deposit<T, KIND, TX, TY, TZ>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), out, out2,
+                       c->M, phi, mass, mass2, a, acc);
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }
 
-template <typename T, int KIND> static int dispatch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
+template <typename T, int KIND>
+static int dispatch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc, T *out2 = nullptr,
+                        double mass2 = 0.0)
 {
     const int N = p->ctx->N;
-    // Tij keeps 6 FP64 accumulators per site: the smaller tile holds 4 blocks per CU
-    constexpr bool big = KIND != DEP_TIJ;
-    if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, phi, mass, a, acc);
-    if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, phi, mass, a, acc);
-    return launch_dep<T, KIND, 4, 4, 4>(p, out, phi, mass, a, acc);
+    // Tij keeps 6 (T00+Tij 7) FP64 accumulators per site: the smaller tile holds 4 blocks per CU
+    constexpr bool big = KIND != DEP_TIJ && KIND != DEP_T00TIJ;
+    if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, out2, phi, mass, mass2, a, acc);
+    if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, out2, phi, mass, mass2, a, acc);
+    return launch_dep<T, KIND, 4, 4, 4>(p, out, out2, phi, mass, mass2, a, acc);
 }
 
 template <typename T> int dep_T00_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
@@ -292,7 +300,14 @@ template <typename T> int dep_Tij_impl(gev_pcls *p, T *out, const T *phi, double
     return dispatch_dep<T, DEP_TIJ>(p, out, phi, mass, a, acc);
 }
 
+template <typename T>
+int dep_T00_Tij_impl(gev_pcls *p, T *T00, T *Tij, const T *phi, double mass00, double massij, double a, int acc)
+{
+    return dispatch_dep<T, DEP_T00TIJ>(p, T00, phi, mass00, a, acc, Tij, massij);
+}
+
 #define INST(T)                                                                   \
+    template int dep_T00_Tij_impl<T>(gev_pcls *, T *, T *, const T *, double, double, double, int); \
     template int dep_T00_impl<T>(gev_pcls *, T *, const T *, double, double, int); \
     template int dep_T0i_impl<T>(gev_pcls *, T *, const T *, double, int);         \
     template int dep_Tij_impl<T>(gev_pcls *, T *, const T *, double, double, int);

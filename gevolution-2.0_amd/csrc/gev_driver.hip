@@ -115,48 +115,43 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
     RC(mark(c, -1));
 
     if (st->gr) {
-        // T00 with the old phi (main.cu:622-651), Sij zeroed by the Tij deposit itself
-        RC(gev_project_T00(p, f->source, a, f->phi, 1.0, 0));
-        RC(mark(c, GEV_PHASE_DEPOSIT_T00));
+        // T00 (main.cu:622-651) and Tij (main.cu:750-758), both with the old phi,
+        // in one pass over the particles (the outputs need no zeroing)
+        RC(gev_project_T00_Tij(p, f->source, f->Sij, a, f->phi, 1.0, 0));
+        RC(mark(c, GEV_PHASE_DEPOSIT_T00_TIJ));
         const double H = gev_hconf(a, fpg, om, ol, orad);
+        double s_res = 0;
         if (dtau_old > 0.) {
-            double s_src = 0, s_res = 0;
+            double s_src = 0;
             RC(gev_prepare_source_T00(c, f->phi, f->chi, f->source, om, f->source, 3. * H * dx * dx / dtau_old,
                                       fpg * dx * dx / a, 3. * H * H * dx * dx, &s_src, &s_res));
             st->T00hom = s_src / (double)M;
-            RC(gev_fft_r2c(c, f->source, f->scalarFT, 1));
-            if (mixed) RC(gev_set_k0(c, f->scalarFT, s_res, 0.0)); // FP64 k=0 mode
             RC(mark(c, GEV_PHASE_SOURCE_T00));
         }
-        // Tij with the old phi (main.cu:750-758)
-        RC(gev_project_Tij(p, f->Sij, a, f->phi, 1.0, 0));
-        RC(mark(c, GEV_PHASE_DEPOSIT_TIJ));
         if (dtau_old > 0.) {
-            RC(gev_poisson(c, f->scalarFT, f->scalarFT, 1. / (dx * dx), 3. * H / dtau_old)); // main.cu:769
-            RC(gev_fft_c2r(c, f->scalarFT, f->phi, 1));
+            // main.cu:742-794; the mixed form takes the k=0 mode from the FP64 sum
+            RC(gev_solve_poisson(c, f->source, f->phi, f->scalarFT, 1. / (dx * dx), 3. * H / dtau_old, mixed ? 1 : 0,
+                                 s_res));
             RC(mark(c, GEV_PHASE_SOLVE_PHI));
         }
         // chi (main.cu:822-902) and B (main.cu:908-955)
         RC(gev_prepare_source_Tij(c, f->phi, f->Sij, f->Sij, 2. * fpg * dx * dx / a));
-        RC(gev_fft_r2c(c, f->Sij, f->SijFT, 6));
         RC(mark(c, GEV_PHASE_SOURCE_TIJ));
         if (st->vector_elliptic) {
             RC(gev_project_T0i(p, f->Bi, f->phi, 1.0, 0));
             RC(mark(c, GEV_PHASE_DEPOSIT_T0I));
-        }
-        RC(gev_project_ft_scalar(c, f->SijFT, f->scalarFT, 0));
-        RC(gev_fft_c2r(c, f->scalarFT, f->chi, 1));
-        RC(mark(c, GEV_PHASE_SOLVE_CHI));
-        if (st->vector_elliptic) {
-            RC(gev_fft_r2c(c, f->Bi, f->BiFT, 3));
-            RC(gev_project_ft_vector(c, f->BiFT, f->BiFT, fpg * dx * dx, 0.0));
+            RC(gev_solve_chi(c, f->Sij, f->chi, f->SijFT, f->scalarFT));
+            RC(mark(c, GEV_PHASE_SOLVE_CHI));
+            RC(gev_solve_vector(c, f->Bi, f->BiFT, fpg * dx * dx, 0.0));
         } else {
+            // parabolic: SijFT feeds evolveFTvector, B persists in k-space
+            RC(gev_fft_r2c(c, f->Sij, f->SijFT, 6));
+            RC(gev_project_ft_scalar(c, f->SijFT, f->scalarFT, 0));
+            RC(gev_fft_c2r(c, f->scalarFT, f->chi, 1));
+            RC(mark(c, GEV_PHASE_SOLVE_CHI));
             RC(gev_evolve_ft_vector(c, f->SijFT, f->BiFT, a * a * dtau_old));
-        }
-        RC(gev_fft_c2r(c, f->BiFT, f->Bi, 3));
-        if (!st->vector_elliptic) {
-            // c2r clobbers its input; parabolic B must persist in k-space
-            RC(gev_fft_r2c(c, f->Bi, f->BiFT, 3));
+            RC(gev_fft_c2r(c, f->BiFT, f->Bi, 3));
+            RC(gev_fft_r2c(c, f->Bi, f->BiFT, 3));  // c2r clobbers its input
         }
         RC(mark(c, GEV_PHASE_SOLVE_B));
         // kick (main.cu:1227-1240), half background step, drift (:1253-1266):
@@ -175,7 +170,7 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
         st->maxvel = mv / sqrt(mv * mv + 1.0); // main.cu:1279-1283
     } else {
         RC(gev_project_mass(p, f->source, 1.0, 0));
-        RC(mark(c, GEV_PHASE_DEPOSIT_T00));
+        RC(mark(c, GEV_PHASE_DEPOSIT_T00_TIJ));
         RC(gev_fft_r2c(c, f->source, f->scalarFT, 1));
         RC(gev_poisson(c, f->scalarFT, f->scalarFT, fpg / a, 0.0)); // main.cu:765
         RC(gev_fft_c2r(c, f->scalarFT, f->phi, 1));

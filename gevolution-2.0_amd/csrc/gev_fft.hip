@@ -2,9 +2,12 @@
 // passes: x (real<->half-complex, contiguous lines), then y and z (complex,
 // strided columns read as TX-wide coalesced row segments).  Radix-8/4/2
 // Stockham stages in LDS, twiddles from a per-size table in global memory.
+#include <type_traits>
+
 #include "gev_common.hpp"
 #include "gev_fft_engine.hpp"
 #include "gev_kernels.hpp"
+#include "gev_kops.hpp"
 
 namespace gev {
 
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(TX * L / 8) void k_col_reg(cx<T> *__restrict__ data
     const int64_t stride = (axis == 1) ? (int64_t)P : (int64_t)P * L;
     cx<T> *base = data + (int64_t)ct * TX + ((axis == 1) ? (int64_t)P * L * o : (int64_t)P * o);
     auto ld = [&](int line, int row) { return base[(int64_t)row * stride + line]; };
-    auto st = [&](int line, int row, cx<T> v) { base[(int64_t)row * stride + line] = v; };
+    auto st = [&](int line, int row, cx<T> v, int) { base[(int64_t)row * stride + line] = v; };
     __syncthreads();
     reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, dir, ld, st);
 }
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(const T *__restrict__
     const int64_t line0 = (int64_t)blockIdx.x * NL;
     const cx<T> *src = reinterpret_cast<const cx<T> *>(in) + line0 * H;
     auto ld = [&](int line, int n) { return (line0 + line < nlines) ? src[line * H + n] : mk((T)0, (T)0); };
-    auto st = [&](int line, int k, cx<T> v) { lds[line * LDL + k] = v; };
+    auto st = [&](int line, int k, cx<T> v, int) { lds[line * LDL + k] = v; };
     __syncthreads();
     reg_fft<T, H, NL, false, 2, true, 1>(lds, twl, -1, ld, st);
     __syncthreads();
@@ -244,7 +247,7 @@ __global__ __launch_bounds__(NL * N / 16) void k_c2r_x_reg(const cx<T> *__restri
         return E + mul_i(O, +1);
     };
     cx<T> *dst = reinterpret_cast<cx<T> *>(out) + line0 * H;
-    auto st = [&](int line, int n, cx<T> v) {
+    auto st = [&](int line, int n, cx<T> v, int) {
         if (line0 + line < nlines) dst[line * H + n] = v;
     };
     reg_fft<T, H, NL, false, 2, false, 1>(lds, twl, +1, ld, st);
@@ -345,6 +348,219 @@ int fft_c2r_impl(gev_ctx *ctx, cx<T> *in, T *out, int ncomp, const cx<T> *tw)
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }
+
+// ===================================== z pass fused with a k-space solve ====
+// Forward z transforms of the NIN input components of TX adjacent kx columns
+// (one ky), the per-mode operator applied in registers, inverse z transforms
+// of the NOUT outputs.  Replaces [z pass, k-space kernel, inverse z pass]:
+// every mode is read once and written once instead of three times each.
+// In place (in == out) is allowed: a workgroup reads all its columns before
+// writing them and no two workgroups share a column.
+enum { ZOP_POISSON = 0, ZOP_SCALAR = 1, ZOP_VECTOR = 2 };
+
+struct ZParams {
+    const double *gk2;
+    const cx<double> *ks;
+    double coeff, modif;  // Poisson: coeff pre-scaled by -1/N^3; vector: as projectFTvector
+    double k0re, k0im;    // Poisson: value taken for the k=0 mode when k0set
+    int k0set;
+};
+
+template <typename T, int L, int TX, int OP>
+__global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *out, int P, int64_t NK,
+                                                       const cx<T> *__restrict__ tw, ZParams zp)
+{
+    constexpr int NT = TX * L / 8, LDL = L + 1;
+    constexpr int NIN = OP == ZOP_SCALAR ? 6 : (OP == ZOP_VECTOR ? 3 : 1);
+    constexpr int NOUT = OP == ZOP_VECTOR ? 3 : 1;
+    constexpr int NV = OP == ZOP_VECTOR ? 3 : 1;  // inputs held at once
+    __shared__ cx<T> lds[TX * LDL];
+    __shared__ cx<T> twl[L];
+    for (int m = threadIdx.x; m < L; m += NT) twl[m] = tw[m];
+    const int ntiles = P / TX;
+    const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int ct = (int)(b % ntiles);
+    const int ky = (int)(b / ntiles);
+    const int64_t stride = (int64_t)P * L;
+    const int64_t off = (int64_t)ct * TX + (int64_t)P * ky;
+    const int H = L / 2;
+    int lines[8], poss[8];
+    cx<T> vin[NV][8];
+    cx<double> acc[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) acc[s] = mk(0.0, 0.0);
+    const double gy = zp.gk2[ky];
+    const cx<double> kyv = zp.ks[ky];
+    // forward z transform of input component c (a compile-time index when the
+    // operator holds all inputs at once, so they stay in registers)
+    auto forward = [&](int c, auto held) {
+        constexpr int CH = decltype(held)::value;
+        const cx<T> *src = in + c * NK + off;
+        auto ld = [&](int line, int row) { return src[(int64_t)row * stride + line]; };
+        auto st = [&](int line, int row, cx<T> v, int slot) {
+            lines[slot] = line;
+            poss[slot] = row;
+            if constexpr (OP == ZOP_SCALAR) {
+                const int kx = ct * TX + line;
+                if (kx <= H) {
+                    const cx<double> w =
+                        kop_scalar_weight(c, zp.gk2[kx], gy, zp.gk2[row], zp.ks[kx], kyv, zp.ks[row]);
+                    acc[slot] = acc[slot] + w * mk((double)v.x, (double)v.y);
+                }
+            } else {
+                vin[CH][slot] = v;
+            }
+        };
+        __syncthreads();  // twiddles loaded; previous transform done with lds
+        reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, -1, ld, st);
+    };
+    if constexpr (OP == ZOP_VECTOR) {
+        forward(0, std::integral_constant<int, 0>{});
+        forward(1, std::integral_constant<int, 1>{});
+        forward(2, std::integral_constant<int, 2>{});
+    } else {
+#pragma unroll 1
+        for (int c = 0; c < NIN; c++) forward(c, std::integral_constant<int, 0>{});
+    }
+    // the operator, per owned mode; the outputs replace the inputs in vin
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int kx = ct * TX + lines[s], kz = poss[s];
+        const bool pad = kx > H;
+        const bool zero_mode = (kx == 0 && ky == 0 && kz == 0);
+        const double kk = pad ? 1.0 : zp.gk2[kx] + gy + zp.gk2[kz];
+        if constexpr (OP == ZOP_POISSON) {
+            cx<double> r = mk(0.0, 0.0);
+            if (!pad) {
+                cx<double> sv = mk((double)vin[0][s].x, (double)vin[0][s].y);
+                if (zero_mode && zp.k0set) sv = mk(zp.k0re, zp.k0im);
+                if (!(zero_mode && zp.modif == 0.0)) r = kop_poisson(sv, kk, zp.coeff, zp.modif);
+            }
+            vin[0][s] = mk((T)r.x, (T)r.y);
+        } else if constexpr (OP == ZOP_SCALAR) {
+            cx<double> r = mk(0.0, 0.0);
+            if (!pad && !zero_mode) r = acc[s] * kop_scalar_norm(kk, L);
+            vin[0][s] = mk((T)r.x, (T)r.y);
+        } else {
+            cx<double> bo[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+            if (!pad && !zero_mode) {
+                const cx<double> sv[3] = {mk((double)vin[0][s].x, (double)vin[0][s].y),
+                                          mk((double)vin[1][s].x, (double)vin[1][s].y),
+                                          mk((double)vin[2][s].x, (double)vin[2][s].y)};
+                kop_vector(sv, kk, zp.ks[kx], kyv, zp.ks[kz], zp.coeff, zp.modif, bo);
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) vin[j][s] = mk((T)bo[j].x, (T)bo[j].y);
+        }
+    }
+    // inverse z transform of output component j
+    auto inverse = [&](int j, auto held) {
+        constexpr int CH = decltype(held)::value;
+        __syncthreads();  // previous transform done with lds
+#pragma unroll
+        for (int s = 0; s < 8; s++) lds[lines[s] * LDL + poss[s]] = vin[CH][s];
+        __syncthreads();
+        cx<T> *dst = out + j * NK + off;
+        auto ld2 = [&](int line, int row) { return lds[line * LDL + row]; };
+        auto st2 = [&](int line, int row, cx<T> v, int) { dst[(int64_t)row * stride + line] = v; };
+        reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, +1, ld2, st2);
+    };
+    inverse(0, std::integral_constant<int, 0>{});
+    if constexpr (NOUT == 3) {
+        inverse(1, std::integral_constant<int, 1>{});
+        inverse(2, std::integral_constant<int, 2>{});
+    }
+}
+
+// at most 512 threads per workgroup: the operator keeps up to 3 x 8 modes per thread in registers
+template <typename T, int L> constexpr int zs_tx() { return (col_tx<T, L>() * L / 8 > 512) ? 4096 / L : col_tx<T, L>(); }
+
+template <typename T, int N, int OP>
+static void launch_zsolve(const cx<T> *in, cx<T> *out, int P, int64_t NK, const cx<T> *tw, const ZParams &zp,
+                          hipStream_t s)
+{
+    constexpr int TX = zs_tx<T, N>();
+    const int64_t nb = (int64_t)(P / TX) * N;
+    hipLaunchKernelGGL((k_zsolve<T, N, TX, OP>), dim3((unsigned)nb), dim3(TX * N / 8), 0, s, in, out, P, NK, tw, zp);
+}
+
+template <typename T> static const cx<T> *twiddles_of(gev_ctx *c)
+{
+    if constexpr (sizeof(T) == 8) return c->tw64;
+    else return c->tw32;
+}
+
+// forward x and y passes (r2c without the z pass) / inverse y and x passes
+template <typename T, int N> static void fwd_xy(gev_ctx *c, const T *in, cx<T> *out)
+{
+    launch_r2c<T, N>(in, out, c->P, (int64_t)N * N, twiddles_of<T>(c), c->stream);
+    launch_col<T, N>(out, c->P, 1, -1, twiddles_of<T>(c), c->stream);
+}
+template <typename T, int N> static void inv_yx(gev_ctx *c, cx<T> *in, T *out)
+{
+    launch_col<T, N>(in, c->P, 1, +1, twiddles_of<T>(c), c->stream);
+    launch_c2r<T, N>(in, out, c->P, (int64_t)N * N, twiddles_of<T>(c), c->stream);
+}
+
+template <typename T, int N>
+static void zsolve_chain(gev_ctx *c, int op, const T *rin, int nin, cx<T> *ftin, cx<T> *ftout, T *rout, int nout,
+                         const ZParams &zp)
+{
+    for (int i = 0; i < nin; i++) fwd_xy<T, N>(c, rin + i * c->M, ftin + i * c->NK);
+    if (op == ZOP_POISSON) launch_zsolve<T, N, ZOP_POISSON>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
+    else if (op == ZOP_SCALAR) launch_zsolve<T, N, ZOP_SCALAR>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
+    else launch_zsolve<T, N, ZOP_VECTOR>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
+    for (int j = 0; j < nout; j++) inv_yx<T, N>(c, ftout + j * c->NK, rout + j * c->M);
+}
+
+template <typename T>
+static int zsolve_dispatch(gev_ctx *c, int op, const T *rin, int nin, cx<T> *ftin, cx<T> *ftout, T *rout, int nout,
+                           const ZParams &zp)
+{
+    switch (c->N) {
+    case 64: zsolve_chain<T, 64>(c, op, rin, nin, ftin, ftout, rout, nout, zp); break;
+    case 128: zsolve_chain<T, 128>(c, op, rin, nin, ftin, ftout, rout, nout, zp); break;
+    case 256: zsolve_chain<T, 256>(c, op, rin, nin, ftin, ftout, rout, nout, zp); break;
+    case 512: zsolve_chain<T, 512>(c, op, rin, nin, ftin, ftout, rout, nout, zp); break;
+    case 1024: zsolve_chain<T, 1024>(c, op, rin, nin, ftin, ftout, rout, nout, zp); break;
+    default: gev_set_error("fused z-pass solve needs 64 <= N <= 1024"); return GEV_ERR_ARG;
+    }
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+bool zsolve_supported(const gev_ctx *c) { return c->N >= 64 && c->N <= 1024; }
+
+// pot = c2r(poisson(r2c(src))) with the k=0 mode optionally replaced first
+template <typename T>
+int solve_poisson_fused_impl(gev_ctx *c, const T *src, T *pot, cx<T> *work, double coeff, double modif, int k0set,
+                             double k0re)
+{
+    ZParams zp{c->gk2, c->kshift, coeff / -((double)c->N * (double)c->N * (double)c->N), modif, k0re, 0.0, k0set};
+    return zsolve_dispatch<T>(c, ZOP_POISSON, src, 1, work, work, pot, 1, zp);
+}
+
+// chi = c2r(projectFTscalar(r2c(Sij)))
+template <typename T> int solve_chi_fused_impl(gev_ctx *c, const T *Sij, T *chi, cx<T> *SijFT, cx<T> *chiFT)
+{
+    ZParams zp{c->gk2, c->kshift, 0.0, 0.0, 0.0, 0.0, 0};
+    return zsolve_dispatch<T>(c, ZOP_SCALAR, Sij, 6, SijFT, chiFT, chi, 1, zp);
+}
+
+// Bi = c2r(projectFTvector(r2c(Bi))), in place on Bi
+template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT, double coeff, double modif)
+{
+    ZParams zp{c->gk2, c->kshift, coeff, modif, 0.0, 0.0, 0};
+    return zsolve_dispatch<T>(c, ZOP_VECTOR, Bi, 3, BiFT, BiFT, Bi, 3, zp);
+}
+
+#define INSTZ(T)                                                                                       \
+    template int solve_poisson_fused_impl<T>(gev_ctx *, const T *, T *, cx<T> *, double, double, int, double); \
+    template int solve_chi_fused_impl<T>(gev_ctx *, const T *, T *, cx<T> *, cx<T> *);                  \
+    template int solve_vector_fused_impl<T>(gev_ctx *, T *, cx<T> *, double, double);
+INSTZ(double)
+INSTZ(float)
+#undef INSTZ
 
 template int fft_r2c_impl<double>(gev_ctx *, const double *, cx<double> *, int, const cx<double> *);
 template int fft_r2c_impl<float>(gev_ctx *, const float *, cx<float> *, int, const cx<float> *);

@@ -55,6 +55,9 @@ template <typename T, int R> __device__ __forceinline__ void dftR(cx<T> *v, int 
 
 // LF (line-fast): consecutive threads walk lines first (coalesced when lines
 // are adjacent columns in memory); otherwise they walk along a line.
+// The store functor is called as st(line, pos, value, slot) with slot in
+// [0, 8) a compile-time constant per call site: the thread's slot-th output,
+// always at the same (line, pos) for a given thread and L, NL, LF.
 // twl[m * TWS] = exp(-2 pi i m / L).  ST_LDS: the store functor writes the
 // same LDS buffer, so the last stage waits for all reads first.
 template <typename T, int L, int NL, bool LF, int TWS, bool ST_LDS, int NS, class LD, class ST>
@@ -96,7 +99,7 @@ __device__ __forceinline__ void reg_fft(cx<T> *lds, const cx<T> *twl, int dir, L
         for (int u = 0; u < BPT; u++) {
             const int j = jt * BPT + u;
 #pragma unroll
-            for (int r = 0; r < R; r++) st(line, (j / NS) * NS * R + (j % NS) + r * NS, v[u][r]);
+            for (int r = 0; r < R; r++) st(line, (j / NS) * NS * R + (j % NS) + r * NS, v[u][r], u * R + r);
         }
     } else {
         __syncthreads();

@@ -18,6 +18,15 @@ int ic_displacement_ft_impl(gev_ctx *ctx, uint32_t seed, int ksphere, double coe
 
 template <typename T> int set_k0_impl(gev_ctx *ctx, cx<T> *ft, double re, double im);
 
+// r2c -> k-space operator -> c2r with the operator applied inside the z pass
+// (64 <= N <= 1024); the FT arguments are work space of the usual layout
+bool zsolve_supported(const gev_ctx *c);
+template <typename T>
+int solve_poisson_fused_impl(gev_ctx *c, const T *src, T *pot, cx<T> *work, double coeff, double modif, int k0set,
+                             double k0re);
+template <typename T> int solve_chi_fused_impl(gev_ctx *c, const T *Sij, T *chi, cx<T> *SijFT, cx<T> *chiFT);
+template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT, double coeff, double modif);
+
 template <typename T>
 int prepare_T00_impl(gev_ctx *ctx, const T *phi, const T *chi, const T *src, double bg, T *res, double c1, double c2,
                      double c3, double *sums);

@@ -4,6 +4,7 @@
 // form stores FP32 but builds the chi constraint combination in FP64.
 #include "gev_common.hpp"
 #include "gev_kernels.hpp"
+#include "gev_kops.hpp"
 
 namespace gev {
 
@@ -49,9 +50,7 @@ __global__ __launch_bounds__(KS_NT) void k_poisson(const cx<T> *__restrict__ src
         pot[e] = mk((T)0, (T)0);
         return;
     }
-    double f = coeff / (gk2[k.kx] + gk2[k.ky] + gk2[k.kz] + modif);
-    cx<double> s = ld(src, e);
-    st(pot, e, s * f);
+    st(pot, e, kop_poisson(ld(src, e), gk2[k.kx] + gk2[k.ky] + gk2[k.kz], coeff, modif));
 }
 
 // projectFTscalar, gevolution.hpp:251-340
@@ -70,13 +69,10 @@ __global__ __launch_bounds__(KS_NT) void k_project_ft_scalar(const cx<T> *__rest
     }
     double g0 = gk2[k.kx], g1 = gk2[k.ky], g2 = gk2[k.kz];
     cx<double> k0 = ks[k.kx], k1 = ks[k.ky], k2 = ks[k.kz];
-    double kk = g0 + g1 + g2;
-    cx<double> acc = ld(S, e) * (g1 + g2 - 2.0 * g0) + ld(S, e + 3 * NK) * (g0 + g2 - 2.0 * g1) +
-                     ld(S, e + 5 * NK) * (g0 + g1 - 2.0 * g2);
-    acc = acc - (k0 * k1 * ld(S, e + NK)) * 6.0;
-    acc = acc - (k0 * k2 * ld(S, e + 2 * NK)) * 6.0;
-    acc = acc - (k1 * k2 * ld(S, e + 4 * NK)) * 6.0;
-    cx<double> r = acc * (1.0 / (2.0 * kk * kk * (double)N));
+    cx<double> acc = mk(0.0, 0.0);
+#pragma unroll
+    for (int c = 0; c < 6; c++) acc = acc + kop_scalar_weight(c, g0, g1, g2, k0, k1, k2) * ld(S, e + c * NK);
+    cx<double> r = acc * kop_scalar_norm(g0 + g1 + g2, N);
     if (add) r = r + ld(chi, e);
     st(chi, e, r);
 }
@@ -98,12 +94,12 @@ __global__ __launch_bounds__(KS_NT) void k_project_ft_vector(const cx<T> *__rest
     }
     double kk = gk2[k.kx] + gk2[k.ky] + gk2[k.kz];
     cx<double> k0 = ks[k.kx], k1 = ks[k.ky], k2 = ks[k.kz];
-    cx<double> s0 = ld(S, e), s1 = ld(S, e + NK), s2 = ld(S, e + 2 * NK);
-    cx<double> tmp = (k0 * s0 + k1 * s1 + k2 * s2) * (1.0 / kk);
-    double f = 4.0 * coeff / (kk + modif);
-    st(B, e, (s0 - conj(k0) * tmp) * f);
-    st(B, e + NK, (s1 - conj(k1) * tmp) * f);
-    st(B, e + 2 * NK, (s2 - conj(k2) * tmp) * f);
+    const cx<double> sv[3] = {ld(S, e), ld(S, e + NK), ld(S, e + 2 * NK)};
+    cx<double> b[3];
+    kop_vector(sv, kk, k0, k1, k2, coeff, modif, b);
+    st(B, e, b[0]);
+    st(B, e + NK, b[1]);
+    st(B, e + 2 * NK, b[2]);
 }
 
 // evolveFTvector, gevolution.hpp:359-429

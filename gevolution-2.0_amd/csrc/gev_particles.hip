@@ -294,7 +294,7 @@ template <typename T> int pcls_sort_counted_impl(gev_pcls *p)
 template <typename T> int pcls_sort_impl(gev_pcls *p)
 {
     gev_ctx *ctx = p->ctx;
-    const int64_t M = ctx->M, n = p->n;
+    const int64_t n = p->n;
     int rc = pcls_clear_counts(p);
     if (rc) return rc;
     if (n > 0)

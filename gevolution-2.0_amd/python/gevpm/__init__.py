@@ -132,6 +132,32 @@ class Context:
         self.bind_stream()
         check(lib.gev_set_k0(self._h, _ptr(F), re, im), "gev_set_k0")
 
+    # ------------------------------------ fused real -> k -> real solves
+    def solve_poisson(self, src, coeff, modif=0.0, k0=None, out=None, work=None):
+        """c2r(poisson(r2c(src))), optionally with the k=0 mode set to k0 first"""
+        out = torch.empty_like(src) if out is None else out
+        work = self.fourier(1) if work is None else work
+        self.bind_stream()
+        check(lib.gev_solve_poisson(self._h, _ptr(src), _ptr(out), _ptr(work), coeff, modif, int(k0 is not None),
+                                    0.0 if k0 is None else float(k0)), "gev_solve_poisson")
+        return out
+
+    def solve_chi(self, Sij, out=None, work6=None, work1=None):
+        """c2r(projectFTscalar(r2c(Sij)))"""
+        out = self.real(1) if out is None else out
+        work6 = self.fourier(6) if work6 is None else work6
+        work1 = self.fourier(1) if work1 is None else work1
+        self.bind_stream()
+        check(lib.gev_solve_chi(self._h, _ptr(Sij), _ptr(out), _ptr(work6), _ptr(work1)), "gev_solve_chi")
+        return out
+
+    def solve_vector(self, Bi, coeff=1.0, modif=0.0, work=None):
+        """Bi <- c2r(projectFTvector(r2c(Bi))), in place"""
+        work = self.fourier(3) if work is None else work
+        self.bind_stream()
+        check(lib.gev_solve_vector(self._h, _ptr(Bi), _ptr(work), coeff, modif), "gev_solve_vector")
+        return Bi
+
     # ----------------------------------------------------------- sources
     def prepare_source_T00(self, phi, chi, source, bg, c1, c2, c3, result=None):
         result = torch.empty_like(source) if result is None else result
@@ -268,6 +294,13 @@ class Particles:
         check(lib.gev_project_Tij(self._h, _ptr(out), a, _ptr(phi), coeff, int(accumulate)), "gev_project_Tij")
         return out
 
+    def project_T00_Tij(self, T00, Tij, a=1.0, phi=None, coeff=1.0, accumulate=False):
+        """project_T00 and project_Tij of the same particles in one pass"""
+        self.ctx.bind_stream()
+        check(lib.gev_project_T00_Tij(self._h, _ptr(T00), _ptr(Tij), a, _ptr(phi), coeff, int(accumulate)),
+              "gev_project_T00_Tij")
+        return T00, Tij
+
     # ------------------------------------------------------------ updates
     def update_q(self, dtau, phi, chi, Bi, nfield, a, bcoeff):
         r = C.c_double()
@@ -393,8 +426,8 @@ def step(pcls, fields, state):
     return state
 
 
-PHASES = ["deposit_T00", "source_T00", "deposit_Tij", "solve_phi", "source_Tij", "deposit_T0i", "solve_chi",
-          "solve_B", "kick_drift", "rebin"]
+PHASES = ["deposit_T00_Tij", "source_T00", "solve_phi", "source_Tij", "deposit_T0i", "solve_chi", "solve_B",
+          "kick_drift", "rebin"]
 
 
 def step_timing(ctx, enable=True):

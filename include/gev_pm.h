@@ -93,6 +93,10 @@ int gev_project_mass(gev_pcls *p, void *rho, double coeff, int accumulate);
 int gev_project_T0i(gev_pcls *p, void *T0i, const void *phi, double coeff, int accumulate);
 /* projection_Tij_project, gevolution.hpp:2999-3033 (functor :2582-2965), no CIC_PROJECT_TIJ */
 int gev_project_Tij(gev_pcls *p, void *Tij, double a, const void *phi, double coeff, int accumulate);
+/* projection_T00_project and projection_Tij_project of the same particles
+ * with the same phi and a (main.cu:623 and 756) in one pass over the
+ * particles; no single reference function.  Results as the two calls. */
+int gev_project_T00_Tij(gev_pcls *p, void *T00, void *Tij, double a, const void *phi, double coeff, int accumulate);
 
 /* ---- real-space sources */
 /* prepareFTsource (T00 form, PHINONLINEAR), gevolution.hpp:160-218.
@@ -114,6 +118,20 @@ int gev_project_ft_vector(gev_ctx *ctx, const void *Si, void *Bi, double coeff, 
 int gev_evolve_ft_vector(gev_ctx *ctx, const void *Sij, void *Bi, double a2dtau);       /* gevolution.hpp:359-429 */
 /* overwrite the k=0 mode (used to restore the FP64 mean in the mixed form) */
 int gev_set_k0(gev_ctx *ctx, void *ft, double re, double im);
+
+/* ---- fused real-space solves.  No single reference function: each is one
+ *      main.cu call sequence with the k-space operator applied inside the z
+ *      pass of the transforms (64 <= N <= 1024; smaller N runs the sequence
+ *      unfused).  Work buffers are Fourier fields of the usual layout; their
+ *      content is undefined afterwards. */
+/* plan_source FFT_FORWARD, [k=0 := k0_re], solveModifiedPoissonFT, plan_phi
+ * FFT_BACKWARD (main.cu:742-794) */
+int gev_solve_poisson(gev_ctx *ctx, const void *src, void *pot, void *workFT, double coeff, double modif, int set_k0,
+                      double k0_re);
+/* plan_Sij FFT_FORWARD, projectFTscalar (add = 0), plan_chi FFT_BACKWARD (main.cu:829-895) */
+int gev_solve_chi(gev_ctx *ctx, const void *Sij, void *chi, void *SijFT, void *chiFT);
+/* plan_Bi FFT_FORWARD, projectFTvector, plan_Bi FFT_BACKWARD (main.cu:918-948), Bi in place */
+int gev_solve_vector(gev_ctx *ctx, void *Bi, void *BiFT, double coeff, double modif);
 
 /* ---- particle updates (perfParticles::updateVel / moveParticles with the
  *      gevolution functors, main.cu:1227-1266).  *max_ret receives the
@@ -178,9 +196,9 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st);
  * step, ids[i] (a GEV_PHASE_* value) and the milliseconds ms[i] spent in
  * that phase.  Returns the number of phases written (<= max). */
 enum {
-    GEV_PHASE_DEPOSIT_T00 = 0, GEV_PHASE_SOURCE_T00 = 1, GEV_PHASE_DEPOSIT_TIJ = 2, GEV_PHASE_SOLVE_PHI = 3,
-    GEV_PHASE_SOURCE_TIJ = 4, GEV_PHASE_DEPOSIT_T0I = 5, GEV_PHASE_SOLVE_CHI = 6, GEV_PHASE_SOLVE_B = 7,
-    GEV_PHASE_KICK_DRIFT = 8, GEV_PHASE_REBIN = 9, GEV_PHASE_COUNT = 10
+    GEV_PHASE_DEPOSIT_T00_TIJ = 0, /* GR: T00 and Tij in one pass; Newton: the mass deposit */
+    GEV_PHASE_SOURCE_T00 = 1, GEV_PHASE_SOLVE_PHI = 2, GEV_PHASE_SOURCE_TIJ = 3, GEV_PHASE_DEPOSIT_T0I = 4,
+    GEV_PHASE_SOLVE_CHI = 5, GEV_PHASE_SOLVE_B = 6, GEV_PHASE_KICK_DRIFT = 7, GEV_PHASE_REBIN = 8, GEV_PHASE_COUNT = 9
 };
 int gev_step_timing(gev_ctx *ctx, int enable);
 int gev_step_timings(gev_ctx *ctx, int *ids, float *ms, int max);

@@ -238,6 +238,12 @@ def test_deposits_fp64(gp, O, N, n, cluster):
     for ph in (None, phi):
         p.project_Tij(o6, a, None if ph is None else gphi, 0.9)
         assert relmax(real_np(o6), O.project_Tij(N, pos, q, m, a=a, phi=ph, coeff=0.9)) < 1e-13
+    # T00 and Tij in one pass
+    o1 = ctx.real(1)
+    for ph in (None, phi):
+        p.project_T00_Tij(o1, o6, a, None if ph is None else gphi, 0.9)
+        assert relmax(real_np(o1)[0], O.project_T00(N, pos, q, m, a=a, phi=ph, coeff=0.9)) < 1e-13
+        assert relmax(real_np(o6), O.project_Tij(N, pos, q, m, a=a, phi=ph, coeff=0.9)) < 1e-13
 
 
 def test_deposits_empty_and_single(gp, O):
@@ -273,6 +279,10 @@ def test_deposits_mixed(gp, O, N):
     assert relmax(real_np(out)[0], O.project_T00(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
     o6 = ctx.real(6)
     p.project_Tij(o6, a, field_gpu(ctx, phi))
+    assert relmax(real_np(o6), O.project_Tij(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
+    o1 = ctx.real(1)
+    p.project_T00_Tij(o1, o6, a, field_gpu(ctx, phi))
+    assert relmax(real_np(o1)[0], O.project_T00(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
     assert relmax(real_np(o6), O.project_Tij(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
 
 
@@ -437,3 +447,35 @@ def test_power_spectrum_vs_oracle(gp, O, N):
     for got, want in zip(res[:4], ref[:4]):
         assert relmax(got.numpy(), want) < 1e-12
     assert np.array_equal(res[4].numpy(), ref[4])
+
+
+# --------------------------------------------- fused z-pass solves
+@pytest.mark.parametrize("N,prec", [(64, "fp64"), (128, "fp64"), (64, "mixed"), (256, "mixed")])
+def test_fused_solves_vs_unfused(gp, N, prec):
+    """gev_solve_poisson / _chi / _vector (operator inside the z pass) against
+    the separate r2c -> k-space kernel -> c2r sequence on the same inputs"""
+    ctx = gp.Context(N, gp.GEV_FP64 if prec == "fp64" else gp.GEV_MIXED)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    g = torch.Generator(device="cpu").manual_seed(N)
+    src = torch.randn(1, N, N, N, generator=g, dtype=torch.float64).to(ctx.rdtype).to(ctx.device)
+    # Poisson with modif (GR form) and with the k=0 mode replaced
+    coeff, modif, k0 = N * N * 1.0, 0.37, 1234.5
+    F = ctx.fft_r2c(src)
+    ctx.set_k0(F, k0)
+    ref = ctx.fft_c2r(ctx.poisson(F, coeff, modif), preserve=False)
+    got = ctx.solve_poisson(src, coeff, modif, k0=k0)
+    assert relmax(to_np(got), to_np(ref)) < tol
+    # Poisson, Newton form (modif = 0, k=0 dropped)
+    ref = ctx.fft_c2r(ctx.poisson(ctx.fft_r2c(src), 0.8), preserve=False)
+    got = ctx.solve_poisson(src, 0.8)
+    assert relmax(to_np(got), to_np(ref)) < tol
+    # chi from a random symmetric tensor
+    S = torch.randn(6, N, N, N, generator=g, dtype=torch.float64).to(ctx.rdtype).to(ctx.device)
+    ref = ctx.fft_c2r(ctx.project_ft_scalar(ctx.fft_r2c(S)), preserve=False)
+    got = ctx.solve_chi(S)
+    assert relmax(to_np(got), to_np(ref)) < tol
+    # B from a random vector source, in place
+    Bsrc = torch.randn(3, N, N, N, generator=g, dtype=torch.float64).to(ctx.rdtype).to(ctx.device)
+    ref = ctx.fft_c2r(ctx.project_ft_vector(ctx.fft_r2c(Bsrc), 0.25, 0.0), preserve=False)
+    got = ctx.solve_vector(Bsrc.clone(), 0.25, 0.0)
+    assert relmax(to_np(got), to_np(ref)) < tol

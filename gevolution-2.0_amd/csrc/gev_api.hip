@@ -35,6 +35,11 @@ int kick_drift_impl(gev_pcls *p, int gr, double dtau_k, double dtau_d, const T *
 template <typename T> int pcls_sort_counted_impl(gev_pcls *p);
 template <typename T> int displace_impl(gev_pcls *p, const T *xi, double coeff, double *max_disp);
 template <typename T> int initialize_q_impl(gev_pcls *p, const T *pot, double coeff, double *max_ret);
+template <typename T> int adj_kick_deposit_impl(gev_pcls *p, const T *lx, T *lq, double C, T *u);
+template <typename T> int adj_divergence_impl(gev_ctx *c, const T *u, T *lphi, double K);
+template <typename T>
+int adj_gather_impl(gev_pcls *p, const T *phi, const T *lrho, const T *lq, T *lx, double K, double mN);
+template <typename T> int adj_ic_lattice_impl(gev_ctx *c, const T *lx, const T *lq, double cx, double cq, T *lpsi);
 }  // namespace gev
 
 using namespace gev;
@@ -386,6 +391,97 @@ int gev_evolve_ft_vector(gev_ctx *c, const void *S, void *B, double a2dtau)
     CHECK(c && S && B, "gev_evolve_ft_vector: NULL argument");
     return PREC_DISPATCH(c, evolve_ft_vector_impl<double>(c, (const cx<double> *)S, (cx<double> *)B, a2dtau),
                          evolve_ft_vector_impl<float>(c, (const cx<float> *)S, (cx<float> *)B, a2dtau));
+}
+
+// ---- container checkpoints (device memory of gev_pcls_state_bytes bytes)
+static int64_t pcls_state_bytes(const gev_pcls *p)
+{
+    const int64_t es = (p->ctx->prec == GEV_FP64) ? 8 : 4;
+    return 8 + 6 * p->n * es + 12 * p->n + 4 * (p->ctx->M + 1);
+}
+
+int gev_pcls_state_bytes(gev_pcls *p, int64_t *bytes)
+{
+    CHECK(p && bytes, "gev_pcls_state_bytes: NULL argument");
+    *bytes = pcls_state_bytes(p);
+    return GEV_OK;
+}
+
+int gev_pcls_save(gev_pcls *p, void *dst)
+{
+    CHECK(p && dst, "gev_pcls_save: NULL argument");
+    NEED_SORTED(p);
+    const int64_t n = p->n, es = (p->ctx->prec == GEV_FP64) ? 8 : 4;
+    hipStream_t s = p->ctx->stream;
+    char *d = (char *)dst;
+    GEV_HIP(hipMemcpyAsync(d, &p->n, 8, hipMemcpyHostToDevice, s));
+    d += 8;
+    for (int k = 0; k < 3; k++, d += n * es) GEV_HIP(hipMemcpyAsync(d, p->pos[k], n * es, hipMemcpyDeviceToDevice, s));
+    for (int k = 0; k < 3; k++, d += n * es) GEV_HIP(hipMemcpyAsync(d, p->q[k], n * es, hipMemcpyDeviceToDevice, s));
+    GEV_HIP(hipMemcpyAsync(d, p->id, 8 * n, hipMemcpyDeviceToDevice, s));
+    d += 8 * n;
+    GEV_HIP(hipMemcpyAsync(d, p->cell, 4 * n, hipMemcpyDeviceToDevice, s));
+    d += 4 * n;
+    GEV_HIP(hipMemcpyAsync(d, p->cell_start, 4 * (p->ctx->M + 1), hipMemcpyDeviceToDevice, s));
+    GEV_HIP(hipStreamSynchronize(s));  // the count went from host memory
+    return GEV_OK;
+}
+
+int gev_pcls_restore(gev_pcls *p, const void *src)
+{
+    CHECK(p && src, "gev_pcls_restore: NULL argument");
+    hipStream_t s = p->ctx->stream;
+    int64_t n = 0;
+    GEV_HIP(hipMemcpyAsync(&n, src, 8, hipMemcpyDeviceToHost, s));
+    GEV_HIP(hipStreamSynchronize(s));
+    CHECK(n >= 0 && n <= p->cap, "gev_pcls_restore: saved state does not fit this container");
+    const int64_t es = (p->ctx->prec == GEV_FP64) ? 8 : 4;
+    const char *d = (const char *)src + 8;
+    for (int k = 0; k < 3; k++, d += n * es) GEV_HIP(hipMemcpyAsync(p->pos[k], d, n * es, hipMemcpyDeviceToDevice, s));
+    for (int k = 0; k < 3; k++, d += n * es) GEV_HIP(hipMemcpyAsync(p->q[k], d, n * es, hipMemcpyDeviceToDevice, s));
+    GEV_HIP(hipMemcpyAsync(p->id, d, 8 * n, hipMemcpyDeviceToDevice, s));
+    d += 8 * n;
+    GEV_HIP(hipMemcpyAsync(p->cell, d, 4 * n, hipMemcpyDeviceToDevice, s));
+    d += 4 * n;
+    GEV_HIP(hipMemcpyAsync(p->cell_start, d, 4 * (p->ctx->M + 1), hipMemcpyDeviceToDevice, s));
+    p->n = n;
+    p->sorted = true;
+    return GEV_OK;
+}
+
+// ---- adjoint of the Newtonian cycle (gev_adjoint.hip)
+int gev_adj_kick_deposit(gev_pcls *p, const void *lx, void *lq, double C, void *u)
+{
+    CHECK(p && lx && lq && u, "gev_adj_kick_deposit: NULL argument");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx, adj_kick_deposit_impl<double>(p, (const double *)lx, (double *)lq, C, (double *)u),
+                         adj_kick_deposit_impl<float>(p, (const float *)lx, (float *)lq, C, (float *)u));
+}
+
+int gev_adj_divergence(gev_ctx *c, const void *u, void *lphi, double K)
+{
+    CHECK(c && u && lphi, "gev_adj_divergence: NULL argument");
+    return PREC_DISPATCH(c, adj_divergence_impl<double>(c, (const double *)u, (double *)lphi, K),
+                         adj_divergence_impl<float>(c, (const float *)u, (float *)lphi, K));
+}
+
+int gev_adj_gather(gev_pcls *p, const void *phi, const void *lrho, const void *lq, void *lx, double K, double mN)
+{
+    CHECK(p && lx && (lrho || phi), "gev_adj_gather: NULL argument");
+    CHECK(!phi || lq, "gev_adj_gather: the kick term needs lq");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx,
+                         adj_gather_impl<double>(p, (const double *)phi, (const double *)lrho, (const double *)lq,
+                                                 (double *)lx, K, mN),
+                         adj_gather_impl<float>(p, (const float *)phi, (const float *)lrho, (const float *)lq,
+                                                (float *)lx, K, mN));
+}
+
+int gev_adj_ic_lattice(gev_ctx *c, const void *lx, const void *lq, double cx, double cq, void *lpsi)
+{
+    CHECK(c && lx && lq && lpsi, "gev_adj_ic_lattice: NULL argument");
+    return PREC_DISPATCH(c, adj_ic_lattice_impl<double>(c, (const double *)lx, (const double *)lq, cx, cq, (double *)lpsi),
+                         adj_ic_lattice_impl<float>(c, (const float *)lx, (const float *)lq, cx, cq, (float *)lpsi));
 }
 
 // ---- fused real -> k-space -> real solves (the operator runs inside the z pass)

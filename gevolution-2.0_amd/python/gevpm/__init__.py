@@ -132,6 +132,16 @@ class Context:
         self.bind_stream()
         check(lib.gev_set_k0(self._h, _ptr(F), re, im), "gev_set_k0")
 
+    def adj_divergence(self, u, lphi, K):
+        self.bind_stream()
+        check(lib.gev_adj_divergence(self._h, _ptr(u), _ptr(lphi), K), "gev_adj_divergence")
+        return lphi
+
+    def adj_ic_lattice(self, lx, lq, cx, cq, lpsi):
+        self.bind_stream()
+        check(lib.gev_adj_ic_lattice(self._h, _ptr(lx), _ptr(lq), cx, cq, _ptr(lpsi)), "gev_adj_ic_lattice")
+        return lpsi
+
     # ------------------------------------ fused real -> k -> real solves
     def solve_poisson(self, src, coeff, modif=0.0, k0=None, out=None, work=None):
         """c2r(poisson(r2c(src))), optionally with the k=0 mode set to k0 first"""
@@ -343,6 +353,34 @@ class Particles:
         self.ctx.bind_stream()
         check(lib.gev_initialize_q(self._h, _ptr(pot), coeff, C.byref(r)), "gev_initialize_q")
         return r.value
+
+    # --------------------------------------------------------- checkpoints
+    def state_bytes(self):
+        b = C.c_int64()
+        check(lib.gev_pcls_state_bytes(self._h, C.byref(b)), "gev_pcls_state_bytes")
+        return b.value
+
+    def save(self, buf=None):
+        """copy the sorted container state into a device byte tensor"""
+        nb = self.state_bytes()
+        buf = torch.empty(nb, dtype=torch.uint8, device=self.ctx.device) if buf is None else buf
+        assert buf.numel() >= nb
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_save(self._h, _ptr(buf)), "gev_pcls_save")
+        return buf
+
+    def restore(self, buf):
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_restore(self._h, _ptr(buf)), "gev_pcls_restore")
+
+    # ------------------------------------------------ adjoint (Newtonian)
+    def adj_kick_deposit(self, lx, lq, C_drift, u):
+        self.ctx.bind_stream()
+        check(lib.gev_adj_kick_deposit(self._h, _ptr(lx), _ptr(lq), C_drift, _ptr(u)), "gev_adj_kick_deposit")
+
+    def adj_gather(self, lx, phi=None, lrho=None, lq=None, K=0.0, mN=0.0):
+        self.ctx.bind_stream()
+        check(lib.gev_adj_gather(self._h, _ptr(phi), _ptr(lrho), _ptr(lq), _ptr(lx), K, mN), "gev_adj_gather")
 
 
 class Fields:

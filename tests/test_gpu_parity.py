@@ -444,8 +444,10 @@ def test_power_spectrum_vs_oracle(gp, O, N):
     F = O.fft_r2c(f)
     res = spectra.power_spectrum(ctx, ft_gpu(ctx, F), 24, deconvolve=True)
     ref = O.power_spectrum(F, 24, deconvolve=True)
-    for got, want in zip(res[:4], ref[:4]):
-        assert relmax(got.numpy(), want) < 1e-12
+    # kbin and power: sums of positive terms; the scatters are square roots of
+    # differences of such sums (cancellation amplifies the summation order)
+    for got, want, tol in zip(res[:4], ref[:4], (1e-12, 1e-12, 1e-8, 1e-8)):
+        assert relmax(got.numpy(), want) < tol
     assert np.array_equal(res[4].numpy(), ref[4])
 
 

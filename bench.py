@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--total-steps", type=int, default=100, help="cycles spanning z_in -> z_end")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg (no P(k) error)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adj-mesh", type=int, default=128)
+    ap.add_argument("--adj-steps", type=int, default=20)
+    ap.add_argument("--adj-modes", type=int, default=64)
     ap.add_argument("--cpu-mesh", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=12)
     return ap.parse_args()
@@ -153,6 +157,9 @@ def main():
     # ----------------------------------------------------- dominant kernel
     roof = kernel_roofline(cm, pm.n, ph_m)
     cpu = None if args.no_cpu else cpu_baseline(args, om, ol, k, P)
+    if not args.no_adjoint:
+        torch.cuda.empty_cache()
+        result["adjoint"] = adjoint_leg(args)
     value = N ** 3 * 1e3 / ms_m
     line = {
         "metric": METRIC,
@@ -227,6 +234,84 @@ def kernel_roofline(ctx, n, phases):
     return {"kernel": name, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic(name), "avg_launch_ms": ms,
             "algorithmic_bytes_per_launch": bytes_per_launch, "bytes_model": desc}
+
+
+def adjoint_leg(args):
+    """Differentiable Newtonian run at adj-mesh^3, L = 2 adj-mesh Mpc/h, adj-steps
+    cycles z=50 -> 0, loss = sum (rho_final - rho_target)^2 with the target from
+    another seed.  Reports the backward / forward time ratio of the mixed form
+    (backward includes the per-cycle recompute), the FP32 gradient against the
+    FP64 one, and the FP64 gradient against central differences on adj-modes
+    random single Fourier modes of psi (eps = 1e-3 of the mode's own scale)."""
+    import numpy as np
+    import torch
+    import gevpm
+    from gevpm.adjoint import PMRun, pm_density
+    N, steps = args.adj_mesh, args.adj_steps
+    rng = np.random.default_rng(7)
+
+    def smooth(seed, amp):
+        r = np.random.default_rng(seed)
+        F = np.fft.rfftn(r.standard_normal((N, N, N)))
+        kk = np.fft.fftfreq(N, 1.0 / N)
+        kz, ky, kx = np.meshgrid(kk, kk, np.arange(N // 2 + 1), indexing="ij")
+        F *= np.exp(-(kx ** 2 + ky ** 2 + kz ** 2) / (2 * (N / 16.0) ** 2))
+        f = np.fft.irfftn(F, s=(N, N, N))
+        return amp * f / f.std()
+
+    amp = 0.3 / N ** 2  # rms displacement ~ a few tenths of a cell at z = 50
+    psi_np, tgt_np = smooth(11, amp), smooth(12, amp)
+    out = {"workload": f"Newtonian PM, {N}^3 particles on {N}^3 mesh, L={2 * N} Mpc/h, {steps} cycles z=50->0, "
+                       "loss sum (rho_final - rho_target)^2, gradient w.r.t. the initial displacement potential"}
+
+    def run_grad(prec):
+        run = PMRun(N, 2.0 * N, 50.0, 0.0, steps, precision=prec, ckpt_every=1)
+        dev = run.ctx.device
+        target = run.forward(torch.from_numpy(tgt_np).to(run.ctx.rdtype).to(dev)).double().clone()
+        psi = torch.from_numpy(psi_np).to(run.ctx.rdtype).to(dev).reshape(1, N, N, N)
+        # warm-up, then timed forward and backward
+        rho = run.forward(psi)
+        run.backward(2 * (rho.double() - target))
+        torch.cuda.synchronize()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        rho = run.forward(psi)
+        e[1].record()
+        g = run.backward(2 * (rho.double() - target))
+        e[2].record()
+        torch.cuda.synchronize()
+        return run, target, g.double().cpu().numpy().reshape(N, N, N), e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+
+    run32, _, g32, fwd_ms, bwd_ms = run_grad(gevpm.GEV_MIXED)
+    out.update(forward_ms=fwd_ms, backward_ms=bwd_ms, adjoint_over_forward=bwd_ms / fwd_ms,
+               checkpoint_bytes=run32.checkpoint_bytes())
+    del run32
+    torch.cuda.empty_cache()
+    run64, target, g64, f64_ms, b64_ms = run_grad(gevpm.GEV_FP64)
+    out.update(fp64_forward_ms=f64_ms, fp64_backward_ms=b64_ms,
+               fp32_vs_fp64_grad_rel_l2=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)))
+    # central differences on random single modes
+    x = np.arange(N)
+    errs = []
+    dev = run64.ctx.device
+
+    def loss(p):
+        rho = run64.forward(torch.from_numpy(p).to(dev).reshape(1, N, N, N)).double()
+        return float(((rho - target) ** 2).sum())
+
+    for _ in range(args.adj_modes):
+        kv = rng.integers(-4, 5, size=3)
+        ph = rng.uniform(0, 2 * np.pi)
+        mode = np.cos(2 * np.pi * (kv[0] * x[None, None, :] + kv[1] * x[None, :, None] + kv[2] * x[:, None, None]) / N
+                      + ph)
+        eps = 1e-3 * amp
+        fd = (loss(psi_np + eps * mode) - loss(psi_np - eps * mode)) / (2 * eps)
+        ad = float((g64 * mode).sum())
+        errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
+    errs = np.array(errs)
+    out.update(fd_modes=int(args.adj_modes), fd_eps_rel=1e-3, fd_rel_err_median=float(np.median(errs)),
+               fd_rel_err_max=float(errs.max()), fd_frac_within_1e5=float((errs <= 1e-5).mean()))
+    return out
 
 
 def cpu_baseline(args, om, ol, k, P):

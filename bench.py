@@ -256,7 +256,7 @@ def adjoint_leg(args):
         kk = np.fft.fftfreq(N, 1.0 / N)
         kz, ky, kx = np.meshgrid(kk, kk, np.arange(N // 2 + 1), indexing="ij")
         F *= np.exp(-(kx ** 2 + ky ** 2 + kz ** 2) / (2 * (N / 16.0) ** 2))
-        f = np.fft.irfftn(F, s=(N, N, N))
+        f = np.fft.irfftn(F, s=(N, N, N), axes=(0, 1, 2))
         return amp * f / f.std()
 
     amp = 0.3 / N ** 2  # rms displacement ~ a few tenths of a cell at z = 50
@@ -299,18 +299,25 @@ def adjoint_leg(args):
         rho = run64.forward(torch.from_numpy(p).to(dev).reshape(1, N, N, N)).double()
         return float(((rho - target) ** 2).sum())
 
+    errs_small = []
     for _ in range(args.adj_modes):
         kv = rng.integers(-4, 5, size=3)
         ph = rng.uniform(0, 2 * np.pi)
         mode = np.cos(2 * np.pi * (kv[0] * x[None, None, :] + kv[1] * x[None, :, None] + kv[2] * x[:, None, None]) / N
                       + ph)
-        eps = 1e-3 * amp
-        fd = (loss(psi_np + eps * mode) - loss(psi_np - eps * mode)) / (2 * eps)
         ad = float((g64 * mode).sum())
-        errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
-    errs = np.array(errs)
+        for rel, lst in ((1e-3, errs), (1e-6, errs_small)):
+            eps = rel * amp
+            fd = (loss(psi_np + eps * mode) - loss(psi_np - eps * mode)) / (2 * eps)
+            lst.append(abs(ad - fd) / max(abs(fd), 1e-300))
+    errs, errs_small = np.array(errs), np.array(errs_small)
+    # eps = 1e-3 of the field scale moves ~1e-4 of the particles across a cell face,
+    # where the CIC loss has a kink; eps = 1e-6 stays on the smooth pieces
     out.update(fd_modes=int(args.adj_modes), fd_eps_rel=1e-3, fd_rel_err_median=float(np.median(errs)),
-               fd_rel_err_max=float(errs.max()), fd_frac_within_1e5=float((errs <= 1e-5).mean()))
+               fd_rel_err_max=float(errs.max()), fd_frac_within_1e5=float((errs <= 1e-5).mean()),
+               fd_small_eps_rel=1e-6, fd_small_rel_err_median=float(np.median(errs_small)),
+               fd_small_rel_err_max=float(errs_small.max()),
+               fd_small_frac_within_1e5=float((errs_small <= 1e-5).mean()))
     return out
 
 

@@ -9,12 +9,14 @@
 //   q*    = q - K g(x; phi),  g_i = 1/2 sum_c w_c (phi(c+e_i) - phi(c-e_i))
 //   x'    = x + C q*,  q' = q*
 // Its adjoint, given (lx', lq'):
-//   lq*   = lq' + C lx'
-//   u_i   = sum_p w_c(x_p) lq*_{p,i}                                  (k_adj_kick_deposit)
+//   lq*   = lq' + C lx'          (done by the gather that finalised lx', see below)
+//   u_i   = sum_p w_c(x_p) lq*_{p,i}          (the tiled CIC deposit, per-ID weights)
 //   lphi  = -K/2 sum_i (u_i(s - e_i) - u_i(s + e_i))                  (k_adj_divergence)
 //   lrho  = G lphi                        (G is symmetric: the same Poisson solve)
 //   lx    = lx' - K N/2 sum_i lq*_i grad1(D_i phi)(x) + m' N grad1(lrho)(x)   (k_adj_gather)
 //   lq    = lq*
+// The gather also applies the next (earlier) cycle's lq <- lq + C_prev lx once
+// lx is final, so every per-particle update happens exactly once.
 // with grad1 the CIC edge gradient (the exact derivative of the CIC
 // interpolant inside a cell) and D_i phi(c) = phi(c+e_i) - phi(c-e_i).
 // Adjoint per-particle arrays are indexed by particle ID, so they survive the
@@ -24,37 +26,6 @@
 #include "gev_pcommon.hpp"
 
 namespace gev {
-
-// lq* = lq + C lx (stored back), u_i += w_c lq*_i at the 8 corners
-template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_adj_kick_deposit(PV<T> v, const T *__restrict__ lx, T *lq, double C, T *u,
-                                                            int64_t M)
-{
-    const int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
-    if (i >= v.n) return;
-    const int N = v.N;
-    const int64_t n = v.n;
-    const int64_t id = v.id[i];
-    int x, y, z;
-    cell_coords(v.cell[i], N, v.lg, x, y, z);
-    T r[3];
-    ref_dist(v, i, x, y, z, r);
-    T l[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        l[d] = lq[d * n + id] + (T)C * lx[d * n + id];
-        lq[d * n + id] = l[d];
-    }
-    const T o = (T)1;
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-        const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-        const T w = (bx ? r[0] : o - r[0]) * (by ? r[1] : o - r[1]) * (bz ? r[2] : o - r[2]);
-        const int64_t s = site(N, x + bx, y + by, z + bz);
-#pragma unroll
-        for (int d = 0; d < 3; d++) atomicAdd(&u[d * M + s], w * l[d]);
-    }
-}
 
 // lphi(s) = -K/2 sum_i (u_i(s - e_i) - u_i(s + e_i))
 template <typename T>
@@ -70,21 +41,13 @@ __global__ __launch_bounds__(256) void k_adj_divergence(const T *__restrict__ u,
     lphi[s] = (T)(-0.5 * K) * d;
 }
 
-// D_i phi at offset (dx,dy,dz) of the cell: phi(c+e_i) - phi(c-e_i)
-template <typename T> struct CentralDiff {
-    const T *f;
-    int N, x, y, z, i;
-    __device__ __forceinline__ T operator()(int dx, int dy, int dz) const
-    {
-        const int ex = i == 0, ey = i == 1, ez = i == 2;
-        return f[site(N, x + dx + ex, y + dy + ey, z + dz + ez)] - f[site(N, x + dx - ex, y + dy - ey, z + dz - ez)];
-    }
-};
-
-// lx += -K N/2 sum_i lq*_i grad1(D_i phi)(x) + mN grad1(lrho)(x)
+// lx += -K N/2 sum_i lq_i grad1(D_i phi)(x) + mN grad1(lrho)(x); then
+// lq += C_prev lx (the drift adjoint of the cycle processed next).
+// sum_i lq_i grad1(D_i phi) = grad1(F) with F(c) = sum_i lq_i (phi(c+e_i) -
+// phi(c-e_i)) at the 8 corners, since lq is constant over the particle.
 template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_adj_gather(PV<T> v, const T *__restrict__ phi, const T *__restrict__ lrho,
-                                                      const T *__restrict__ lq, T *lx, double K, double mN)
+                                                      T *lq, T *lx, double K, double mN, double C_prev)
 {
     const int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
     if (i >= v.n) return;
@@ -102,19 +65,29 @@ __global__ __launch_bounds__(PT_NT) void k_adj_gather(PV<T> v, const T *__restri
 #pragma unroll
         for (int j = 0; j < 3; j++) acc[j] = (T)mN * g[j];
     }
+    T l[3] = {lq[id], lq[n + id], lq[2 * n + id]};
     if (phi) {
-        const T l[3] = {lq[id], lq[n + id], lq[2 * n + id]};
+        T Fc[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int cx = x + ((b >> 2) & 1), cy = y + ((b >> 1) & 1), cz = z + (b & 1);
+            Fc[b] = l[0] * (phi[site(N, cx + 1, cy, cz)] - phi[site(N, cx - 1, cy, cz)]) +
+                    l[1] * (phi[site(N, cx, cy + 1, cz)] - phi[site(N, cx, cy - 1, cz)]) +
+                    l[2] * (phi[site(N, cx, cy, cz + 1)] - phi[site(N, cx, cy, cz - 1)]);
+        }
+        auto F = [&](int dx, int dy, int dz) { return Fc[(dx << 2) | (dy << 1) | dz]; };
+        T h[3];
+        grad1(F, r, h);
         const T kf = (T)(-0.5 * K * (double)N);
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            T h[3];
-            grad1(CentralDiff<T>{phi, N, x, y, z, k}, r, h);
-#pragma unroll
-            for (int j = 0; j < 3; j++) acc[j] += kf * l[k] * h[j];
-        }
+        for (int j = 0; j < 3; j++) acc[j] += kf * h[j];
     }
 #pragma unroll
-    for (int j = 0; j < 3; j++) lx[j * n + id] += acc[j];
+    for (int j = 0; j < 3; j++) {
+        const T nx = lx[j * n + id] + acc[j];
+        lx[j * n + id] = nx;
+        if (C_prev != 0.0) lq[j * n + id] = l[j] + (T)C_prev * nx;
+    }
 }
 
 // adjoint of the lattice Zel'dovich map (x = x_lat + cx N grad1(psi), q = cq N
@@ -146,18 +119,6 @@ __global__ __launch_bounds__(256) void k_adj_ic_lattice(const T *__restrict__ lx
 
 static inline unsigned nblocks(int64_t n, int nt) { return (unsigned)((n + nt - 1) / nt); }
 
-template <typename T> int adj_kick_deposit_impl(gev_pcls *p, const T *lx, T *lq, double C, T *u)
-{
-    gev_ctx *c = p->ctx;
-    int rc = field_fill_impl<T>(c, u, 3 * c->M, (T)0);
-    if (rc) return rc;
-    if (p->n > 0)
-        hipLaunchKernelGGL((k_adj_kick_deposit<T>), dim3(nblocks(p->n, PT_NT)), dim3(PT_NT), 0, c->stream, make_pv<T>(p),
-                           lx, lq, C, u, c->M);
-    GEV_HIP(hipGetLastError());
-    return GEV_OK;
-}
-
 template <typename T> int adj_divergence_impl(gev_ctx *c, const T *u, T *lphi, double K)
 {
     hipLaunchKernelGGL((k_adj_divergence<T>), dim3(nblocks(c->M, 256)), dim3(256), 0, c->stream, u, lphi, c->N, c->M, K);
@@ -165,12 +126,13 @@ template <typename T> int adj_divergence_impl(gev_ctx *c, const T *u, T *lphi, d
     return GEV_OK;
 }
 
-template <typename T> int adj_gather_impl(gev_pcls *p, const T *phi, const T *lrho, const T *lq, T *lx, double K, double mN)
+template <typename T>
+int adj_gather_impl(gev_pcls *p, const T *phi, const T *lrho, T *lq, T *lx, double K, double mN, double C_prev)
 {
     gev_ctx *c = p->ctx;
     if (p->n > 0)
         hipLaunchKernelGGL((k_adj_gather<T>), dim3(nblocks(p->n, PT_NT)), dim3(PT_NT), 0, c->stream, make_pv<T>(p), phi,
-                           lrho, lq, lx, K, mN);
+                           lrho, lq, lx, K, mN, C_prev);
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }
@@ -184,9 +146,8 @@ template <typename T> int adj_ic_lattice_impl(gev_ctx *c, const T *lx, const T *
 }
 
 #define INST(T)                                                                                   \
-    template int adj_kick_deposit_impl<T>(gev_pcls *, const T *, T *, double, T *);               \
     template int adj_divergence_impl<T>(gev_ctx *, const T *, T *, double);                       \
-    template int adj_gather_impl<T>(gev_pcls *, const T *, const T *, const T *, T *, double, double); \
+    template int adj_gather_impl<T>(gev_pcls *, const T *, const T *, T *, T *, double, double, double); \
     template int adj_ic_lattice_impl<T>(gev_ctx *, const T *, const T *, double, double, T *);
 INST(double)
 INST(float)

@@ -35,10 +35,10 @@ int kick_drift_impl(gev_pcls *p, int gr, double dtau_k, double dtau_d, const T *
 template <typename T> int pcls_sort_counted_impl(gev_pcls *p);
 template <typename T> int displace_impl(gev_pcls *p, const T *xi, double coeff, double *max_disp);
 template <typename T> int initialize_q_impl(gev_pcls *p, const T *pot, double coeff, double *max_ret);
-template <typename T> int adj_kick_deposit_impl(gev_pcls *p, const T *lx, T *lq, double C, T *u);
+template <typename T> int dep_vec3_impl(gev_pcls *p, const T *w, T *u);
 template <typename T> int adj_divergence_impl(gev_ctx *c, const T *u, T *lphi, double K);
 template <typename T>
-int adj_gather_impl(gev_pcls *p, const T *phi, const T *lrho, const T *lq, T *lx, double K, double mN);
+int adj_gather_impl(gev_pcls *p, const T *phi, const T *lrho, T *lq, T *lx, double K, double mN, double C_prev);
 template <typename T> int adj_ic_lattice_impl(gev_ctx *c, const T *lx, const T *lq, double cx, double cq, T *lpsi);
 }  // namespace gev
 
@@ -450,12 +450,12 @@ int gev_pcls_restore(gev_pcls *p, const void *src)
 }
 
 // ---- adjoint of the Newtonian cycle (gev_adjoint.hip)
-int gev_adj_kick_deposit(gev_pcls *p, const void *lx, void *lq, double C, void *u)
+int gev_adj_deposit(gev_pcls *p, const void *lq, void *u)
 {
-    CHECK(p && lx && lq && u, "gev_adj_kick_deposit: NULL argument");
+    CHECK(p && lq && u, "gev_adj_deposit: NULL argument");
     NEED_SORTED(p);
-    return PREC_DISPATCH(p->ctx, adj_kick_deposit_impl<double>(p, (const double *)lx, (double *)lq, C, (double *)u),
-                         adj_kick_deposit_impl<float>(p, (const float *)lx, (float *)lq, C, (float *)u));
+    return PREC_DISPATCH(p->ctx, dep_vec3_impl<double>(p, (const double *)lq, (double *)u),
+                         dep_vec3_impl<float>(p, (const float *)lq, (float *)u));
 }
 
 int gev_adj_divergence(gev_ctx *c, const void *u, void *lphi, double K)
@@ -465,16 +465,16 @@ int gev_adj_divergence(gev_ctx *c, const void *u, void *lphi, double K)
                          adj_divergence_impl<float>(c, (const float *)u, (float *)lphi, K));
 }
 
-int gev_adj_gather(gev_pcls *p, const void *phi, const void *lrho, const void *lq, void *lx, double K, double mN)
+int gev_adj_gather(gev_pcls *p, const void *phi, const void *lrho, void *lq, void *lx, double K, double mN,
+                   double C_prev)
 {
-    CHECK(p && lx && (lrho || phi), "gev_adj_gather: NULL argument");
-    CHECK(!phi || lq, "gev_adj_gather: the kick term needs lq");
+    CHECK(p && lx && lq && (lrho || phi), "gev_adj_gather: NULL argument");
     NEED_SORTED(p);
     return PREC_DISPATCH(p->ctx,
-                         adj_gather_impl<double>(p, (const double *)phi, (const double *)lrho, (const double *)lq,
-                                                 (double *)lx, K, mN),
-                         adj_gather_impl<float>(p, (const float *)phi, (const float *)lrho, (const float *)lq,
-                                                (float *)lx, K, mN));
+                         adj_gather_impl<double>(p, (const double *)phi, (const double *)lrho, (double *)lq,
+                                                 (double *)lx, K, mN, C_prev),
+                         adj_gather_impl<float>(p, (const float *)phi, (const float *)lrho, (float *)lq, (float *)lx,
+                                                K, mN, C_prev));
 }
 
 int gev_adj_ic_lattice(gev_ctx *c, const void *lx, const void *lq, double cx, double cq, void *lpsi)

@@ -22,9 +22,12 @@
 namespace gev {
 
 // DEP_T00TIJ: T00 and Tij of the same particles and phi in one pass (7 outputs)
-enum { DEP_T00 = 0, DEP_T0I = 1, DEP_TIJ = 2, DEP_T00TIJ = 3 };
+// DEP_VEC3: CIC deposit of a per-particle 3-vector indexed by particle ID
+// (the adjoint's u_i = sum_p w_c lq_{p,i}), weights read from `aux`
+enum { DEP_T00 = 0, DEP_T0I = 1, DEP_TIJ = 2, DEP_T00TIJ = 3, DEP_VEC3 = 4 };
 template <int KIND> struct DepC {
-    static constexpr int C = KIND == DEP_T00 ? 1 : (KIND == DEP_T0I ? 3 : (KIND == DEP_TIJ ? 6 : 7));
+    static constexpr int C =
+        KIND == DEP_T00 ? 1 : (KIND == DEP_T0I || KIND == DEP_VEC3 ? 3 : (KIND == DEP_TIJ ? 6 : 7));
 };
 
 // FP64 LDS accumulators in both precisions: ds_add_f64 is the fast LDS atomic
@@ -36,7 +39,8 @@ __device__ __forceinline__ void lds_add(double *a, double v)
 
 template <typename T, int KIND, int TX, int TY, int TZ>
 __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int64_t M, const T *__restrict__ phi,
-                                                   double mass_d, double mass2_d, double a_d, int accumulate)
+                                                   double mass_d, double mass2_d, double a_d, int accumulate,
+                                                   const T *__restrict__ aux)
 {
     constexpr int C = DepC<KIND>::C;
     constexpr int NS = TX * TY * TZ;                    // owned sites
@@ -127,7 +131,12 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
             const int64_t p = pu[u];
             cu[u] = (run[u] & 1) ? (uint32_t)v.cell[p] : 0u;
             xu[u][0] = v.p0[p]; xu[u][1] = v.p1[p]; xu[u][2] = v.p2[p];
-            if (need_q) { qu[u][0] = v.q0[p]; qu[u][1] = v.q1[p]; qu[u][2] = v.q2[p]; }
+            if constexpr (KIND == DEP_VEC3) {
+                const int64_t id = v.id[p];
+                qu[u][0] = aux[id]; qu[u][1] = aux[v.n + id]; qu[u][2] = aux[2 * v.n + id];
+            } else {
+                if (need_q) { qu[u][0] = v.q0[p]; qu[u][1] = v.q1[p]; qu[u][2] = v.q2[p]; }
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -148,6 +157,19 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
         };
         auto sidx = [&](int bx, int by, int bz) { return sb + bx + TX * (by + TY * bz); };
         const int pb = lx + PX * (ly + PY * lz);
+        if constexpr (KIND == DEP_VEC3) {
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+                if (own(bx, by, bz)) {
+                    const T w = (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]);
+                    const int si = sidx(bx, by, bz);
+                    lds_add(&acc[0][si], w * qu[u][0]);
+                    lds_add(&acc[1][si], w * qu[u][1]);
+                    lds_add(&acc[2][si], w * qu[u][2]);
+                }
+            }
+        }
         if constexpr (KIND == DEP_T00 || KIND == DEP_T00TIJ) {
             // projection_T00 functor, gevolution.hpp:1729-1854
             if (use_phi) {
@@ -262,27 +284,34 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
 }
 
 template <typename T, int KIND, int TX, int TY, int TZ>
-static int launch_dep(gev_pcls *p, T *out, T *out2, const T *phi, double mass, double mass2, double a, int acc)
+static int launch_dep(gev_pcls *p, T *out, T *out2, const T *phi, double mass, double mass2, double a, int acc,
+                      const T *aux)
 {
     gev_ctx *c = p->ctx;
     const int N = c->N;
     const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
     hipLaunchKernelGGL((k_deposit<T, KIND, TX, TY, TZ>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), out, out2,
-                       c->M, phi, mass, mass2, a, acc);
+                       c->M, phi, mass, mass2, a, acc, aux);
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }
 
 template <typename T, int KIND>
 static int dispatch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc, T *out2 = nullptr,
-                        double mass2 = 0.0)
+                        double mass2 = 0.0, const T *aux = nullptr)
 {
     const int N = p->ctx->N;
     // Tij keeps 6 (T00+Tij 7) FP64 accumulators per site: the smaller tile holds 4 blocks per CU
     constexpr bool big = KIND != DEP_TIJ && KIND != DEP_T00TIJ;
-    if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, out2, phi, mass, mass2, a, acc);
-    if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, out2, phi, mass, mass2, a, acc);
-    return launch_dep<T, KIND, 4, 4, 4>(p, out, out2, phi, mass, mass2, a, acc);
+    if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux);
+    if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux);
+    return launch_dep<T, KIND, 4, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux);
+}
+
+// u = CIC deposit of the per-ID 3-vector w ([3][n]), unit mass
+template <typename T> int dep_vec3_impl(gev_pcls *p, const T *w, T *u)
+{
+    return dispatch_dep<T, DEP_VEC3>(p, u, nullptr, 1.0, 1.0, 0, nullptr, 0.0, w);
 }
 
 template <typename T> int dep_T00_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc)
@@ -307,6 +336,7 @@ int dep_T00_Tij_impl(gev_pcls *p, T *T00, T *Tij, const T *phi, double mass00, d
 }
 
 #define INST(T)                                                                   \
+    template int dep_vec3_impl<T>(gev_pcls *, const T *, T *);                       \
     template int dep_T00_Tij_impl<T>(gev_pcls *, T *, T *, const T *, double, double, double, int); \
     template int dep_T00_impl<T>(gev_pcls *, T *, const T *, double, double, int); \
     template int dep_T0i_impl<T>(gev_pcls *, T *, const T *, double, int);         \

@@ -127,7 +127,9 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
             T *q = qu[k];
             FieldLds<T> Fp{fld[0], o, SX, SX * SY}, Fc{fld[1], o, SX, SX * SY};
             FieldLds<T> B0{fld[2], o, SX, SX * SY}, B1{fld[3], o, SX, SX * SY}, B2{fld[4], o, SX, SX * SY};
-            if constexpr ((F & (UPD_KICK_GR | UPD_DRIFT_GR)) == (UPD_KICK_GR | UPD_DRIFT_GR)) {
+            // FP32: kick and drift share weights and stencil reads; FP64 keeps
+            // them apart (the shared form needs more registers than it saves)
+            if constexpr ((F & (UPD_KICK_GR | UPD_DRIFT_GR)) == (UPD_KICK_GR | UPD_DRIFT_GR) && sizeof(T) == 4) {
                 double d[3];
                 T rv = phys_kick_drift<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn, nf_d,
                                           (T)u.a_d, (T)u.bc_d, u.dtau_d, d);

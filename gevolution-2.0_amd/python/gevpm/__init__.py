@@ -374,13 +374,14 @@ class Particles:
         check(lib.gev_pcls_restore(self._h, _ptr(buf)), "gev_pcls_restore")
 
     # ------------------------------------------------ adjoint (Newtonian)
-    def adj_kick_deposit(self, lx, lq, C_drift, u):
+    def adj_deposit(self, lq, u):
         self.ctx.bind_stream()
-        check(lib.gev_adj_kick_deposit(self._h, _ptr(lx), _ptr(lq), C_drift, _ptr(u)), "gev_adj_kick_deposit")
+        check(lib.gev_adj_deposit(self._h, _ptr(lq), _ptr(u)), "gev_adj_deposit")
 
-    def adj_gather(self, lx, phi=None, lrho=None, lq=None, K=0.0, mN=0.0):
+    def adj_gather(self, lx, lq, phi=None, lrho=None, K=0.0, mN=0.0, C_prev=0.0):
         self.ctx.bind_stream()
-        check(lib.gev_adj_gather(self._h, _ptr(phi), _ptr(lrho), _ptr(lq), _ptr(lx), K, mN), "gev_adj_gather")
+        check(lib.gev_adj_gather(self._h, _ptr(phi), _ptr(lrho), _ptr(lq), _ptr(lx), K, mN, C_prev),
+              "gev_adj_gather")
 
 
 class Fields:

@@ -80,9 +80,9 @@ SIGNATURES = {
     "gev_pcls_state_bytes": (_i, [_vp, C.POINTER(C.c_int64)]),
     "gev_pcls_save": (_i, [_vp, _vp]),
     "gev_pcls_restore": (_i, [_vp, _vp]),
-    "gev_adj_kick_deposit": (_i, [_vp, _vp, _vp, _d, _vp]),
+    "gev_adj_deposit": (_i, [_vp, _vp, _vp]),
     "gev_adj_divergence": (_i, [_vp, _vp, _vp, _d]),
-    "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d]),
+    "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "gev_adj_ic_lattice": (_i, [_vp, _vp, _vp, _d, _d, _vp]),
     "gev_ic_displacement_ft": (_i, [_vp, _u32, _i, _d, _i, _dp, _dp, _vp, _vp]),
     "gev_displace_particles": (_i, [_vp, _vp, _d, _dp]),

@@ -93,8 +93,12 @@ class PMRun:
         g_rho = g_rho.reshape(1, self.N, self.N, self.N).to(c.rdtype).contiguous()
         lx = torch.zeros(3, n, dtype=c.rdtype, device=c.device)
         lq = torch.zeros_like(lx)
+        def c_of(k):  # drift factor of cycle k (0 before the first cycle)
+            return self.sched[k]["dtau"] / self.sched[k]["a_d"] if k >= 0 else 0.0
+
         p.restore(self._final)
-        p.adj_gather(lx, lrho=g_rho, mN=self.mN)  # d rho_final / d x_final
+        # d rho_final / d x_final, then lq += C lx for the last cycle
+        p.adj_gather(lx, lq, lrho=g_rho, mN=self.mN, C_prev=c_of(self.steps - 1))
         starts = sorted(self._ckpts)
         for si in range(len(starts) - 1, -1, -1):
             k0 = starts[si]
@@ -112,10 +116,10 @@ class PMRun:
                 p.restore(states[k])
                 p.project_mass(self.rho)
                 c.solve_poisson(self.rho, self.fpg / s["a"], 0.0, out=self.phi, work=self.work)
-                p.adj_kick_deposit(lx, lq, s["dtau"] / s["a_d"], self.u)
+                p.adj_deposit(lq, self.u)
                 c.adj_divergence(self.u, self.lphi, K)
                 c.solve_poisson(self.lphi, self.fpg / s["a"], 0.0, out=self.lrho, work=self.work)
-                p.adj_gather(lx, phi=self.phi, lrho=self.lrho, lq=lq, K=K, mN=self.mN)
+                p.adj_gather(lx, lq, phi=self.phi, lrho=self.lrho, K=K, mN=self.mN, C_prev=c_of(k - 1))
             del states
         lpsi = c.real(1)
         c.adj_ic_lattice(lx, lq, self.cx, self.cq, lpsi)

@@ -179,18 +179,21 @@ int gev_pcls_restore(gev_pcls *p, const void *src);
  *      Poisson solve, gevolution.hpp:1051-1118, 1270-1273, 1902-1905, 806-852).
  *      Per-particle adjoints lx, lq are [3][n] arrays indexed by particle ID
  *      (IDs in [0, n)), in the context precision.  With K = dtau_kick a N and
- *      C = dtau_drift / a_drift of the forward cycle:
- *   gev_adj_kick_deposit: lq <- lq + C lx; u (3 components, overwritten) =
- *                         CIC deposit of lq at the particle positions
- *   gev_adj_divergence:   lphi = -K/2 sum_i (u_i(s - e_i) - u_i(s + e_i))
+ *      C = dtau_drift / a_drift of the forward cycle, one reverse cycle is
+ *   (lq <- lq + C lx, applied by the gather that finished lx, see C_prev)
+ *   gev_adj_deposit:    u (3 components, overwritten) = CIC deposit of lq
+ *   gev_adj_divergence: lphi = -K/2 sum_i (u_i(s - e_i) - u_i(s + e_i))
  *   (lrho = the Poisson solve of lphi with the forward coefficients)
- *   gev_adj_gather:       lx += -K N/2 sum_i lq_i grad1(D_i phi) + mN grad1(lrho),
- *                         mN = particle mass / dx^3 * N; phi or lrho may be NULL
- *   gev_adj_ic_lattice:   lpsi from the adjoints (lx, lq) of the lattice map
- *                         x = x_lat + cx N grad1(psi), q = cq N grad1(psi), ID = cell */
-int gev_adj_kick_deposit(gev_pcls *p, const void *lx, void *lq, double C, void *u);
+ *   gev_adj_gather:     lx += -K N/2 sum_i lq_i grad1(D_i phi) + mN grad1(lrho),
+ *                       then lq += C_prev lx when C_prev != 0 (C of the cycle
+ *                       reversed next); mN = particle mass / dx^3 * N; phi or
+ *                       lrho may be NULL
+ *   gev_adj_ic_lattice: lpsi from the adjoints (lx, lq) of the lattice map
+ *                       x = x_lat + cx N grad1(psi), q = cq N grad1(psi), ID = cell */
+int gev_adj_deposit(gev_pcls *p, const void *lq, void *u);
 int gev_adj_divergence(gev_ctx *ctx, const void *u, void *lphi, double K);
-int gev_adj_gather(gev_pcls *p, const void *phi, const void *lrho, const void *lq, void *lx, double K, double mN);
+int gev_adj_gather(gev_pcls *p, const void *phi, const void *lrho, void *lq, void *lx, double K, double mN,
+                   double C_prev);
 int gev_adj_ic_lattice(gev_ctx *ctx, const void *lx, const void *lq, double cx, double cq, void *lpsi);
 
 /* ---- background (background.hpp:200-290), Omega_rad/ncdm/fld as given */

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
     const T a = (T)a_d, o = (T)1, ia2 = (T)(1.0 / (a_d * a_d));
 
     // U particles per thread and pass: their loads are issued together
-    constexpr int U = (KIND == DEP_TIJ || KIND == DEP_T00TIJ) ? 2 : 4;
+    constexpr int U = (KIND == DEP_TIJ || KIND == DEP_T00TIJ) ? 1 : 2;
     constexpr int TB = KIND == DEP_T00TIJ ? 1 : 0;  // first Tij accumulator
     const bool need_q = KIND != DEP_T00 || use_phi;
     for (int32_t i0 = t; i0 < total; i0 += U * PT_NT) {
@@ -303,7 +303,9 @@ static int dispatch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a
     const int N = p->ctx->N;
     // Tij keeps 6 (T00+Tij 7) FP64 accumulators per site: the smaller tile holds 4 blocks per CU
     constexpr bool big = KIND != DEP_TIJ && KIND != DEP_T00TIJ;
+    if (KIND == DEP_T0I && N >= 32) return launch_dep<T, KIND, 32, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux);
     if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux);
+    if (KIND == DEP_T00TIJ && N >= 16) return launch_dep<T, KIND, 16, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux);
     if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux);
     return launch_dep<T, KIND, 4, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux);
 }

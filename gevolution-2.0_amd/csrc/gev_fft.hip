@@ -220,26 +220,25 @@ __global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(const T *__restrict__
     }
 }
 
-// x pass c2r: N/2+1 complex (pitch P) -> N reals; DC and Nyquist taken as real
+// x pass c2r: N/2+1 complex (pitch P) -> N reals; DC and Nyquist taken as real.
+// The first stage reads X[k] and X[H-k] straight from global memory (the
+// second read of each element hits L2), so LDS holds only the exchange buffer.
 template <typename T, int N, int NL>
 __global__ __launch_bounds__(NL * N / 16) void k_c2r_x_reg(const cx<T> *__restrict__ in, T *__restrict__ out, int P,
                                                            int64_t nlines, const cx<T> *__restrict__ tw)
 {
     constexpr int H = N / 2, NT = NL * H / 8, LDL = H + 1;
-    __shared__ cx<T> xin[NL * LDL];
     __shared__ cx<T> lds[NL * LDL];
     __shared__ cx<T> twl[N];
     for (int m = threadIdx.x; m < N; m += NT) twl[m] = tw[m];
     const int64_t line0 = (int64_t)blockIdx.x * NL;
     const cx<T> *src = in + line0 * P;
-    for (int e = threadIdx.x; e < NL * P; e += NT) {
-        int l = e / P, k = e % P;
-        if (k <= H) xin[l * LDL + k] = (line0 + l < nlines) ? src[e] : mk((T)0, (T)0);
-    }
     __syncthreads();
     auto ld = [&](int line, int k) {
-        cx<T> Xk = xin[line * LDL + k];
-        cx<T> Xm = xin[line * LDL + (H - k)];
+        if (line0 + line >= nlines) return mk((T)0, (T)0);
+        const cx<T> *row = src + (int64_t)line * P;
+        cx<T> Xk = row[k];
+        cx<T> Xm = row[H - k];
         if (k == 0) { Xk.y = 0; Xm.y = 0; }
         cx<T> cm = conj(Xm);
         cx<T> E = Xk + cm;
@@ -472,14 +471,19 @@ __global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *o
     }
 }
 
-// at most 512 threads per workgroup: the operator keeps up to 3 x 8 modes per thread in registers
-template <typename T, int L> constexpr int zs_tx() { return (col_tx<T, L>() * L / 8 > 512) ? 4096 / L : col_tx<T, L>(); }
+// the vector operator keeps 3 x 8 modes per thread in registers: at most 512
+// threads per workgroup there; Poisson and scalar use the column pass's width
+template <typename T, int L, int OP> constexpr int zs_tx()
+{
+    constexpr bool narrow = OP == ZOP_VECTOR || (OP == ZOP_SCALAR && L >= 1024);
+    return (narrow && col_tx<T, L>() * L / 8 > 512) ? 4096 / L : col_tx<T, L>();
+}
 
 template <typename T, int N, int OP>
 static void launch_zsolve(const cx<T> *in, cx<T> *out, int P, int64_t NK, const cx<T> *tw, const ZParams &zp,
                           hipStream_t s)
 {
-    constexpr int TX = zs_tx<T, N>();
+    constexpr int TX = zs_tx<T, N, OP>();
     const int64_t nb = (int64_t)(P / TX) * N;
     hipLaunchKernelGGL((k_zsolve<T, N, TX, OP>), dim3((unsigned)nb), dim3(TX * N / 8), 0, s, in, out, P, NK, tw, zp);
 }

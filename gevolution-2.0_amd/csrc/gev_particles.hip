@@ -85,9 +85,10 @@ template <typename T> __global__ void k_store(PV<T> v, double *__restrict__ pos,
 // ------------------------------------------------------------- sorting ----
 __global__ void k_count(const int32_t *__restrict__ cell, int32_t *__restrict__ rank, int32_t *count, int64_t n)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    rank[i] = atomicAdd(&count[cell[i]], 1);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    const int32_t r = wave_rank_add(count, active ? cell[i] : 0, active);
+    if (active) rank[i] = r;
 }
 
 constexpr int SCAN_NT = 256, SCAN_PER = 8, SCAN_CHUNK = SCAN_NT * SCAN_PER;

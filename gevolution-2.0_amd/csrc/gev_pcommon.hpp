@@ -438,6 +438,29 @@ __device__ __forceinline__ T phys_kick_drift(const T r[3], T q[3], const FA &Fp,
     return ret;
 }
 
+// rank = atomicAdd(&count[key], 1) for every active lane, with one atomic per
+// run of consecutive lanes that share the key (cell-sorted particles put the
+// members of a dense cell in neighbouring lanes, which would otherwise queue
+// on one counter).  Lanes outside the current execution mask count as
+// inactive (their ballot bits are 0); inactive lanes get -1.
+__device__ __forceinline__ int32_t wave_rank_add(int32_t *count, int32_t key, bool active)
+{
+    const int lane = threadIdx.x & 63;
+    const int32_t prev = __shfl_up(key, 1, 64);
+    const unsigned long long act = __ballot(active);
+    const bool prev_active = lane > 0 && ((act >> (lane - 1)) & 1ull);
+    const bool head = !active || lane == 0 || !prev_active || prev != key;
+    const unsigned long long heads = __ballot(head) | ~act;
+    const unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const int h = 63 - __clzll(heads & le);
+    const unsigned long long above = heads & ~le;
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    int32_t old = 0;
+    if (active && lane == h) old = atomicAdd(&count[key], next - h);
+    const int32_t base = __shfl(old, h, 64);
+    return active ? base + (lane - h) : -1;
+}
+
 __device__ __forceinline__ void block_max_to(double v, unsigned long long *slot)
 {
     __shared__ double sh[PT_NT / 64];

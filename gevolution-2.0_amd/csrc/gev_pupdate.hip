@@ -118,7 +118,9 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
         }
 #pragma unroll
         for (int k = 0; k < U; k++) {
-            if (pu[k] < 0) continue;
+            const bool active = pu[k] >= 0;
+            int32_t newc = 0;
+            if (active) {
             const int64_t p = pu[k];
             const int x = cu[k] & (N - 1), y = (cu[k] >> v.lg) & (N - 1), z = cu[k] >> (2 * v.lg);
             const int o = (x - x0 + 1) + SX * ((y - y0 + 1) + SY * (z - z0 + 1));
@@ -135,8 +137,7 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
                                           (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
                 ret = fmax(ret, (double)rv);
                 v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2];
-                const int32_t c = move_by(v, p, xu[k], x, y, z, d);
-                if (u.flags & UPD_COUNT) rank[p] = atomicAdd(&count[c], 1);
+                newc = move_by(v, p, xu[k], x, y, z, d);
             } else {
             if constexpr (F & UPD_KICK_GR) {
                 T rv = phys_update_q<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn);
@@ -155,9 +156,14 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
                     d[1] = u.dtau_d * (double)q[1] / u.a_d;
                     d[2] = u.dtau_d * (double)q[2] / u.a_d;
                 }
-                const int32_t c = move_by(v, p, xu[k], x, y, z, d);
-                if (u.flags & UPD_COUNT) rank[p] = atomicAdd(&count[c], 1);
+                newc = move_by(v, p, xu[k], x, y, z, d);
             }
+            }
+            }
+            // re-binning rank in the new cell, one counter atomic per run of equal cells
+            if (drift && (u.flags & UPD_COUNT)) {
+                const int32_t rk = wave_rank_add(count, newc, active);
+                if (active) rank[pu[k]] = rk;
             }
         }
     }

@@ -37,138 +37,12 @@ __device__ __forceinline__ void lds_add(double *a, double v)
     __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// The values one particle adds, as NSLOT (component, corner) slots: slot k
-// goes to accumulator comp(k) of the site cell + (bx, by, bz)(k).
-template <int KIND> struct DepSlot {
-    static constexpr int N = KIND == DEP_T00 ? 8 : KIND == DEP_T0I ? 12 : KIND == DEP_TIJ ? 30 : KIND == DEP_T00TIJ ? 38 : 24;
-    // Tij slot j (0..29) -> component and corner code (bx<<2 | by<<1 | bz)
-    static constexpr int tij_comp(int j) { return j < 24 ? (j < 8 ? 0 : (j < 16 ? 3 : 5)) : (j < 26 ? 1 : (j < 28 ? 2 : 4)); }
-    static constexpr int tij_corner(int j)
-    {
-        return j < 24 ? (j & 7) : (j == 25 ? 1 : (j == 27 ? 2 : (j == 29 ? 4 : 0)));
-    }
-    // T0i link k: component and corner (gevolution.hpp:2060-2071 staggering)
-    static constexpr int t0i_comp(int k) { return (k == 0 || k == 5 || k == 7 || k == 9) ? 0 : ((k == 1 || k == 3 || k == 8 || k == 10) ? 1 : 2); }
-    static constexpr int t0i_corner(int k)
-    {
-        return k <= 2 ? 0 : (k <= 4 ? 4 : (k <= 6 ? 2 : (k <= 8 ? 1 : (k == 9 ? 3 : (k == 10 ? 5 : 6)))));
-    }
-    static constexpr int comp(int k)
-    {
-        return KIND == DEP_T00 ? 0
-             : KIND == DEP_T0I ? t0i_comp(k)
-             : KIND == DEP_TIJ ? tij_comp(k)
-             : KIND == DEP_T00TIJ ? (k < 8 ? 0 : 1 + tij_comp(k - 8))
-                                  : k / 8;
-    }
-    static constexpr int corner(int k)
-    {
-        return KIND == DEP_T00 ? k
-             : KIND == DEP_T0I ? t0i_corner(k)
-             : KIND == DEP_TIJ ? tij_corner(k)
-             : KIND == DEP_T00TIJ ? (k < 8 ? k : tij_corner(k - 8))
-                                  : k % 8;
-    }
-    static constexpr int bx(int k) { return (corner(k) >> 2) & 1; }
-    static constexpr int by(int k) { return (corner(k) >> 1) & 1; }
-    static constexpr int bz(int k) { return corner(k) & 1; }
-};
-
-// per-particle values of the reference functors; rd: position in the cell,
-// q: momentum (DEP_VEC3: the deposited vector), P: phi box at the cell with
-// strides (1, SY, SZ)
-template <typename T, int KIND>
-__device__ __forceinline__ void dep_values(const T rd[3], const T q[3], const T *P, int SY, int SZ, bool use_phi, T a,
-                                           T ia2, T *val)
-{
-    const T o = (T)1;
-    auto w = [&](int b) {
-        const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-        return (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]);
-    };
-    auto ph = [&](int b) { return P[((b >> 2) & 1) + SY * ((b >> 1) & 1) + SZ * (b & 1)]; };
-    if constexpr (KIND == DEP_VEC3) {
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const T wb = w(b);
-            val[b] = wb * q[0]; val[8 + b] = wb * q[1]; val[16 + b] = wb * q[2];
-        }
-    }
-    if constexpr (KIND == DEP_T00 || KIND == DEP_T00TIJ) {
-        // projection_T00 functor, gevolution.hpp:1729-1854
-        if (use_phi) {
-            T f2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-            const T e = fsqrt(f2 + a * a), ie = frcp(e);
-            const T f = (T)3 * e + f2 * ie;
-            f2 = ((T)0.5 * f * f + f2 * frcp(o + f2 * ia2)) * ie;
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const T p = ph(b);
-                val[b] = w(b) * (e + f * p + f2 * p * p);
-            }
-        } else {
-#pragma unroll
-            for (int b = 0; b < 8; b++) val[b] = w(b);
-        }
-    }
-    if constexpr (KIND == DEP_T0I) {
-        // projection_T0i functor, gevolution.hpp:2031-2201: 12 links per cell,
-        // each weighted by (1 + phi_a + phi_b) of its two end points
-        const T v0 = q[0], v1 = q[1], v2 = q[2];
-        val[0] = (o - rd[1]) * (o - rd[2]) * v0; val[1] = (o - rd[0]) * (o - rd[2]) * v1;
-        val[2] = (o - rd[0]) * (o - rd[1]) * v2; val[3] = rd[0] * (o - rd[2]) * v1;
-        val[4] = rd[0] * (o - rd[1]) * v2;       val[5] = rd[1] * (o - rd[2]) * v0;
-        val[6] = rd[1] * (o - rd[0]) * v2;       val[7] = rd[2] * (o - rd[1]) * v0;
-        val[8] = rd[2] * (o - rd[0]) * v1;       val[9] = rd[1] * rd[2] * v0;
-        val[10] = rd[0] * rd[2] * v1;            val[11] = rd[0] * rd[1] * v2;
-        if (use_phi) {
-            T p[8];
-#pragma unroll
-            for (int b = 0; b < 8; b++) p[b] = ph(b);
-            val[0] *= o + p[0] + p[4]; val[1] *= o + p[0] + p[2]; val[2] *= o + p[0] + p[1];
-            val[3] *= o + p[4] + p[6]; val[4] *= o + p[4] + p[5]; val[5] *= o + p[2] + p[6];
-            val[6] *= o + p[2] + p[3]; val[7] *= o + p[1] + p[5]; val[8] *= o + p[1] + p[3];
-            val[9] *= o + p[3] + p[7]; val[10] *= o + p[5] + p[7]; val[11] *= o + p[6] + p[7];
-        }
-    }
-    if constexpr (KIND == DEP_TIJ || KIND == DEP_T00TIJ) {
-        // projection_Tij functor, gevolution.hpp:2582-2965 (no CIC_PROJECT_TIJ):
-        // diagonal parts CIC onto the 8 corners, off-diagonal parts onto 2 sites
-        constexpr int S0 = KIND == DEP_T00TIJ ? 8 : 0;
-        T f = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-        const T e2 = f + a * a, ie = frsqrt(e2);
-        if (use_phi) {
-            f = (T)4 + a * a * frcp(e2);
-            f *= (o - rd[0]) * ((o - rd[1]) * ((o - rd[2]) * P[0] + rd[2] * P[SZ]) +
-                                rd[1] * ((o - rd[2]) * P[SY] + rd[2] * P[SY + SZ])) +
-                 rd[0] * ((o - rd[1]) * ((o - rd[2]) * P[1] + rd[2] * P[1 + SZ]) +
-                          rd[1] * ((o - rd[2]) * P[1 + SY] + rd[2] * P[1 + SY + SZ]));
-            f += o;
-        } else {
-            f = o;
-        }
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const T wd = f * q[d] * q[d] * ie;
-#pragma unroll
-            for (int b = 0; b < 8; b++) val[S0 + 8 * d + b] = wd * w(b);
-        }
-        T wo = f * q[0] * q[1] * ie;
-        val[S0 + 24] = wo * (o - rd[2]); val[S0 + 25] = wo * rd[2];
-        wo = f * q[0] * q[2] * ie;
-        val[S0 + 26] = wo * (o - rd[1]); val[S0 + 27] = wo * rd[1];
-        wo = f * q[1] * q[2] * ie;
-        val[S0 + 28] = wo * (o - rd[0]); val[S0 + 29] = wo * rd[0];
-    }
-}
-
 template <typename T, int KIND, int TX, int TY, int TZ>
 __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int64_t M, const T *__restrict__ phi,
                                                    double mass_d, double mass2_d, double a_d, int accumulate,
                                                    const T *__restrict__ aux)
 {
     constexpr int C = DepC<KIND>::C;
-    constexpr int NSLOT = DepSlot<KIND>::N;
     constexpr int NS = TX * TY * TZ;                    // owned sites
     constexpr int HY = TY + 1, NR = HY * (TZ + 1);      // halo rows of cells
     constexpr int NSEG = 2 * NR;                        // runs: halo cell, tile cells
@@ -223,17 +97,18 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
     if (t == NSEG - 1) rbeg[NSEG] = off + incl;
     __syncthreads();
     const int32_t total = rbeg[NSEG];
-    const T a = (T)a_d, ia2 = (T)(1.0 / (a_d * a_d));
+    const T a = (T)a_d, o = (T)1, ia2 = (T)(1.0 / (a_d * a_d));
 
     // U particles per thread and pass: their loads are issued together
     constexpr int U = (KIND == DEP_TIJ || KIND == DEP_T00TIJ) ? 1 : 2;
+    constexpr int TB = KIND == DEP_T00TIJ ? 1 : 0;  // first Tij accumulator
     const bool need_q = KIND != DEP_T00 || use_phi;
-    for (int32_t i0 = t; i0 < total; i0 += U * PT_NT) {
+    for (int32_t i0 = 0; i0 < total; i0 += U * PT_NT) {
         int run[U];
         int64_t pu[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int32_t i = i0 + u * PT_NT;
+            const int32_t i = i0 + u * PT_NT + t;
             run[u] = -1;
             pu[u] = 0;
             if (i < total) {
@@ -265,62 +140,132 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const bool active = run[u] >= 0;
-            const int rr = active ? run[u] : 0;
-            const int r = rr >> 1, ly = r % HY, lz = r / HY;
-            const int lx = (rr & 1) ? 1 + (int)((cu[u] & msk) - (uint32_t)x0) : 0;
-            const int gx = (x0 - 1 + lx) & (int)msk, gy = (y0 - 1 + ly) & (int)msk, gz = (z0 - 1 + lz) & (int)msk;
-            // site of corner (bx,by,bz) in tile coordinates and whether the tile owns it
-            const int sx = lx - 1, sy = ly - 1, sz = lz - 1;
-            const bool inx0 = sx >= 0, inx1 = sx + 1 < TX;
-            const bool iny0 = sy >= 0, iny1 = sy + 1 < TY;
-            const bool inz0 = sz >= 0, inz1 = sz + 1 < TZ;
-            const int sb = sx + TX * (sy + TY * sz);
-            auto own = [&](int bx, int by, int bz) {
-                return (bx ? inx1 : inx0) && (by ? iny1 : iny0) && (bz ? inz1 : inz0);
-            };
-            auto sidx = [&](int bx, int by, int bz) { return sb + bx + TX * (by + TY * bz); };
-            T val[NSLOT];
-            if (active) {
-                T rd[3];
-                cell_rel(xu[u], gx, gy, gz, N, rd);
-                dep_values<T, KIND>(rd, qu[u], phs + lx + PX * (ly + PY * lz), PX, PX * PY, use_phi, a, ia2, val);
-            } else {
+        if (run[u] < 0) continue;
+        const int r = run[u] >> 1, ly = r % HY, lz = r / HY;
+        const int lx = (run[u] & 1) ? 1 + (int)((cu[u] & msk) - (uint32_t)x0) : 0;
+        const int gx = (x0 - 1 + lx) & (int)msk, gy = (y0 - 1 + ly) & (int)msk, gz = (z0 - 1 + lz) & (int)msk;
+        T rd[3];
+        cell_rel(xu[u], gx, gy, gz, N, rd);
+        // site of corner (bx,by,bz) in tile coordinates and whether the tile owns it
+        const int sx = lx - 1, sy = ly - 1, sz = lz - 1;
+        const bool inx0 = sx >= 0, inx1 = sx + 1 < TX;
+        const bool iny0 = sy >= 0, iny1 = sy + 1 < TY;
+        const bool inz0 = sz >= 0, inz1 = sz + 1 < TZ;
+        const int sb = sx + TX * (sy + TY * sz);
+        auto own = [&](int bx, int by, int bz) {
+            return (bx ? inx1 : inx0) && (by ? iny1 : iny0) && (bz ? inz1 : inz0);
+        };
+        auto sidx = [&](int bx, int by, int bz) { return sb + bx + TX * (by + TY * bz); };
+        const int pb = lx + PX * (ly + PY * lz);
+        if constexpr (KIND == DEP_VEC3) {
 #pragma unroll
-                for (int k = 0; k < NSLOT; k++) val[k] = (T)0;
+            for (int b = 0; b < 8; b++) {
+                const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+                if (own(bx, by, bz)) {
+                    const T w = (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]);
+                    const int si = sidx(bx, by, bz);
+                    lds_add(&acc[0][si], w * qu[u][0]);
+                    lds_add(&acc[1][si], w * qu[u][1]);
+                    lds_add(&acc[2][si], w * qu[u][2]);
+                }
             }
-            // lanes of one cell (consecutive in the cell-sorted runs) first sum
-            // their values, so a dense cell issues one LDS atomic per slot
-            const int key = active ? rr * (TX + 2) + lx : -1;
-            const int prevkey = __shfl_up(key, 1, 64);
-            const unsigned long long act = __ballot(active);
-            const bool same = active && lane > 0 && ((act >> (lane - 1)) & 1ull) && prevkey == key;
-            const unsigned long long dup = __ballot(same);
-            bool emit = active;
-            if (dup) {
-                // a lane starts a run unless it repeats its left neighbour; h: its run's first lane
-                const unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-                const int h = 63 - __clzll(~dup & le);
-                // segmented inclusive scan, as many steps as the longest run needs
-                for (int o = 1; o < 64 && __ballot(lane - h >= o); o <<= 1) {
+        }
+        if constexpr (KIND == DEP_T00 || KIND == DEP_T00TIJ) {
+            // projection_T00 functor, gevolution.hpp:1729-1854
+            if (use_phi) {
+                const T q0 = qu[u][0], q1 = qu[u][1], q2 = qu[u][2];
+                T f2 = q0 * q0 + q1 * q1 + q2 * q2;
+                const T e = fsqrt(f2 + a * a), ie = frcp(e);
+                const T f = (T)3 * e + f2 * ie;
+                f2 = ((T)0.5 * f * f + f2 * frcp(o + f2 * ia2)) * ie;
 #pragma unroll
-                    for (int k = 0; k < NSLOT; k++) {
-                        const T y = __shfl_up(val[k], o, 64);
-                        if (lane - o >= h) val[k] += y;
+                for (int b = 0; b < 8; b++) {
+                    const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+                    if (own(bx, by, bz)) {
+                        const T ph = phs[pb + bx + PX * (by + PY * bz)];
+                        const T w = (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]);
+                        lds_add(&acc[0][sidx(bx, by, bz)], w * (e + f * ph + f2 * ph * ph));
                     }
                 }
-                // the last lane of each run holds the run's sums
-                const bool next_same = lane < 63 && ((dup >> (lane + 1)) & 1ull);
-                emit = active && !next_same;
-            }
-            if (emit) {
+            } else {
 #pragma unroll
-                for (int k = 0; k < NSLOT; k++) {
-                    const int c = DepSlot<KIND>::comp(k), bx = DepSlot<KIND>::bx(k), by = DepSlot<KIND>::by(k),
-                              bz = DepSlot<KIND>::bz(k);
-                    if (own(bx, by, bz)) lds_add(&acc[c][sidx(bx, by, bz)], (double)val[k]);
+                for (int b = 0; b < 8; b++) {
+                    const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+                    if (own(bx, by, bz))
+                        lds_add(&acc[0][sidx(bx, by, bz)],
+                                (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
                 }
             }
+        }
+        if constexpr (KIND == DEP_T0I) {
+            // projection_T0i functor, gevolution.hpp:2031-2201: 12 links per cell,
+            // each weighted by (1 + phi_a + phi_b) of its two end points
+            const T v0 = qu[u][0], v1 = qu[u][1], v2 = qu[u][2];
+            T g[12] = {(o - rd[1]) * (o - rd[2]) * v0, (o - rd[0]) * (o - rd[2]) * v1, (o - rd[0]) * (o - rd[1]) * v2,
+                       rd[0] * (o - rd[2]) * v1,       rd[0] * (o - rd[1]) * v2,       rd[1] * (o - rd[2]) * v0,
+                       rd[1] * (o - rd[0]) * v2,       rd[2] * (o - rd[1]) * v0,       rd[2] * (o - rd[0]) * v1,
+                       rd[1] * rd[2] * v0,             rd[0] * rd[2] * v1,             rd[0] * rd[1] * v2};
+            if (use_phi) {
+                T ph[8];
+#pragma unroll
+                for (int b = 0; b < 8; b++) ph[b] = phs[pb + ((b >> 2) & 1) + PX * (((b >> 1) & 1) + PY * (b & 1))];
+                g[0] *= o + ph[0] + ph[4]; g[1] *= o + ph[0] + ph[2]; g[2] *= o + ph[0] + ph[1];
+                g[3] *= o + ph[4] + ph[6]; g[4] *= o + ph[4] + ph[5]; g[5] *= o + ph[2] + ph[6];
+                g[6] *= o + ph[2] + ph[3]; g[7] *= o + ph[1] + ph[5]; g[8] *= o + ph[1] + ph[3];
+                g[9] *= o + ph[3] + ph[7]; g[10] *= o + ph[5] + ph[7]; g[11] *= o + ph[6] + ph[7];
+            }
+            // link k lands on component comp[k] of the site cell + offset[k]
+            if (own(0, 0, 0)) {
+                lds_add(&acc[0][sidx(0, 0, 0)], g[0]);
+                lds_add(&acc[1][sidx(0, 0, 0)], g[1]);
+                lds_add(&acc[2][sidx(0, 0, 0)], g[2]);
+            }
+            if (own(1, 0, 0)) { lds_add(&acc[1][sidx(1, 0, 0)], g[3]); lds_add(&acc[2][sidx(1, 0, 0)], g[4]); }
+            if (own(0, 1, 0)) { lds_add(&acc[0][sidx(0, 1, 0)], g[5]); lds_add(&acc[2][sidx(0, 1, 0)], g[6]); }
+            if (own(0, 0, 1)) { lds_add(&acc[0][sidx(0, 0, 1)], g[7]); lds_add(&acc[1][sidx(0, 0, 1)], g[8]); }
+            if (own(0, 1, 1)) lds_add(&acc[0][sidx(0, 1, 1)], g[9]);
+            if (own(1, 0, 1)) lds_add(&acc[1][sidx(1, 0, 1)], g[10]);
+            if (own(1, 1, 0)) lds_add(&acc[2][sidx(1, 1, 0)], g[11]);
+        }
+        if constexpr (KIND == DEP_TIJ || KIND == DEP_T00TIJ) {
+            // projection_Tij functor, gevolution.hpp:2582-2965 (no CIC_PROJECT_TIJ):
+            // diagonal parts CIC onto the 8 corners, off-diagonal parts onto 2 sites
+            const T vv[3] = {qu[u][0], qu[u][1], qu[u][2]};
+            T f = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
+            const T e2 = f + a * a, ie = frsqrt(e2);
+            if (use_phi) {
+                const T *P = phs + pb;
+                f = (T)4 + a * a * frcp(e2);
+                f *= (o - rd[0]) * ((o - rd[1]) * ((o - rd[2]) * P[0] + rd[2] * P[PX * PY]) +
+                                    rd[1] * ((o - rd[2]) * P[PX] + rd[2] * P[PX + PX * PY])) +
+                     rd[0] * ((o - rd[1]) * ((o - rd[2]) * P[1] + rd[2] * P[1 + PX * PY]) +
+                              rd[1] * ((o - rd[2]) * P[1 + PX] + rd[2] * P[1 + PX + PX * PY]));
+                f += o;
+            } else {
+                f = o;
+            }
+            constexpr int dcomp[3] = {TB + 0, TB + 3, TB + 5};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const T w = f * vv[d] * vv[d] * ie;
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+                    if (own(bx, by, bz))
+                        lds_add(&acc[dcomp[d]][sidx(bx, by, bz)],
+                                w * (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
+                }
+            }
+            T w = f * vv[0] * vv[1] * ie;
+            if (own(0, 0, 0)) lds_add(&acc[TB + 1][sidx(0, 0, 0)], w * (o - rd[2]));
+            if (own(0, 0, 1)) lds_add(&acc[TB + 1][sidx(0, 0, 1)], w * rd[2]);
+            w = f * vv[0] * vv[2] * ie;
+            if (own(0, 0, 0)) lds_add(&acc[TB + 2][sidx(0, 0, 0)], w * (o - rd[1]));
+            if (own(0, 1, 0)) lds_add(&acc[TB + 2][sidx(0, 1, 0)], w * rd[1]);
+            w = f * vv[1] * vv[2] * ie;
+            if (own(0, 0, 0)) lds_add(&acc[TB + 4][sidx(0, 0, 0)], w * (o - rd[0]));
+            if (own(1, 0, 0)) lds_add(&acc[TB + 4][sidx(1, 0, 0)], w * rd[0]);
+        }
         }
     }
     __syncthreads();

@@ -443,7 +443,14 @@ __device__ __forceinline__ T phys_kick_drift(const T r[3], T q[3], const FA &Fp,
 // members of a dense cell in neighbouring lanes, which would otherwise queue
 // on one counter).  Lanes outside the current execution mask count as
 // inactive (their ballot bits are 0); inactive lanes get -1.
-__device__ __forceinline__ int32_t wave_rank_add(int32_t *count, int32_t key, bool active)
+// The rank comes back with the counter atomic; wave_rank_issue starts it and
+// wave_rank_finish (a wave-wide shuffle, so every lane calls it) reads it, so a
+// caller can overlap the atomic's round trip with other work.
+struct RankPending {
+    int32_t old;  // the counter value before the run's add (valid in the run's first lane)
+    int h;        // first lane of this lane's run
+};
+__device__ __forceinline__ RankPending wave_rank_issue(int32_t *count, int32_t key, bool active)
 {
     const int lane = threadIdx.x & 63;
     const int32_t prev = __shfl_up(key, 1, 64);
@@ -457,20 +464,29 @@ __device__ __forceinline__ int32_t wave_rank_add(int32_t *count, int32_t key, bo
     const int next = above ? __ffsll((long long)above) - 1 : 64;
     int32_t old = 0;
     if (active && lane == h) old = atomicAdd(&count[key], next - h);
-    const int32_t base = __shfl(old, h, 64);
-    return active ? base + (lane - h) : -1;
+    return RankPending{old, h};
+}
+__device__ __forceinline__ int32_t wave_rank_finish(const RankPending &r)
+{
+    const int lane = threadIdx.x & 63;
+    return __shfl(r.old, r.h, 64) + (lane - r.h);
+}
+__device__ __forceinline__ int32_t wave_rank_add(int32_t *count, int32_t key, bool active)
+{
+    const int32_t r = wave_rank_finish(wave_rank_issue(count, key, active));
+    return active ? r : -1;
 }
 
-__device__ __forceinline__ void block_max_to(double v, unsigned long long *slot)
+template <int NT = PT_NT> __device__ __forceinline__ void block_max_to(double v, unsigned long long *slot)
 {
-    __shared__ double sh[PT_NT / 64];
+    __shared__ double sh[NT / 64];
     v = wave_max(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) sh[w] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = sh[0];
-        for (int j = 1; j < PT_NT / 64; j++) m = fmax(m, sh[j]);
+        for (int j = 1; j < NT / 64; j++) m = fmax(m, sh[j]);
         // all blocks hit one address: skip the atomic unless it can raise the maximum
         unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((unsigned long long)__double_as_longlong(m) > cur) atomic_max_nonneg(slot, m);

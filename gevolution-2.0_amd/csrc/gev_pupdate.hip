@@ -21,17 +21,26 @@ struct UpdParams {
     int nf_k, nf_d, flags;
 };
 
-template <typename T, int TX, int TY, int TZ, int F>
-__global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, const T *__restrict__ phi,
+// threads per workgroup
+#ifndef UPD_NT_F32
+#define UPD_NT_F32 512
+#endif
+#ifndef UPD_NT_F64
+#define UPD_NT_F64 256
+#endif
+template <typename T> struct UpdShape {
+    static constexpr int NT = sizeof(T) == 4 ? UPD_NT_F32 : UPD_NT_F64;
+};
+
+template <typename T, int TX, int TY, int TZ, int F, int NT = UpdShape<T>::NT>
+__global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const T *__restrict__ phi,
                                                         const T *__restrict__ chi, const T *__restrict__ B0g,
                                                         const T *__restrict__ B1g, const T *__restrict__ B2g,
                                                         unsigned long long *maxslot, int32_t *count, int32_t *rank)
 {
     constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3, NS = SX * SY * SZ, NROW = TY * TZ;
-    static_assert(NROW <= 64 && SX <= 64, "tile too large for the wave-level staging");
+    static_assert(NROW <= 64 && SX <= 64 && (NROW & (NROW - 1)) == 0, "tile shape unsupported by the wave-level staging");
     __shared__ T fld[5][NS];
-    __shared__ int32_t rs[NROW + 1];
-    __shared__ int32_t rbeg[NROW];
     const int N = v.N;
     const uint32_t msk = (uint32_t)N - 1;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -46,7 +55,7 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
     const bool use_phi = kick || nf_d >= 1, use_chi = nfmax >= 2, use_B = nfmax >= 3;
     // stage [x0-1, x0+TX+1] x ... : each wave covers RPW box rows per pass, lane -> (row, x)
     {
-        constexpr int RPW = 64 / SX, NR = SY * SZ, NW = PT_NT / 64;
+        constexpr int RPW = 64 / SX, NR = SY * SZ, NW = NT / 64;
         const int sub = lane / SX, lx = lane - sub * SX;
         const uint32_t gx = (uint32_t)(x0 - 1 + lx) & msk;
         if (sub < RPW) {
@@ -65,68 +74,67 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
             }
         }
     }
-    // particle ranges of the tile's TY*TZ rows and their prefix (one wave)
-    if (wv == 0) {
-        int32_t cnt = 0, s0 = 0;
-        if (lane < NROW) {
-            const int ry = lane % TY, rz = lane / TY;
-            const uint32_t rb = (uint32_t)x0 + (uint32_t)N * ((uint32_t)(y0 + ry) + (uint32_t)N * (uint32_t)(z0 + rz));
-            s0 = v.start[rb];
-            cnt = v.start[rb + TX] - s0;
-        }
-        int32_t incl = cnt;
-        for (int o = 1; o < 64; o <<= 1) {
-            int32_t y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
-        if (lane < NROW) {
-            rbeg[lane] = s0;
-            rs[lane + 1] = incl;
-        }
-        if (lane == 0) rs[0] = 0;
+    // particle ranges of the tile's TY*TZ rows, computed by every wave for
+    // itself (no barrier before the first particle loads): lane r < NROW
+    // holds the first container index of row r and the inclusive prefix of the
+    // row lengths
+    int32_t rb0 = 0, incl = 0;
+    if (lane < NROW) {
+        const int ry = lane % TY, rz = lane / TY;
+        const uint32_t rb = (uint32_t)x0 + (uint32_t)N * ((uint32_t)(y0 + ry) + (uint32_t)N * (uint32_t)(z0 + rz));
+        rb0 = v.start[rb];
+        incl = v.start[rb + TX] - rb0;
     }
-    __syncthreads();
-    const int total = rs[NROW];
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    const int total = __shfl(incl, NROW - 1, 64);
+    const int niter = (total + NT - 1) / NT;  // the same for every lane of the workgroup
+    // container index of flattened particle f; every lane runs the same steps
+    // (the shuffles need all lanes), lanes with f >= total get -1
+    auto locate = [&](int f) -> int64_t {
+        int lo = 0;
+#pragma unroll
+        for (int step = NROW / 2; step >= 1; step >>= 1) {
+            const int j = lo + step < NROW ? lo + step : NROW - 1;
+            const int32_t rsj = __shfl(incl, j - 1, 64);  // first flattened index of row j
+            if (lo + step < NROW && rsj <= f) lo = j;
+        }
+        const int32_t rsl = __shfl(incl, lo > 0 ? lo - 1 : 0, 64);
+        const int32_t rbl = __shfl(rb0, lo, 64);
+        return f < total ? (int64_t)rbl + (f - (lo > 0 ? rsl : 0)) : (int64_t)-1;
+    };
+    // the next particle of this lane, loaded one pass ahead of its use
+    int64_t pn = locate(tid);
+    int32_t cn = 0;
+    T xn[3] = {(T)0, (T)0, (T)0}, qn[3] = {(T)0, (T)0, (T)0};
+    auto fetch = [&]() {
+        if (pn >= 0) {
+            cn = v.cell[pn];
+            xn[0] = v.p0[pn]; xn[1] = v.p1[pn]; xn[2] = v.p2[pn];
+            qn[0] = v.q0[pn]; qn[1] = v.q1[pn]; qn[2] = v.q2[pn];
+        }
+    };
+    fetch();
+    __syncthreads();  // staged fields
     const T Nn = (T)N;
     double ret = 0.0;
-    // U particles per thread and pass: their loads are issued together
-    constexpr int U = sizeof(T) == 4 ? 2 : 1;
-    for (int f0 = tid; f0 < total; f0 += U * PT_NT) {
-        int64_t pu[U];
-#pragma unroll
-        for (int k = 0; k < U; k++) {
-            const int f = f0 + k * PT_NT;
-            pu[k] = -1;
-            if (f < total) {
-                int lo = 0, hi = NROW;
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (rs[mid] <= f) lo = mid; else hi = mid;
-                }
-                pu[k] = (int64_t)rbeg[lo] + (f - rs[lo]);
-            }
-        }
-        int32_t cu[U];
-        T xu[U][3], qu[U][3];
-#pragma unroll
-        for (int k = 0; k < U; k++) {
-            if (pu[k] < 0) continue;
-            const int64_t p = pu[k];
-            cu[k] = v.cell[p];
-            xu[k][0] = v.p0[p]; xu[k][1] = v.p1[p]; xu[k][2] = v.p2[p];
-            qu[k][0] = v.q0[p]; qu[k][1] = v.q1[p]; qu[k][2] = v.q2[p];
-        }
-#pragma unroll
-        for (int k = 0; k < U; k++) {
-            const bool active = pu[k] >= 0;
-            int32_t newc = 0;
-            if (active) {
-            const int64_t p = pu[k];
-            const int x = cu[k] & (N - 1), y = (cu[k] >> v.lg) & (N - 1), z = cu[k] >> (2 * v.lg);
+    RankPending rpend{0, 0};
+    int64_t ppend = -1;
+    for (int it = 0; it < niter; it++) {
+        const int64_t p = pn;
+        const int32_t cc = cn;
+        T xk[3] = {xn[0], xn[1], xn[2]}, q[3] = {qn[0], qn[1], qn[2]};
+        pn = locate(tid + (it + 1) * NT);
+        fetch();
+        const bool active = p >= 0;
+        int32_t newc = 0;
+        if (active) {
+            const int x = cc & (N - 1), y = (cc >> v.lg) & (N - 1), z = cc >> (2 * v.lg);
             const int o = (x - x0 + 1) + SX * ((y - y0 + 1) + SY * (z - z0 + 1));
             T r[3];
-            cell_rel(xu[k], x, y, z, N, r);
-            T *q = qu[k];
+            cell_rel(xk, x, y, z, N, r);
             FieldLds<T> Fp{fld[0], o, SX, SX * SY}, Fc{fld[1], o, SX, SX * SY};
             FieldLds<T> B0{fld[2], o, SX, SX * SY}, B1{fld[3], o, SX, SX * SY}, B2{fld[4], o, SX, SX * SY};
             // FP32: kick and drift share weights and stencil reads; FP64 keeps
@@ -137,37 +145,46 @@ __global__ __launch_bounds__(PT_NT) void k_update_tiled(PV<T> v, UpdParams u, co
                                           (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
                 ret = fmax(ret, (double)rv);
                 v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2];
-                newc = move_by(v, p, xu[k], x, y, z, d);
+                newc = move_by(v, p, xk, x, y, z, d);
             } else {
-            if constexpr (F & UPD_KICK_GR) {
-                T rv = phys_update_q<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn);
-                ret = fmax(ret, (double)rv);
-            } else if constexpr (F & UPD_KICK_NEWTON) {
-                T rv = phys_update_q_newton<T>(r, q, Fp, Fc, nf_k, (T)u.a_k, (T)u.dtau_k, Nn);
-                ret = fmax(ret, (double)rv);
-            }
-            if (kick) { v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2]; }
-            if (drift) {
-                double d[3];
-                if constexpr (F & UPD_DRIFT_GR) {
-                    phys_update_pos<T>(r, q, Fp, Fc, B0, B1, B2, nf_d, (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
-                } else {
-                    d[0] = u.dtau_d * (double)q[0] / u.a_d;
-                    d[1] = u.dtau_d * (double)q[1] / u.a_d;
-                    d[2] = u.dtau_d * (double)q[2] / u.a_d;
+                if constexpr (F & UPD_KICK_GR) {
+                    T rv = phys_update_q<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn);
+                    ret = fmax(ret, (double)rv);
+                } else if constexpr (F & UPD_KICK_NEWTON) {
+                    T rv = phys_update_q_newton<T>(r, q, Fp, Fc, nf_k, (T)u.a_k, (T)u.dtau_k, Nn);
+                    ret = fmax(ret, (double)rv);
                 }
-                newc = move_by(v, p, xu[k], x, y, z, d);
-            }
-            }
-            }
-            // re-binning rank in the new cell, one counter atomic per run of equal cells
-            if (drift && (u.flags & UPD_COUNT)) {
-                const int32_t rk = wave_rank_add(count, newc, active);
-                if (active) rank[pu[k]] = rk;
+                if (kick) { v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2]; }
+                if (drift) {
+                    double d[3];
+                    if constexpr (F & UPD_DRIFT_GR) {
+                        phys_update_pos<T>(r, q, Fp, Fc, B0, B1, B2, nf_d, (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
+                    } else {
+                        d[0] = u.dtau_d * (double)q[0] / u.a_d;
+                        d[1] = u.dtau_d * (double)q[1] / u.a_d;
+                        d[2] = u.dtau_d * (double)q[2] / u.a_d;
+                    }
+                    newc = move_by(v, p, xk, x, y, z, d);
+                }
             }
         }
+        // re-binning rank in the new cell, one counter atomic per run of equal
+        // cells; each pass stores the rank of the previous one, so the
+        // atomic's round trip overlaps a pass of work
+        if (drift && (u.flags & UPD_COUNT)) {
+            if (it > 0) {
+                const int32_t rk = wave_rank_finish(rpend);
+                if (ppend >= 0) rank[ppend] = rk;
+            }
+            rpend = wave_rank_issue(count, newc, active);
+            ppend = p;
+        }
     }
-    if (kick) block_max_to(ret, maxslot);
+    if (drift && (u.flags & UPD_COUNT) && niter > 0) {
+        const int32_t rk = wave_rank_finish(rpend);
+        if (ppend >= 0) rank[ppend] = rk;
+    }
+    if (kick) block_max_to<NT>(ret, maxslot);
 }
 
 template <typename T> int pcls_sort_impl(gev_pcls *p);
@@ -181,7 +198,7 @@ static void launch_tile(gev_pcls *p, const UpdParams &u, const T *phi, const T *
     const int N = c->N;
     const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
     if constexpr (!(sizeof(T) == 8 && TX == 16))  // the large tile exists for FP32 only
-        hipLaunchKernelGGL((k_update_tiled<T, TX, TY, TZ, F>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), u, phi,
+        hipLaunchKernelGGL((k_update_tiled<T, TX, TY, TZ, F>), dim3(nb), dim3(UpdShape<T>::NT), 0, c->stream, make_pv<T>(p), u, phi,
                            chi, B0, B1, B2, slot, p->cell_count, p->rank);
 }
 

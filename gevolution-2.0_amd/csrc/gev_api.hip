@@ -78,6 +78,7 @@ int gev_ctx_create(int N, int precision, void *hip_stream, gev_ctx **out)
     c->M = (int64_t)N * N * N;
     c->NK = (int64_t)c->P * N * N;
     GEV_HIP(hipGetDevice(&c->device));
+    GEV_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     if (hip_stream) {
         c->stream = (hipStream_t)hip_stream;
     } else {
@@ -150,7 +151,9 @@ int gev_pcls_create(gev_ctx *c, int64_t capacity, double mass, gev_pcls **out)
     p->cap = capacity;
     p->mass = mass;
     const size_t es = (c->prec == GEV_FP64) ? 8 : 4;
-    const size_t cap = (size_t)std::max<int64_t>(capacity, 1);
+    // one slot past the capacity: the sink that the particle kernels read
+    // and write for idle lanes, so their memory instructions need no branch
+    const size_t cap = (size_t)std::max<int64_t>(capacity, 1) + 1;
     for (int d = 0; d < 3; d++) {
         GEV_HIP(hipMalloc(&p->pos[d], es * cap));
         GEV_HIP(hipMalloc(&p->q[d], es * cap));

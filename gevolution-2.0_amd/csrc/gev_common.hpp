@@ -81,6 +81,7 @@ struct gev_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int device = 0;
+    int ncu = 0;           // compute units of the device
     // twiddles exp(-2 pi i m / N), m in [0,N): double2 and float2 copies
     gev::cx<double> *tw64 = nullptr;
     gev::cx<float> *tw32 = nullptr;

@@ -158,24 +158,44 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(int32_t *__restrict__ cn
                                                         const int32_t *__restrict__ bsum, int32_t *__restrict__ start,
                                                         int32_t total)
 {
-    int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK + (int64_t)threadIdx.x * SCAN_PER;
+    static_assert(SCAN_PER == 8, "two 16-byte vectors per thread");
+    // each thread owns SCAN_PER consecutive cells: 16-byte loads and stores
+    // where the whole run is inside [0, M) (always when M is a multiple of 8)
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK + (int64_t)threadIdx.x * SCAN_PER;
+    const bool full = base + SCAN_PER <= M;
     int32_t v[SCAN_PER];
+    if (full) {
+        int4 *c4 = (int4 *)(cnt + base);
+        const int4 a = c4[0], b = c4[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        c4[0] = make_int4(0, 0, 0, 0);
+        c4[1] = make_int4(0, 0, 0, 0);
+    } else {
+        for (int j = 0; j < SCAN_PER; j++) {
+            const int64_t i = base + j;
+            v[j] = (i < M) ? cnt[i] : 0;
+            if (i < M) cnt[i] = 0;
+        }
+    }
     int32_t s = 0;
-    for (int j = 0; j < SCAN_PER; j++) {
-        int64_t i = base + j;
-        v[j] = (i < M) ? cnt[i] : 0;
-        s += v[j];
-    }
-    for (int j = 0; j < SCAN_PER; j++) {
-        int64_t i = base + j;
-        if (i < M) cnt[i] = 0;
-    }
+#pragma unroll
+    for (int j = 0; j < SCAN_PER; j++) s += v[j];
     int32_t tot;
     int32_t off = block_excl_scan(s, tot) + bsum[blockIdx.x];
+    int32_t o[SCAN_PER];
+#pragma unroll
     for (int j = 0; j < SCAN_PER; j++) {
-        int64_t i = base + j;
-        if (i < M) start[i] = off;
+        o[j] = off;
         off += v[j];
+    }
+    if (full) {
+        int4 *s4 = (int4 *)(start + base);
+        s4[0] = make_int4(o[0], o[1], o[2], o[3]);
+        s4[1] = make_int4(o[4], o[5], o[6], o[7]);
+    } else {
+        for (int j = 0; j < SCAN_PER; j++)
+            if (base + j < M) start[base + j] = o[j];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) start[M] = total;
 }

@@ -17,6 +17,7 @@ template <typename T> struct PV {
     int32_t *cell;
     const int32_t *start;
     int64_t n;
+    int64_t sink;  // the slot past the capacity, read and written by idle lanes
     int N, lg;
 };
 
@@ -29,6 +30,7 @@ template <typename T> static PV<T> make_pv(gev_pcls *p)
     v.cell = p->cell;
     v.start = p->cell_start;
     v.n = p->n;
+    v.sink = std::max<int64_t>(p->cap, 1);
     v.N = p->ctx->N;
     v.lg = __builtin_ctz(p->ctx->N);
     return v;

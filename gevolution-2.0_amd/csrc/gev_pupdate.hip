@@ -32,6 +32,113 @@ template <typename T> struct UpdShape {
     static constexpr int NT = sizeof(T) == 4 ? UPD_NT_F32 : UPD_NT_F64;
 };
 
+
+// position of an idle lane's stand-in particle: the centre of cell (x0, y0, z0)
+// (FP64 stores box coordinates, FP32 offsets in the cell)
+template <typename T> __device__ __forceinline__ void idle_pos(int x0, int y0, int z0, int N, T x[3])
+{
+    if constexpr (sizeof(T) == 8) {
+        x[0] = (x0 + 0.5) / N; x[1] = (y0 + 0.5) / N; x[2] = (z0 + 0.5) / N;
+    } else {
+        x[0] = x[1] = x[2] = (T)0.5;
+    }
+}
+
+// One particle of a kick / drift pass: the stencil values come from the
+// staged box fields[f * FS + ...] of a tile with origin (x0, y0, z0) and box
+// pitch SX, SY.  Writes q, position and cell; returns the new cell.
+template <typename T, int F, int SX, int SY>
+__device__ __forceinline__ int32_t upd_particle(const PV<T> &v, const UpdParams &u, int64_t p, int32_t cc, T xk[3],
+                                                T q[3], const T *fields, int x0, int y0, int z0, int FS, double &ret)
+{
+    constexpr bool kick = F & (UPD_KICK_GR | UPD_KICK_NEWTON);
+    constexpr bool drift = F & (UPD_DRIFT_GR | UPD_DRIFT_NEWTON);
+    const int N = v.N;
+    const int nf_k = u.nf_k;
+    const int nf_d = (F & UPD_DRIFT_GR) ? u.nf_d : 0;
+    const T Nn = (T)N;
+    int32_t newc = 0;
+    const int x = cc & (N - 1), y = (cc >> v.lg) & (N - 1), z = cc >> (2 * v.lg);
+    const int o = (x - x0 + 1) + SX * ((y - y0 + 1) + SY * (z - z0 + 1));
+    T r[3];
+    cell_rel(xk, x, y, z, N, r);
+    FieldLds<T> Fp{fields, o, SX, SX * SY}, Fc{fields + FS, o, SX, SX * SY};
+    FieldLds<T> B0{fields + 2 * FS, o, SX, SX * SY}, B1{fields + 3 * FS, o, SX, SX * SY},
+        B2{fields + 4 * FS, o, SX, SX * SY};
+    // FP32: kick and drift share weights and stencil reads; FP64 keeps
+    // them apart (the shared form needs more registers than it saves)
+    if constexpr ((F & (UPD_KICK_GR | UPD_DRIFT_GR)) == (UPD_KICK_GR | UPD_DRIFT_GR) && sizeof(T) == 4) {
+        double d[3];
+        T rv = phys_kick_drift<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn, nf_d, (T)u.a_d,
+                                  (T)u.bc_d, u.dtau_d, d);
+        ret = fmax(ret, (double)rv);
+        v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2];
+        newc = move_by(v, p, xk, x, y, z, d);
+    } else {
+        if constexpr (F & UPD_KICK_GR) {
+            T rv = phys_update_q<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn);
+            ret = fmax(ret, (double)rv);
+        } else if constexpr (F & UPD_KICK_NEWTON) {
+            T rv = phys_update_q_newton<T>(r, q, Fp, Fc, nf_k, (T)u.a_k, (T)u.dtau_k, Nn);
+            ret = fmax(ret, (double)rv);
+        }
+        if (kick) { v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2]; }
+        if (drift) {
+            double d[3];
+            if constexpr (F & UPD_DRIFT_GR) {
+                phys_update_pos<T>(r, q, Fp, Fc, B0, B1, B2, nf_d, (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
+            } else {
+                d[0] = u.dtau_d * (double)q[0] / u.a_d;
+                d[1] = u.dtau_d * (double)q[1] / u.a_d;
+                d[2] = u.dtau_d * (double)q[2] / u.a_d;
+            }
+            newc = move_by(v, p, xk, x, y, z, d);
+        }
+    }
+    return newc;
+}
+
+// row prefix of a tile: lane r < NROW gets the first container index of row r
+// and the inclusive prefix of the row lengths (after wave_row_scan)
+template <int TX, int TY, int NROW>
+__device__ __forceinline__ void tile_rows_load(const int32_t *start, int N, int x0, int y0, int z0, int lane,
+                                               int32_t &rb0, int32_t &len)
+{
+    rb0 = 0;
+    len = 0;
+    if (lane < NROW) {
+        const int ry = lane % TY, rz = lane / TY;
+        const uint32_t rb = (uint32_t)x0 + (uint32_t)N * ((uint32_t)(y0 + ry) + (uint32_t)N * (uint32_t)(z0 + rz));
+        rb0 = start[rb];
+        len = start[rb + TX] - rb0;
+    }
+}
+__device__ __forceinline__ int32_t wave_row_scan(int32_t len, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(len, o, 64);
+        if (lane >= o) len += y;
+    }
+    return len;
+}
+// container index of flattened particle f of a tile with row prefix (rb0,
+// incl); every lane runs the same steps (the shuffles need all lanes), lanes
+// with f beyond the tile's particles get -1
+template <int NROW> __device__ __forceinline__ int64_t tile_locate(int32_t rb0, int32_t incl, int f)
+{
+    const int32_t total = __shfl(incl, NROW - 1, 64);
+    int lo = 0;
+#pragma unroll
+    for (int step = NROW / 2; step >= 1; step >>= 1) {
+        const int j = lo + step < NROW ? lo + step : NROW - 1;
+        const int32_t rsj = __shfl(incl, j - 1, 64);  // first flattened index of row j
+        if (lo + step < NROW && rsj <= f) lo = j;
+    }
+    const int32_t rsl = __shfl(incl, lo > 0 ? lo - 1 : 0, 64);
+    const int32_t rbl = __shfl(rb0, lo, 64);
+    return f < total ? (int64_t)rbl + (f - (lo > 0 ? rsl : 0)) : (int64_t)-1;
+}
+
 template <typename T, int TX, int TY, int TZ, int F, int NT = UpdShape<T>::NT>
 __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const T *__restrict__ phi,
                                                         const T *__restrict__ chi, const T *__restrict__ B0g,
@@ -46,6 +153,9 @@ __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int x0, y0, z0;
     tile_origin<TX, TY, TZ>(N, x0, y0, z0);
+    const int32_t cell0 = (int32_t)((uint32_t)x0 + (uint32_t)N * ((uint32_t)y0 + (uint32_t)N * (uint32_t)z0));
+    T idle_x[3];
+    idle_pos<T>(x0, y0, z0, N, idle_x);
     // F: the kick / drift forms (compile time); UPD_COUNT in u.flags (run time)
     constexpr bool kick = F & (UPD_KICK_GR | UPD_KICK_NEWTON);
     constexpr bool drift = F & (UPD_DRIFT_GR | UPD_DRIFT_NEWTON);
@@ -75,106 +185,53 @@ __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const
         }
     }
     // particle ranges of the tile's TY*TZ rows, computed by every wave for
-    // itself (no barrier before the first particle loads): lane r < NROW
-    // holds the first container index of row r and the inclusive prefix of the
-    // row lengths
-    int32_t rb0 = 0, incl = 0;
-    if (lane < NROW) {
-        const int ry = lane % TY, rz = lane / TY;
-        const uint32_t rb = (uint32_t)x0 + (uint32_t)N * ((uint32_t)(y0 + ry) + (uint32_t)N * (uint32_t)(z0 + rz));
-        rb0 = v.start[rb];
-        incl = v.start[rb + TX] - rb0;
-    }
-    for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
+    // itself (no barrier before the first particle loads)
+    int32_t rb0, incl;
+    tile_rows_load<TX, TY, NROW>(v.start, N, x0, y0, z0, lane, rb0, incl);
+    incl = wave_row_scan(incl, lane);
     const int total = __shfl(incl, NROW - 1, 64);
     const int niter = (total + NT - 1) / NT;  // the same for every lane of the workgroup
-    // container index of flattened particle f; every lane runs the same steps
-    // (the shuffles need all lanes), lanes with f >= total get -1
-    auto locate = [&](int f) -> int64_t {
-        int lo = 0;
-#pragma unroll
-        for (int step = NROW / 2; step >= 1; step >>= 1) {
-            const int j = lo + step < NROW ? lo + step : NROW - 1;
-            const int32_t rsj = __shfl(incl, j - 1, 64);  // first flattened index of row j
-            if (lo + step < NROW && rsj <= f) lo = j;
-        }
-        const int32_t rsl = __shfl(incl, lo > 0 ? lo - 1 : 0, 64);
-        const int32_t rbl = __shfl(rb0, lo, 64);
-        return f < total ? (int64_t)rbl + (f - (lo > 0 ? rsl : 0)) : (int64_t)-1;
-    };
+    auto locate = [&](int f) { return tile_locate<NROW>(rb0, incl, f); };
     // the next particle of this lane, loaded one pass ahead of its use
     int64_t pn = locate(tid);
     int32_t cn = 0;
     T xn[3] = {(T)0, (T)0, (T)0}, qn[3] = {(T)0, (T)0, (T)0};
+    // idle lanes load the sink slot: the loads, like the stores below, run
+    // without a branch, which keeps the compiler's memory-counter waits exact
     auto fetch = [&]() {
-        if (pn >= 0) {
-            cn = v.cell[pn];
-            xn[0] = v.p0[pn]; xn[1] = v.p1[pn]; xn[2] = v.p2[pn];
-            qn[0] = v.q0[pn]; qn[1] = v.q1[pn]; qn[2] = v.q2[pn];
-        }
+        const int64_t lp = pn >= 0 ? pn : v.sink;
+        cn = v.cell[lp];
+        xn[0] = v.p0[lp]; xn[1] = v.p1[lp]; xn[2] = v.p2[lp];
+        qn[0] = v.q0[lp]; qn[1] = v.q1[lp]; qn[2] = v.q2[lp];
     };
     fetch();
     __syncthreads();  // staged fields
-    const T Nn = (T)N;
     double ret = 0.0;
     RankPending rpend{0, 0};
     int64_t ppend = -1;
     for (int it = 0; it < niter; it++) {
         const int64_t p = pn;
-        const int32_t cc = cn;
-        T xk[3] = {xn[0], xn[1], xn[2]}, q[3] = {qn[0], qn[1], qn[2]};
+        const bool active = p >= 0;
+        // idle lanes update a particle at rest in the tile's first cell and
+        // store it to the sink slot
+        const int32_t cc = active ? cn : cell0;
+        T xk[3], q[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            xk[d] = active ? xn[d] : idle_x[d];
+            q[d] = active ? qn[d] : (T)0;
+        }
         pn = locate(tid + (it + 1) * NT);
         fetch();
-        const bool active = p >= 0;
-        int32_t newc = 0;
-        if (active) {
-            const int x = cc & (N - 1), y = (cc >> v.lg) & (N - 1), z = cc >> (2 * v.lg);
-            const int o = (x - x0 + 1) + SX * ((y - y0 + 1) + SY * (z - z0 + 1));
-            T r[3];
-            cell_rel(xk, x, y, z, N, r);
-            FieldLds<T> Fp{fld[0], o, SX, SX * SY}, Fc{fld[1], o, SX, SX * SY};
-            FieldLds<T> B0{fld[2], o, SX, SX * SY}, B1{fld[3], o, SX, SX * SY}, B2{fld[4], o, SX, SX * SY};
-            // FP32: kick and drift share weights and stencil reads; FP64 keeps
-            // them apart (the shared form needs more registers than it saves)
-            if constexpr ((F & (UPD_KICK_GR | UPD_DRIFT_GR)) == (UPD_KICK_GR | UPD_DRIFT_GR) && sizeof(T) == 4) {
-                double d[3];
-                T rv = phys_kick_drift<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn, nf_d,
-                                          (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
-                ret = fmax(ret, (double)rv);
-                v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2];
-                newc = move_by(v, p, xk, x, y, z, d);
-            } else {
-                if constexpr (F & UPD_KICK_GR) {
-                    T rv = phys_update_q<T>(r, q, Fp, Fc, B0, B1, B2, nf_k, (T)u.a_k, (T)u.dtau_k, (T)u.bc_k, Nn);
-                    ret = fmax(ret, (double)rv);
-                } else if constexpr (F & UPD_KICK_NEWTON) {
-                    T rv = phys_update_q_newton<T>(r, q, Fp, Fc, nf_k, (T)u.a_k, (T)u.dtau_k, Nn);
-                    ret = fmax(ret, (double)rv);
-                }
-                if (kick) { v.q0[p] = q[0]; v.q1[p] = q[1]; v.q2[p] = q[2]; }
-                if (drift) {
-                    double d[3];
-                    if constexpr (F & UPD_DRIFT_GR) {
-                        phys_update_pos<T>(r, q, Fp, Fc, B0, B1, B2, nf_d, (T)u.a_d, (T)u.bc_d, u.dtau_d, d);
-                    } else {
-                        d[0] = u.dtau_d * (double)q[0] / u.a_d;
-                        d[1] = u.dtau_d * (double)q[1] / u.a_d;
-                        d[2] = u.dtau_d * (double)q[2] / u.a_d;
-                    }
-                    newc = move_by(v, p, xk, x, y, z, d);
-                }
-            }
-        }
+        const int32_t newc =
+            upd_particle<T, F, SX, SY>(v, u, active ? p : v.sink, cc, xk, q, fld[0], x0, y0, z0, NS, ret);
         // re-binning rank in the new cell, one counter atomic per run of equal
         // cells; each pass stores the rank of the previous one, so the
         // atomic's round trip overlaps a pass of work
         if (drift && (u.flags & UPD_COUNT)) {
             if (it > 0) {
                 const int32_t rk = wave_rank_finish(rpend);
-                if (ppend >= 0) rank[ppend] = rk;
+                rank[ppend >= 0 ? ppend : v.sink] = rk;
             }
             rpend = wave_rank_issue(count, newc, active);
             ppend = p;
@@ -182,10 +239,11 @@ __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const
     }
     if (drift && (u.flags & UPD_COUNT) && niter > 0) {
         const int32_t rk = wave_rank_finish(rpend);
-        if (ppend >= 0) rank[ppend] = rk;
+        rank[ppend >= 0 ? ppend : v.sink] = rk;
     }
     if (kick) block_max_to<NT>(ret, maxslot);
 }
+
 
 template <typename T> int pcls_sort_impl(gev_pcls *p);
 template <typename T> int pcls_sort_counted_impl(gev_pcls *p);
@@ -198,8 +256,8 @@ static void launch_tile(gev_pcls *p, const UpdParams &u, const T *phi, const T *
     const int N = c->N;
     const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
     if constexpr (!(sizeof(T) == 8 && TX == 16))  // the large tile exists for FP32 only
-        hipLaunchKernelGGL((k_update_tiled<T, TX, TY, TZ, F>), dim3(nb), dim3(UpdShape<T>::NT), 0, c->stream, make_pv<T>(p), u, phi,
-                           chi, B0, B1, B2, slot, p->cell_count, p->rank);
+        hipLaunchKernelGGL((k_update_tiled<T, TX, TY, TZ, F>), dim3(nb), dim3(UpdShape<T>::NT), 0, c->stream,
+                           make_pv<T>(p), u, phi, chi, B0, B1, B2, slot, p->cell_count, p->rank);
 }
 
 template <typename T, int TX, int TY, int TZ>

@@ -262,11 +262,12 @@ def test_deposits_empty_and_single(gp, O):
     assert relmax(real_np(out)[0], O.project_T00(N, pos, np.zeros((3, 1)), 1.0)) < 1e-14
 
 
-@pytest.mark.parametrize("N", [16, 64])
-def test_deposits_mixed(gp, O, N):
+@pytest.mark.parametrize("N,cluster", [(16, False), (64, False), (64, True)])
+def test_deposits_mixed(gp, O, N, cluster):
+    """clustered: half the particles in 8 cells (LDS atomics on shared sites)"""
     ctx = gp.Context(N, gp.GEV_MIXED)
     n = 4 * N ** 3 // 8
-    pos, q, pid = make_particles(N, n, 400 + N, vmax=0.2)
+    pos, q, pid = make_particles(N, n, 400 + N, vmax=0.2, cluster=cluster)
     m, a = 0.31 / n, 0.05
     p = gpu_pcls(gp, ctx, pos, q, pid, m)
     # the mixed form rounds offsets and momenta to FP32: compare with the oracle on those inputs
@@ -430,6 +431,39 @@ def test_fused_kick_drift_vs_oracle(gp, O, gr):
     assert np.isclose(ret, rref, rtol=1e-13)
     cells = to_np(p.cells())
     c = np.floor(gpos * N).astype(np.int64)
+    assert np.array_equal(np.sort(cells), np.sort(c[0] + N * (c[1] + N * c[2])))
+
+
+@pytest.mark.parametrize("N,cluster", [(32, False), (128, False), (128, True)])
+def test_kick_drift_mixed(gp, O, N, cluster):
+    """mixed-form fused GR kick+drift (FP32 offsets, momenta and fields) against
+    the FP64 oracle run on the same FP32-rounded inputs; the clustered case
+    puts half the particles in 8 cells (many passes over one tile)"""
+    ctx = gp.Context(N, gp.GEV_MIXED)
+    n = N ** 3 // 2
+    pos, q, pid = make_particles(N, n, 1300 + N, vmax=0.3, cluster=cluster)
+    rng = np.random.default_rng(N + 5)
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    phi, chi = f32(1e-3 * rng.standard_normal(N ** 3)), f32(1e-4 * rng.standard_normal(N ** 3))
+    B = f32(1e-2 * rng.standard_normal((3, N ** 3)))
+    p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
+    gpos, gq, gid = by_id(p)
+    a_k, a_d = 0.2, 0.21
+    args = (field_gpu(ctx, phi), field_gpu(ctx, chi), field_gpu(ctx, B, 3))
+    ret = p.kick_drift(1, 0.013, 0.02, *args, 3, 3, a_k, a_d, a_k ** 2 * N, a_d ** 2 * N)
+    qo, po = gq.copy(), gpos.copy()
+    rref = O.update_q(N, po, qo, 0.013, phi, chi, B, 3, a_k, a_k ** 2 * N)
+    O.update_pos(N, po, qo, 0.02, phi, chi, B, 3, a_d, a_d ** 2 * N)
+    npos, nq, nid = by_id(p)
+    assert np.array_equal(nid, gid)
+    # FP32 stencil sums: a few ulp of the kick; offsets round to 2^-24 of a
+    # cell, and the drift carries the kick's FP32 error
+    assert relmax(nq, qo) < 3e-6
+    assert np.abs(((npos - po) + 0.5) % 1.0 - 0.5).max() < 2e-6 / N
+    assert np.isclose(ret, rref, rtol=1e-5)
+    cells = to_np(p.cells())
+    assert np.all(np.diff(cells) >= 0)
+    c = np.floor(npos * N).astype(np.int64) % N
     assert np.array_equal(np.sort(cells), np.sort(c[0] + N * (c[1] + N * c[2])))
 
 

@@ -520,6 +520,21 @@ int gev_solve_chi(gev_ctx *c, const void *Sij, void *chi, void *SijFT, void *chi
     return rc;
 }
 
+int gev_solve_chi_Tij(gev_ctx *c, const void *phi, void *Tij, double coeff, void *chi, void *SijFT, void *chiFT)
+{
+    CHECK(c && phi && Tij && chi && SijFT && chiFT, "gev_solve_chi_Tij: NULL argument");
+    if (chi_from_Tij_supported(c))
+        return PREC_DISPATCH(c,
+                             solve_chi_Tij_fused_impl<double>(c, (const double *)phi, (const double *)Tij, coeff,
+                                                              (double *)chi, (cx<double> *)SijFT, (cx<double> *)chiFT),
+                             solve_chi_Tij_fused_impl<float>(c, (const float *)phi, (const float *)Tij, coeff,
+                                                             (float *)chi, (cx<float> *)SijFT, (cx<float> *)chiFT));
+    // small meshes: Sij in place over Tij, then the chi solve
+    int rc = gev_prepare_source_Tij(c, phi, Tij, Tij, coeff);
+    if (!rc) rc = gev_solve_chi(c, Tij, chi, SijFT, chiFT);
+    return rc;
+}
+
 int gev_solve_vector(gev_ctx *c, void *Bi, void *BiFT, double coeff, double modif)
 {
     CHECK(c && Bi && BiFT, "gev_solve_vector: NULL argument");

@@ -135,15 +135,16 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
             RC(mark(c, GEV_PHASE_SOLVE_PHI));
         }
         // chi (main.cu:822-902) and B (main.cu:908-955)
-        RC(gev_prepare_source_Tij(c, f->phi, f->Sij, f->Sij, 2. * fpg * dx * dx / a));
-        RC(mark(c, GEV_PHASE_SOURCE_TIJ));
         if (st->vector_elliptic) {
+            RC(mark(c, GEV_PHASE_SOURCE_TIJ));  // formed inside the chi solve's x passes
             RC(gev_project_T0i(p, f->Bi, f->phi, 1.0, 0));
             RC(mark(c, GEV_PHASE_DEPOSIT_T0I));
-            RC(gev_solve_chi(c, f->Sij, f->chi, f->SijFT, f->scalarFT));
+            RC(gev_solve_chi_Tij(c, f->phi, f->Sij, 2. * fpg * dx * dx / a, f->chi, f->SijFT, f->scalarFT));
             RC(mark(c, GEV_PHASE_SOLVE_CHI));
             RC(gev_solve_vector(c, f->Bi, f->BiFT, fpg * dx * dx, 0.0));
         } else {
+            RC(gev_prepare_source_Tij(c, f->phi, f->Sij, f->Sij, 2. * fpg * dx * dx / a));
+            RC(mark(c, GEV_PHASE_SOURCE_TIJ));
             // parabolic: SijFT feeds evolveFTvector, B persists in k-space
             RC(gev_fft_r2c(c, f->Sij, f->SijFT, 6));
             RC(gev_project_ft_scalar(c, f->SijFT, f->scalarFT, 0));

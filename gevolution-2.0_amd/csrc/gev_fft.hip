@@ -188,18 +188,73 @@ __global__ __launch_bounds__(TX * L / 8) void k_col_reg(cx<T> *__restrict__ data
     reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, dir, ld, st);
 }
 
+// Sources of the r2c x pass: the pair of reals (2n, 2n+1) of line L (= y + N z)
+template <typename T, int N> struct SrcPlain {
+    const T *in;
+    __device__ __forceinline__ cx<T> operator()(int64_t L, int n) const
+    {
+        return reinterpret_cast<const cx<T> *>(in)[L * (N / 2) + n];
+    }
+};
+// component C of the Sij source of prepareFTsource (gevolution.hpp:1490-1560),
+// computed from Tij and phi while the pass reads it (the same expressions as
+// k_prepare_Tij); C: 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz
+template <typename T, int N, int C> struct SrcSij {
+    const T *phi, *Tc;  // Tc: component C of Tij
+    T coeff;
+    __device__ __forceinline__ cx<T> operator()(int64_t L, int n) const
+    {
+        constexpr int lg = __builtin_ctz(N), M1 = N - 1;
+        const int y = (int)(L & M1), z = (int)(L >> lg);
+        const int x = 2 * n, xm = (x - 1) & M1, xp2 = (x + 2) & M1;
+        auto row = [&](int dy, int dz) {
+            return phi + (int64_t)((y + dy) & M1) * N + (int64_t)((z + dz) & M1) * N * N;
+        };
+        const cx<T> t = reinterpret_cast<const cx<T> *>(Tc + L * N)[n];
+        const T *r0 = phi + L * N;
+        const cx<T> pp = reinterpret_cast<const cx<T> *>(r0)[n];  // phi at x, x+1
+        const T h = (T)0.5;
+        cx<T> out;
+        if constexpr (C == 0) {
+            const T pm = r0[xm], p2 = r0[xp2];
+            out.x = h * (pp.y - pm) * (pp.y - pm) + coeff * t.x;
+            out.y = h * (p2 - pp.x) * (p2 - pp.x) + coeff * t.y;
+        } else if constexpr (C == 3 || C == 5) {
+            const T *ra = C == 3 ? row(1, 0) : row(0, 1), *rb = C == 3 ? row(-1, 0) : row(0, -1);
+            const cx<T> a = reinterpret_cast<const cx<T> *>(ra)[n], b = reinterpret_cast<const cx<T> *>(rb)[n];
+            out.x = h * (a.x - b.x) * (a.x - b.x) + coeff * t.x;
+            out.y = h * (a.y - b.y) * (a.y - b.y) + coeff * t.y;
+        } else if constexpr (C == 1 || C == 2) {
+            // px py - p pxy + (p^2 - px^2 - py^2 + pxy^2)/2 with y -> z for C = 2
+            const T *r1 = C == 1 ? row(1, 0) : row(0, 1);
+            const cx<T> q = reinterpret_cast<const cx<T> *>(r1)[n];
+            const T p2 = r0[xp2], q2 = r1[xp2];
+            auto f = [&](T p, T px, T py, T pxy) { return px * py - p * pxy + h * (p * p - px * px - py * py + pxy * pxy); };
+            out.x = f(pp.x, pp.y, q.x, q.y) + coeff * t.x;
+            out.y = f(pp.y, p2, q.y, q2) + coeff * t.y;
+        } else {
+            // py pz - p pyz + (p^2 - py^2 - pz^2 + pyz^2)/2
+            const cx<T> a = reinterpret_cast<const cx<T> *>(row(1, 0))[n], b = reinterpret_cast<const cx<T> *>(row(0, 1))[n],
+                        d = reinterpret_cast<const cx<T> *>(row(1, 1))[n];
+            auto f = [&](T p, T py, T pz, T pyz) { return py * pz - p * pyz + h * (p * p - py * py - pz * pz + pyz * pyz); };
+            out.x = f(pp.x, a.x, b.x, d.x) + coeff * t.x;
+            out.y = f(pp.y, a.y, b.y, d.y) + coeff * t.y;
+        }
+        return out;
+    }
+};
+
 // x pass r2c: NL consecutive lines of N reals -> N/2+1 complex (pitch P)
-template <typename T, int N, int NL>
-__global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(const T *__restrict__ in, cx<T> *__restrict__ out, int P,
-                                                           int64_t nlines, const cx<T> *__restrict__ tw)
+template <typename T, int N, int NL, typename Src>
+__global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(Src src, cx<T> *__restrict__ out, int P, int64_t nlines,
+                                                           const cx<T> *__restrict__ tw)
 {
     constexpr int H = N / 2, NT = NL * H / 8, LDL = H + 1;
     __shared__ cx<T> lds[NL * LDL];
     __shared__ cx<T> twl[N];
     for (int m = threadIdx.x; m < N; m += NT) twl[m] = tw[m];
     const int64_t line0 = (int64_t)blockIdx.x * NL;
-    const cx<T> *src = reinterpret_cast<const cx<T> *>(in) + line0 * H;
-    auto ld = [&](int line, int n) { return (line0 + line < nlines) ? src[line * H + n] : mk((T)0, (T)0); };
+    auto ld = [&](int line, int n) { return (line0 + line < nlines) ? src(line0 + line, n) : mk((T)0, (T)0); };
     auto st = [&](int line, int k, cx<T> v, int) { lds[line * LDL + k] = v; };
     __syncthreads();
     reg_fft<T, H, NL, false, 2, true, 1>(lds, twl, -1, ld, st);
@@ -254,14 +309,20 @@ __global__ __launch_bounds__(NL * N / 16) void k_c2r_x_reg(const cx<T> *__restri
 
 // ---------------------------------------------------------- dispatchers ---
 
+template <typename T, int N, typename Src>
+static void launch_r2c_src(const Src &src, cx<T> *out, int P, int64_t nlines, const cx<T> *tw, hipStream_t s)
+{
+    constexpr int NL = xlines<T, N>();
+    int64_t nb = (nlines + NL - 1) / NL;
+    hipLaunchKernelGGL((k_r2c_x_reg<T, N, NL, Src>), dim3((unsigned)nb), dim3(NL * N / 16), 0, s, src, out, P, nlines,
+                       tw);
+}
+
 template <typename T, int N>
 static void launch_r2c(const T *in, cx<T> *out, int P, int64_t nlines, const cx<T> *tw, hipStream_t s)
 {
     if constexpr (N >= 128) {
-        constexpr int NL = xlines<T, N>();
-        int64_t nb = (nlines + NL - 1) / NL;
-        hipLaunchKernelGGL((k_r2c_x_reg<T, N, NL>), dim3((unsigned)nb), dim3(NL * N / 16), 0, s, in, out, P, nlines,
-                           tw);
+        launch_r2c_src<T, N>(SrcPlain<T, N>{in}, out, P, nlines, tw, s);
     } else {
         constexpr int NLINES = (4096 / N > 1024) ? 1024 : 4096 / N;
         int64_t nb = (nlines + NLINES - 1) / NLINES;
@@ -551,6 +612,45 @@ template <typename T> int solve_chi_fused_impl(gev_ctx *c, const T *Sij, T *chi,
     return zsolve_dispatch<T>(c, ZOP_SCALAR, Sij, 6, SijFT, chiFT, chi, 1, zp);
 }
 
+// chi from Tij: the x passes compute Sij = prepareFTsource(phi, Tij, coeff)
+// on the fly (N >= 128; the caller runs the unfused sequence below that)
+template <typename T, int N>
+static void chi_from_Tij_chain(gev_ctx *c, const T *phi, const T *Tij, T coeff, cx<T> *SijFT, cx<T> *chiFT, T *chi)
+{
+    const int64_t nl = (int64_t)N * N;
+    const cx<T> *tw = twiddles_of<T>(c);
+    auto xy = [&](const auto &src, int i) {
+        launch_r2c_src<T, N>(src, SijFT + i * c->NK, c->P, nl, tw, c->stream);
+        launch_col<T, N>(SijFT + i * c->NK, c->P, 1, -1, tw, c->stream);
+    };
+    const int64_t M = c->M;
+    xy(SrcSij<T, N, 0>{phi, Tij, coeff}, 0);
+    xy(SrcSij<T, N, 1>{phi, Tij + M, coeff}, 1);
+    xy(SrcSij<T, N, 2>{phi, Tij + 2 * M, coeff}, 2);
+    xy(SrcSij<T, N, 3>{phi, Tij + 3 * M, coeff}, 3);
+    xy(SrcSij<T, N, 4>{phi, Tij + 4 * M, coeff}, 4);
+    xy(SrcSij<T, N, 5>{phi, Tij + 5 * M, coeff}, 5);
+    ZParams zp{c->gk2, c->kshift, 0.0, 0.0, 0.0, 0.0, 0};
+    launch_zsolve<T, N, ZOP_SCALAR>(SijFT, chiFT, c->P, c->NK, tw, zp, c->stream);
+    inv_yx<T, N>(c, chiFT, chi);
+}
+
+bool chi_from_Tij_supported(const gev_ctx *c) { return c->N >= 128 && c->N <= 1024; }
+
+template <typename T>
+int solve_chi_Tij_fused_impl(gev_ctx *c, const T *phi, const T *Tij, double coeff, T *chi, cx<T> *SijFT, cx<T> *chiFT)
+{
+    switch (c->N) {
+    case 128: chi_from_Tij_chain<T, 128>(c, phi, Tij, (T)coeff, SijFT, chiFT, chi); break;
+    case 256: chi_from_Tij_chain<T, 256>(c, phi, Tij, (T)coeff, SijFT, chiFT, chi); break;
+    case 512: chi_from_Tij_chain<T, 512>(c, phi, Tij, (T)coeff, SijFT, chiFT, chi); break;
+    case 1024: chi_from_Tij_chain<T, 1024>(c, phi, Tij, (T)coeff, SijFT, chiFT, chi); break;
+    default: gev_set_error("chi from Tij needs 128 <= N <= 1024"); return GEV_ERR_ARG;
+    }
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
 // Bi = c2r(projectFTvector(r2c(Bi))), in place on Bi
 template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT, double coeff, double modif)
 {
@@ -559,6 +659,7 @@ template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT
 }
 
 #define INSTZ(T)                                                                                       \
+    template int solve_chi_Tij_fused_impl<T>(gev_ctx *, const T *, const T *, double, T *, cx<T> *, cx<T> *); \
     template int solve_poisson_fused_impl<T>(gev_ctx *, const T *, T *, cx<T> *, double, double, int, double); \
     template int solve_chi_fused_impl<T>(gev_ctx *, const T *, T *, cx<T> *, cx<T> *);                  \
     template int solve_vector_fused_impl<T>(gev_ctx *, T *, cx<T> *, double, double);

@@ -21,10 +21,13 @@ template <typename T> int set_k0_impl(gev_ctx *ctx, cx<T> *ft, double re, double
 // r2c -> k-space operator -> c2r with the operator applied inside the z pass
 // (64 <= N <= 1024); the FT arguments are work space of the usual layout
 bool zsolve_supported(const gev_ctx *c);
+bool chi_from_Tij_supported(const gev_ctx *c);
 template <typename T>
 int solve_poisson_fused_impl(gev_ctx *c, const T *src, T *pot, cx<T> *work, double coeff, double modif, int k0set,
                              double k0re);
 template <typename T> int solve_chi_fused_impl(gev_ctx *c, const T *Sij, T *chi, cx<T> *SijFT, cx<T> *chiFT);
+template <typename T>
+int solve_chi_Tij_fused_impl(gev_ctx *c, const T *phi, const T *Tij, double coeff, T *chi, cx<T> *SijFT, cx<T> *chiFT);
 template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT, double coeff, double modif);
 
 template <typename T>

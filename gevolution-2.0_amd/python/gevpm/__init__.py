@@ -161,6 +161,16 @@ class Context:
         check(lib.gev_solve_chi(self._h, _ptr(Sij), _ptr(out), _ptr(work6), _ptr(work1)), "gev_solve_chi")
         return out
 
+    def solve_chi_Tij(self, phi, Tij, coeff, out=None, work6=None, work1=None):
+        """c2r(projectFTscalar(r2c(prepareFTsource(phi, Tij, coeff)))); Tij holds Sij afterwards for N < 128"""
+        out = self.real(1) if out is None else out
+        work6 = self.fourier(6) if work6 is None else work6
+        work1 = self.fourier(1) if work1 is None else work1
+        self.bind_stream()
+        check(lib.gev_solve_chi_Tij(self._h, _ptr(phi), _ptr(Tij), coeff, _ptr(out), _ptr(work6), _ptr(work1)),
+              "gev_solve_chi_Tij")
+        return out
+
     def solve_vector(self, Bi, coeff=1.0, modif=0.0, work=None):
         """Bi <- c2r(projectFTvector(r2c(Bi))), in place"""
         work = self.fourier(3) if work is None else work

@@ -70,6 +70,7 @@ SIGNATURES = {
     "gev_set_k0": (_i, [_vp, _vp, _d, _d]),
     "gev_solve_poisson": (_i, [_vp, _vp, _vp, _vp, _d, _d, _i, _d]),
     "gev_solve_chi": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "gev_solve_chi_Tij": (_i, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gev_solve_vector": (_i, [_vp, _vp, _vp, _d, _d]),
     "gev_update_q": (_i, [_vp, _d, _vp, _vp, _vp, _i, _d, _d, _dp]),
     "gev_update_q_newton": (_i, [_vp, _d, _vp, _vp, _i, _d, _dp]),

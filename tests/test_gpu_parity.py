@@ -510,6 +510,16 @@ def test_fused_solves_vs_unfused(gp, N, prec):
     ref = ctx.fft_c2r(ctx.project_ft_scalar(ctx.fft_r2c(S)), preserve=False)
     got = ctx.solve_chi(S)
     assert relmax(to_np(got), to_np(ref)) < tol
+    # chi with Sij formed inside the x passes (N >= 128; the sequence below that)
+    phi = 1e-3 * torch.randn(1, N, N, N, generator=g, dtype=torch.float64).to(ctx.rdtype).to(ctx.device)
+    T = torch.randn(6, N, N, N, generator=g, dtype=torch.float64).to(ctx.rdtype).to(ctx.device)
+    Sref = ctx.prepare_source_Tij(phi, T, 0.7)
+    ref = ctx.solve_chi(Sref)
+    T2 = T.clone()
+    got = ctx.solve_chi_Tij(phi, T2, 0.7)
+    assert relmax(to_np(got), to_np(ref)) < tol
+    if N >= 128:
+        assert torch.equal(T2, T)  # Tij is only read
     # B from a random vector source, in place
     Bsrc = torch.randn(3, N, N, N, generator=g, dtype=torch.float64).to(ctx.rdtype).to(ctx.device)
     ref = ctx.fft_c2r(ctx.project_ft_vector(ctx.fft_r2c(Bsrc), 0.25, 0.0), preserve=False)

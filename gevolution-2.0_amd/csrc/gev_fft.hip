@@ -478,9 +478,54 @@ __global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *o
         forward(0, std::integral_constant<int, 0>{});
         forward(1, std::integral_constant<int, 1>{});
         forward(2, std::integral_constant<int, 2>{});
+    } else if constexpr (OP == ZOP_POISSON) {
+        forward(0, std::integral_constant<int, 0>{});
     } else {
-#pragma unroll 1
-        for (int c = 0; c < NIN; c++) forward(c, std::integral_constant<int, 0>{});
+        // projectFTscalar: chi = sum_c w_c S_c, and every w_c is linear in the
+        // kz factors g2 = gk2[kz] and k2 = ks[kz]; three combinations formed
+        // before the z transform carry the whole projection:
+        //   A = (g1 - 2 g0) S_xx - 6 k0 k1 S_xy + (g0 - 2 g1) S_yy + (g0 + g1) S_zz
+        //   B = S_xx + S_yy - 2 S_zz,   C = -6 (k0 S_xz + k1 S_yz)
+        //   sum_c w_c S_c = FFT_z(A) + g2 FFT_z(B) + k2 FFT_z(C)
+        // Each thread combines the 8 elements reg_fft's first stage reads
+        // (line t % TX, rows t / TX + r L/8).
+        constexpr int NB = L / 8;
+        const int line = threadIdx.x % TX, jt = threadIdx.x / TX;
+        const int kx = ct * TX + line;
+        const bool valid = kx <= H;  // padding columns give 0 below
+        const double g0 = valid ? zp.gk2[kx] : 0.0;
+        const cx<double> k0 = valid ? zp.ks[kx] : mk(0.0, 0.0);
+        const cx<double> k01 = (k0 * kyv) * -6.0, k0m = k0 * -6.0, k1m = kyv * -6.0;
+        const T wa0 = (T)(gy - 2.0 * g0), wa3 = (T)(g0 - 2.0 * gy), wa5 = (T)(g0 + gy);
+        const cx<T> wa1 = mk((T)k01.x, (T)k01.y), wc2 = mk((T)k0m.x, (T)k0m.y), wc4 = mk((T)k1m.x, (T)k1m.y);
+        cx<T> A[8], B[8], C[8];
+        const cx<T> *src = in + off + line;
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int64_t e = (int64_t)(jt + r * NB) * stride;
+            const cx<T> s0 = src[e], s1 = src[NK + e], s2 = src[2 * NK + e], s3 = src[3 * NK + e],
+                        s4 = src[4 * NK + e], s5 = src[5 * NK + e];
+            A[r] = s0 * wa0 + wa1 * s1 + s3 * wa3 + s5 * wa5;
+            B[r] = s0 + s3 - s5 * (T)2;
+            C[r] = wc2 * s2 + wc4 * s4;
+        }
+        auto fft = [&](const cx<T> *pre, auto kind) {
+            constexpr int K = decltype(kind)::value;
+            auto ld = [&](int, int row) { return pre[(row - jt) / NB]; };
+            auto st = [&](int ln, int row, cx<T> v, int slot) {
+                lines[slot] = ln;
+                poss[slot] = row;
+                const cx<double> d = mk((double)v.x, (double)v.y);
+                if constexpr (K == 0) acc[slot] = d;
+                else if constexpr (K == 1) acc[slot] = acc[slot] + d * zp.gk2[row];
+                else acc[slot] = acc[slot] + zp.ks[row] * d;
+            };
+            __syncthreads();  // twiddles loaded; previous transform done with lds
+            reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, -1, ld, st);
+        };
+        fft(A, std::integral_constant<int, 0>{});
+        fft(B, std::integral_constant<int, 1>{});
+        fft(C, std::integral_constant<int, 2>{});
     }
     // the operator, per owned mode; the outputs replace the inputs in vin
 #pragma unroll
@@ -536,7 +581,7 @@ __global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *o
 // threads per workgroup there; Poisson and scalar use the column pass's width
 template <typename T, int L, int OP> constexpr int zs_tx()
 {
-    constexpr bool narrow = OP == ZOP_VECTOR || (OP == ZOP_SCALAR && L >= 1024);
+    constexpr bool narrow = OP == ZOP_VECTOR || OP == ZOP_SCALAR;
     return (narrow && col_tx<T, L>() * L / 8 > 512) ? 4096 / L : col_tx<T, L>();
 }
 

@@ -475,9 +475,30 @@ __global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *o
         reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, -1, ld, st);
     };
     if constexpr (OP == ZOP_VECTOR) {
-        forward(0, std::integral_constant<int, 0>{});
-        forward(1, std::integral_constant<int, 1>{});
-        forward(2, std::integral_constant<int, 2>{});
+        // all 24 first-stage operands (3 components x the 8 elements reg_fft's
+        // first stage reads: line t % TX, rows t / TX + r L/8) are loaded
+        // before the first transform
+        constexpr int NB = L / 8;
+        const int line = threadIdx.x % TX, jt = threadIdx.x / TX;
+        cx<T> pre[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 8; r++) pre[c][r] = in[c * NK + off + (int64_t)(jt + r * NB) * stride + line];
+        auto fwd = [&](auto held) {
+            constexpr int CH = decltype(held)::value;
+            auto ld = [&](int, int row) { return pre[CH][(row - jt) / NB]; };
+            auto st = [&](int ln, int row, cx<T> v, int slot) {
+                lines[slot] = ln;
+                poss[slot] = row;
+                vin[CH][slot] = v;
+            };
+            __syncthreads();  // twiddles loaded; previous transform done with lds
+            reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, -1, ld, st);
+        };
+        fwd(std::integral_constant<int, 0>{});
+        fwd(std::integral_constant<int, 1>{});
+        fwd(std::integral_constant<int, 2>{});
     } else if constexpr (OP == ZOP_POISSON) {
         forward(0, std::integral_constant<int, 0>{});
     } else {

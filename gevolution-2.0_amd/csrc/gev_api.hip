@@ -85,6 +85,9 @@ int gev_ctx_create(int N, int precision, void *hip_stream, gev_ctx **out)
         GEV_HIP(hipStreamCreate(&c->stream));
         c->own_stream = true;
     }
+    GEV_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));  // no implicit sync with the null stream
+    GEV_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+    GEV_HIP(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
     // twiddles exp(-2 pi i m/N) and k-space tables (gevolution.hpp:269-274)
     std::vector<cx<double>> tw(N), ks(N);
     std::vector<cx<float>> twf(N);
@@ -126,6 +129,12 @@ int gev_ctx_destroy(gev_ctx *c)
     (void)hipFree(c->scan_blocks);
     for (int i = 0; i < gev_ctx::MAX_MARKS; i++)
         if (c->marks[i]) (void)hipEventDestroy(c->marks[i]);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GEV_OK;

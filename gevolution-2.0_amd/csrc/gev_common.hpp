@@ -80,6 +80,10 @@ struct gev_ctx {
     int64_t NK = 0;        // P*N*N
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // second stream of gev_step (default priority, like the first): the T0i
+    // deposit and the B solve run on it beside the chi solve
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     int device = 0;
     int ncu = 0;           // compute units of the device
     // twiddles exp(-2 pi i m / N), m in [0,N): double2 and float2 copies

@@ -136,12 +136,26 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
         }
         // chi (main.cu:822-902) and B (main.cu:908-955)
         if (st->vector_elliptic) {
-            RC(mark(c, GEV_PHASE_SOURCE_TIJ));  // formed inside the chi solve's x passes
-            RC(gev_project_T0i(p, f->Bi, f->phi, 1.0, 0));
+            // Sij is formed inside the chi solve's x passes.  The T0i deposit
+            // and the B solve (Bi, BiFT) share no buffer with the chi solve
+            // (chi, SijFT, scalarFT; both read phi): they run on the side
+            // stream while the chi solve runs on the main one, which waits for
+            // them before the kick.  Phases: deposit_T0i is the fork (~0),
+            // solve_chi the chi solve beside them, solve_B the wait after it.
+            RC(mark(c, GEV_PHASE_SOURCE_TIJ));
+            GEV_HIP(hipEventRecord(c->fork, c->stream));
+            GEV_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
+            hipStream_t main_stream = c->stream;
+            c->stream = c->side;
+            int rc_side = gev_project_T0i(p, f->Bi, f->phi, 1.0, 0);
+            if (!rc_side) rc_side = gev_solve_vector(c, f->Bi, f->BiFT, fpg * dx * dx, 0.0);
+            c->stream = main_stream;
+            GEV_HIP(hipEventRecord(c->join, c->side));
+            RC(rc_side);
             RC(mark(c, GEV_PHASE_DEPOSIT_T0I));
             RC(gev_solve_chi_Tij(c, f->phi, f->Sij, 2. * fpg * dx * dx / a, f->chi, f->SijFT, f->scalarFT));
             RC(mark(c, GEV_PHASE_SOLVE_CHI));
-            RC(gev_solve_vector(c, f->Bi, f->BiFT, fpg * dx * dx, 0.0));
+            GEV_HIP(hipStreamWaitEvent(c->stream, c->join, 0));
         } else {
             RC(gev_prepare_source_Tij(c, f->phi, f->Sij, f->Sij, 2. * fpg * dx * dx / a));
             RC(mark(c, GEV_PHASE_SOURCE_TIJ));

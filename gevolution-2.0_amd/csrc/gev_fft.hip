@@ -431,7 +431,6 @@ __global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *o
                                                        const cx<T> *__restrict__ tw, ZParams zp)
 {
     constexpr int NT = TX * L / 8, LDL = L + 1;
-    constexpr int NIN = OP == ZOP_SCALAR ? 6 : (OP == ZOP_VECTOR ? 3 : 1);
     constexpr int NOUT = OP == ZOP_VECTOR ? 3 : 1;
     constexpr int NV = OP == ZOP_VECTOR ? 3 : 1;  // inputs held at once
     __shared__ cx<T> lds[TX * LDL];

@@ -228,7 +228,10 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st);
  * -DBENCHMARK, main.cu:1340-1371).  When enabled, gev_step records HIP
  * events at its phase boundaries; gev_step_timings returns, for the last
  * step, ids[i] (a GEV_PHASE_* value) and the milliseconds ms[i] spent in
- * that phase.  Returns the number of phases written (<= max). */
+ * that phase.  Returns the number of phases written (<= max).  In the GR
+ * elliptic cycle the T0i deposit and the B solve run on a second stream
+ * beside the chi solve: there DEPOSIT_T0I marks the fork (~0 ms), SOLVE_CHI
+ * is the chi solve running beside them and SOLVE_B the wait that follows. */
 enum {
     GEV_PHASE_DEPOSIT_T00_TIJ = 0, /* GR: T00 and Tij in one pass; Newton: the mass deposit */
     GEV_PHASE_SOURCE_T00 = 1, GEV_PHASE_SOLVE_PHI = 2, GEV_PHASE_SOURCE_TIJ = 3, GEV_PHASE_DEPOSIT_T0I = 4,

@@ -196,7 +196,7 @@ template <typename T, int N> struct SrcPlain {
         return reinterpret_cast<const cx<T> *>(in)[L * (N / 2) + n];
     }
 };
-// component C of the Sij source of prepareFTsource (gevolution.hpp:1490-1560),
+// component C of the Sij source of prepareFTsource_Tij (gevolution.hpp:64-107),
 // computed from Tij and phi while the pass reads it (the same expressions as
 // k_prepare_Tij); C: 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz
 template <typename T, int N, int C> struct SrcSij {

@@ -130,7 +130,7 @@ int gev_solve_poisson(gev_ctx *ctx, const void *src, void *pot, void *workFT, do
                       double k0_re);
 /* plan_Sij FFT_FORWARD, projectFTscalar (add = 0), plan_chi FFT_BACKWARD (main.cu:829-895) */
 int gev_solve_chi(gev_ctx *ctx, const void *Sij, void *chi, void *SijFT, void *chiFT);
-/* prepareFTsource<Cplx>(phi, Sij, Sij, coeff) (gevolution.hpp:1490-1560,
+/* prepareFTsource<Cplx>(phi, Sij, Sij, coeff) (gevolution.hpp:64-119,
  * main.cu:822-828) followed by gev_solve_chi: for 128 <= N <= 1024 the x
  * passes read Tij and phi and form Sij on the fly, so Sij is never stored;
  * Tij is left unchanged there and holds Sij on return for smaller N. */

@@ -37,6 +37,10 @@ __device__ __forceinline__ void lds_add(double *a, double v)
     __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+#ifndef DEP_SPREAD_X4
+#define DEP_SPREAD_X4 5  // four times the mean particles per halo-box cell above which a tile spreads
+#endif
+
 template <typename T, int KIND, int TX, int TY, int TZ>
 __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int64_t M, const T *__restrict__ phi,
                                                    double mass_d, double mass2_d, double a_d, int accumulate,
@@ -103,12 +107,20 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
     constexpr int U = (KIND == DEP_TIJ || KIND == DEP_T00TIJ) ? 1 : 2;
     constexpr int TB = KIND == DEP_T00TIJ ? 1 : 0;  // first Tij accumulator
     const bool need_q = KIND != DEP_T00 || use_phi;
+    // A dense tile (on average DEP_SPREAD_X4 / 4 or more particles per cell of
+    // its halo box) hands consecutive particles to different waves, so the members
+    // of one crowded cell contend in PT_NT/64 LDS atomic instructions at a
+    // time instead of all in one; sparse tiles keep the lane-contiguous order,
+    // whose loads coalesce fully.
+    constexpr int NCELL = (TX + 1) * (TY + 1) * (TZ + 1), NW = PT_NT / 64;
+    const bool spread = DEP_SPREAD_X4 > 0 && U == 1 && 4 * total >= DEP_SPREAD_X4 * NCELL;
+    const int slot = spread ? lane * NW + wv : t;
     for (int32_t i0 = 0; i0 < total; i0 += U * PT_NT) {
         int run[U];
         int64_t pu[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int32_t i = i0 + u * PT_NT + t;
+            const int32_t i = i0 + u * PT_NT + slot;
             run[u] = -1;
             pu[u] = 0;
             if (i < total) {

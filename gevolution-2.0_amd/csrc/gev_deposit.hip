@@ -113,7 +113,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
     // time instead of all in one; sparse tiles keep the lane-contiguous order,
     // whose loads coalesce fully.
     constexpr int NCELL = (TX + 1) * (TY + 1) * (TZ + 1), NW = PT_NT / 64;
-    const bool spread = DEP_SPREAD_X4 > 0 && U == 1 && 4 * total >= DEP_SPREAD_X4 * NCELL;
+    const bool spread = DEP_SPREAD_X4 > 0 && 4 * total >= DEP_SPREAD_X4 * NCELL;
     const int slot = spread ? lane * NW + wv : t;
     for (int32_t i0 = 0; i0 < total; i0 += U * PT_NT) {
         int run[U];

@@ -1,0 +1,211 @@
+// gev_snapshot.hip -- Gadget-2 particle snapshots (single file, type-1 particles):
+// perfParticles_gevolution::saveGadget2 / loadGadget2
+// (Particles_gevolution.hpp:597-800, 2455-2640) with the header of
+// metadata.hpp:159-178 and the unit conventions of metadata.hpp:110-120.
+//
+// File layout (Fortran-style records, each framed by its byte count):
+//   [256] header [256]  [12 n] positions float3 [12 n]  [12 n] velocities float3 [12 n]
+//   [8 n] IDs int64 [8 n]
+// Positions are in the header's length unit (BoxSize per box), velocities
+// are q / (a^1.5 GADGET_VELOCITY_CONVERSION).  The writer selects the
+// particles whose ID is a multiple of the tracer factor and, as the reference
+// built with EXACT_OUTPUT_REDSHIFTS (config/defaults.mk:21) does, moves them to
+// the snapshot time: positions drift by dtau_pos with the GR velocity
+// q (1 + (4 - (2q^2 + a^2)/e^2) phi) / e, velocities kick by dtau_vel with
+// (2q^2 + a^2)/e times the CIC edge gradient of phi (phi optional).
+//
+// The conversion runs on the device into the container's ping-pong arrays
+// (free between cycles), one copy brings it to the host, and the host writes
+// the records in container (cell) order.
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "gev_common.hpp"
+#include "gev_kernels.hpp"
+#include "gev_pcommon.hpp"
+
+namespace gev {
+
+constexpr double GADGET_VELOCITY_CONVERSION = 3.335640952e-6;  // Gadget velocity unit / c (metadata.hpp:117)
+static_assert(sizeof(gev_gadget2_header) == 256, "the Gadget-2 header is 256 bytes");
+
+// per particle: snapshot position (length units) and velocity, in the order of the container
+template <typename T>
+__global__ __launch_bounds__(PT_NT) void k_gadget_convert(PV<T> v, double dtau_pos, double dtau_vel, double a,
+                                                          double boxsize, const T *__restrict__ phi, float *px,
+                                                          float *py, float *pz, float *vx, float *vy, float *vz)
+{
+    const int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
+    if (i >= v.n) return;
+    const int N = v.N;
+    int x, y, z;
+    cell_coords(v.cell[i], N, v.lg, x, y, z);
+    const T xs[3] = {v.p0[i], v.p1[i], v.p2[i]};
+    T fr[3];
+    cell_rel(xs, x, y, z, N, fr);
+    const double q[3] = {(double)v.q0[i], (double)v.q1[i], (double)v.q2[i]};
+    double phip = 0.0, g[3] = {0.0, 0.0, 0.0};
+    if (phi) {
+        // CIC value and edge gradient (differences, not yet divided by dx)
+        auto P = [&](int dx_, int dy_, int dz_) { return (double)phi[site(N, x + dx_, y + dy_, z + dz_)]; };
+        const double f0 = fr[0], f1 = fr[1], f2 = fr[2], m0 = 1.0 - f0, m1 = 1.0 - f1, m2 = 1.0 - f2;
+        phip = P(0, 0, 0) * m0 * m1 * m2 + P(1, 0, 0) * f0 * m1 * m2 + P(0, 1, 0) * m0 * f1 * m2 +
+               P(1, 1, 0) * f0 * f1 * m2 + P(0, 0, 1) * m0 * m1 * f2 + P(1, 0, 1) * f0 * m1 * f2 +
+               P(0, 1, 1) * m0 * f1 * f2 + P(1, 1, 1) * f0 * f1 * f2;
+        g[0] = m1 * m2 * (P(1, 0, 0) - P(0, 0, 0)) + f1 * m2 * (P(1, 1, 0) - P(0, 1, 0)) +
+               m1 * f2 * (P(1, 0, 1) - P(0, 0, 1)) + f1 * f2 * (P(1, 1, 1) - P(0, 1, 1));
+        g[1] = m0 * m2 * (P(0, 1, 0) - P(0, 0, 0)) + f0 * m2 * (P(1, 1, 0) - P(1, 0, 0)) +
+               m0 * f2 * (P(0, 1, 1) - P(0, 0, 1)) + f0 * f2 * (P(1, 1, 1) - P(1, 0, 1));
+        g[2] = m0 * m1 * (P(0, 0, 1) - P(0, 0, 0)) + f0 * m1 * (P(1, 0, 1) - P(1, 0, 0)) +
+               m0 * f1 * (P(0, 1, 1) - P(0, 1, 0)) + f0 * f1 * (P(1, 1, 1) - P(1, 1, 0));
+    }
+    const double q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const double e2 = q2 + a * a, e = sqrt(e2);
+    const double w = 2.0 * q2 + a * a;  // q^2 + e^2
+    const double vfac = 1.0 + (4.0 - w / e2) * phip;
+    const double rescale = 1.0 / sqrt(a) / GADGET_VELOCITY_CONVERSION;
+    const double cell[3] = {(double)x, (double)y, (double)z};
+    float *po[3] = {px, py, pz}, *vo[3] = {vx, vy, vz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        // absolute position in box units, drifted, wrapped to [0, 1)
+        const double p = (cell[d] + (double)fr[d]) / N + dtau_pos * q[d] * vfac / e;
+        double wrapped = p - floor(p);
+        if (wrapped >= 1.0) wrapped = 0.0;
+        po[d][i] = (float)(wrapped * boxsize);
+        vo[d][i] = (float)((q[d] - dtau_vel * w * g[d] * N / e) * rescale / a);
+    }
+}
+
+template <typename T>
+static int gadget_convert(gev_pcls *p, double dtau_pos, double dtau_vel, double a, double boxsize, const T *phi,
+                          std::vector<float> &pos, std::vector<float> &vel, std::vector<int64_t> &id)
+{
+    gev_ctx *c = p->ctx;
+    const int64_t n = p->n;
+    // staging: the ping-pong position and momentum arrays hold n floats each
+    float *px = (float *)p->pos2[0], *py = (float *)p->pos2[1], *pz = (float *)p->pos2[2];
+    float *vx = (float *)p->q2[0], *vy = (float *)p->q2[1], *vz = (float *)p->q2[2];
+    if (n > 0)
+        hipLaunchKernelGGL((k_gadget_convert<T>), dim3((unsigned)((n + PT_NT - 1) / PT_NT)), dim3(PT_NT), 0, c->stream,
+                           make_pv<T>(p), dtau_pos, dtau_vel, a, boxsize, phi, px, py, pz, vx, vy, vz);
+    GEV_HIP(hipGetLastError());
+    std::vector<float> tmp((size_t)n);
+    pos.assign(3 * (size_t)n, 0.0f);
+    vel.assign(3 * (size_t)n, 0.0f);
+    id.assign((size_t)n, 0);
+    const float *src[6] = {px, py, pz, vx, vy, vz};
+    for (int k = 0; k < 6; k++) {
+        if (n == 0) break;
+        GEV_HIP(hipMemcpyAsync(tmp.data(), src[k], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+        GEV_HIP(hipStreamSynchronize(c->stream));
+        std::vector<float> &dst = k < 3 ? pos : vel;
+        for (int64_t j = 0; j < n; j++) dst[3 * j + (k % 3)] = tmp[j];
+    }
+    if (n > 0) {
+        GEV_HIP(hipMemcpyAsync(id.data(), p->id, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+        GEV_HIP(hipStreamSynchronize(c->stream));
+    }
+    return GEV_OK;
+}
+
+static bool put_record(FILE *f, const void *data, uint32_t bytes)
+{
+    return fwrite(&bytes, 4, 1, f) == 1 && (bytes == 0 || fwrite(data, 1, bytes, f) == bytes) &&
+           fwrite(&bytes, 4, 1, f) == 1;
+}
+
+static bool get_record(FILE *f, void *data, uint32_t bytes)
+{
+    uint32_t b0 = 0, b1 = 0;
+    if (fread(&b0, 4, 1, f) != 1 || b0 != bytes) return false;
+    if (bytes > 0 && fread(data, 1, bytes, f) != bytes) return false;
+    return fread(&b1, 4, 1, f) == 1 && b1 == bytes;
+}
+
+}  // namespace gev
+
+using namespace gev;
+
+extern "C" {
+
+int gev_pcls_save_gadget2(gev_pcls *p, const char *path, const gev_gadget2_header *hdr, int tracer_factor,
+                          double dtau_pos, double dtau_vel, const void *phi)
+{
+    if (!p || !path || !hdr) { gev_set_error("gev_pcls_save_gadget2: NULL argument"); return GEV_ERR_ARG; }
+    if (tracer_factor < 1) { gev_set_error("gev_pcls_save_gadget2: tracer_factor must be >= 1"); return GEV_ERR_ARG; }
+    if (hdr->num_files != 1) { gev_set_error("gev_pcls_save_gadget2: only single-file snapshots (num_files = 1)"); return GEV_ERR_ARG; }
+    if (!p->sorted) { gev_set_error("gev_pcls_save_gadget2: particles are not binned"); return GEV_ERR_STATE; }
+    std::vector<float> pos, vel;
+    std::vector<int64_t> id;
+    const int rc = p->ctx->prec == GEV_FP64
+                       ? gadget_convert<double>(p, dtau_pos, dtau_vel, hdr->time, hdr->BoxSize, (const double *)phi, pos,
+                                                vel, id)
+                       : gadget_convert<float>(p, dtau_pos, dtau_vel, hdr->time, hdr->BoxSize, (const float *)phi, pos,
+                                               vel, id);
+    if (rc) return rc;
+    // tracer selection (ID multiple of the factor, Particles_gevolution.hpp:444,494), order kept
+    int64_t m = 0;
+    for (size_t j = 0; j < id.size(); j++) {
+        if (id[j] % tracer_factor != 0) continue;
+        for (int d = 0; d < 3; d++) { pos[3 * m + d] = pos[3 * j + d]; vel[3 * m + d] = vel[3 * j + d]; }
+        id[m++] = id[j];
+    }
+    const uint64_t declared = (uint64_t)hdr->npart[1] + ((uint64_t)hdr->npartTotalHW[1] << 32);
+    if (declared != (uint64_t)m || hdr->npartTotalHW[1] != 0) {
+        gev_set_error("gev_pcls_save_gadget2: header npart[1] does not match the selected particles (or exceeds 2^32-1)");
+        return GEV_ERR_ARG;
+    }
+    if ((uint64_t)m * 12 > 0xffffffffull) { gev_set_error("gev_pcls_save_gadget2: record exceeds 4 GiB"); return GEV_ERR_ARG; }
+    FILE *f = fopen(path, "wb");
+    if (!f) { gev_set_error(std::string("gev_pcls_save_gadget2: cannot open ") + path); return GEV_ERR_ARG; }
+    const uint32_t b3 = (uint32_t)(12 * m), b1 = (uint32_t)(8 * m);
+    bool ok = put_record(f, hdr, sizeof(gev_gadget2_header)) && put_record(f, pos.data(), b3) &&
+              put_record(f, vel.data(), b3) && put_record(f, id.data(), b1);
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) { gev_set_error(std::string("gev_pcls_save_gadget2: write failed on ") + path); return GEV_ERR_ARG; }
+    return GEV_OK;
+}
+
+int gev_pcls_load_gadget2(gev_pcls *p, const char *path, gev_gadget2_header *hdr_out)
+{
+    if (!p || !path) { gev_set_error("gev_pcls_load_gadget2: NULL argument"); return GEV_ERR_ARG; }
+    FILE *f = fopen(path, "rb");
+    if (!f) { gev_set_error(std::string("gev_pcls_load_gadget2: cannot open ") + path); return GEV_ERR_ARG; }
+    gev_gadget2_header hdr;
+    bool ok = get_record(f, &hdr, sizeof(hdr));
+    const int64_t n = ok ? (int64_t)hdr.npart[1] : 0;
+    if (ok && n > p->cap) {
+        fclose(f);
+        gev_set_error("gev_pcls_load_gadget2: the snapshot holds more particles than the container's capacity");
+        return GEV_ERR_ARG;
+    }
+    std::vector<float> pos(3 * (size_t)n), vel(3 * (size_t)n);
+    std::vector<int64_t> id((size_t)n);
+    ok = ok && get_record(f, pos.data(), (uint32_t)(12 * n)) && get_record(f, vel.data(), (uint32_t)(12 * n)) &&
+         get_record(f, id.data(), (uint32_t)(8 * n));
+    fclose(f);
+    if (!ok) { gev_set_error(std::string("gev_pcls_load_gadget2: not a single-file Gadget-2 snapshot: ") + path); return GEV_ERR_ARG; }
+    // units back to the box (loadGadget2, Particles_gevolution.hpp:2563-2569)
+    const double rescale = 1.0 / GADGET_VELOCITY_CONVERSION / sqrt(hdr.time);
+    std::vector<double> dp(3 * (size_t)n), dq(3 * (size_t)n);
+    for (size_t j = 0; j < 3 * (size_t)n; j++) {
+        float x = (float)((double)pos[j] / hdr.BoxSize);  // the reference divides a float in place
+        while (x < 0.0f) x += 1.0f;
+        while (x >= 1.0f) x -= 1.0f;
+        dp[j] = (double)x;
+        dq[j] = (double)(float)((double)vel[j] * (hdr.time / rescale));
+    }
+    // [3][n] layout for gev_pcls_load
+    std::vector<double> P(3 * (size_t)n), Q(3 * (size_t)n);
+    for (int64_t j = 0; j < n; j++)
+        for (int d = 0; d < 3; d++) { P[d * n + j] = dp[3 * j + d]; Q[d * n + j] = dq[3 * j + d]; }
+    const int rc = gev_pcls_load(p, n, P.data(), Q.data(), id.data(), 0);
+    if (rc) return rc;
+    if (hdr_out) *hdr_out = hdr;
+    return GEV_OK;
+}
+
+}  // extern "C"

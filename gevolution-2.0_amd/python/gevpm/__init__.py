@@ -11,11 +11,13 @@ Tensor layouts:
   Fourier field  (ncomp, N, N, P) complex, indexed [c, kz, ky, kx], P = ctx.P
 """
 import ctypes as C
+import os
 import math
 
 import torch
 
-from ._lib import (GEV_FP64, GEV_MIXED, GevError, check, gev_fields, gev_pcls_view, gev_step_state, lib)
+from ._lib import (GEV_FP64, GEV_MIXED, GevError, check, gev_fields, gev_gadget2_header, gev_pcls_view,
+                   gev_step_state, lib)
 
 __all__ = ["Context", "Particles", "Fields", "StepState", "GEV_FP64", "GEV_MIXED", "GevError"]
 
@@ -363,6 +365,21 @@ class Particles:
         self.ctx.bind_stream()
         check(lib.gev_initialize_q(self._h, _ptr(pot), coeff, C.byref(r)), "gev_initialize_q")
         return r.value
+
+    # ------------------------------------------------------ Gadget-2 files
+    def save_gadget2(self, path, header, tracer_factor=1, dtau_pos=0.0, dtau_vel=0.0, phi=None):
+        """saveGadget2: the particles with ID % tracer_factor == 0; `header` is a
+        gev_gadget2_header whose npart[1] equals their number"""
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_save_gadget2(self._h, os.fsencode(path), C.byref(header), int(tracer_factor), dtau_pos,
+                                        dtau_vel, _ptr(phi)), "gev_pcls_save_gadget2")
+
+    def load_gadget2(self, path):
+        """loadGadget2: replaces the particles with the snapshot's; returns its header"""
+        hdr = gev_gadget2_header()
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_load_gadget2(self._h, os.fsencode(path), C.byref(hdr)), "gev_pcls_load_gadget2")
+        return hdr
 
     # --------------------------------------------------------- checkpoints
     def state_bytes(self):

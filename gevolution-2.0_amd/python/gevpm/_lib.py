@@ -28,6 +28,16 @@ class gev_fields(C.Structure):
                 ("Sij", C.c_void_p), ("scalarFT", C.c_void_p), ("SijFT", C.c_void_p), ("BiFT", C.c_void_p)]
 
 
+class gev_gadget2_header(C.Structure):
+    """gadget2_header (metadata.hpp:159-178), 256 bytes"""
+    _fields_ = [("npart", C.c_uint32 * 6), ("mass", C.c_double * 6), ("time", C.c_double), ("redshift", C.c_double),
+                ("flag_sfr", C.c_int32), ("flag_feedback", C.c_int32), ("npartTotal", C.c_uint32 * 6),
+                ("flag_cooling", C.c_int32), ("num_files", C.c_int32), ("BoxSize", C.c_double),
+                ("Omega0", C.c_double), ("OmegaLambda", C.c_double), ("HubbleParam", C.c_double),
+                ("flag_age", C.c_int32), ("flag_metals", C.c_int32), ("npartTotalHW", C.c_uint32 * 6),
+                ("fill", C.c_char * 64)]
+
+
 class gev_step_state(C.Structure):
     _fields_ = [("gr", C.c_int), ("vector_elliptic", C.c_int), ("fourpiG", C.c_double), ("Omega_m", C.c_double),
                 ("Omega_Lambda", C.c_double), ("Omega_rad", C.c_double), ("z_relax", C.c_double),
@@ -81,6 +91,8 @@ SIGNATURES = {
     "gev_pcls_state_bytes": (_i, [_vp, C.POINTER(C.c_int64)]),
     "gev_pcls_save": (_i, [_vp, _vp]),
     "gev_pcls_restore": (_i, [_vp, _vp]),
+    "gev_pcls_save_gadget2": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header), _i, _d, _d, _vp]),
+    "gev_pcls_load_gadget2": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),
     "gev_adj_deposit": (_i, [_vp, _vp, _vp]),
     "gev_adj_divergence": (_i, [_vp, _vp, _vp, _d]),
     "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _d]),

@@ -179,6 +179,41 @@ int gev_pcls_state_bytes(gev_pcls *p, int64_t *bytes);
 int gev_pcls_save(gev_pcls *p, void *dst);
 int gev_pcls_restore(gev_pcls *p, const void *src);
 
+/* ---- Gadget-2 particle snapshots, single file, particle type 1
+ *      (perfParticles_gevolution::saveGadget2 / loadGadget2,
+ *      Particles_gevolution.hpp:597-800 and 2455-2640).  The header is the
+ *      reference's gadget2_header (metadata.hpp:159-178), 256 bytes. */
+typedef struct gev_gadget2_header {
+    uint32_t npart[6];
+    double mass[6];
+    double time;
+    double redshift;
+    int32_t flag_sfr;
+    int32_t flag_feedback;
+    uint32_t npartTotal[6];
+    int32_t flag_cooling;
+    int32_t num_files;
+    double BoxSize;
+    double Omega0;
+    double OmegaLambda;
+    double HubbleParam;
+    int32_t flag_age;
+    int32_t flag_metals;
+    uint32_t npartTotalHW[6];
+    char fill[64];
+} gev_gadget2_header;
+/* Writes the particles with ID % tracer_factor == 0 (hdr->npart[1] must equal
+ * their number; hdr->num_files must be 1): positions p*BoxSize, velocities
+ * q/(a^1.5 GADGET_VELOCITY_CONVERSION) with a = hdr->time, both moved to the
+ * snapshot time as EXACT_OUTPUT_REDSHIFTS does (drift dtau_pos, kick dtau_vel
+ * with the gradient of phi; phi may be NULL).  Records in container order. */
+int gev_pcls_save_gadget2(gev_pcls *p, const char *path, const gev_gadget2_header *hdr, int tracer_factor,
+                          double dtau_pos, double dtau_vel, const void *phi);
+/* Replaces the container's particles with the snapshot's (positions wrapped
+ * into the box, velocities converted back to q/m) and re-bins; *hdr (may be
+ * NULL) receives the header. */
+int gev_pcls_load_gadget2(gev_pcls *p, const char *path, gev_gadget2_header *hdr);
+
 /* ---- adjoint of the Newtonian cycle (no reference counterpart; reverse mode
  *      of update_q_Newton, update_pos_Newton, the CIC mass deposit and the
  *      Poisson solve, gevolution.hpp:1051-1118, 1270-1273, 1902-1905, 806-852).

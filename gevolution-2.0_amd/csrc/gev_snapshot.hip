@@ -17,6 +17,13 @@
 // The conversion runs on the device into the container's ping-pong arrays
 // (free between cycles), one copy brings it to the host, and the host writes
 // the records in container (cell) order.
+//
+// The express format (saveExpress / loadExpress, Particles_gevolution.hpp:
+// 2642-2900) is the raw restart state of one rank: a 120-byte express header,
+// positions (box units) and momenta as Real triplets, IDs as int64, then the
+// Gadget-2 header.  Real is the container's precision (4 bytes in the mixed
+// form, as in the reference's single-precision build); this build is one rank
+// of one, so the file is <name>.0.
 #include <stdio.h>
 #include <string.h>
 
@@ -125,6 +132,45 @@ static bool get_record(FILE *f, void *data, uint32_t bytes)
     return fread(&b1, 4, 1, f) == 1 && b1 == bytes;
 }
 
+
+// express_header (Particles_gevolution.hpp:98-120), 120 bytes
+struct ExpressHeader {
+    char magic[8];
+    uint32_t version, header_size, endian, sizeof_real, sizeof_long, sizeof_gadget2_header;
+    int32_t rank, size, grid_size[2], lat_size[3], lat_size_local[3], coord_skip[2];
+    uint64_t local_npart, global_npart, position_bytes, momentum_bytes, id_bytes;
+};
+static_assert(sizeof(ExpressHeader) == 120, "express header layout");
+static const char EXPRESS_MAGIC[8] = {'G', 'E', 'X', 'P', 'C', 'D', 'M', '\0'};
+constexpr uint32_t EXPRESS_VERSION = 1, EXPRESS_ENDIAN = 0x01020304u;
+
+// express_rank_filename (Particles_gevolution.hpp:125-144) for rank 0
+static std::string express_name(const char *path)
+{
+    std::string f(path);
+    const size_t dot = f.find_last_of('.');
+    if (dot != std::string::npos && dot + 1 < f.size() &&
+        f.find_first_not_of("0123456789", dot + 1) == std::string::npos)
+        f.resize(dot);
+    return f + ".0";
+}
+
+template <typename R> static void put_triplets(std::vector<char> &buf, const std::vector<double> &a, int64_t n)
+{
+    // a: [3][n] -> R[n][3]
+    const size_t o = buf.size();
+    buf.resize(o + sizeof(R) * 3 * n);
+    R *d = (R *)(buf.data() + o);
+    for (int64_t j = 0; j < n; j++)
+        for (int k = 0; k < 3; k++) d[3 * j + k] = (R)a[k * n + j];
+}
+template <typename R> static void get_triplets(const char *src, std::vector<double> &a, int64_t n)
+{
+    const R *s = (const R *)src;
+    for (int64_t j = 0; j < n; j++)
+        for (int k = 0; k < 3; k++) a[k * n + j] = (double)s[3 * j + k];
+}
+
 }  // namespace gev
 
 using namespace gev;
@@ -203,6 +249,100 @@ int gev_pcls_load_gadget2(gev_pcls *p, const char *path, gev_gadget2_header *hdr
     for (int64_t j = 0; j < n; j++)
         for (int d = 0; d < 3; d++) { P[d * n + j] = dp[3 * j + d]; Q[d * n + j] = dq[3 * j + d]; }
     const int rc = gev_pcls_load(p, n, P.data(), Q.data(), id.data(), 0);
+    if (rc) return rc;
+    if (hdr_out) *hdr_out = hdr;
+    return GEV_OK;
+}
+
+int gev_pcls_save_express(gev_pcls *p, const char *path, gev_gadget2_header *hdr)
+{
+    if (!p || !path || !hdr) { gev_set_error("gev_pcls_save_express: NULL argument"); return GEV_ERR_ARG; }
+    if (hdr->num_files != 1) { gev_set_error("gev_pcls_save_express: one file per task, and this build runs one task"); return GEV_ERR_ARG; }
+    const int64_t n = p->n;
+    std::vector<double> pos(3 * (size_t)n), q(3 * (size_t)n);
+    std::vector<int64_t> id((size_t)n);
+    if (n > 0) {
+        const int rc = gev_pcls_store(p, pos.data(), q.data(), id.data(), 0);
+        if (rc) return rc;
+    }
+    hdr->npart[1] = (uint32_t)((uint64_t)n & 0xffffffffu);
+    hdr->npartTotal[1] = hdr->npart[1];
+    hdr->npartTotalHW[1] = (uint32_t)((uint64_t)n >> 32);
+    const int N = p->ctx->N;
+    const uint32_t rs = p->ctx->prec == GEV_FP64 ? 8 : 4;
+    ExpressHeader e;
+    memset(&e, 0, sizeof(e));
+    memcpy(e.magic, EXPRESS_MAGIC, sizeof(e.magic));
+    e.version = EXPRESS_VERSION;
+    e.header_size = sizeof(e);
+    e.endian = EXPRESS_ENDIAN;
+    e.sizeof_real = rs;
+    e.sizeof_long = sizeof(int64_t);
+    e.sizeof_gadget2_header = sizeof(gev_gadget2_header);
+    e.rank = 0;
+    e.size = 1;
+    e.grid_size[0] = e.grid_size[1] = 1;
+    for (int i = 0; i < 3; i++) e.lat_size[i] = e.lat_size_local[i] = N;
+    e.local_npart = e.global_npart = (uint64_t)n;
+    e.position_bytes = e.momentum_bytes = 3 * (uint64_t)n * rs;
+    e.id_bytes = (uint64_t)n * sizeof(int64_t);
+    std::vector<char> buf((const char *)&e, (const char *)&e + sizeof(e));
+    if (rs == 8) { put_triplets<double>(buf, pos, n); put_triplets<double>(buf, q, n); }
+    else { put_triplets<float>(buf, pos, n); put_triplets<float>(buf, q, n); }
+    buf.insert(buf.end(), (const char *)id.data(), (const char *)(id.data() + n));
+    buf.insert(buf.end(), (const char *)hdr, (const char *)hdr + sizeof(gev_gadget2_header));
+    const std::string name = express_name(path);
+    FILE *f = fopen(name.c_str(), "wb");
+    if (!f) { gev_set_error("gev_pcls_save_express: cannot open " + name); return GEV_ERR_ARG; }
+    const bool ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    if ((fclose(f) != 0) || !ok) { gev_set_error("gev_pcls_save_express: write failed on " + name); return GEV_ERR_ARG; }
+    return GEV_OK;
+}
+
+int gev_pcls_load_express(gev_pcls *p, const char *path, gev_gadget2_header *hdr_out)
+{
+    if (!p || !path) { gev_set_error("gev_pcls_load_express: NULL argument"); return GEV_ERR_ARG; }
+    const std::string name = express_name(path);
+    FILE *f = fopen(name.c_str(), "rb");
+    if (!f) { gev_set_error("gev_pcls_load_express: cannot open " + name); return GEV_ERR_ARG; }
+    ExpressHeader e;
+    bool ok = fread(&e, sizeof(e), 1, f) == 1;
+    const int N = p->ctx->N;
+    const uint32_t rs = p->ctx->prec == GEV_FP64 ? 8 : 4;
+    // the layout checks of loadExpress (Particles_gevolution.hpp:2786-2808)
+    ok = ok && memcmp(e.magic, EXPRESS_MAGIC, sizeof(e.magic)) == 0 && e.version == EXPRESS_VERSION &&
+         e.header_size == sizeof(e) && e.endian == EXPRESS_ENDIAN && e.sizeof_real == rs &&
+         e.sizeof_long == sizeof(int64_t) && e.sizeof_gadget2_header == sizeof(gev_gadget2_header) && e.rank == 0 &&
+         e.size == 1 && e.grid_size[0] == 1 && e.grid_size[1] == 1 && e.coord_skip[0] == 0 && e.coord_skip[1] == 0;
+    for (int i = 0; ok && i < 3; i++) ok = e.lat_size[i] == N && e.lat_size_local[i] == N;
+    ok = ok && e.position_bytes == 3 * e.local_npart * rs && e.momentum_bytes == 3 * e.local_npart * rs &&
+         e.id_bytes == e.local_npart * sizeof(int64_t);
+    if (!ok) {
+        fclose(f);
+        gev_set_error("gev_pcls_load_express: " + name + " is incompatible with this lattice, precision or ABI");
+        return GEV_ERR_ARG;
+    }
+    const int64_t n = (int64_t)e.local_npart;
+    if (n > p->cap) {
+        fclose(f);
+        gev_set_error("gev_pcls_load_express: the file holds more particles than the container's capacity");
+        return GEV_ERR_ARG;
+    }
+    std::vector<char> raw(2 * e.position_bytes + e.id_bytes);
+    gev_gadget2_header hdr;
+    ok = fread(raw.data(), 1, raw.size(), f) == raw.size() && fread(&hdr, sizeof(hdr), 1, f) == 1;
+    fclose(f);
+    const uint64_t hdr_global = (uint64_t)hdr.npartTotal[1] + ((uint64_t)hdr.npartTotalHW[1] << 32);
+    if (!ok || hdr_global != e.global_npart || hdr.num_files != e.size) {
+        gev_set_error("gev_pcls_load_express: " + name + " is truncated or its Gadget-2 metadata does not match");
+        return GEV_ERR_ARG;
+    }
+    std::vector<double> pos(3 * (size_t)n), q(3 * (size_t)n);
+    std::vector<int64_t> id((size_t)n);
+    if (rs == 8) { get_triplets<double>(raw.data(), pos, n); get_triplets<double>(raw.data() + e.position_bytes, q, n); }
+    else { get_triplets<float>(raw.data(), pos, n); get_triplets<float>(raw.data() + e.position_bytes, q, n); }
+    memcpy(id.data(), raw.data() + 2 * e.position_bytes, e.id_bytes);
+    const int rc = gev_pcls_load(p, n, pos.data(), q.data(), id.data(), 0);
     if (rc) return rc;
     if (hdr_out) *hdr_out = hdr;
     return GEV_OK;

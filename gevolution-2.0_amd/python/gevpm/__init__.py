@@ -381,6 +381,19 @@ class Particles:
         check(lib.gev_pcls_load_gadget2(self._h, os.fsencode(path), C.byref(hdr)), "gev_pcls_load_gadget2")
         return hdr
 
+    def save_express(self, path, header):
+        """saveExpress: the raw particle state to "<path>.0"; fills header's particle counts"""
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_save_express(self._h, os.fsencode(path), C.byref(header)), "gev_pcls_save_express")
+        return header
+
+    def load_express(self, path):
+        """loadExpress: replaces the particles with the file's; returns its Gadget-2 header"""
+        hdr = gev_gadget2_header()
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_load_express(self._h, os.fsencode(path), C.byref(hdr)), "gev_pcls_load_express")
+        return hdr
+
     # --------------------------------------------------------- checkpoints
     def state_bytes(self):
         b = C.c_int64()

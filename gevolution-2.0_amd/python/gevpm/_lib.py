@@ -93,6 +93,8 @@ SIGNATURES = {
     "gev_pcls_restore": (_i, [_vp, _vp]),
     "gev_pcls_save_gadget2": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header), _i, _d, _d, _vp]),
     "gev_pcls_load_gadget2": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),
+    "gev_pcls_save_express": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),
+    "gev_pcls_load_express": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),
     "gev_adj_deposit": (_i, [_vp, _vp, _vp]),
     "gev_adj_divergence": (_i, [_vp, _vp, _vp, _d]),
     "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _d]),

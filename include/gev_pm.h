@@ -213,6 +213,14 @@ int gev_pcls_save_gadget2(gev_pcls *p, const char *path, const gev_gadget2_heade
  * into the box, velocities converted back to q/m) and re-bins; *hdr (may be
  * NULL) receives the header. */
 int gev_pcls_load_gadget2(gev_pcls *p, const char *path, gev_gadget2_header *hdr);
+/* saveExpress / loadExpress (Particles_gevolution.hpp:2642-2900): the raw
+ * particle state as one rank's file "<path>.0" (a numeric suffix of path is
+ * replaced): express header, positions (box units) and momenta as Real
+ * triplets (Real = 8 bytes in FP64, 4 in the mixed form), int64 IDs, then
+ * the Gadget-2 header.  save fills hdr's particle counts, as the reference
+ * does; load checks the file against this lattice and precision. */
+int gev_pcls_save_express(gev_pcls *p, const char *path, gev_gadget2_header *hdr);
+int gev_pcls_load_express(gev_pcls *p, const char *path, gev_gadget2_header *hdr);
 
 /* ---- adjoint of the Newtonian cycle (no reference counterpart; reverse mode
  *      of update_q_Newton, update_pos_Newton, the CIC mass deposit and the

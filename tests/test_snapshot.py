@@ -1,6 +1,7 @@
-"""Gadget-2 particle snapshots (saveGadget2 / loadGadget2,
-Particles_gevolution.hpp:597-800, 2455-2640): the byte layout of the file
-read back with numpy, the unit conversions, the tracer selection, the
+"""Particle files: Gadget-2 snapshots (saveGadget2 / loadGadget2,
+Particles_gevolution.hpp:597-800, 2455-2640) and the express restart format
+(saveExpress / loadExpress, :2642-2900).  The byte layout of each file read
+back with numpy, the unit conversions, the tracer selection, the
 exact-redshift drift and kick against closed forms, and save -> load round
 trips in both precision forms."""
 import ctypes as C
@@ -151,3 +152,52 @@ def test_gadget2_rejects_bad_requests(tmp_path):
     p.save_gadget2(str(tmp_path / "ok"), header(gp, 10, 1.0, 1.0))
     with pytest.raises(gp.GevError):
         small.load_gadget2(str(tmp_path / "ok"))  # capacity 5 < 10 particles
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_express_layout_and_round_trip(tmp_path, prec):
+    """saveExpress / loadExpress: 120-byte express header, Real triplets of
+    positions and momenta, int64 IDs, the Gadget-2 header last; the file name
+    gets the rank suffix .0 in place of a numeric one"""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gevpm as gp
+    N, n = 16, 4000
+    ctx = gp.Context(N, gp.GEV_FP64 if prec == "fp64" else gp.GEV_MIXED)
+    rng = np.random.default_rng(12)
+    p = gp.Particles(ctx, n, 1.0)
+    p.load(rng.uniform(0, 1, (3, n)), rng.uniform(-0.3, 0.3, (3, n)), rng.permutation(n).astype(np.int64) + 100)
+    gpos, gq, gid = (t.cpu().numpy() for t in p.store())
+    h = p.save_express(str(tmp_path / "restart.7"), header(gp, 0, 0.1, 640000.0))
+    assert h.npart[1] == n and h.npartTotal[1] == n
+    path = tmp_path / "restart.0"
+    raw = path.read_bytes()
+    rs = 8 if prec == "fp64" else 4
+    assert raw[:8] == b"GEXPCDM\x00"
+    u32 = np.frombuffer(raw, np.uint32, 6, 8)
+    assert tuple(u32) == (1, 120, 0x01020304, rs, 8, 256)
+    # rank, size, grid_size[2], lat_size[3], lat_size_local[3], coord_skip[2]
+    assert tuple(np.frombuffer(raw, np.int32, 12, 32)) == (0, 1, 1, 1, N, N, N, N, N, N, 0, 0)
+    u64 = np.frombuffer(raw, np.uint64, 5, 80)
+    assert tuple(u64) == (n, n, 3 * n * rs, 3 * n * rs, 8 * n)
+    assert len(raw) == 120 + 6 * n * rs + 8 * n + 256
+    dt = np.float64 if rs == 8 else np.float32
+    fpos = np.frombuffer(raw, dt, 3 * n, 120).reshape(n, 3)
+    fid = np.frombuffer(raw, np.int64, n, 120 + 6 * n * rs)
+    assert np.array_equal(fid, gid)  # container order
+    assert np.array_equal(fpos, gpos.T.astype(dt))
+    p2 = gp.Particles(ctx, n, 1.0)
+    h2 = p2.load_express(str(tmp_path / "restart"))
+    assert h2.npart[1] == n and h2.BoxSize == 640000.0
+    lpos, lq, lid = (t.cpu().numpy() for t in p2.store())
+    o, o2 = np.argsort(gid), np.argsort(lid)
+    assert np.array_equal(lid[o2], gid[o])
+    tol = 0.0 if prec == "fp64" else 1.2e-7
+    assert np.abs(((lpos[:, o2] - gpos[:, o]) + 0.5) % 1.0 - 0.5).max() <= tol
+    assert np.array_equal(lq[:, o2], gq[:, o])  # momenta are stored in the container's own precision
+    # another lattice is refused
+    other = gp.Particles(gp.Context(32, ctx.precision), n, 1.0)
+    with pytest.raises(gp.GevError):
+        other.load_express(str(tmp_path / "restart"))

@@ -6,7 +6,8 @@
 //   Field<Real> / Field<Cplx>     LATfield2 Field in real / Fourier layout
 //   PlanFFT<Cplx>                 LATfield2 PlanFFT (execute FFT_FORWARD / FFT_BACKWARD)
 //   perfParticles_gevolution<>    Particles_gevolution.hpp:176-211 (+ perfParticles base:
-//                                 updateVel, moveParticles, num_particles, sampleParticles)
+//                                 updateVel, moveParticles, num_particles, sampleParticles,
+//                                 saveGadget2, loadGadget2, saveExpress, loadExpress)
 //   projection_* / prepareFTsource / solveModifiedPoissonFT /
 //   projectFTscalar / projectFTvector / evolveFTvector
 //                                 gevolution.hpp free functions, same defaults
@@ -146,6 +147,7 @@ template <typename T> class PlanFFT {
 // ------------------------------------------------------------- cosmology
 struct cosmology {
     double Omega_m = 0.31, Omega_Lambda = 0.69, Omega_rad = 0.0;
+    double Omega_cdm = 0.31, h = 0.67;  // snapshot metadata only (output.hpp:423-457)
 };
 inline double Hconf(double a, double fourpiG, const cosmology &c)
 {
@@ -170,6 +172,9 @@ struct part_simple_info {
     double mass = 0.0;
     const char *type_name = "part_simple";
 };
+
+// metadata.hpp:159-178, byte for byte
+using gadget2_header = gev_gadget2_header;
 
 // functor tags: which gevolution functor updateVel / moveParticles apply
 struct update_q_functor {};         // gevolution.hpp:888-1011
@@ -262,6 +267,29 @@ template <class part = part_simple, class part_info = part_simple_info> class pe
     void moveParticles(update_pos_Newton_functor, double dtau, Field<Real> **, int, const double *params)
     {
         check(gev_update_pos_newton(p_, dtau, params[0]), "moveParticles(update_pos_Newton)");
+    }
+
+    // Particle files (Particles_gevolution.hpp:170-186): Gadget-2 snapshot,
+    // single file, particle type 1, optional EXACT_OUTPUT_REDSHIFTS drift and
+    // kick; express restart file "<filename>.0" with this lattice's layout
+    void saveGadget2(std::string filename, gadget2_header &hdr, const int tracer_factor = 1, double dtau_pos = 0.,
+                     double dtau_vel = 0., Field<Real> *phi = nullptr)
+    {
+        check(gev_pcls_save_gadget2(p_, filename.c_str(), &hdr, tracer_factor, dtau_pos, dtau_vel,
+                                    phi ? phi->data() : nullptr),
+              "saveGadget2");
+    }
+    void loadGadget2(std::string filename, gadget2_header &hdr)
+    {
+        check(gev_pcls_load_gadget2(p_, filename.c_str(), &hdr), "loadGadget2");
+    }
+    void saveExpress(std::string filename, gadget2_header &hdr)
+    {
+        check(gev_pcls_save_express(p_, filename.c_str(), &hdr), "saveExpress");
+    }
+    void loadExpress(std::string filename, gadget2_header &hdr)
+    {
+        check(gev_pcls_load_express(p_, filename.c_str(), &hdr), "loadExpress");
     }
 
     gev_pcls *handle() const { return p_; }

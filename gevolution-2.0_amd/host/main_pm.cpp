@@ -4,12 +4,14 @@
 // function by function through the gevolution-named interfaces.
 //
 //   main_pm -n N -L boxsize -zin z -zend z -steps S [-newton] [-seed s]
-//           [-zrelax z] -pk mpk.txt [-out final.bin]
+//           [-zrelax z] -pk mpk.txt [-out final.bin] [-gadget snap] [-express restart]
 //
 // The mPk file has the loadPowerSpectrum format (ic_basic.hpp:370-449): two
 // columns, k [h/Mpc] and P(k) [(Mpc/h)^3].  Steps are uniform in conformal
 // time between z_in and z_end.  -out writes the final particles as
-// int64 n, int64 id[n], double pos[3][n], double q[3][n].
+// int64 n, int64 id[n], double pos[3][n], double q[3][n].  -gadget writes them
+// as a Gadget-2 snapshot with the header output.hpp:418-466 fills (Omega_cdm
+// particles, no tracer thinning), -express as an express restart file.
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -23,12 +25,15 @@ using namespace gevpm;
 namespace {
 
 constexpr double C_SPEED_OF_LIGHT = 2997.92458;  // metadata.hpp:97 (100 km/s units)
+constexpr double C_RHO_CRIT = 2.77459457e11;     // metadata.hpp:98 [M_sun h^2 / Mpc^3]
+constexpr double GADGET_LENGTH_CONVERSION = 0.001;  // metadata.hpp:111 (kpc/h)
+constexpr double GADGET_MASS_CONVERSION = 1.0e10;   // metadata.hpp:114 (1e10 M_sun/h)
 
 struct Options {
     int N = 32, steps = 5, newton = 0;
     uint32_t seed = 42;
     double L = 320.0, zin = 100.0, zend = 50.0, zrelax = -2.0;
-    std::string pk, out;
+    std::string pk, out, gadget, express;
 };
 
 Options parse(int argc, char **argv)
@@ -50,6 +55,8 @@ Options parse(int argc, char **argv)
         else if (a == "-zrelax") o.zrelax = std::atof(next());
         else if (a == "-pk") o.pk = next();
         else if (a == "-out") o.out = next();
+        else if (a == "-gadget") o.gadget = next();
+        else if (a == "-express") o.express = next();
         else throw std::runtime_error("unknown option " + a);
     }
     if (o.pk.empty()) throw std::runtime_error("-pk <mPk file> is required");
@@ -236,6 +243,23 @@ int main(int argc, char **argv)
             f.write((const char *)pos.data(), sizeof(double) * 3 * n);
             f.write((const char *)q.data(), sizeof(double) * 3 * n);
             if (!f) throw std::runtime_error("writing " + opt.out + " failed");
+        }
+        if (!opt.gadget.empty() || !opt.express.empty()) {
+            // output.hpp:418-466 at the current time (no exact-redshift step)
+            gadget2_header hdr{};
+            const int64_t n = pcls.num_particles();
+            hdr.num_files = 1;
+            hdr.Omega0 = cosmo.Omega_m;
+            hdr.OmegaLambda = cosmo.Omega_Lambda;
+            hdr.HubbleParam = cosmo.h;
+            hdr.BoxSize = opt.L / GADGET_LENGTH_CONVERSION;
+            hdr.time = a;
+            hdr.redshift = 1. / a - 1.;
+            hdr.npart[1] = hdr.npartTotal[1] = (uint32_t)(n % (1ll << 32));
+            hdr.npartTotalHW[1] = (uint32_t)(n / (1ll << 32));
+            hdr.mass[1] = C_RHO_CRIT * cosmo.Omega_cdm * opt.L * opt.L * opt.L / (double)n / GADGET_MASS_CONVERSION;
+            if (!opt.gadget.empty()) pcls.saveGadget2(opt.gadget, hdr, 1, 0., 0., &phi);
+            if (!opt.express.empty()) pcls.saveExpress(opt.express, hdr);
         }
         std::printf("done: %d cycles, final z=%.6f\n", opt.steps, 1. / a - 1.);
         return 0;

@@ -35,6 +35,37 @@ def _read_out(path):
     return pid[o], pos[:, o], q[:, o]
 
 
+def _check_files(snap, restart, pid, pos, q, L, single):
+    """the façade's saveGadget2 / saveExpress output against -out: kpc/h
+    positions, q / (a^1.5 GADGET_VELOCITY_CONVERSION) velocities, header
+    metadata of output.hpp:418-466"""
+    raw = open(snap, "rb").read()
+    n = len(pid)
+    assert np.frombuffer(raw, np.uint32, 1, 0)[0] == 256
+    hb = raw[4:260]
+    a = np.frombuffer(hb, np.float64, 1, 72)[0]
+    assert np.frombuffer(hb, np.uint32, 6, 0)[1] == n and np.frombuffer(hb, np.float64, 1, 128)[0] == L * 1000
+    m = 2.77459457e11 * 0.31 * L ** 3 / n / 1e10
+    assert abs(np.frombuffer(hb, np.float64, 6, 24)[1] / m - 1) < 1e-12
+    # records: [4|256|4] header, then [4|12n|4] positions, velocities, [4|8n|4] IDs
+    fpos = np.frombuffer(raw, np.float32, 3 * n, 268).reshape(n, 3)
+    fvel = np.frombuffer(raw, np.float32, 3 * n, 276 + 12 * n).reshape(n, 3)
+    fid = np.frombuffer(raw, np.int64, n, 284 + 24 * n)
+    f = np.argsort(fid)
+    assert np.array_equal(fid[f], pid)
+    assert np.allclose(fpos[f], (pos * L * 1000).T, rtol=1e-6, atol=L * 1000 * 1e-6)
+    assert np.allclose(fvel[f], (q / (a ** 1.5 * 3.335640952e-6)).T, rtol=1e-6, atol=1e-3)
+    rs = 4 if single else 8
+    er = open(restart, "rb").read()
+    assert er[:8] == b"GEXPCDM\x00" and np.frombuffer(er, np.uint32, 1, 20)[0] == rs
+    eid = np.frombuffer(er, np.int64, n, 120 + 6 * n * rs)
+    e = np.argsort(eid)
+    assert np.array_equal(eid[e], pid)
+    eq = np.frombuffer(er, np.float32 if single else np.float64, 3 * n, 120 + 3 * n * rs).reshape(n, 3)
+    assert np.allclose(eq[e], q.T, rtol=1e-6, atol=1e-12)
+    assert er[-256:] == hb  # the Gadget-2 header closes the express file
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("newton,single", [(1, False), (0, False), (0, True)])
 def test_driver_matches_gev_step(tmp_path, newton, single):
@@ -48,14 +79,17 @@ def test_driver_matches_gev_step(tmp_path, newton, single):
     mpk = str(tmp_path / "pk.txt")
     cosmo.write_mpk(mpk, k, P)
     out = str(tmp_path / "final.bin")
+    snap, restart = str(tmp_path / "snap_cdm"), str(tmp_path / "restart")
     exe = os.path.join(LIB, "main_pm_single" if single else "main_pm")
     cmd = [exe, "-n", str(N), "-L", str(L), "-zin", str(z_in), "-zend", str(z_end), "-steps", str(steps),
-           "-seed", str(seed), "-pk", mpk, "-out", out] + (["-newton"] if newton else [])
+           "-seed", str(seed), "-pk", mpk, "-out", out, "-gadget", snap, "-express", restart] + \
+        (["-newton"] if newton else [])
     print("running", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     pid, pos, q = _read_out(out)
     assert np.array_equal(pid, np.arange(N ** 3))
+    _check_files(snap, restart + ".0", pid, pos, q, L, single)
 
     # the same run through gev_step (fused kick+drift, one re-binning)
     fpg = gevpm.fourpiG_of(L)

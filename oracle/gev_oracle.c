@@ -935,3 +935,164 @@ OR_API double or_particle_horizon(double a, double fourpiG, double om, double ol
     }
     return sum / sqrt(fourpiG);
 }
+
+/* ------------------------------------------------------------------------
+ * Lightcone particle output (single rank).
+ * ------------------------------------------------------------------------ */
+
+#define OR_IDCHECK_ZONE 0.05        /* LIGHTCONE_IDCHECK_ZONE, metadata.hpp:38-40 */
+#define OR_GADGET_VEL 3.335640952e-6 /* GADGET_VELOCITY_CONVERSION, metadata.hpp:117 */
+
+static double dist3(const double a[3], const double b[3])
+{
+    return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+/* findIntersectingLightcones (tools.hpp:619-862): the periodic copies
+ * lcvertex + (u,v,w) of the observer, |u|,|v|,|w| <= ceil(outer)+1, visited
+ * u-major, whose look-back shell [inner, outer] and cone may meet the box
+ * domain = {lo[0..2], hi[0..2]} (box units).  Rejected: copies farther than
+ * outer from the nearest point of the box, and copies with every box corner
+ * nearer than inner.  Accepted: copies inside the box's enclosing sphere,
+ * copies that see the box centre inside the opening, and copies whose
+ * opening meets the enclosing sphere (two angular tests, outside and inside
+ * the outer radius).  Returns the number found; the first cap are stored
+ * (the reference stores MAX_INTERSECTS = 12). */
+OR_API int or_lightcone_vertices(const double lcvertex[3], const double dir[3], double opening, double outer,
+                                 double inner, const double domain[6], double *vertex, int cap)
+{
+    const int range = (int)ceil(outer) + 1;
+    const double *lo = domain, *hi = domain + 3;
+    double corner[8][3];
+    for (int c = 0; c < 8; c++) {
+        corner[c][0] = (c & 1) ? hi[0] : lo[0];
+        corner[c][1] = (c & 2) ? hi[1] : lo[1];
+        corner[c][2] = (c & 4) ? hi[2] : lo[2];
+    }
+    const double centre[3] = {0.5 * lo[0] + 0.5 * hi[0], 0.5 * lo[1] + 0.5 * hi[1], 0.5 * lo[2] + 0.5 * hi[2]};
+    const double rdom = 0.5 * sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                   (hi[2] - lo[2]) * (hi[2] - lo[2]));
+    int n = 0;
+    for (int u = -range; u <= range; u++)
+        for (int v = -range; v <= range; v++)
+            for (int w = -range; w <= range; w++) {
+                const double vx[3] = {lcvertex[0] + u, lcvertex[1] + v, lcvertex[2] + w};
+                /* distance to the nearest point of the box */
+                double s2 = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    const double e = vx[d] < lo[d] ? lo[d] - vx[d] : (vx[d] > hi[d] ? vx[d] - hi[d] : 0.0);
+                    s2 += e * e;
+                }
+                if (sqrt(s2) > outer) continue;
+                int all_in = 1;
+                for (int c = 0; c < 8 && all_in; c++)
+                    if (!(dist3(corner[c], vx) < inner)) all_in = 0;
+                if (all_in) continue;
+                const double dist = dist3(centre, vx);
+                int take = dist <= rdom;
+                if (!take) {
+                    const double cosang = ((centre[0] - vx[0]) * dir[0] + (centre[1] - vx[1]) * dir[1] +
+                                           (centre[2] - vx[2]) * dir[2]) / dist;
+                    if (cosang >= opening)
+                        take = 1;
+                    else if (dist > outer &&
+                             acos(cosang) - acos(opening) <=
+                                 acos((outer * outer + dist * dist - rdom * rdom) / (2. * outer * dist)))
+                        take = 1;
+                    else if (dist <= outer && acos(cosang) - acos(opening) <= asin(rdom / dist))
+                        take = 1;
+                }
+                if (take) {
+                    if (n < cap)
+                        for (int d = 0; d < 3; d++) vertex[3 * n + d] = vx[d];
+                    n++;
+                }
+            }
+    return n;
+}
+
+static int backlog_has(const long *b, long nb, long id)
+{
+    long lo = 0, hi = nb;
+    while (lo < hi) {
+        long mid = lo + (hi - lo) / 2;
+        if (b[mid] < id) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < nb && b[lo] == id;
+}
+
+/* saveGadget2, lightcone form, device path (Particles_gevolution.hpp:1375-1470
+ * buffer_tracer_particles + bufferTracerParticle :500-592 with
+ * EXACT_OUTPUT_REDSHIFTS, the backlog check :2050-2115 and the prelog bins
+ * :2120-2170 for one rank).  Walks the particles in order and writes the
+ * selected ones in that order:
+ *   - ID % tracer_factor == 0;
+ *   - the first vertex v with inner <= d < outer (d = |pos - v|,
+ *     inner = dist - dtau/2, outer = dist + (1/2 + ZONE) dtau_old) and, unless
+ *     opening == -1, (pos - v).direction / d > opening;
+ *   - not in the backlog when outer - d <= 2 ZONE dtau_old.
+ * Each selected particle is moved to the light-cone crossing,
+ * dtau_pos = (dist - d) / (1 + vlos), and kicked by dtau_pos + dtau_old/2
+ * with the CIC edge gradient of phi at a + dtau_pos dadtau; positions are
+ * shifted by (lightcone vertex - v), times boxsize, not wrapped.
+ * prelog[j] = 1 when d - inner < 2 ZONE dtau (the ID goes to the next
+ * backlog).  margin[i] (every particle) is the smallest distance of any
+ * selection quantity to its threshold, over all vertices, for parity tests.
+ * Returns the number written. */
+OR_API long or_lightcone_select(int N, long np, const double *pos, const double *q, const long *id,
+                                const double *phi, int tracer_factor, const double lcvertex[3], const double dir[3],
+                                double opening, double dist, double dtau, double dtau_old, double dadtau, double a,
+                                double boxsize, const double *vertex, int nv, const long *backlog, long nbacklog,
+                                float *posout, float *velout, long *idout, unsigned char *prelog, double *margin)
+{
+    const double inner = dist - 0.5 * dtau, outer = dist + (0.5 + OR_IDCHECK_ZONE) * dtau_old;
+    const double zc = 2. * OR_IDCHECK_ZONE * dtau_old, zp = 2. * OR_IDCHECK_ZONE * dtau;
+    long nw = 0;
+    for (long i = 0; i < np; i++) {
+        double m = INFINITY;
+        const double P[3] = {pos[i], pos[np + i], pos[2 * np + i]};
+        int hit = -1;
+        double dh = 0.0;
+        for (int v = 0; v < nv; v++) {
+            const double *vx = vertex + 3 * v;
+            const double d = dist3(P, vx);
+            const double cosang = ((P[0] - vx[0]) * dir[0] + (P[1] - vx[1]) * dir[1] + (P[2] - vx[2]) * dir[2]) / d;
+            double t[5] = {fabs(d - inner), fabs(d - outer), opening == -1. ? INFINITY : fabs(cosang - opening),
+                           fabs(outer - d - zc), fabs(d - inner - zp)};
+            for (int k = 0; k < 5; k++) m = t[k] < m ? t[k] : m;
+            if (hit < 0 && d >= inner && d < outer && (opening == -1. || cosang > opening)) {
+                hit = v;
+                dh = d;
+            }
+        }
+        if (margin) margin[i] = m;
+        if (tracer_factor <= 0 || id[i] % tracer_factor != 0 || hit < 0) continue;
+        if (outer - dh <= zc && nbacklog > 0 && backlog_has(backlog, nbacklog, id[i])) continue;
+        const double *vx = vertex + 3 * hit;
+        const double Q[3] = {q[i], q[np + i], q[2 * np + i]};
+        const double q2 = Q[0] * Q[0] + Q[1] * Q[1] + Q[2] * Q[2];
+        const double vlos = (Q[0] * (P[0] - vx[0]) + Q[1] * (P[1] - vx[1]) + Q[2] * (P[2] - vx[2])) / dh /
+                            sqrt(q2 + a * (a + dtau_old * dadtau));
+        const double dtp = (dist - dh) / (1. + vlos), dtv = dtp + 0.5 * dtau_old, ap = a + dtp * dadtau;
+        double phip = 0.0, g[3] = {0.0, 0.0, 0.0};
+        if (phi) {
+            long c[3];
+            double r[3];
+            cell_of(N, pos, np, i, c, r);
+            phip = cic_interp(N, phi, c[0], c[1], c[2], r);
+            grad1(N, phi, c[0], c[1], c[2], r, g);
+        }
+        const double e2 = q2 + ap * ap, e = sqrt(e2), w = q2 + e2;
+        const double vfac = 1. + (4. - w / e2) * phip;
+        const double rescale = 1. / sqrt(ap) / OR_GADGET_VEL;
+        for (int d = 0; d < 3; d++) {
+            posout[3 * nw + d] = (float)((P[d] + dtp * Q[d] * vfac / e + (lcvertex[d] - vx[d])) * boxsize);
+            velout[3 * nw + d] = (float)((Q[d] - dtv * w * g[d] * (double)N / e) * rescale / ap);
+        }
+        idout[nw] = id[i];
+        prelog[nw] = (dh - inner < zp) ? 1 : 0;
+        nw++;
+    }
+    return nw;
+}

@@ -25,6 +25,8 @@ _cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
 def build():
@@ -80,6 +82,9 @@ def lib():
                                _dp, _dp, C.POINTER(C.c_double)]),
         "or_step": (V, [I, Lg, I, D, D, D, D, D, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_power_spectrum": (V, [I, _cp, I, I, _dp, _dp, _dp, _dp, _i32p]),
+        "or_lightcone_vertices": (I, [_dp, _dp, D, D, D, _dp, _dp, I]),
+        "or_lightcone_select": (Lg, [I, Lg, _dp, _dp, _ip, vp, I, _dp, _dp, D, D, D, D, D, D, D, _dp, I, vp, Lg,
+                                     _f32p, _f32p, _ip, _u8p, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -252,3 +257,37 @@ def power_spectrum(FT, numbins, deconvolve=True):
     occ = np.zeros(numbins, np.int32)
     lib().or_power_spectrum(N, np.ascontiguousarray(FT), numbins, int(deconvolve), kb, pw, ks, ps, occ)
     return kb, pw, ks, ps, occ
+
+
+MAX_INTERSECTS = 12
+IDCHECK_ZONE = 0.05
+
+
+def lightcone_vertices(lc_vertex, direction, opening, outer, inner, domain, cap=MAX_INTERSECTS):
+    """findIntersectingLightcones: (count, vertices[min(count, cap), 3])"""
+    out = np.zeros((max(cap, 1), 3))
+    n = lib().or_lightcone_vertices(_c(lc_vertex), _c(direction), float(opening), float(outer), float(inner),
+                                    _c(domain), out, int(cap))
+    return n, out[:min(n, cap)].copy()
+
+
+def lightcone_select(N, pos, q, pid, phi, tracer_factor, lc_vertex, direction, opening, dist, dtau, dtau_old,
+                     dadtau, a, boxsize, vertices, backlog=None):
+    """lightcone form of saveGadget2 for one rank: (pos [n,3] f32, vel [n,3] f32,
+    id [n], prelog flag [n] bool, margin [np]) in particle order"""
+    pos, q = _c(pos), _c(q)
+    np_ = pos.shape[1]
+    pid = np.ascontiguousarray(pid, np.int64)
+    vertices = _c(np.asarray(vertices, np.float64).reshape(-1, 3))
+    bl = None if backlog is None or len(backlog) == 0 else np.ascontiguousarray(backlog, np.int64)
+    po = np.zeros((max(np_, 1), 3), np.float32)
+    vo = np.zeros((max(np_, 1), 3), np.float32)
+    io = np.zeros(max(np_, 1), np.int64)
+    pl = np.zeros(max(np_, 1), np.uint8)
+    margin = np.zeros(max(np_, 1))
+    ph = None if phi is None else _c(phi).reshape(-1)
+    n = lib().or_lightcone_select(N, np_, pos, q, pid, _ptr(ph), int(tracer_factor), _c(lc_vertex), _c(direction),
+                                  float(opening), float(dist), float(dtau), float(dtau_old), float(dadtau), float(a),
+                                  float(boxsize), vertices, len(vertices), _ptr(bl), 0 if bl is None else len(bl),
+                                  po, vo, io, pl, margin)
+    return po[:n].copy(), vo[:n].copy(), io[:n].copy(), pl[:n].astype(bool), margin[:np_].copy()

@@ -1,6 +1,8 @@
 // gev_kernels.hpp -- internal launchers shared between the HIP translation
 // units of the library (not part of the C ABI).
 #pragma once
+#include <stdio.h>
+
 #include "gev_common.hpp"
 
 namespace gev {
@@ -42,6 +44,13 @@ int read_max_slot(gev_ctx *c, double *out);
 
 // zero the cell counters unless the last scan already left them zero
 int pcls_clear_counts(gev_pcls *p);
+
+// start[0..M) = exclusive prefix of cnt[0..M), start[M] = the sum; clears cnt.
+// M at most the context's lattice size.
+int scan_exclusive_counts(gev_ctx *ctx, int32_t *cnt, int64_t M, int32_t *start);
+
+// one Fortran-style record: byte count, data, byte count
+bool gadget_put_record(FILE *f, const void *data, uint32_t bytes);
 
 // deterministic two-level FP64 sum: per-block partials then one block
 int reduce_partials(gev_ctx *ctx, const double *partials, int nparts, int nvals, double *out_dev);

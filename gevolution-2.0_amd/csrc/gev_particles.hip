@@ -153,7 +153,8 @@ __global__ __launch_bounds__(1024) void k_scan_mid(int32_t *bsum, int64_t nb)
         }
 }
 
-// writes the exclusive prefix of cnt to start and clears cnt for the next count
+// writes the exclusive prefix of cnt to start and clears cnt for the next count;
+// start[M] = total, or the sum of all counts when total < 0
 __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(int32_t *__restrict__ cnt, int64_t M,
                                                         const int32_t *__restrict__ bsum, int32_t *__restrict__ start,
                                                         int32_t total)
@@ -197,7 +198,13 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(int32_t *__restrict__ cn
         for (int j = 0; j < SCAN_PER; j++)
             if (base + j < M) start[base + j] = o[j];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) start[M] = total;
+    if (total >= 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) start[M] = total;
+    } else {
+#pragma unroll
+        for (int j = 0; j < SCAN_PER; j++)
+            if (base + j == M - 1) start[M] = o[j] + v[j];
+    }
 }
 
 template <typename T> __global__ void k_scatter(PV<T> s, PV<T> d, const int32_t *__restrict__ rank)
@@ -309,6 +316,22 @@ template <typename T> int pcls_sort_counted_impl(gev_pcls *p)
     }
     GEV_HIP(hipGetLastError());
     p->sorted = true;
+    return GEV_OK;
+}
+
+int scan_exclusive_counts(gev_ctx *ctx, int32_t *cnt, int64_t M, int32_t *start)
+{
+    const int64_t nb = (M + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (M <= 0 || nb > ctx->scan_blocks_n) {
+        gev_set_error("scan_exclusive_counts: length outside the context's scan space");
+        return GEV_ERR_ARG;
+    }
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_NT), 0, s, cnt, M, ctx->scan_blocks);
+    hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(1024), 0, s, ctx->scan_blocks, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_NT), 0, s, cnt, M, ctx->scan_blocks, start,
+                       (int32_t)-1);
+    GEV_HIP(hipGetLastError());
     return GEV_OK;
 }
 

@@ -118,7 +118,7 @@ static int gadget_convert(gev_pcls *p, double dtau_pos, double dtau_vel, double 
     return GEV_OK;
 }
 
-static bool put_record(FILE *f, const void *data, uint32_t bytes)
+bool gadget_put_record(FILE *f, const void *data, uint32_t bytes)
 {
     return fwrite(&bytes, 4, 1, f) == 1 && (bytes == 0 || fwrite(data, 1, bytes, f) == bytes) &&
            fwrite(&bytes, 4, 1, f) == 1;
@@ -208,8 +208,8 @@ int gev_pcls_save_gadget2(gev_pcls *p, const char *path, const gev_gadget2_heade
     FILE *f = fopen(path, "wb");
     if (!f) { gev_set_error(std::string("gev_pcls_save_gadget2: cannot open ") + path); return GEV_ERR_ARG; }
     const uint32_t b3 = (uint32_t)(12 * m), b1 = (uint32_t)(8 * m);
-    bool ok = put_record(f, hdr, sizeof(gev_gadget2_header)) && put_record(f, pos.data(), b3) &&
-              put_record(f, vel.data(), b3) && put_record(f, id.data(), b1);
+    bool ok = gadget_put_record(f, hdr, sizeof(gev_gadget2_header)) && gadget_put_record(f, pos.data(), b3) &&
+              gadget_put_record(f, vel.data(), b3) && gadget_put_record(f, id.data(), b1);
     ok = (fclose(f) == 0) && ok;
     if (!ok) { gev_set_error(std::string("gev_pcls_save_gadget2: write failed on ") + path); return GEV_ERR_ARG; }
     return GEV_OK;

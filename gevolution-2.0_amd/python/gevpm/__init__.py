@@ -387,6 +387,25 @@ class Particles:
         check(lib.gev_pcls_save_express(self._h, os.fsencode(path), C.byref(header)), "gev_pcls_save_express")
         return header
 
+    def save_gadget2_lightcone(self, path, header, lightcone, dist, dtau, dtau_old, dadtau, vertices, idlog=None,
+                               phi=None, tracer_factor=1):
+        """saveGadget2, lightcone form: the particles crossing the light cone of
+        `lightcone` (a gevpm.lightcone.Geometry) seen from `vertices` during this
+        cycle, moved to the crossing; fills header's particle counts and returns
+        the number written (no file when zero).  `idlog` (gevpm.lightcone.IDLog)
+        supplies the backlog and receives the prelog."""
+        import numpy as np
+        from .lightcone import Geometry
+        geo = lightcone.c() if isinstance(lightcone, Geometry) else lightcone
+        v = np.ascontiguousarray(np.asarray(vertices, np.float64).reshape(-1, 3))
+        written = C.c_int64(0)
+        self.ctx.bind_stream()
+        check(lib.gev_pcls_save_gadget2_lightcone(
+            self._h, os.fsencode(path), C.byref(header), C.byref(geo), float(dist), float(dtau), float(dtau_old),
+            float(dadtau), v.ctypes.data_as(C.POINTER(C.c_double)), len(v), None if idlog is None else idlog._h,
+            _ptr(phi), int(tracer_factor), C.byref(written)), "gev_pcls_save_gadget2_lightcone")
+        return written.value
+
     def load_express(self, path):
         """loadExpress: replaces the particles with the file's; returns its Gadget-2 header"""
         hdr = gev_gadget2_header()

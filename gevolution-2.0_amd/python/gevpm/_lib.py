@@ -38,6 +38,12 @@ class gev_gadget2_header(C.Structure):
                 ("fill", C.c_char * 64)]
 
 
+class gev_lightcone_geometry(C.Structure):
+    """lightcone_geometry (metadata.hpp:205-212)"""
+    _fields_ = [("vertex", C.c_double * 3), ("z", C.c_double), ("direction", C.c_double * 3),
+                ("opening", C.c_double), ("distance", C.c_double * 2)]
+
+
 class gev_step_state(C.Structure):
     _fields_ = [("gr", C.c_int), ("vector_elliptic", C.c_int), ("fourpiG", C.c_double), ("Omega_m", C.c_double),
                 ("Omega_Lambda", C.c_double), ("Omega_rad", C.c_double), ("z_relax", C.c_double),
@@ -95,6 +101,17 @@ SIGNATURES = {
     "gev_pcls_load_gadget2": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),
     "gev_pcls_save_express": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),
     "gev_pcls_load_express": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),
+    "gev_lightcone_vertices": (_i, [C.POINTER(gev_lightcone_geometry), _d, _d, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), _i, C.POINTER(C.c_int)]),
+    "gev_idlog_create": (_i, [_vp, C.POINTER(_vp)]),
+    "gev_idlog_destroy": (_i, [_vp]),
+    "gev_idlog_sizes": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "gev_idlog_get": (_i, [_vp, _i, C.POINTER(C.c_int64)]),
+    "gev_idlog_set_backlog": (_i, [_vp, C.POINTER(C.c_int64), C.c_int64]),
+    "gev_idlog_commit": (_i, [_vp]),
+    "gev_pcls_save_gadget2_lightcone": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header),
+                                             C.POINTER(gev_lightcone_geometry), _d, _d, _d, _d,
+                                             C.POINTER(C.c_double), _i, _vp, _vp, _i, C.POINTER(C.c_int64)]),
     "gev_adj_deposit": (_i, [_vp, _vp, _vp]),
     "gev_adj_divergence": (_i, [_vp, _vp, _vp, _d]),
     "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _d]),

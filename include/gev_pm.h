@@ -222,6 +222,54 @@ int gev_pcls_load_gadget2(gev_pcls *p, const char *path, gev_gadget2_header *hdr
 int gev_pcls_save_express(gev_pcls *p, const char *path, gev_gadget2_header *hdr);
 int gev_pcls_load_express(gev_pcls *p, const char *path, gev_gadget2_header *hdr);
 
+/* ---- lightcone particle output (one rank) */
+#define GEV_MAX_INTERSECTS 12            /* MAX_INTERSECTS, metadata.hpp:34-36 */
+#define GEV_LIGHTCONE_MAX_VERTICES 32    /* vertices one output call accepts */
+#define GEV_LIGHTCONE_IDCHECK_ZONE 0.05  /* LIGHTCONE_IDCHECK_ZONE, metadata.hpp:38-40 */
+/* lightcone_geometry (metadata.hpp:205-212): vertex in box units, opening =
+ * cos of the half-angle (-1: full sky), distance[2] the far and near edges. */
+typedef struct gev_lightcone_geometry {
+    double vertex[3];
+    double z;
+    double direction[3];
+    double opening;
+    double distance[2];
+} gev_lightcone_geometry;
+/* findIntersectingLightcones (tools.hpp:619-862): the periodic copies of the
+ * observer whose look-back shell [inner, outer] and cone may meet the box
+ * domain[0..2] (low corner) to domain[3..5] (high corner), box units, in the
+ * reference's u-major order.  *count = the number found; the first cap are
+ * stored in vertex[cap][3]. */
+int gev_lightcone_vertices(const gev_lightcone_geometry *lc, double outer, double inner, const double domain[6],
+                           double *vertex, int cap, int *count);
+/* The ID log of one lightcone group: LightconeIDBacklog
+ * (lightcone_id_backlog.hpp:16-125) plus the IDprelog vector writeLightcones
+ * fills and hands over (output.hpp:1017-1185, 2166-2880).  The lightcone
+ * output skips check-zone particles whose ID is in the backlog and appends
+ * the IDs it writes near the inner edge to the prelog; commit makes the
+ * prelog (sorted, unique) the next backlog.  which: 0 = backlog, 1 = prelog. */
+typedef struct gev_idlog gev_idlog;
+int gev_idlog_create(gev_ctx *ctx, gev_idlog **out);
+int gev_idlog_destroy(gev_idlog *log);
+int gev_idlog_sizes(const gev_idlog *log, int64_t *nbacklog, int64_t *nprelog);
+int gev_idlog_get(const gev_idlog *log, int which, int64_t *ids);
+int gev_idlog_set_backlog(gev_idlog *log, const int64_t *ids, int64_t n);
+int gev_idlog_commit(gev_idlog *log);
+/* saveGadget2, lightcone form (perfParticles_gevolution, Particles_gevolution.hpp:
+ * 172,1566-2317, device path with EXACT_OUTPUT_REDSHIFTS): writes the
+ * particles whose ID is a multiple of tracer_factor and that lie in the shell
+ * dist - dtau/2 <= d < dist + (1/2 + ZONE) dtau_old around one of the
+ * vertices (and inside the opening), moved to their light-cone crossing and
+ * kicked with the gradient of phi (may be NULL), as a single-file Gadget-2
+ * snapshot, in container order.  hdr->time is a, hdr->BoxSize the length
+ * scale; hdr's type-1 counts are set to the number written (*written, may
+ * be NULL).  No file is written when it is zero.  log may be NULL (no
+ * backlog check, no prelog). */
+int gev_pcls_save_gadget2_lightcone(gev_pcls *p, const char *path, gev_gadget2_header *hdr,
+                                    const gev_lightcone_geometry *lc, double dist, double dtau, double dtau_old,
+                                    double dadtau, const double *vertex, int vertexcount, gev_idlog *log,
+                                    const void *phi, int tracer_factor, int64_t *written);
+
 /* ---- adjoint of the Newtonian cycle (no reference counterpart; reverse mode
  *      of update_q_Newton, update_pos_Newton, the CIC mass deposit and the
  *      Poisson solve, gevolution.hpp:1051-1118, 1270-1273, 1902-1905, 806-852).

@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -29,6 +30,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "gev_pm.h"
@@ -175,6 +177,65 @@ struct part_simple_info {
 
 // metadata.hpp:159-178, byte for byte
 using gadget2_header = gev_gadget2_header;
+// metadata.hpp:205-212
+using lightcone_geometry = gev_lightcone_geometry;
+constexpr int MAX_INTERSECTS = GEV_MAX_INTERSECTS;
+
+// LightconeIDBacklog (lightcone_id_backlog.hpp:16-125) together with the
+// IDprelog vector writeLightcones keeps beside it (output.hpp:1017-1185):
+// the lightcone saveGadget2 reads the backlog and appends to the prelog;
+// finalize() makes the prelog, sorted and unique, the next backlog (the
+// hand-over at the end of writeLightcones, output.hpp:2166-2880, one rank).
+class LightconeIDBacklog {
+  public:
+    explicit LightconeIDBacklog(Lattice *lat) { check(gev_idlog_create(lat->ctx(), &h_), "LightconeIDBacklog"); }
+    ~LightconeIDBacklog() { gev_idlog_destroy(h_); }
+    LightconeIDBacklog(const LightconeIDBacklog &) = delete;
+    LightconeIDBacklog &operator=(const LightconeIDBacklog &) = delete;
+
+    size_t size() const { return (size_t)sizes().first; }
+    bool empty() const { return size() == 0; }
+    size_t prelog_size() const { return (size_t)sizes().second; }
+    std::vector<int64_t> ids() const { return get(0); }
+    std::vector<int64_t> prelog() const { return get(1); }
+    bool contains(int64_t id) const
+    {
+        const std::vector<int64_t> v = ids();
+        return std::binary_search(v.begin(), v.end(), id);
+    }
+    // clear + append + finalize of the reference class
+    void assign(const std::vector<int64_t> &v) { check(gev_idlog_set_backlog(h_, v.data(), (int64_t)v.size()), "LightconeIDBacklog::assign"); }
+    void clear() { assign({}); }
+    void finalize() { check(gev_idlog_commit(h_), "LightconeIDBacklog::finalize"); }
+    gev_idlog *handle() const { return h_; }
+
+  private:
+    std::pair<int64_t, int64_t> sizes() const
+    {
+        int64_t a = 0, b = 0;
+        check(gev_idlog_sizes(h_, &a, &b), "LightconeIDBacklog::size");
+        return {a, b};
+    }
+    std::vector<int64_t> get(int which) const
+    {
+        const auto s = sizes();
+        std::vector<int64_t> v((size_t)(which == 0 ? s.first : s.second));
+        check(gev_idlog_get(h_, which, v.data()), "LightconeIDBacklog::ids");
+        return v;
+    }
+    gev_idlog *h_ = nullptr;
+};
+
+// findIntersectingLightcones (tools.hpp:619-862); returns the number found,
+// of which the first MAX_INTERSECTS are stored
+inline int findIntersectingLightcones(lightcone_geometry &lightcone, double outer, double inner, double *domain,
+                                      double vertex[MAX_INTERSECTS][3])
+{
+    int n = 0;
+    check(gev_lightcone_vertices(&lightcone, outer, inner, domain, &vertex[0][0], MAX_INTERSECTS, &n),
+          "findIntersectingLightcones");
+    return n;
+}
 
 // functor tags: which gevolution functor updateVel / moveParticles apply
 struct update_q_functor {};         // gevolution.hpp:888-1011
@@ -282,6 +343,18 @@ template <class part = part_simple, class part_info = part_simple_info> class pe
     void loadGadget2(std::string filename, gadget2_header &hdr)
     {
         check(gev_pcls_load_gadget2(p_, filename.c_str(), &hdr), "loadGadget2");
+    }
+    // saveGadget2, lightcone form (Particles_gevolution.hpp:172,1566-2317); the
+    // prelog lives in IDbacklog (its prelog()) instead of a separate vector
+    void saveGadget2(std::string filename, gadget2_header &hdr, lightcone_geometry &lightcone, double dist,
+                     double dtau, double dtau_old, double dadtau, double vertex[][3], const int vertexcount,
+                     LightconeIDBacklog &IDbacklog, Field<Real> *phi, const int tracer_factor = 1)
+    {
+        check(gev_pcls_save_gadget2_lightcone(p_, filename.c_str(), &hdr, &lightcone, dist, dtau, dtau_old, dadtau,
+                                              vertexcount > 0 ? &vertex[0][0] : nullptr, vertexcount,
+                                              IDbacklog.handle(), phi ? phi->data() : nullptr, tracer_factor,
+                                              nullptr),
+              "saveGadget2(lightcone)");
     }
     void saveExpress(std::string filename, gadget2_header &hdr)
     {

@@ -5,6 +5,7 @@
 //
 //   main_pm -n N -L boxsize -zin z -zend z -steps S [-newton] [-seed s]
 //           [-zrelax z] -pk mpk.txt [-out final.bin] [-gadget snap] [-express restart]
+//           [-lightcone prefix -lcvertex x,y,z -lcz z]
 //
 // The mPk file has the loadPowerSpectrum format (ic_basic.hpp:370-449): two
 // columns, k [h/Mpc] and P(k) [(Mpc/h)^3].  Steps are uniform in conformal
@@ -12,6 +13,10 @@
 // int64 n, int64 id[n], double pos[3][n], double q[3][n].  -gadget writes them
 // as a Gadget-2 snapshot with the header output.hpp:418-466 fills (Omega_cdm
 // particles, no tracer thinning), -express as an express restart file.
+// -lightcone writes, each cycle before the kick, the full-sky particle
+// lightcone of an observer at lcvertex (box units) and redshift lcz as
+// <prefix>_<cycle>_cdm (writeLightcones' Gadget branch, output.hpp:2094-2160,
+// and the ID backlog hand-over, :2166-2880).
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -33,7 +38,8 @@ struct Options {
     int N = 32, steps = 5, newton = 0;
     uint32_t seed = 42;
     double L = 320.0, zin = 100.0, zend = 50.0, zrelax = -2.0;
-    std::string pk, out, gadget, express;
+    std::string pk, out, gadget, express, lightcone;
+    double lcvertex[3] = {0.5, 0.5, 0.5}, lcz = 0.0;
 };
 
 Options parse(int argc, char **argv)
@@ -57,6 +63,11 @@ Options parse(int argc, char **argv)
         else if (a == "-out") o.out = next();
         else if (a == "-gadget") o.gadget = next();
         else if (a == "-express") o.express = next();
+        else if (a == "-lightcone") o.lightcone = next();
+        else if (a == "-lcvertex") {
+            if (std::sscanf(next(), "%lf,%lf,%lf", &o.lcvertex[0], &o.lcvertex[1], &o.lcvertex[2]) != 3)
+                throw std::runtime_error("-lcvertex wants x,y,z");
+        } else if (a == "-lcz") o.lcz = std::atof(next());
         else throw std::runtime_error("unknown option " + a);
     }
     if (o.pk.empty()) throw std::runtime_error("-pk <mPk file> is required");
@@ -164,6 +175,44 @@ int main(int argc, char **argv)
         Field<Real> *update_fields[3] = {&phi, &chi, &Bi};
         double f_params[2];
 
+        lightcone_geometry lc{};
+        for (int d = 0; d < 3; d++) lc.vertex[d] = opt.lcvertex[d];
+        lc.z = opt.lcz;
+        lc.direction[2] = 1.0;
+        lc.opening = -1.0;
+        lc.distance[0] = 1e30;
+        lc.distance[1] = 0.0;
+        LightconeIDBacklog IDbacklog(&lat);
+        auto write_lightcone = [&](int cycle) {
+            if (opt.lightcone.empty()) return;
+            const double d = particleHorizon(1. / (1. + lc.z), fourpiG, cosmo);  // output.hpp:1213
+            const double shell = d - tau + 0.5 * dtau_old;
+            if (lc.distance[0] > shell && lc.distance[1] <= shell && shell > 0.) {
+                double domain[6], vertex[MAX_INTERSECTS][3];
+                for (int k = 0; k < 3; k++) {
+                    domain[k] = -0.5 / N;
+                    domain[k + 3] = (N + 0.5) / N;
+                }
+                const int nv = findIntersectingLightcones(lc, d - tau + (0.5 + GEV_LIGHTCONE_IDCHECK_ZONE) * dtau_old,
+                                                          d - tau - 0.5 * dtau, domain, vertex);
+                if (nv > MAX_INTERSECTS) throw std::runtime_error("lightcone meets more than MAX_INTERSECTS copies");
+                gadget2_header hdr{};
+                hdr.num_files = 1;
+                hdr.Omega0 = cosmo.Omega_m;
+                hdr.OmegaLambda = cosmo.Omega_Lambda;
+                hdr.HubbleParam = cosmo.h;
+                hdr.BoxSize = opt.L / GADGET_LENGTH_CONVERSION;
+                hdr.time = a;
+                hdr.redshift = 1. / a - 1.;
+                hdr.mass[1] = C_RHO_CRIT * cosmo.Omega_cdm * opt.L * opt.L * opt.L / (double)M / GADGET_MASS_CONVERSION;
+                char name[32];
+                std::snprintf(name, sizeof(name), "_%04d_cdm", cycle);
+                pcls.saveGadget2(opt.lightcone + name, hdr, lc, d - tau, dtau, dtau_old, a * Hconf(a, fourpiG, cosmo),
+                                 vertex, nv, IDbacklog, &phi, 1);
+            }
+            IDbacklog.finalize();
+        };
+
         for (int cycle = 0; cycle < opt.steps; cycle++) {
             if (!opt.newton) {
                 // T00 (main.cu:615-651) with the old phi
@@ -193,6 +242,7 @@ int main(int argc, char **argv)
                 plan_Bi.execute(FFT_FORWARD);              // main.cu:918
                 projectFTvector(BiFT, BiFT, fourpiG * dx * dx);  // main.cu:925
                 plan_Bi.execute(FFT_BACKWARD);                   // main.cu:948
+                write_lightcone(cycle);  // main.cu:1013-1131 outputs, before the particle update
                 // kick (main.cu:1227-1240), background half step, drift (:1253-1266)
                 f_params[0] = a;
                 f_params[1] = a * a * N;
@@ -211,6 +261,7 @@ int main(int argc, char **argv)
                 plan_source.execute(FFT_FORWARD);                      // main.cu:742
                 solveModifiedPoissonFT(scalarFT, scalarFT, fourpiG / a);  // main.cu:765
                 plan_phi.execute(FFT_BACKWARD);                           // main.cu:794
+                write_lightcone(cycle);
                 f_params[0] = a;
                 maxvel = pcls.updateVel(update_q_Newton_functor(), (dtau + dtau_old) / 2., update_fields, 1, f_params);
                 rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau);

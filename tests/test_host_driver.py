@@ -128,3 +128,58 @@ def test_driver_matches_gev_step(tmp_path, newton, single):
     last = [ln for ln in r.stdout.splitlines() if ln.startswith("done")]
     # RK4 in a(tau) over 4 large steps lands within ~1e-5 of the target redshift
     assert last and abs(float(last[0].split("z=")[1]) - z_end) < 1e-4
+
+
+@pytest.mark.gpu
+def test_driver_lightcone_output(tmp_path):
+    """main_pm -lightcone: the façade's findIntersectingLightcones,
+    LightconeIDBacklog and lightcone saveGadget2 inside the cycle; each file
+    holds distinct IDs at distances from the observer inside that cycle's
+    shell, and few IDs appear in two consecutive files"""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gevpm
+    import gevpm.cosmology as cosmo
+    N, L, z_in, z_end, steps = 16, 320.0, 100.0, 60.0, 8
+    fpg = gevpm.fourpiG_of(L)
+    k, P = cosmo.bbks_table(z=z_in)
+    mpk = str(tmp_path / "pk.txt")
+    cosmo.write_mpk(mpk, k, P)
+    tau_in = gevpm.particle_horizon(1 / (1 + z_in), fpg, 0.31, 0.69)
+    tau_end = gevpm.particle_horizon(1 / (1 + z_end), fpg, 0.31, 0.69)
+    dtau = (tau_end - tau_in) / steps
+    # observer whose horizon is 0.45 box beyond the initial time: shells of
+    # radius <= 0.5 box meet at most the faces' periodic copies
+    target = tau_in + 0.45
+    lo, hi = 1 / (1 + z_in), 1.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if gevpm.particle_horizon(mid, fpg, 0.31, 0.69) < target else (lo, mid)
+    lcz = 1 / lo - 1
+    lcv = (0.5, 0.45, 0.55)
+    prefix = str(tmp_path / "lc")
+    cmd = [os.path.join(LIB, "main_pm"), "-n", str(N), "-L", str(L), "-zin", str(z_in), "-zend", str(z_end),
+           "-steps", str(steps), "-pk", mpk, "-lightcone", prefix, "-lcvertex", "%g,%g,%g" % lcv, "-lcz", repr(lcz)]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    files = sorted(f for f in os.listdir(tmp_path) if f.startswith("lc_") and f.endswith("_cdm"))
+    assert len(files) >= 2
+    ids = []
+    for name in files:
+        cycle = int(name.split("_")[1])
+        raw = open(tmp_path / name, "rb").read()
+        n = int(np.frombuffer(raw, np.uint32, 6, 4)[1])
+        pos = np.frombuffer(raw, np.float32, 3 * n, 268).reshape(n, 3) / (L * 1000)
+        pid = np.frombuffer(raw, np.int64, n, 284 + 24 * n)
+        assert len(np.unique(pid)) == n
+        tau = tau_in + cycle * dtau
+        dist = target - tau  # d(lcz) - tau
+        inner, outer = dist - 0.5 * dtau, dist + 0.55 * dtau
+        d = np.linalg.norm(pos - np.array(lcv), axis=1)
+        # moved to the crossing by at most |v| dtau (|v| << 1 at z > 60)
+        assert (d > inner - 0.01 * dtau).all() and (d < outer + 0.01 * dtau).all()
+        ids.append(set(pid.tolist()))
+    for a_, b_ in zip(ids, ids[1:]):
+        assert len(a_ & b_) <= 0.05 * max(len(b_), 1)

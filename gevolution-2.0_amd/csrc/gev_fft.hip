@@ -276,8 +276,9 @@ __global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(Src src, cx<T> *__res
 }
 
 // x pass c2r: N/2+1 complex (pitch P) -> N reals; DC and Nyquist taken as real.
-// The first stage reads X[k] and X[H-k] straight from global memory (the
-// second read of each element hits L2), so LDS holds only the exchange buffer.
+// The lines are staged in LDS with one coalesced read per element, the first
+// stage pairs X[k] with X[H-k] from there, and the last stage's outputs go
+// back through LDS so that the global stores are consecutive per lane group.
 template <typename T, int N, int NL>
 __global__ __launch_bounds__(NL * N / 16) void k_c2r_x_reg(const cx<T> *__restrict__ in, T *__restrict__ out, int P,
                                                            int64_t nlines, const cx<T> *__restrict__ tw)
@@ -288,10 +289,14 @@ __global__ __launch_bounds__(NL * N / 16) void k_c2r_x_reg(const cx<T> *__restri
     for (int m = threadIdx.x; m < N; m += NT) twl[m] = tw[m];
     const int64_t line0 = (int64_t)blockIdx.x * NL;
     const cx<T> *src = in + line0 * P;
+    static_assert(LDL == H + 1, "a staged line holds X[0..H]");
+    for (int e = threadIdx.x; e < NL * LDL; e += NT) {
+        const int l = e / LDL;
+        lds[e] = (line0 + l < nlines) ? src[(int64_t)l * P + (e - l * LDL)] : mk((T)0, (T)0);
+    }
     __syncthreads();
     auto ld = [&](int line, int k) {
-        if (line0 + line >= nlines) return mk((T)0, (T)0);
-        const cx<T> *row = src + (int64_t)line * P;
+        const cx<T> *row = lds + line * LDL;
         cx<T> Xk = row[k];
         cx<T> Xm = row[H - k];
         if (k == 0) { Xk.y = 0; Xm.y = 0; }
@@ -301,10 +306,13 @@ __global__ __launch_bounds__(NL * N / 16) void k_c2r_x_reg(const cx<T> *__restri
         return E + mul_i(O, +1);
     };
     cx<T> *dst = reinterpret_cast<cx<T> *>(out) + line0 * H;
-    auto st = [&](int line, int n, cx<T> v, int) {
-        if (line0 + line < nlines) dst[line * H + n] = v;
-    };
-    reg_fft<T, H, NL, false, 2, false, 1>(lds, twl, +1, ld, st);
+    auto st = [&](int line, int n, cx<T> v, int) { lds[line * LDL + n] = v; };
+    reg_fft<T, H, NL, false, 2, true, 1>(lds, twl, +1, ld, st);
+    __syncthreads();
+    for (int e = threadIdx.x; e < NL * H; e += NT) {
+        const int l = e / H;
+        if (line0 + l < nlines) dst[e] = lds[l * LDL + (e - l * H)];
+    }
 }
 
 // ---------------------------------------------------------- dispatchers ---
@@ -597,11 +605,12 @@ __global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *o
     }
 }
 
-// the vector operator keeps 3 x 8 modes per thread in registers: at most 512
-// threads per workgroup there; Poisson and scalar use the column pass's width
+// the vector and scalar operators keep 3 x 8 values per thread in registers:
+// at most 512 threads per workgroup there; Poisson takes 512 too up to L = 512
+// (three workgroups per CU instead of one, 4 % faster at L = 512)
 template <typename T, int L, int OP> constexpr int zs_tx()
 {
-    constexpr bool narrow = OP == ZOP_VECTOR || OP == ZOP_SCALAR;
+    constexpr bool narrow = OP == ZOP_VECTOR || OP == ZOP_SCALAR || L <= 512;
     return (narrow && col_tx<T, L>() * L / 8 > 512) ? 4096 / L : col_tx<T, L>();
 }
 

@@ -21,6 +21,9 @@ int dep_T00_Tij_impl(gev_pcls *p, T *T00, T *Tij, const T *phi, double mass00, d
 template <typename T> int dep_T0i_impl(gev_pcls *p, T *out, const T *phi, double mass, int acc);
 template <typename T> int dep_Tij_impl(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc);
 template <typename T>
+int dep_T00_Tij_links_impl(gev_pcls *p, T *T00, T *Tij, T *links, const T *phi, double mass00, double massij,
+                           double mass0i, double a);
+template <typename T>
 int update_q_impl(gev_pcls *p, double dtau, const T *phi, const T *chi, const T *Bi, int nfield, double a,
                   double bcoeff, double *max_ret);
 template <typename T>
@@ -338,6 +341,21 @@ int gev_project_T00_Tij(gev_pcls *p, void *T00, void *Tij, double a, const void 
                                                  accumulate));
 }
 
+int gev_project_T00_Tij_T0i_links(gev_pcls *p, void *T00, void *Tij, void *links, double a, const void *phi,
+                                  double coeff)
+{
+    CHECK(p && T00 && Tij && links, "gev_project_T00_Tij_T0i_links: NULL argument");
+    NEED_SORTED(p);
+    const double dx = 1.0 / (double)p->ctx->N;
+    const double m = p->mass * coeff / (dx * dx * dx);
+    const double m00 = phi ? m / a : m, mij = m / a;  // as gev_project_T00 / gev_project_Tij
+    return PREC_DISPATCH(p->ctx,
+                         dep_T00_Tij_links_impl<double>(p, (double *)T00, (double *)Tij, (double *)links,
+                                                        (const double *)phi, m00, mij, m, a),
+                         dep_T00_Tij_links_impl<float>(p, (float *)T00, (float *)Tij, (float *)links,
+                                                       (const float *)phi, m00, mij, m, a));
+}
+
 // -------------------------------------------------------------- sources ---
 int gev_prepare_source_T00(gev_ctx *c, const void *phi, const void *chi, const void *source, double bg, void *result,
                            double c1, double c2, double c3, double *sum_source, double *sum_result)
@@ -553,6 +571,21 @@ int gev_solve_vector(gev_ctx *c, void *Bi, void *BiFT, double coeff, double modi
     int rc = gev_fft_r2c(c, Bi, BiFT, 3);
     if (!rc) rc = gev_project_ft_vector(c, BiFT, BiFT, coeff, modif);
     if (!rc) rc = gev_fft_c2r(c, BiFT, Bi, 3);
+    return rc;
+}
+
+int gev_solve_vector_T0i_links(gev_ctx *c, void *Bi, const void *phi, void *BiFT, double coeff, double modif)
+{
+    CHECK(c && Bi && phi && BiFT, "gev_solve_vector_T0i_links: NULL argument");
+    if (chi_from_Tij_supported(c))
+        return PREC_DISPATCH(c,
+                             solve_vector_links_fused_impl<double>(c, (double *)Bi, (const double *)phi,
+                                                                   (cx<double> *)BiFT, coeff, modif),
+                             solve_vector_links_fused_impl<float>(c, (float *)Bi, (const float *)phi,
+                                                                  (cx<float> *)BiFT, coeff, modif));
+    int rc = PREC_DISPATCH(c, t0i_link_factor_impl<double>(c, (double *)Bi, (const double *)phi),
+                           t0i_link_factor_impl<float>(c, (float *)Bi, (const float *)phi));
+    if (!rc) rc = gev_solve_vector(c, Bi, BiFT, coeff, modif);
     return rc;
 }
 

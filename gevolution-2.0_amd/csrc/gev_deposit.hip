@@ -22,12 +22,18 @@
 namespace gev {
 
 // DEP_T00TIJ: T00 and Tij of the same particles and phi in one pass (7 outputs)
+// DEP_T00TIJL: DEP_T00TIJ plus the particle part of T0i on each link (10 outputs):
+// projection_T0i weights the 12 link values of a particle by 1 + phi_a + phi_b
+// of the link's end points, a factor common to every particle on the link, so
+// the links can be summed here (with the old phi's T00 and Tij pass) and
+// multiplied by the new phi's factor later, inside the B solve
 // DEP_VEC3: CIC deposit of a per-particle 3-vector indexed by particle ID
 // (the adjoint's u_i = sum_p w_c lq_{p,i}), weights read from `aux`
-enum { DEP_T00 = 0, DEP_T0I = 1, DEP_TIJ = 2, DEP_T00TIJ = 3, DEP_VEC3 = 4 };
+enum { DEP_T00 = 0, DEP_T0I = 1, DEP_TIJ = 2, DEP_T00TIJ = 3, DEP_VEC3 = 4, DEP_T00TIJL = 5 };
 template <int KIND> struct DepC {
-    static constexpr int C =
-        KIND == DEP_T00 ? 1 : (KIND == DEP_T0I || KIND == DEP_VEC3 ? 3 : (KIND == DEP_TIJ ? 6 : 7));
+    static constexpr int C = KIND == DEP_T00 ? 1
+                             : (KIND == DEP_T0I || KIND == DEP_VEC3 ? 3
+                                                                    : (KIND == DEP_TIJ ? 6 : (KIND == DEP_T00TIJ ? 7 : 10)));
 };
 
 // FP64 LDS accumulators in both precisions: ds_add_f64 is the fast LDS atomic
@@ -42,8 +48,9 @@ __device__ __forceinline__ void lds_add(double *a, double v)
 #endif
 
 template <typename T, int KIND, int TX, int TY, int TZ>
-__global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int64_t M, const T *__restrict__ phi,
-                                                   double mass_d, double mass2_d, double a_d, int accumulate,
+__global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *out3, int64_t M,
+                                                   const T *__restrict__ phi, double mass_d, double mass2_d,
+                                                   double mass3_d, double a_d, int accumulate,
                                                    const T *__restrict__ aux)
 {
     constexpr int C = DepC<KIND>::C;
@@ -104,8 +111,9 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
     const T a = (T)a_d, o = (T)1, ia2 = (T)(1.0 / (a_d * a_d));
 
     // U particles per thread and pass: their loads are issued together
-    constexpr int U = (KIND == DEP_TIJ || KIND == DEP_T00TIJ) ? 1 : 2;
-    constexpr int TB = KIND == DEP_T00TIJ ? 1 : 0;  // first Tij accumulator
+    constexpr bool T00TIJ = KIND == DEP_T00TIJ || KIND == DEP_T00TIJL;
+    constexpr int U = (KIND == DEP_TIJ || T00TIJ) ? 1 : 2;
+    constexpr int TB = T00TIJ ? 1 : 0;  // first Tij accumulator
     const bool need_q = KIND != DEP_T00 || use_phi;
     // A dense tile (on average DEP_SPREAD_X4 / 4 or more particles per cell of
     // its halo box) hands consecutive particles to different waves, so the members
@@ -182,7 +190,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
                 }
             }
         }
-        if constexpr (KIND == DEP_T00 || KIND == DEP_T00TIJ) {
+        if constexpr (KIND == DEP_T00 || T00TIJ) {
             // projection_T00 functor, gevolution.hpp:1729-1854
             if (use_phi) {
                 const T q0 = qu[u][0], q1 = qu[u][1], q2 = qu[u][2];
@@ -209,15 +217,17 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
                 }
             }
         }
-        if constexpr (KIND == DEP_T0I) {
+        if constexpr (KIND == DEP_T0I || KIND == DEP_T00TIJL) {
             // projection_T0i functor, gevolution.hpp:2031-2201: 12 links per cell,
-            // each weighted by (1 + phi_a + phi_b) of its two end points
+            // each weighted by (1 + phi_a + phi_b) of its two end points (DEP_T0I;
+            // DEP_T00TIJL leaves the factor to the B solve)
+            constexpr int L0 = KIND == DEP_T0I ? 0 : 7;  // first T0i accumulator
             const T v0 = qu[u][0], v1 = qu[u][1], v2 = qu[u][2];
             T g[12] = {(o - rd[1]) * (o - rd[2]) * v0, (o - rd[0]) * (o - rd[2]) * v1, (o - rd[0]) * (o - rd[1]) * v2,
                        rd[0] * (o - rd[2]) * v1,       rd[0] * (o - rd[1]) * v2,       rd[1] * (o - rd[2]) * v0,
                        rd[1] * (o - rd[0]) * v2,       rd[2] * (o - rd[1]) * v0,       rd[2] * (o - rd[0]) * v1,
                        rd[1] * rd[2] * v0,             rd[0] * rd[2] * v1,             rd[0] * rd[1] * v2};
-            if (use_phi) {
+            if (KIND == DEP_T0I && use_phi) {
                 T ph[8];
 #pragma unroll
                 for (int b = 0; b < 8; b++) ph[b] = phs[pb + ((b >> 2) & 1) + PX * (((b >> 1) & 1) + PY * (b & 1))];
@@ -228,18 +238,18 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
             }
             // link k lands on component comp[k] of the site cell + offset[k]
             if (own(0, 0, 0)) {
-                lds_add(&acc[0][sidx(0, 0, 0)], g[0]);
-                lds_add(&acc[1][sidx(0, 0, 0)], g[1]);
-                lds_add(&acc[2][sidx(0, 0, 0)], g[2]);
+                lds_add(&acc[L0 + 0][sidx(0, 0, 0)], g[0]);
+                lds_add(&acc[L0 + 1][sidx(0, 0, 0)], g[1]);
+                lds_add(&acc[L0 + 2][sidx(0, 0, 0)], g[2]);
             }
-            if (own(1, 0, 0)) { lds_add(&acc[1][sidx(1, 0, 0)], g[3]); lds_add(&acc[2][sidx(1, 0, 0)], g[4]); }
-            if (own(0, 1, 0)) { lds_add(&acc[0][sidx(0, 1, 0)], g[5]); lds_add(&acc[2][sidx(0, 1, 0)], g[6]); }
-            if (own(0, 0, 1)) { lds_add(&acc[0][sidx(0, 0, 1)], g[7]); lds_add(&acc[1][sidx(0, 0, 1)], g[8]); }
-            if (own(0, 1, 1)) lds_add(&acc[0][sidx(0, 1, 1)], g[9]);
-            if (own(1, 0, 1)) lds_add(&acc[1][sidx(1, 0, 1)], g[10]);
-            if (own(1, 1, 0)) lds_add(&acc[2][sidx(1, 1, 0)], g[11]);
+            if (own(1, 0, 0)) { lds_add(&acc[L0 + 1][sidx(1, 0, 0)], g[3]); lds_add(&acc[L0 + 2][sidx(1, 0, 0)], g[4]); }
+            if (own(0, 1, 0)) { lds_add(&acc[L0 + 0][sidx(0, 1, 0)], g[5]); lds_add(&acc[L0 + 2][sidx(0, 1, 0)], g[6]); }
+            if (own(0, 0, 1)) { lds_add(&acc[L0 + 0][sidx(0, 0, 1)], g[7]); lds_add(&acc[L0 + 1][sidx(0, 0, 1)], g[8]); }
+            if (own(0, 1, 1)) lds_add(&acc[L0 + 0][sidx(0, 1, 1)], g[9]);
+            if (own(1, 0, 1)) lds_add(&acc[L0 + 1][sidx(1, 0, 1)], g[10]);
+            if (own(1, 1, 0)) lds_add(&acc[L0 + 2][sidx(1, 1, 0)], g[11]);
         }
-        if constexpr (KIND == DEP_TIJ || KIND == DEP_T00TIJ) {
+        if constexpr (KIND == DEP_TIJ || T00TIJ) {
             // projection_Tij functor, gevolution.hpp:2582-2965 (no CIC_PROJECT_TIJ):
             // diagonal parts CIC onto the 8 corners, off-diagonal parts onto 2 sites
             const T vv[3] = {qu[u][0], qu[u][1], qu[u][2]};
@@ -287,8 +297,11 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
         const uint32_t gs = (uint32_t)(x0 + sx) + (uint32_t)N * ((uint32_t)(y0 + sy) + (uint32_t)N * (uint32_t)(z0 + sz));
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            T *oc = (KIND == DEP_T00TIJ && c > 0) ? out2 + (c - 1) * M : out + c * M;
-            T val = (T)(acc[c][s] * ((KIND == DEP_T00TIJ && c > 0) ? mass2_d : mass_d));
+            T *oc = out + c * M;
+            double ms = mass_d;
+            if (T00TIJ && c > 0 && c < 7) { oc = out2 + (c - 1) * M; ms = mass2_d; }
+            if (KIND == DEP_T00TIJL && c >= 7) { oc = out3 + (c - 7) * M; ms = mass3_d; }
+            T val = (T)(acc[c][s] * ms);
             if (accumulate) val += oc[gs];
             oc[gs] = val;
         }
@@ -297,29 +310,30 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, int
 
 template <typename T, int KIND, int TX, int TY, int TZ>
 static int launch_dep(gev_pcls *p, T *out, T *out2, const T *phi, double mass, double mass2, double a, int acc,
-                      const T *aux)
+                      const T *aux, T *out3, double mass3)
 {
     gev_ctx *c = p->ctx;
     const int N = c->N;
     const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
     hipLaunchKernelGGL((k_deposit<T, KIND, TX, TY, TZ>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), out, out2,
-                       c->M, phi, mass, mass2, a, acc, aux);
+                       out3, c->M, phi, mass, mass2, mass3, a, acc, aux);
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }
 
 template <typename T, int KIND>
 static int dispatch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a, int acc, T *out2 = nullptr,
-                        double mass2 = 0.0, const T *aux = nullptr)
+                        double mass2 = 0.0, const T *aux = nullptr, T *out3 = nullptr, double mass3 = 0.0)
 {
     const int N = p->ctx->N;
     // Tij keeps 6 (T00+Tij 7) FP64 accumulators per site: the smaller tile holds 4 blocks per CU
-    constexpr bool big = KIND != DEP_TIJ && KIND != DEP_T00TIJ;
-    if (KIND == DEP_T0I && N >= 32) return launch_dep<T, KIND, 32, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux);
-    if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux);
-    if (KIND == DEP_T00TIJ && N >= 16) return launch_dep<T, KIND, 16, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux);
-    if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux);
-    return launch_dep<T, KIND, 4, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux);
+    constexpr bool big = KIND != DEP_TIJ && KIND != DEP_T00TIJ && KIND != DEP_T00TIJL;
+    if (KIND == DEP_T0I && N >= 32) return launch_dep<T, KIND, 32, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
+    if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
+    if ((KIND == DEP_T00TIJ || KIND == DEP_T00TIJL) && N >= 16)
+        return launch_dep<T, KIND, 16, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
+    if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
+    return launch_dep<T, KIND, 4, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
 }
 
 // u = CIC deposit of the per-ID 3-vector w ([3][n]), unit mass
@@ -349,8 +363,18 @@ int dep_T00_Tij_impl(gev_pcls *p, T *T00, T *Tij, const T *phi, double mass00, d
     return dispatch_dep<T, DEP_T00TIJ>(p, T00, phi, mass00, a, acc, Tij, massij);
 }
 
+// T00 and Tij as dep_T00_Tij_impl, and in links[3][M] the T0i link sums
+// without their 1 + phi_a + phi_b factor (mass0i per particle)
+template <typename T>
+int dep_T00_Tij_links_impl(gev_pcls *p, T *T00, T *Tij, T *links, const T *phi, double mass00, double massij,
+                           double mass0i, double a)
+{
+    return dispatch_dep<T, DEP_T00TIJL>(p, T00, phi, mass00, a, 0, Tij, massij, nullptr, links, mass0i);
+}
+
 #define INST(T)                                                                   \
     template int dep_vec3_impl<T>(gev_pcls *, const T *, T *);                       \
+    template int dep_T00_Tij_links_impl<T>(gev_pcls *, T *, T *, T *, const T *, double, double, double, double); \
     template int dep_T00_Tij_impl<T>(gev_pcls *, T *, T *, const T *, double, double, double, int); \
     template int dep_T00_impl<T>(gev_pcls *, T *, const T *, double, double, int); \
     template int dep_T0i_impl<T>(gev_pcls *, T *, const T *, double, int);         \

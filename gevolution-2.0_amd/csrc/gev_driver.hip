@@ -116,8 +116,13 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
 
     if (st->gr) {
         // T00 (main.cu:622-651) and Tij (main.cu:750-758), both with the old phi,
-        // in one pass over the particles (the outputs need no zeroing)
-        RC(gev_project_T00_Tij(p, f->source, f->Sij, a, f->phi, 1.0, 0));
+        // in one pass over the particles (the outputs need no zeroing).  With
+        // the elliptic B on 128 <= N <= 1024 the same pass sums T0i
+        // (main.cu:858) on the links into Bi, whose old B no step before the B
+        // solve reads; the new phi's link factor is applied in the B solve.
+        const bool t0i_links = st->vector_elliptic && gev::chi_from_Tij_supported(c);
+        if (t0i_links) RC(gev_project_T00_Tij_T0i_links(p, f->source, f->Sij, f->Bi, a, f->phi, 1.0));
+        else RC(gev_project_T00_Tij(p, f->source, f->Sij, a, f->phi, 1.0, 0));
         RC(mark(c, GEV_PHASE_DEPOSIT_T00_TIJ));
         const double H = gev_hconf(a, fpg, om, ol, orad);
         double s_res = 0;
@@ -136,19 +141,25 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
         }
         // chi (main.cu:822-902) and B (main.cu:908-955)
         if (st->vector_elliptic) {
-            // Sij is formed inside the chi solve's x passes.  The T0i deposit
-            // and the B solve (Bi, BiFT) share no buffer with the chi solve
-            // (chi, SijFT, scalarFT; both read phi): they run on the side
-            // stream while the chi solve runs on the main one, which waits for
-            // them before the kick.  Phases: deposit_T0i is the fork (~0),
-            // solve_chi the chi solve beside them, solve_B the wait after it.
+            // Sij is formed inside the chi solve's x passes.  The B solve (and
+            // the T0i deposit where it is not summed on the links above) share
+            // no buffer with the chi solve (Bi, BiFT against chi, SijFT,
+            // scalarFT; both read phi): they run on the side stream while the
+            // chi solve runs on the main one, which waits for them before the
+            // kick.  Phases: deposit_T0i is the fork (~0), solve_chi the chi
+            // solve beside them, solve_B the wait after it.
             RC(mark(c, GEV_PHASE_SOURCE_TIJ));
             GEV_HIP(hipEventRecord(c->fork, c->stream));
             GEV_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
             hipStream_t main_stream = c->stream;
             c->stream = c->side;
-            int rc_side = gev_project_T0i(p, f->Bi, f->phi, 1.0, 0);
-            if (!rc_side) rc_side = gev_solve_vector(c, f->Bi, f->BiFT, fpg * dx * dx, 0.0);
+            int rc_side;
+            if (t0i_links) {
+                rc_side = gev_solve_vector_T0i_links(c, f->Bi, f->phi, f->BiFT, fpg * dx * dx, 0.0);
+            } else {
+                rc_side = gev_project_T0i(p, f->Bi, f->phi, 1.0, 0);
+                if (!rc_side) rc_side = gev_solve_vector(c, f->Bi, f->BiFT, fpg * dx * dx, 0.0);
+            }
             c->stream = main_stream;
             GEV_HIP(hipEventRecord(c->join, c->side));
             RC(rc_side);

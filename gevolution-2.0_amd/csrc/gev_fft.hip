@@ -244,6 +244,29 @@ template <typename T, int N, int C> struct SrcSij {
     }
 };
 
+// component C of T0i from its link sums s (the particle parts of
+// projection_T0i, gevolution.hpp:2031-2201) and the link factor
+// 1 + phi(x) + phi(x + e_C) of the link from site x along axis C
+template <typename T, int N, int C> struct SrcT0iLink {
+    const T *phi, *s;  // s: component C of the link sums
+    __device__ __forceinline__ cx<T> operator()(int64_t L, int n) const
+    {
+        constexpr int lg = __builtin_ctz(N), M1 = N - 1;
+        const int y = (int)(L & M1), z = (int)(L >> lg);
+        const cx<T> sv = reinterpret_cast<const cx<T> *>(s + L * N)[n];
+        const T *r0 = phi + L * N;
+        const cx<T> p = reinterpret_cast<const cx<T> *>(r0)[n];  // phi at x, x+1
+        cx<T> e;  // phi at the far end of the links from x and x+1
+        if constexpr (C == 0) {
+            e = mk(p.y, r0[(2 * n + 2) & M1]);
+        } else {
+            const int64_t row = C == 1 ? (int64_t)((y + 1) & M1) + (int64_t)z * N : (int64_t)y + (int64_t)((z + 1) & M1) * N;
+            e = reinterpret_cast<const cx<T> *>(phi + row * N)[n];
+        }
+        return mk(sv.x * ((T)1 + p.x + e.x), sv.y * ((T)1 + p.y + e.y));
+    }
+};
+
 // x pass r2c: NL consecutive lines of N reals -> N/2+1 complex (pitch P)
 template <typename T, int N, int NL, typename Src>
 __global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(Src src, cx<T> *__restrict__ out, int P, int64_t nlines,
@@ -732,11 +755,66 @@ template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT
     return zsolve_dispatch<T>(c, ZOP_VECTOR, Bi, 3, BiFT, BiFT, Bi, 3, zp);
 }
 
+// the same from T0i link sums: the x passes apply the link factor with phi
+template <typename T, int N>
+static void vector_from_links_chain(gev_ctx *c, T *Bi, const T *phi, cx<T> *BiFT, const ZParams &zp)
+{
+    const int64_t nl = (int64_t)N * N, M = c->M;
+    const cx<T> *tw = twiddles_of<T>(c);
+    auto xy = [&](const auto &src, int i) {
+        launch_r2c_src<T, N>(src, BiFT + i * c->NK, c->P, nl, tw, c->stream);
+        launch_col<T, N>(BiFT + i * c->NK, c->P, 1, -1, tw, c->stream);
+    };
+    xy(SrcT0iLink<T, N, 0>{phi, Bi}, 0);
+    xy(SrcT0iLink<T, N, 1>{phi, Bi + M}, 1);
+    xy(SrcT0iLink<T, N, 2>{phi, Bi + 2 * M}, 2);
+    launch_zsolve<T, N, ZOP_VECTOR>(BiFT, BiFT, c->P, c->NK, tw, zp, c->stream);
+    for (int j = 0; j < 3; j++) inv_yx<T, N>(c, BiFT + j * c->NK, Bi + j * M);
+}
+
+// the link factor alone, in place (meshes below the fused chain's range)
+template <typename T> __global__ void k_t0i_link_factor(T *Bi, const T *__restrict__ phi, int N, int64_t M)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int lg = __builtin_ctz(N), M1 = N - 1;
+    const int x = (int)(i & M1), y = (int)((i >> lg) & M1), z = (int)(i >> (2 * lg));
+    const T p = phi[i];
+    Bi[i] *= (T)1 + p + phi[site(N, x + 1, y, z)];
+    Bi[M + i] *= (T)1 + p + phi[site(N, x, y + 1, z)];
+    Bi[2 * M + i] *= (T)1 + p + phi[site(N, x, y, z + 1)];
+}
+
+template <typename T> int t0i_link_factor_impl(gev_ctx *c, T *Bi, const T *phi)
+{
+    hipLaunchKernelGGL((k_t0i_link_factor<T>), dim3((unsigned)((c->M + 255) / 256)), dim3(256), 0, c->stream, Bi, phi,
+                       c->N, c->M);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+template <typename T>
+int solve_vector_links_fused_impl(gev_ctx *c, T *Bi, const T *phi, cx<T> *BiFT, double coeff, double modif)
+{
+    ZParams zp{c->gk2, c->kshift, coeff, modif, 0.0, 0.0, 0};
+    switch (c->N) {
+    case 128: vector_from_links_chain<T, 128>(c, Bi, phi, BiFT, zp); break;
+    case 256: vector_from_links_chain<T, 256>(c, Bi, phi, BiFT, zp); break;
+    case 512: vector_from_links_chain<T, 512>(c, Bi, phi, BiFT, zp); break;
+    case 1024: vector_from_links_chain<T, 1024>(c, Bi, phi, BiFT, zp); break;
+    default: gev_set_error("B from T0i links needs 128 <= N <= 1024"); return GEV_ERR_ARG;
+    }
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
 #define INSTZ(T)                                                                                       \
     template int solve_chi_Tij_fused_impl<T>(gev_ctx *, const T *, const T *, double, T *, cx<T> *, cx<T> *); \
     template int solve_poisson_fused_impl<T>(gev_ctx *, const T *, T *, cx<T> *, double, double, int, double); \
     template int solve_chi_fused_impl<T>(gev_ctx *, const T *, T *, cx<T> *, cx<T> *);                  \
-    template int solve_vector_fused_impl<T>(gev_ctx *, T *, cx<T> *, double, double);
+    template int solve_vector_fused_impl<T>(gev_ctx *, T *, cx<T> *, double, double);                    \
+    template int solve_vector_links_fused_impl<T>(gev_ctx *, T *, const T *, cx<T> *, double, double);    \
+    template int t0i_link_factor_impl<T>(gev_ctx *, T *, const T *);
 INSTZ(double)
 INSTZ(float)
 #undef INSTZ

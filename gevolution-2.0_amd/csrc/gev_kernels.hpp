@@ -31,6 +31,12 @@ template <typename T> int solve_chi_fused_impl(gev_ctx *c, const T *Sij, T *chi,
 template <typename T>
 int solve_chi_Tij_fused_impl(gev_ctx *c, const T *phi, const T *Tij, double coeff, T *chi, cx<T> *SijFT, cx<T> *chiFT);
 template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT, double coeff, double modif);
+// Bi holds T0i link sums (no 1 + phi_a + phi_b factor) on entry, B on return
+// (128 <= N <= 1024, as chi_from_Tij_supported)
+template <typename T>
+int solve_vector_links_fused_impl(gev_ctx *c, T *Bi, const T *phi, cx<T> *BiFT, double coeff, double modif);
+// Bi[c] *= 1 + phi(x) + phi(x + e_c), in place
+template <typename T> int t0i_link_factor_impl(gev_ctx *c, T *Bi, const T *phi);
 
 template <typename T>
 int prepare_T00_impl(gev_ctx *ctx, const T *phi, const T *chi, const T *src, double bg, T *res, double c1, double c2,

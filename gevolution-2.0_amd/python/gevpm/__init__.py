@@ -180,6 +180,14 @@ class Context:
         check(lib.gev_solve_vector(self._h, _ptr(Bi), _ptr(work), coeff, modif), "gev_solve_vector")
         return Bi
 
+    def solve_vector_T0i_links(self, Bi, phi, coeff=1.0, modif=0.0, work=None):
+        """Bi (T0i link sums) <- B: the link factor 1 + phi_a + phi_b, then solve_vector"""
+        work = self.fourier(3) if work is None else work
+        self.bind_stream()
+        check(lib.gev_solve_vector_T0i_links(self._h, _ptr(Bi), _ptr(phi), _ptr(work), coeff, modif),
+              "gev_solve_vector_T0i_links")
+        return Bi
+
     # ----------------------------------------------------------- sources
     def prepare_source_T00(self, phi, chi, source, bg, c1, c2, c3, result=None):
         result = torch.empty_like(source) if result is None else result
@@ -322,6 +330,14 @@ class Particles:
         check(lib.gev_project_T00_Tij(self._h, _ptr(T00), _ptr(Tij), a, _ptr(phi), coeff, int(accumulate)),
               "gev_project_T00_Tij")
         return T00, Tij
+
+    def project_T00_Tij_T0i_links(self, T00, Tij, links, a=1.0, phi=None, coeff=1.0):
+        """project_T00 and project_Tij, and the T0i link sums without their
+        1 + phi_a + phi_b factor (see Context.solve_vector_T0i_links)"""
+        self.ctx.bind_stream()
+        check(lib.gev_project_T00_Tij_T0i_links(self._h, _ptr(T00), _ptr(Tij), _ptr(links), a, _ptr(phi), coeff),
+              "gev_project_T00_Tij_T0i_links")
+        return T00, Tij, links
 
     # ------------------------------------------------------------ updates
     def update_q(self, dtau, phi, chi, Bi, nfield, a, bcoeff):

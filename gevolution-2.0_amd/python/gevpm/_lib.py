@@ -75,6 +75,7 @@ SIGNATURES = {
     "gev_project_T0i": (_i, [_vp, _vp, _vp, _d, _i]),
     "gev_project_Tij": (_i, [_vp, _vp, _d, _vp, _d, _i]),
     "gev_project_T00_Tij": (_i, [_vp, _vp, _vp, _d, _vp, _d, _i]),
+    "gev_project_T00_Tij_T0i_links": (_i, [_vp, _vp, _vp, _vp, _d, _vp, _d]),
     "gev_prepare_source_T00": (_i, [_vp, _vp, _vp, _vp, _d, _vp, _d, _d, _d, _dp, _dp]),
     "gev_prepare_source_Tij": (_i, [_vp, _vp, _vp, _vp, _d]),
     "gev_fft_r2c": (_i, [_vp, _vp, _vp, _i]),
@@ -88,6 +89,7 @@ SIGNATURES = {
     "gev_solve_chi": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "gev_solve_chi_Tij": (_i, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gev_solve_vector": (_i, [_vp, _vp, _vp, _d, _d]),
+    "gev_solve_vector_T0i_links": (_i, [_vp, _vp, _vp, _vp, _d, _d]),
     "gev_update_q": (_i, [_vp, _d, _vp, _vp, _vp, _i, _d, _d, _dp]),
     "gev_update_q_newton": (_i, [_vp, _d, _vp, _vp, _i, _d, _dp]),
     "gev_update_pos": (_i, [_vp, _d, _vp, _vp, _vp, _i, _d, _d]),

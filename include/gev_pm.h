@@ -97,6 +97,16 @@ int gev_project_Tij(gev_pcls *p, void *Tij, double a, const void *phi, double co
  * with the same phi and a (main.cu:623 and 756) in one pass over the
  * particles; no single reference function.  Results as the two calls. */
 int gev_project_T00_Tij(gev_pcls *p, void *T00, void *Tij, double a, const void *phi, double coeff, int accumulate);
+/* No single reference function: gev_project_T00_Tij (with phi) and, in
+ * links[3][N^3], projection_T0i_project's link sums without their
+ * 1 + phi_a + phi_b factor (gevolution.hpp:2031-2201 multiplies each of a
+ * particle's 12 link values by the phi of the link's two end points, which is
+ * the same for every particle on the link).  gev_solve_vector_T0i_links
+ * applies the factor with the phi of its own call, so T0i can be deposited in
+ * the same pass over the particles as T00 and Tij although main.cu projects
+ * it after the phi solve (main.cu:858).  Overwrites its outputs. */
+int gev_project_T00_Tij_T0i_links(gev_pcls *p, void *T00, void *Tij, void *links, double a, const void *phi,
+                                  double coeff);
 
 /* ---- real-space sources */
 /* prepareFTsource (T00 form, PHINONLINEAR), gevolution.hpp:160-218.
@@ -137,6 +147,10 @@ int gev_solve_chi(gev_ctx *ctx, const void *Sij, void *chi, void *SijFT, void *c
 int gev_solve_chi_Tij(gev_ctx *ctx, const void *phi, void *Tij, double coeff, void *chi, void *SijFT, void *chiFT);
 /* plan_Bi FFT_FORWARD, projectFTvector, plan_Bi FFT_BACKWARD (main.cu:918-948), Bi in place */
 int gev_solve_vector(gev_ctx *ctx, void *Bi, void *BiFT, double coeff, double modif);
+/* gev_solve_vector of T0i given as link sums (gev_project_T00_Tij_T0i_links):
+ * Bi[c](x) *= 1 + phi(x) + phi(x + e_c), then the solve; for
+ * 128 <= N <= 1024 the factor is applied inside the forward x passes. */
+int gev_solve_vector_T0i_links(gev_ctx *ctx, void *Bi, const void *phi, void *BiFT, double coeff, double modif);
 
 /* ---- particle updates (perfParticles::updateVel / moveParticles with the
  *      gevolution functors, main.cu:1227-1266).  *max_ret receives the

@@ -525,3 +525,65 @@ def test_fused_solves_vs_unfused(gp, N, prec):
     ref = ctx.fft_c2r(ctx.project_ft_vector(ctx.fft_r2c(Bsrc), 0.25, 0.0), preserve=False)
     got = ctx.solve_vector(Bsrc.clone(), 0.25, 0.0)
     assert relmax(to_np(got), to_np(ref)) < tol
+
+
+# ------------------------------- T0i deposited as link sums (one particle pass)
+@pytest.mark.parametrize("N,prec,cluster", [(32, "fp64", False), (128, "fp64", True), (128, "mixed", False),
+                                            (128, "mixed", True)])
+def test_t0i_links_match_separate_projection(gp, N, prec, cluster):
+    """gev_project_T00_Tij_T0i_links gives gev_project_T00_Tij's T00 and Tij,
+    and link sums that, times 1 + phi_a + phi_b of a second phi, are
+    gev_project_T0i with that phi; gev_solve_vector_T0i_links equals
+    gev_solve_vector of that T0i (x passes with the factor for N >= 128,
+    the separate factor kernel below)"""
+    ctx = gp.Context(N, gp.GEV_FP64 if prec == "fp64" else gp.GEV_MIXED)
+    n = 3 * N ** 3 // 2
+    pos, q, pid = make_particles(N, n, 1700 + N, vmax=0.2, cluster=cluster)
+    p = gpu_pcls(gp, ctx, pos, q, pid, 0.31 / n)
+    g = torch.Generator(device="cpu").manual_seed(N + 3)
+    phi_old = (1e-4 * torch.randn(1, N, N, N, generator=g, dtype=torch.float64)).to(ctx.rdtype).to(ctx.device)
+    phi_new = (1e-4 * torch.randn(1, N, N, N, generator=g, dtype=torch.float64)).to(ctx.rdtype).to(ctx.device)
+    a = 0.3
+    T00a, Tija = p.project_T00_Tij(ctx.real(1), ctx.real(6), a, phi_old)
+    T00b, Tijb, Lk = p.project_T00_Tij_T0i_links(ctx.real(1), ctx.real(6), ctx.real(3), a, phi_old)
+    tol = 1e-12 if prec == "fp64" else 1e-6
+    assert relmax(to_np(T00b), to_np(T00a)) < tol
+    assert relmax(to_np(Tijb), to_np(Tija)) < tol
+    # the link factor applied on the host, against the projection with phi_new
+    T0i = to_np(p.project_T0i(ctx.real(3), phi_new)).astype(np.float64)
+    ph = to_np(phi_new)[0].astype(np.float64)  # [z, y, x]
+    L = to_np(Lk).astype(np.float64)
+    fac = [1 + ph + np.roll(ph, -1, axis=2), 1 + ph + np.roll(ph, -1, axis=1), 1 + ph + np.roll(ph, -1, axis=0)]
+    assert relmax(np.stack([L[c] * fac[c] for c in range(3)]), T0i) < (1e-12 if prec == "fp64" else 2e-6)
+    # the B solve from the link sums
+    Bref = ctx.solve_vector(p.project_T0i(ctx.real(3), phi_new), 0.25, 0.0)
+    Bgot = ctx.solve_vector_T0i_links(Lk.clone(), phi_new, 0.25, 0.0)
+    assert relmax(to_np(Bgot), to_np(Bref)) < (1e-10 if prec == "fp64" else 2e-5)
+
+
+def test_gr_steps_128_vs_oracle(gp, O):
+    """GR cycles at 128^3, where gev_step deposits T0i on the links together
+    with T00 and Tij and applies the link factor inside the B solve, against
+    the oracle's main.cu-ordered cycle"""
+    N, L, steps = 128, 640.0, 3
+    ctx, p, f, st, md, _ = gpu_ic(gp, N, L, 1)
+    pos0, q0, pid = by_id(p)
+    M = N ** 3
+    ost = dict(pos=pos0.copy(), q=q0.copy(), phi=real_np(f.phi)[0].copy(), chi=real_np(f.chi)[0].copy(),
+               Bi=real_np(f.Bi).copy(), source=np.zeros(M), Sij=np.zeros((6, M)), a=st.a, tau=st.tau,
+               dtau_old=0.0, maxvel=st.maxvel)
+    fpg = gp.fourpiG_of(L)
+    t1 = gp.particle_horizon(1 / 90.0, fpg, 0.31, 0.69)
+    dtau = (t1 - st.tau) / steps
+    for _ in range(steps):
+        st.dtau = dtau
+        gp.step(p, f, st)
+        ost["dtau"] = dtau
+        O.step(N, ost, 1, 0.31 / M, fpg, 0.31, 0.69, 100.0)
+    gpos, gq, _ = by_id(p)
+    dpos = np.abs(((gpos - ost["pos"]) + 0.5) % 1.0 - 0.5).max()
+    assert dpos < 1e-11 / N, dpos
+    assert relmax(gq, ost["q"]) < 1e-9
+    assert relmax(real_np(f.phi)[0], ost["phi"]) < 1e-9
+    assert relmax(real_np(f.chi)[0], ost["chi"]) < 1e-7
+    assert relmax(real_np(f.Bi), ost["Bi"]) < 1e-7

@@ -190,15 +190,17 @@ def main():
 # Algorithmic HBM bytes per launch of the kernels that can dominate the mixed
 # cycle (R = 4-byte reals, n particles, M cells): each input read once, each
 # output written once, no halo or re-read counted.
-def _bytes_models(M, n, R=4):
+def _bytes_models(M, n, R=4, links=True):
+    # links: the GR step sums T0i on the links in the T00+Tij pass (128 <= N <= 1024)
+    nc = 10 if links else 7
     return {
         "kick_drift": (n * (7 * R + 7 * R + 8) + M * 5 * R,
                        "k_update_tiled (fused GR kick + drift): per particle 3 offsets + 3 momenta + cell read, "
                        "3 offsets + 3 momenta + cell + rank written, cell counter read + written; per cell phi, "
                        "chi, B0..B2 read"),
-        "deposit_T00_Tij": (n * 7 * R + M * (4 + R + 7 * R),
-                            "k_deposit T00+Tij: per particle 3 offsets + 3 momenta + cell read; per cell start + "
-                            "phi read, 7 components written"),
+        "deposit_T00_Tij": (n * 7 * R + M * (4 + R + nc * R),
+                            f"k_deposit T00+Tij{'+T0i links' if links else ''}: per particle 3 offsets + 3 momenta "
+                            f"+ cell read; per cell start + phi read, {nc} components written"),
         "deposit_T0i": (n * 7 * R + M * (4 + R + 3 * R),
                         "k_deposit T0i: per particle 3 offsets + 3 momenta + cell read; per cell start + phi read, "
                         "3 components written"),
@@ -226,7 +228,7 @@ def kernel_roofline(ctx, n, phases):
     launch duration is the library's own HIP-event timing of that phase on the
     library stream, averaged over the timed cycles (the phase brackets exactly
     that one kernel)."""
-    models = _bytes_models(ctx.M, n)
+    models = _bytes_models(ctx.M, n, links=128 <= ctx.N <= 1024)
     name = max(models, key=lambda kk: phases.get(kk, 0.0))
     bytes_per_launch, desc = models[name]
     ms = phases[name]

@@ -43,6 +43,9 @@ __device__ __forceinline__ void lds_add(double *a, double v)
     __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+#ifndef DEP_CHUNK_X4
+#define DEP_CHUNK_X4 5  // FP32 single-particle kinds: as DEP_SPREAD_X4, per-lane chunks
+#endif
 #ifndef DEP_SPREAD_X4
 #define DEP_SPREAD_X4 5  // four times the mean particles per halo-box cell above which a tile spreads
 #endif
@@ -123,12 +126,20 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     constexpr int NCELL = (TX + 1) * (TY + 1) * (TZ + 1), NW = PT_NT / 64;
     const bool spread = DEP_SPREAD_X4 > 0 && 4 * total >= DEP_SPREAD_X4 * NCELL;
     const int slot = spread ? lane * NW + wv : t;
+    // In the FP32 form a crowded tile of a one-particle-per-pass kind instead
+    // gives every lane its own contiguous chunk of Q particles, so the lanes of
+    // one instruction sit Q particles apart, in different cells (late-time
+    // T00+Tij+T0i links 10.7 -> 9.7 ms at 512^3; FP64 and two-particle kinds
+    // measured slower with it)
+    const bool chunk = sizeof(T) == 4 && U == 1 && DEP_CHUNK_X4 > 0 && 4 * total >= DEP_CHUNK_X4 * NCELL;
+    const int32_t Q = (total + 63) / 64;
     for (int32_t i0 = 0; i0 < total; i0 += U * PT_NT) {
         int run[U];
         int64_t pu[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int32_t i = i0 + u * PT_NT + slot;
+            const int32_t sq = (i0 / PT_NT + u) * NW + wv;  // position inside the lane's chunk
+            const int32_t i = chunk ? (sq < Q ? lane * Q + sq : total) : i0 + u * PT_NT + slot;
             run[u] = -1;
             pu[u] = 0;
             if (i < total) {

@@ -111,7 +111,10 @@ int gev_ctx_create(int N, int precision, void *hip_stream, gev_ctx **out)
     GEV_HIP(hipMemcpy(c->tw32, twf.data(), sizeof(cx<float>) * N, hipMemcpyHostToDevice));
     GEV_HIP(hipMemcpy(c->gk2, gk2.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     GEV_HIP(hipMemcpy(c->kshift, ks.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice));
-    GEV_HIP(hipMalloc(&c->red, sizeof(double) * (64 + 2048 * 8)));
+    // results, then workgroup partials: the field sums' 2048 x 8 and the
+    // summing x pass's two per workgroup (N^2 / NL workgroups, NL >= 4096 / N)
+    const int64_t nparts = std::max<int64_t>(2048 * 8, 2 * ((int64_t)N * N * N / 4096 + 1));
+    GEV_HIP(hipMalloc(&c->red, sizeof(double) * (64 + nparts)));
     GEV_HIP(hipHostMalloc(&c->red_host, sizeof(double) * 64, hipHostMallocDefault));
     c->scan_blocks_n = (c->M + 2047) / 2048 + 1;
     GEV_HIP(hipMalloc(&c->scan_blocks, sizeof(int32_t) * c->scan_blocks_n));
@@ -572,6 +575,42 @@ int gev_solve_vector(gev_ctx *c, void *Bi, void *BiFT, double coeff, double modi
     if (!rc) rc = gev_project_ft_vector(c, BiFT, BiFT, coeff, modif);
     if (!rc) rc = gev_fft_c2r(c, BiFT, Bi, 3);
     return rc;
+}
+
+int gev_solve_poisson_T00(gev_ctx *c, const void *phi, const void *chi, void *source, double bg, double c1,
+                          double c2, double c3, void *pot, void *work, double coeff, double modif, int k0set,
+                          double *sums)
+{
+    CHECK(c && phi && chi && source && pot && work, "gev_solve_poisson_T00: NULL argument");
+    if (chi_from_Tij_supported(c)) {
+        int rc = PREC_DISPATCH(c,
+                               solve_poisson_T00_fused_impl<double>(c, (const double *)phi, (const double *)chi,
+                                                                    (const double *)source, bg, c1, c2, c3,
+                                                                    (double *)pot, (cx<double> *)work, coeff, modif,
+                                                                    k0set),
+                               solve_poisson_T00_fused_impl<float>(c, (const float *)phi, (const float *)chi,
+                                                                   (const float *)source, bg, c1, c2, c3,
+                                                                   (float *)pot, (cx<float> *)work, coeff, modif,
+                                                                   k0set));
+        if (rc || !sums) return rc;
+        return gev_ctx_sums(c, sums);
+    }
+    // small meshes: the source pass in place, then the solve
+    double s2[2] = {0.0, 0.0};
+    int rc = gev_prepare_source_T00(c, phi, chi, source, bg, source, c1, c2, c3, &s2[0], &s2[1]);
+    if (!rc) rc = gev_solve_poisson(c, source, pot, work, coeff, modif, k0set, s2[1]);
+    if (!rc && sums) { sums[0] = s2[0]; sums[1] = s2[1]; }
+    return rc;
+}
+
+int gev_ctx_sums(gev_ctx *c, double sums[2])
+{
+    CHECK(c && sums, "gev_ctx_sums: NULL argument");
+    GEV_HIP(hipMemcpyAsync(c->red_host, c->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GEV_HIP(hipStreamSynchronize(c->stream));
+    sums[0] = c->red_host[0];
+    sums[1] = c->red_host[1];
+    return GEV_OK;
 }
 
 int gev_solve_vector_T0i_links(gev_ctx *c, void *Bi, const void *phi, void *BiFT, double coeff, double modif)

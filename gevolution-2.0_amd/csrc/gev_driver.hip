@@ -125,16 +125,24 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
         else RC(gev_project_T00_Tij(p, f->source, f->Sij, a, f->phi, 1.0, 0));
         RC(mark(c, GEV_PHASE_DEPOSIT_T00_TIJ));
         const double H = gev_hconf(a, fpg, om, ol, orad);
-        double s_res = 0;
-        if (dtau_old > 0.) {
-            double s_src = 0;
+        // prepareFTsource (main.cu:725) and the phi solve (main.cu:742-794); the
+        // mixed form takes the k=0 mode from the FP64 sum.  For 128 <= N <= 1024
+        // the source is formed inside the solve's x pass and its sums stay on
+        // the device until the end of the cycle (no wait here).
+        bool t00_sums_pending = false;
+        if (dtau_old > 0. && gev::chi_from_Tij_supported(c)) {
+            RC(mark(c, GEV_PHASE_SOURCE_T00));
+            RC(gev_solve_poisson_T00(c, f->phi, f->chi, f->source, om, 3. * H * dx * dx / dtau_old, fpg * dx * dx / a,
+                                     3. * H * H * dx * dx, f->phi, f->scalarFT, 1. / (dx * dx), 3. * H / dtau_old,
+                                     mixed ? 1 : 0, nullptr));
+            t00_sums_pending = true;
+            RC(mark(c, GEV_PHASE_SOLVE_PHI));
+        } else if (dtau_old > 0.) {
+            double s_src = 0, s_res = 0;
             RC(gev_prepare_source_T00(c, f->phi, f->chi, f->source, om, f->source, 3. * H * dx * dx / dtau_old,
                                       fpg * dx * dx / a, 3. * H * H * dx * dx, &s_src, &s_res));
             st->T00hom = s_src / (double)M;
             RC(mark(c, GEV_PHASE_SOURCE_T00));
-        }
-        if (dtau_old > 0.) {
-            // main.cu:742-794; the mixed form takes the k=0 mode from the FP64 sum
             RC(gev_solve_poisson(c, f->source, f->phi, f->scalarFT, 1. / (dx * dx), 3. * H / dtau_old, mixed ? 1 : 0,
                                  s_res));
             RC(mark(c, GEV_PHASE_SOLVE_PHI));
@@ -191,6 +199,11 @@ int gev_step(gev_pcls *p, const gev_fields *f, gev_step_state *st)
         RC(gev_pcls_rebin_counted(p));
         RC(mark(c, GEV_PHASE_REBIN));
         RC(gev::read_max_slot(c, &maxv2));  // the scatter leaves the kick's maximum alone
+        if (t00_sums_pending) {
+            double sums[2];
+            RC(gev_ctx_sums(c, sums));  // nothing since the phi solve reduced into these slots
+            st->T00hom = sums[0] / (double)M;
+        }
         a = gev_rk4bg(a_d, fpg, om, ol, orad, 0.5 * dtau);
         double mv = sqrt(maxv2);
         st->maxvel = mv / sqrt(mv * mv + 1.0); // main.cu:1279-1283

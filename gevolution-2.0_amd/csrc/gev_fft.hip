@@ -190,6 +190,7 @@ __global__ __launch_bounds__(TX * L / 8) void k_col_reg(cx<T> *__restrict__ data
 
 // Sources of the r2c x pass: the pair of reals (2n, 2n+1) of line L (= y + N z)
 template <typename T, int N> struct SrcPlain {
+    static constexpr bool kSums = false;
     const T *in;
     __device__ __forceinline__ cx<T> operator()(int64_t L, int n) const
     {
@@ -200,6 +201,7 @@ template <typename T, int N> struct SrcPlain {
 // computed from Tij and phi while the pass reads it (the same expressions as
 // k_prepare_Tij); C: 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz
 template <typename T, int N, int C> struct SrcSij {
+    static constexpr bool kSums = false;
     const T *phi, *Tc;  // Tc: component C of Tij
     T coeff;
     __device__ __forceinline__ cx<T> operator()(int64_t L, int n) const
@@ -248,6 +250,7 @@ template <typename T, int N, int C> struct SrcSij {
 // projection_T0i, gevolution.hpp:2031-2201) and the link factor
 // 1 + phi(x) + phi(x + e_C) of the link from site x along axis C
 template <typename T, int N, int C> struct SrcT0iLink {
+    static constexpr bool kSums = false;
     const T *phi, *s;  // s: component C of the link sums
     __device__ __forceinline__ cx<T> operator()(int64_t L, int n) const
     {
@@ -267,20 +270,76 @@ template <typename T, int N, int C> struct SrcT0iLink {
     }
 };
 
+// prepareFTsource's T00 form (gevolution.hpp:160-171, the same expressions as
+// k_prepare_T00) evaluated while the Poisson solve's x pass reads it; every
+// copy sums the source and the result it produced (FP64) for the k=0 mode
+template <typename T, int N> struct SrcT00 {
+    static constexpr bool kSums = true;
+    const T *phi, *chi, *src;
+    T bg, c1, c2, c3;
+    double acc[2];
+    __device__ __forceinline__ cx<T> operator()(int64_t L, int n)
+    {
+        constexpr int lg = __builtin_ctz(N), M1 = N - 1;
+        const int y = (int)(L & M1), z = (int)(L >> lg);
+        auto pair = [&](const T *f, int64_t line) { return reinterpret_cast<const cx<T> *>(f + line * N)[n]; };
+        const int64_t ym = (int64_t)((y - 1) & M1) + (int64_t)z * N, yp = (int64_t)((y + 1) & M1) + (int64_t)z * N;
+        const int64_t zm = (int64_t)y + (int64_t)((z - 1) & M1) * N, zp = (int64_t)y + (int64_t)((z + 1) & M1) * N;
+        const T *r0 = phi + L * N;
+        const cx<T> p = pair(phi, L), s = pair(src, L), ch = pair(chi, L);
+        const cx<T> a = pair(phi, ym), b = pair(phi, yp), d = pair(phi, zm), e = pair(phi, zp);
+        const T pm = r0[(2 * n - 1) & M1], p2 = r0[(2 * n + 2) & M1];
+        auto res = [&](T sv, T pv, T xm, T xp, T ym_, T yp_, T zm_, T zp_, T cv) {
+            T r = c2 * (sv - bg);
+            r *= (T)1 - (T)2 * pv * ((T)1 - pv);
+            const T d0 = xm - xp, d1 = ym_ - yp_, d2 = zm_ - zp_;
+            r += (T)0.125 * d0 * d0;
+            r += (T)0.125 * d1 * d1;
+            r += (T)0.125 * d2 * d2;
+            r += (c3 * ((T)1 - (T)3 * pv) - c1) * pv - c3 * cv;
+            return r;
+        };
+        const T r0v = res(s.x, p.x, pm, p.y, a.x, b.x, d.x, e.x, ch.x);
+        const T r1v = res(s.y, p.y, p.x, p2, a.y, b.y, d.y, e.y, ch.y);
+        acc[0] += (double)s.x + (double)s.y;
+        acc[1] += (double)r0v + (double)r1v;
+        return mk(r0v, r1v);
+    }
+};
+
 // x pass r2c: NL consecutive lines of N reals -> N/2+1 complex (pitch P)
 template <typename T, int N, int NL, typename Src>
 __global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(Src src, cx<T> *__restrict__ out, int P, int64_t nlines,
-                                                           const cx<T> *__restrict__ tw)
+                                                           const cx<T> *__restrict__ tw, double *partials)
 {
     constexpr int H = N / 2, NT = NL * H / 8, LDL = H + 1;
     __shared__ cx<T> lds[NL * LDL];
     __shared__ cx<T> twl[N];
     for (int m = threadIdx.x; m < N; m += NT) twl[m] = tw[m];
     const int64_t line0 = (int64_t)blockIdx.x * NL;
-    auto ld = [&](int line, int n) { return (line0 + line < nlines) ? src(line0 + line, n) : mk((T)0, (T)0); };
+    Src sr = src;  // a source that sums keeps its sums per thread
+    if constexpr (Src::kSums) sr.acc[0] = sr.acc[1] = 0.0;
+    auto ld = [&](int line, int n) { return (line0 + line < nlines) ? sr(line0 + line, n) : mk((T)0, (T)0); };
     auto st = [&](int line, int k, cx<T> v, int) { lds[line * LDL + k] = v; };
     __syncthreads();
     reg_fft<T, H, NL, false, 2, true, 1>(lds, twl, -1, ld, st);
+    if constexpr (Src::kSums) {
+        // workgroup partials of the two sums, one slot pair per workgroup
+        __shared__ double ws[NT / 64][2];
+        double v0 = sr.acc[0], v1 = sr.acc[1];
+        for (int o = 32; o > 0; o >>= 1) {
+            v0 += __shfl_xor(v0, o, 64);
+            v1 += __shfl_xor(v1, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) { ws[threadIdx.x >> 6][0] = v0; ws[threadIdx.x >> 6][1] = v1; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t0 = 0.0, t1 = 0.0;
+            for (int w = 0; w < NT / 64; w++) { t0 += ws[w][0]; t1 += ws[w][1]; }
+            partials[2 * (int64_t)blockIdx.x] = t0;
+            partials[2 * (int64_t)blockIdx.x + 1] = t1;
+        }
+    }
     __syncthreads();
     cx<T> *dst = out + line0 * P;
     for (int e = threadIdx.x; e < NL * P; e += NT) {
@@ -341,12 +400,13 @@ __global__ __launch_bounds__(NL * N / 16) void k_c2r_x_reg(const cx<T> *__restri
 // ---------------------------------------------------------- dispatchers ---
 
 template <typename T, int N, typename Src>
-static void launch_r2c_src(const Src &src, cx<T> *out, int P, int64_t nlines, const cx<T> *tw, hipStream_t s)
+static void launch_r2c_src(const Src &src, cx<T> *out, int P, int64_t nlines, const cx<T> *tw, hipStream_t s,
+                           double *partials = nullptr)
 {
     constexpr int NL = xlines<T, N>();
     int64_t nb = (nlines + NL - 1) / NL;
     hipLaunchKernelGGL((k_r2c_x_reg<T, N, NL, Src>), dim3((unsigned)nb), dim3(NL * N / 16), 0, s, src, out, P, nlines,
-                       tw);
+                       tw, partials);
 }
 
 template <typename T, int N>
@@ -455,6 +515,7 @@ struct ZParams {
     double coeff, modif;  // Poisson: coeff pre-scaled by -1/N^3; vector: as projectFTvector
     double k0re, k0im;    // Poisson: value taken for the k=0 mode when k0set
     int k0set;
+    const double *k0dev;  // when set, the real part of that mode is read here instead
 };
 
 template <typename T, int L, int TX, int OP>
@@ -589,7 +650,7 @@ __global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *o
             cx<double> r = mk(0.0, 0.0);
             if (!pad) {
                 cx<double> sv = mk((double)vin[0][s].x, (double)vin[0][s].y);
-                if (zero_mode && zp.k0set) sv = mk(zp.k0re, zp.k0im);
+                if (zero_mode && zp.k0set) sv = mk(zp.k0dev ? *zp.k0dev : zp.k0re, zp.k0im);
                 if (!(zero_mode && zp.modif == 0.0)) r = kop_poisson(sv, kk, zp.coeff, zp.modif);
             }
             vin[0][s] = mk((T)r.x, (T)r.y);
@@ -698,14 +759,52 @@ template <typename T>
 int solve_poisson_fused_impl(gev_ctx *c, const T *src, T *pot, cx<T> *work, double coeff, double modif, int k0set,
                              double k0re)
 {
-    ZParams zp{c->gk2, c->kshift, coeff / -((double)c->N * (double)c->N * (double)c->N), modif, k0re, 0.0, k0set};
+    ZParams zp{c->gk2, c->kshift, coeff / -((double)c->N * (double)c->N * (double)c->N), modif, k0re, 0.0, k0set,
+               nullptr};
     return zsolve_dispatch<T>(c, ZOP_POISSON, src, 1, work, work, pot, 1, zp);
+}
+
+// pot = c2r(poisson(r2c(prepareFTsource_T00(phi, chi, src)))) with the source
+// formed inside the x pass; its workgroup sums go to ctx->red + 64 and are
+// reduced to ctx->red[0] (source) and ctx->red[1] (result) before the z pass,
+// which takes the k=0 mode from ctx->red[1] when zp.k0set
+template <typename T, int N>
+static void poisson_T00_chain(gev_ctx *c, const SrcT00<T, N> &src, T *pot, cx<T> *work, const ZParams &zp)
+{
+    constexpr int NL = xlines<T, N>();
+    const int64_t nl = (int64_t)N * N, nb = (nl + NL - 1) / NL;
+    const cx<T> *tw = twiddles_of<T>(c);
+    launch_r2c_src<T, N>(src, work, c->P, nl, tw, c->stream, c->red + 64);
+    launch_col<T, N>(work, c->P, 1, -1, tw, c->stream);
+    reduce_partials(c, c->red + 64, (int)nb, 2, c->red);
+    launch_zsolve<T, N, ZOP_POISSON>(work, work, c->P, c->NK, tw, zp, c->stream);
+    inv_yx<T, N>(c, work, pot);
+}
+
+template <typename T>
+int solve_poisson_T00_fused_impl(gev_ctx *c, const T *phi, const T *chi, const T *src, double bg, double c1,
+                                 double c2, double c3, T *pot, cx<T> *work, double coeff, double modif, int k0set)
+{
+    ZParams zp{c->gk2, c->kshift, coeff / -((double)c->N * (double)c->N * (double)c->N), modif, 0.0, 0.0, k0set,
+               k0set ? c->red + 1 : nullptr};
+    switch (c->N) {
+#define X(n)                                                                                                     \
+    case n:                                                                                                      \
+        poisson_T00_chain<T, n>(c, SrcT00<T, n>{phi, chi, src, (T)bg, (T)c1, (T)c2, (T)c3, {0.0, 0.0}}, pot, work, \
+                                zp);                                                                             \
+        break;
+        X(128) X(256) X(512) X(1024)
+#undef X
+    default: gev_set_error("prepareFTsource T00 inside the Poisson solve needs 128 <= N <= 1024"); return GEV_ERR_ARG;
+    }
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
 }
 
 // chi = c2r(projectFTscalar(r2c(Sij)))
 template <typename T> int solve_chi_fused_impl(gev_ctx *c, const T *Sij, T *chi, cx<T> *SijFT, cx<T> *chiFT)
 {
-    ZParams zp{c->gk2, c->kshift, 0.0, 0.0, 0.0, 0.0, 0};
+    ZParams zp{c->gk2, c->kshift, 0.0, 0.0, 0.0, 0.0, 0, nullptr};
     return zsolve_dispatch<T>(c, ZOP_SCALAR, Sij, 6, SijFT, chiFT, chi, 1, zp);
 }
 
@@ -727,7 +826,7 @@ static void chi_from_Tij_chain(gev_ctx *c, const T *phi, const T *Tij, T coeff, 
     xy(SrcSij<T, N, 3>{phi, Tij + 3 * M, coeff}, 3);
     xy(SrcSij<T, N, 4>{phi, Tij + 4 * M, coeff}, 4);
     xy(SrcSij<T, N, 5>{phi, Tij + 5 * M, coeff}, 5);
-    ZParams zp{c->gk2, c->kshift, 0.0, 0.0, 0.0, 0.0, 0};
+    ZParams zp{c->gk2, c->kshift, 0.0, 0.0, 0.0, 0.0, 0, nullptr};
     launch_zsolve<T, N, ZOP_SCALAR>(SijFT, chiFT, c->P, c->NK, tw, zp, c->stream);
     inv_yx<T, N>(c, chiFT, chi);
 }
@@ -751,7 +850,7 @@ int solve_chi_Tij_fused_impl(gev_ctx *c, const T *phi, const T *Tij, double coef
 // Bi = c2r(projectFTvector(r2c(Bi))), in place on Bi
 template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT, double coeff, double modif)
 {
-    ZParams zp{c->gk2, c->kshift, coeff, modif, 0.0, 0.0, 0};
+    ZParams zp{c->gk2, c->kshift, coeff, modif, 0.0, 0.0, 0, nullptr};
     return zsolve_dispatch<T>(c, ZOP_VECTOR, Bi, 3, BiFT, BiFT, Bi, 3, zp);
 }
 
@@ -796,7 +895,7 @@ template <typename T> int t0i_link_factor_impl(gev_ctx *c, T *Bi, const T *phi)
 template <typename T>
 int solve_vector_links_fused_impl(gev_ctx *c, T *Bi, const T *phi, cx<T> *BiFT, double coeff, double modif)
 {
-    ZParams zp{c->gk2, c->kshift, coeff, modif, 0.0, 0.0, 0};
+    ZParams zp{c->gk2, c->kshift, coeff, modif, 0.0, 0.0, 0, nullptr};
     switch (c->N) {
     case 128: vector_from_links_chain<T, 128>(c, Bi, phi, BiFT, zp); break;
     case 256: vector_from_links_chain<T, 256>(c, Bi, phi, BiFT, zp); break;
@@ -814,7 +913,9 @@ int solve_vector_links_fused_impl(gev_ctx *c, T *Bi, const T *phi, cx<T> *BiFT, 
     template int solve_chi_fused_impl<T>(gev_ctx *, const T *, T *, cx<T> *, cx<T> *);                  \
     template int solve_vector_fused_impl<T>(gev_ctx *, T *, cx<T> *, double, double);                    \
     template int solve_vector_links_fused_impl<T>(gev_ctx *, T *, const T *, cx<T> *, double, double);    \
-    template int t0i_link_factor_impl<T>(gev_ctx *, T *, const T *);
+    template int t0i_link_factor_impl<T>(gev_ctx *, T *, const T *);                                      \
+    template int solve_poisson_T00_fused_impl<T>(gev_ctx *, const T *, const T *, const T *, double, double, double, \
+                                                 double, T *, cx<T> *, double, double, int);
 INSTZ(double)
 INSTZ(float)
 #undef INSTZ

@@ -35,6 +35,12 @@ template <typename T> int solve_vector_fused_impl(gev_ctx *c, T *Bi, cx<T> *BiFT
 // (128 <= N <= 1024, as chi_from_Tij_supported)
 template <typename T>
 int solve_vector_links_fused_impl(gev_ctx *c, T *Bi, const T *phi, cx<T> *BiFT, double coeff, double modif);
+// prepareFTsource T00 form formed inside the Poisson solve's x pass (128 <= N
+// <= 1024); ctx->red[0] / ctx->red[1] receive the FP64 sums of the source and
+// of the result, and the k=0 mode is taken from ctx->red[1] when k0set
+template <typename T>
+int solve_poisson_T00_fused_impl(gev_ctx *c, const T *phi, const T *chi, const T *src, double bg, double c1,
+                                 double c2, double c3, T *pot, cx<T> *work, double coeff, double modif, int k0set);
 // Bi[c] *= 1 + phi(x) + phi(x + e_c), in place
 template <typename T> int t0i_link_factor_impl(gev_ctx *c, T *Bi, const T *phi);
 

@@ -154,6 +154,19 @@ class Context:
                                     0.0 if k0 is None else float(k0)), "gev_solve_poisson")
         return out
 
+    def solve_poisson_T00(self, phi, chi, source, bg, c1, c2, c3, coeff, modif=0.0, k0set=False, out=None,
+                          work=None):
+        """prepareFTsource (T00 form) and the Poisson solve in one chain; returns
+        (potential, (sum of source, sum of the source terms)).  source is only
+        read for 128 <= N <= 1024 and overwritten by the source terms below."""
+        out = torch.empty_like(phi) if out is None else out
+        work = self.fourier(1) if work is None else work
+        sums = (C.c_double * 2)()
+        self.bind_stream()
+        check(lib.gev_solve_poisson_T00(self._h, _ptr(phi), _ptr(chi), _ptr(source), bg, c1, c2, c3, _ptr(out),
+                                        _ptr(work), coeff, modif, int(k0set), sums), "gev_solve_poisson_T00")
+        return out, (sums[0], sums[1])
+
     def solve_chi(self, Sij, out=None, work6=None, work1=None):
         """c2r(projectFTscalar(r2c(Sij)))"""
         out = self.real(1) if out is None else out

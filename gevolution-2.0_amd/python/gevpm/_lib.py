@@ -90,6 +90,8 @@ SIGNATURES = {
     "gev_solve_chi_Tij": (_i, [_vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "gev_solve_vector": (_i, [_vp, _vp, _vp, _d, _d]),
     "gev_solve_vector_T0i_links": (_i, [_vp, _vp, _vp, _vp, _d, _d]),
+    "gev_solve_poisson_T00": (_i, [_vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _d, _d, _i, _dp]),
+    "gev_ctx_sums": (_i, [_vp, _dp]),
     "gev_update_q": (_i, [_vp, _d, _vp, _vp, _vp, _i, _d, _d, _dp]),
     "gev_update_q_newton": (_i, [_vp, _d, _vp, _vp, _i, _d, _dp]),
     "gev_update_pos": (_i, [_vp, _d, _vp, _vp, _vp, _i, _d, _d]),

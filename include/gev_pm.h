@@ -147,6 +147,21 @@ int gev_solve_chi(gev_ctx *ctx, const void *Sij, void *chi, void *SijFT, void *c
 int gev_solve_chi_Tij(gev_ctx *ctx, const void *phi, void *Tij, double coeff, void *chi, void *SijFT, void *chiFT);
 /* plan_Bi FFT_FORWARD, projectFTvector, plan_Bi FFT_BACKWARD (main.cu:918-948), Bi in place */
 int gev_solve_vector(gev_ctx *ctx, void *Bi, void *BiFT, double coeff, double modif);
+/* prepareFTsource, T00 form (gevolution.hpp:160-183; main.cu:725), and
+ * solveModifiedPoissonFT with its transforms (main.cu:742-794) in one chain,
+ * pot = phi allowed: for 128 <= N <= 1024 the source is formed inside the
+ * forward x pass and never stored, and source is only read; below that the
+ * source pass runs in place over source first.  k0set: the k=0 mode of the
+ * transformed source becomes the FP64 sum of the source terms (as
+ * gev_solve_poisson with k0re = sums[1]).  sums (may be NULL): the FP64 sums
+ * of source and of the source terms, as gev_prepare_source_T00 returns them;
+ * with NULL the call does not wait for the GPU, and gev_ctx_sums reads them
+ * later (until the context's next reduction). */
+int gev_solve_poisson_T00(gev_ctx *ctx, const void *phi, const void *chi, void *source, double bg, double c1,
+                          double c2, double c3, void *pot, void *work, double coeff, double modif, int k0set,
+                          double *sums);
+/* the two sums of the last gev_solve_poisson_T00 (waits for the context's stream) */
+int gev_ctx_sums(gev_ctx *ctx, double sums[2]);
 /* gev_solve_vector of T0i given as link sums (gev_project_T00_Tij_T0i_links):
  * Bi[c](x) *= 1 + phi(x) + phi(x + e_c), then the solve; for
  * 128 <= N <= 1024 the factor is applied inside the forward x passes. */

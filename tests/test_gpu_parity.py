@@ -587,3 +587,58 @@ def test_gr_steps_128_vs_oracle(gp, O):
     assert relmax(real_np(f.phi)[0], ost["phi"]) < 1e-9
     assert relmax(real_np(f.chi)[0], ost["chi"]) < 1e-7
     assert relmax(real_np(f.Bi), ost["Bi"]) < 1e-7
+
+
+# ------------------------- prepareFTsource T00 formed inside the Poisson solve
+@pytest.mark.parametrize("N,prec", [(64, "fp64"), (128, "fp64"), (128, "mixed"), (256, "mixed")])
+def test_solve_poisson_T00_vs_separate(gp, N, prec):
+    """gev_solve_poisson_T00 (source formed in the x pass for N >= 128) against
+    gev_prepare_source_T00 followed by gev_solve_poisson with the k=0 mode set
+    from the FP64 sum; both sums returned as the source pass returns them"""
+    ctx = gp.Context(N, gp.GEV_FP64 if prec == "fp64" else gp.GEV_MIXED)
+    g = torch.Generator(device="cpu").manual_seed(5 * N)
+    rnd = lambda s: (s * torch.randn(1, N, N, N, generator=g, dtype=torch.float64)).to(ctx.rdtype).to(ctx.device)
+    phi, chi, src = rnd(1e-4), rnd(1e-5), 1.0 + rnd(0.3)
+    bg, c1, c2, c3 = 0.31, 2.3e-3, 4.1e-4, 7.7e-7
+    coeff, modif = float(N * N), 0.21
+    k0set = prec == "mixed"
+    res, s_src, s_res = ctx.prepare_source_T00(phi, chi, src, bg, c1, c2, c3)
+    ref = ctx.solve_poisson(res, coeff, modif, k0=s_res if k0set else None)
+    src2 = src.clone()
+    got, (g_src, g_res) = ctx.solve_poisson_T00(phi, chi, src2, bg, c1, c2, c3, coeff, modif, k0set=k0set)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    assert relmax(to_np(got), to_np(ref)) < tol
+    assert abs(g_src - s_src) <= 1e-12 * abs(s_src) and abs(g_res - s_res) <= 1e-9 * max(abs(s_res), 1.0)
+    if N >= 128:
+        assert torch.equal(src2, src)  # the source is only read
+
+
+def test_gr_steps_128_mixed_vs_fp64(gp):
+    """the mixed GR cycle at 128^3 (T00 source inside the phi solve with the FP64
+    k=0 mode, T0i on the links, FP32 particles) against the FP64 cycle from the
+    same IC"""
+    N, L, steps = 128, 640.0, 3
+    ctx, p, f, st, md, _ = gpu_ic(gp, N, L, 1)
+    cm = gp.Context(N, gp.GEV_MIXED)
+    pm = gp.Particles(cm, N ** 3, 0.31 / N ** 3)
+    a_, b_, c_ = p.store()
+    pm.load(a_, b_, c_)
+    fm = gp.Fields(cm)
+    fm.copy_from(f)
+    sm = gp.StepState(1, st.fourpiG, 0.31, 0.69, z_relax=100.0)
+    for key in ("a", "tau", "dtau_old", "maxvel"):
+        setattr(sm, key, getattr(st, key))
+    t1 = gp.particle_horizon(1 / 90.0, st.fourpiG, 0.31, 0.69)
+    dtau = (t1 - st.tau) / steps
+    for _ in range(steps):
+        st.dtau = dtau
+        sm.dtau = dtau
+        gp.step(p, f, st)
+        gp.step(pm, fm, sm)
+    gpos, gq, _ = by_id(p)
+    mpos, mq, _ = by_id(pm)
+    assert np.abs(((mpos - gpos) + 0.5) % 1.0 - 0.5).max() * N < 2e-5
+    assert relmax(mq, gq) < 2e-5
+    assert relmax(real_np(fm.phi)[0], real_np(f.phi)[0]) < 1e-3
+    assert np.isclose(sm.T00hom, st.T00hom, rtol=1e-6)
+    assert np.isclose(sm.a, st.a, rtol=1e-12)

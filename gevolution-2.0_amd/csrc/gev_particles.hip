@@ -129,13 +129,22 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const int32_t *__restri
 
 __global__ __launch_bounds__(1024) void k_scan_mid(int32_t *bsum, int64_t nb)
 {
-    // one block of 1024 threads, each owns a contiguous run
+    // one block of 1024 threads, each owns a contiguous run (read 16 bytes at
+    // a time when the runs are whole multiples of four)
     __shared__ int32_t sh[1024];
     int64_t per = (nb + 1023) / 1024;
     int64_t b0 = threadIdx.x * per;
     int32_t s = 0;
-    for (int64_t j = 0; j < per; j++)
-        if (b0 + j < nb) s += bsum[b0 + j];
+    if (per % 4 == 0 && b0 + per <= nb) {
+        const int4 *v = reinterpret_cast<const int4 *>(bsum + b0);
+        for (int64_t j = 0; j < per / 4; j++) {
+            const int4 q = v[j];
+            s += q.x + q.y + q.z + q.w;
+        }
+    } else {
+        for (int64_t j = 0; j < per; j++)
+            if (b0 + j < nb) s += bsum[b0 + j];
+    }
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
@@ -145,12 +154,25 @@ __global__ __launch_bounds__(1024) void k_scan_mid(int32_t *bsum, int64_t nb)
         __syncthreads();
     }
     int32_t run = sh[threadIdx.x] - s;
-    for (int64_t j = 0; j < per; j++)
-        if (b0 + j < nb) {
-            int32_t t = bsum[b0 + j];
-            bsum[b0 + j] = run;
-            run += t;
+    if (per % 4 == 0 && b0 + per <= nb) {
+        int4 *v = reinterpret_cast<int4 *>(bsum + b0);
+        for (int64_t j = 0; j < per / 4; j++) {
+            const int4 q = v[j];
+            int4 o;
+            o.x = run; run += q.x;
+            o.y = run; run += q.y;
+            o.z = run; run += q.z;
+            o.w = run; run += q.w;
+            v[j] = o;
         }
+    } else {
+        for (int64_t j = 0; j < per; j++)
+            if (b0 + j < nb) {
+                int32_t t = bsum[b0 + j];
+                bsum[b0 + j] = run;
+                run += t;
+            }
+    }
 }
 
 // writes the exclusive prefix of cnt to start and clears cnt for the next count;

@@ -197,10 +197,12 @@ def by_id(p):
     return np.ascontiguousarray(pos[:, o]), np.ascontiguousarray(q[:, o]), pid[o]
 
 
-def test_sort_invariants(gp):
-    N = 32
+@pytest.mark.parametrize("N,n", [(32, 50000), (256, 400000)])
+def test_sort_invariants(gp, N, n):
+    """cell keys, order and the cell-start table; at 256^3 the block sums of
+    the scan are long enough for its 16-byte path"""
     ctx = gp.Context(N, gp.GEV_FP64)
-    pos, q, pid = make_particles(N, 50000, 1, cluster=True)
+    pos, q, pid = make_particles(N, n, 1, cluster=True)
     p = gpu_pcls(gp, ctx, pos, q, pid, 1.0)
     gpos, gq, gid = (to_np(t) for t in p.store())
     assert np.array_equal(np.sort(gid), np.sort(pid))
@@ -209,7 +211,7 @@ def test_sort_invariants(gp):
     assert np.array_equal(cells, c[0] + N * (c[1] + N * c[2]))          # bit-exact keys
     assert np.all(np.diff(cells) >= 0)                                    # sorted by cell
     start = to_np(p.cell_start())
-    assert start[0] == 0 and start[-1] == 50000
+    assert start[0] == 0 and start[-1] == n
     assert np.array_equal(np.diff(start), np.bincount(cells, minlength=N ** 3))
     o = np.argsort(gid); oi = np.argsort(pid)
     assert np.array_equal(gpos[:, o], pos[:, oi]) and np.array_equal(gq[:, o], q[:, oi])

@@ -80,7 +80,8 @@ def main():
     a_end = 1.0 / (1.0 + args.z_end)
     dtau = (gevpm.particle_horizon(a_end, fpg, om, ol) - st64.tau) / total
     torch.cuda.synchronize()
-    log(f"IC {N}^3 generated in {time.time() - t0:.1f}s, a={st64.a:.5f} tau={st64.tau:.4f} dtau={dtau:.5f}")
+    ic_s = time.time() - t0
+    log(f"IC {N}^3 generated in {ic_s:.1f}s, a={st64.a:.5f} tau={st64.tau:.4f} dtau={dtau:.5f}")
 
     # ------------------------------------------------------- mixed-form copy
     cm = gevpm.Context(N, gevpm.GEV_MIXED)
@@ -122,7 +123,7 @@ def main():
         return ms, {kk: v / args.steps for kk, v in phases.items()}
 
     ms_m, ph_m = run(cm, pm, fm, stm, "mixed")
-    result = {}
+    result = {"ic_fp64_s": ic_s, "rows_next": rows_next(cm, pm, fm, stm, dtau, L, om, ol, fpg)}
     if not args.no_fp64:
         ms_64, ph_64 = run(c64, p64, f64, st64, "fp64")
         # ---------------------------------------------- errors of the mixed form
@@ -185,6 +186,56 @@ def main():
     }
     line.update(result)
     print(json.dumps(line), flush=True)
+
+
+def rows_next(cm, pm, fm, st, dtau, L, om, ol, fpg):
+    """The callers beside the step, timed once on the final mixed state (each
+    is a whole library call, so host work and PCIe copies are included where
+    the call has them): P(k) of the CIC density (deposit, r2c, binning), a
+    Gadget-2 snapshot of every 64th particle and the particle lightcone of a
+    full-sky observer at the box centre whose shell (radius 0.4 box) lies
+    inside the box, every 8th particle, written to a temporary directory."""
+    import tempfile
+    import torch
+    import gevpm
+    from gevpm import lightcone as lcm
+    from gevpm.spectra import density_power
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.time()
+        r = fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.time() - t), r
+
+    out = {}
+    density_power(cm, pm, 64, work=fm.source, F=fm.scalarFT)  # warm
+    out["pk_ms"], _ = timed(lambda: density_power(cm, pm, 64, work=fm.source, F=fm.scalarFT))
+    with tempfile.TemporaryDirectory() as d:
+        hdr = gevpm.gev_gadget2_header()
+        hdr.num_files = 1
+        hdr.time, hdr.BoxSize = st.a, L * 1000.0
+        ids = pm.store()[2]
+        sel = int((ids % 64 == 0).sum())
+        del ids
+        hdr.npart[1] = hdr.npartTotal[1] = sel
+        out["gadget2_ms"], _ = timed(lambda: pm.save_gadget2(os.path.join(d, "snap"), hdr, tracer_factor=64,
+                                                             phi=fm.phi))
+        out["gadget2_particles"] = sel
+        geo = lcm.Geometry(vertex=(0.5, 0.5, 0.5))
+        dist = 0.4
+        nv, verts = lcm.vertices(geo, dist + (0.5 + lcm.IDCHECK_ZONE) * dtau, dist - 0.5 * dtau,
+                                 lcm.box_domain(cm.N))
+        log_ = lcm.IDLog(cm)
+        h2 = gevpm.gev_gadget2_header()
+        h2.num_files = 1
+        h2.time, h2.BoxSize = st.a, L * 1000.0
+        dadtau = st.a * gevpm.hconf(st.a, fpg, om, ol)
+        out["lightcone_ms"], w = timed(lambda: pm.save_gadget2_lightcone(
+            os.path.join(d, "lc"), h2, geo, dist, dtau, dtau, dadtau, verts, log_, fm.phi, tracer_factor=8))
+        out["lightcone_particles"] = w
+        out["lightcone_vertices"] = nv
+    return out
 
 
 # Algorithmic HBM bytes per launch of the kernels that can dominate the mixed

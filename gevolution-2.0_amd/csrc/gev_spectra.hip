@@ -1,0 +1,203 @@
+// gev_spectra.hip -- binned power spectrum of a Fourier field on the device:
+// extractPowerSpectrum (tools.hpp:65-342) with KTYPE_LINEAR bins, optionally
+// with the CIC window divided out (the deconvolved density spectrum of
+// output.hpp:3006-3033), sums in FP64.
+//
+// One thread owns one segment of a (kx, ky) column of the half spectrum and
+// walks it along the slowest index kz, so the lanes of a wavefront read
+// consecutive kx of the same row (coalesced).  Along the column |k| rises for
+// kz = 0..N/2 and falls after, so the bin changes only a few times per
+// segment: the thread keeps the five sums of its current bin in registers and
+// adds them to its workgroup's LDS bins when the bin changes.  A second
+// kernel adds the workgroups' bins in a fixed order.  The LDS adds inside one
+// workgroup come in no fixed order, so repeated calls may differ in the last
+// bits of the sums.
+#include <math.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "gev_common.hpp"
+#include "gev_kernels.hpp"
+
+namespace gev {
+
+constexpr int PK_NT = 256;
+constexpr int PK_MAX_BINS = 1024;
+
+// Per bin: occupation, sum k, sum k^2, sum P k^3 / W^2, sum (P k^3 / W^2)^2,
+// with the weight 2 of a mode whose conjugate is not stored (tools.hpp:180-290).
+// grid: x over columns (PK_NT per workgroup), y over kz segments of zlen;
+// partials[j * groups + g], g = blockIdx.y * gridDim.x + blockIdx.x
+template <typename T>
+__global__ __launch_bounds__(PK_NT) void k_pk_columns(const cx<T> *__restrict__ F, int N, int P, int numbins,
+                                                      int deconvolve, int zlen, double *__restrict__ partials)
+{
+    extern __shared__ double sh[];
+    double *bins = sh;               // [5][numbins]
+    double *tk2 = sh + 5 * numbins;  // (2 pi |n|)^2 per index
+    double *isn = tk2 + N;           // pi |n| / (N sin(pi |n| / N)), or 1
+    const int H = N / 2;
+    for (int j = threadIdx.x; j < 5 * numbins; j += PK_NT) bins[j] = 0.0;
+    for (int i = threadIdx.x; i < N; i += PK_NT) {
+        const int lin = i <= H ? i : N - i;
+        double k = 2.0 * M_PI * lin;
+        tk2[i] = k * k;
+        isn[i] = (deconvolve && lin > 0) ? M_PI * (double)lin / (sin(M_PI * (double)lin / (double)N) * (double)N) : 1.0;
+    }
+    __syncthreads();
+    const double kmax = sqrt(3.0 * tk2[H]), inv_kmax = 1.0 / kmax;
+    const int64_t col = (int64_t)blockIdx.x * PK_NT + threadIdx.x;
+    if (col < (int64_t)N * (H + 1)) {
+        const int kx = (int)(col % (H + 1)), ky = (int)(col / (H + 1));
+        const double kxy2 = tk2[kx] + tk2[ky], ixy = isn[kx] * isn[ky];
+        const double w = (kx == 0 || kx == H) ? 1.0 : 2.0;
+        const cx<T> *f = F + (int64_t)ky * P + kx;
+        const int64_t zstride = (int64_t)N * P;
+        int cur = -1;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+        const int z0 = blockIdx.y * zlen, z1 = min(N, z0 + zlen);
+        for (int kz = z0; kz < z1; kz++) {
+            const cx<T> v = f[kz * zstride];
+            if (kz == 0 && kx == 0 && ky == 0) continue;
+            const double k2 = kxy2 + tk2[kz];
+            const double sk = sqrt(k2);
+            // the bin is floor(numbins sk / kmax) with a correctly rounded
+            // division; the product with 1 / kmax agrees except within a few
+            // ulp of a bin edge, where the division is done
+            const double q = (double)numbins * sk;
+            double r = q * inv_kmax, fl = floor(r);
+            if (r - fl < 1e-9 || fl + 1.0 - r < 1e-9) fl = floor(q / kmax);
+            const int b = (int)fl;
+            if (b >= numbins) continue;
+            if (b != cur) {
+                if (cur >= 0) {
+                    atomicAdd(&bins[cur], a0);
+                    atomicAdd(&bins[numbins + cur], a1);
+                    atomicAdd(&bins[2 * numbins + cur], a2);
+                    atomicAdd(&bins[3 * numbins + cur], a3);
+                    atomicAdd(&bins[4 * numbins + cur], a4);
+                }
+                a0 = a1 = a2 = a3 = a4 = 0.0;
+                cur = b;
+            }
+            double is = ixy * isn[kz];
+            is *= is;
+            const double p = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            const double pk = p * k2 * sk * is;
+            a0 += w;
+            a1 += w * sk;
+            a2 += w * k2;
+            a3 += w * pk;
+            a4 += w * (pk * pk);
+        }
+        if (cur >= 0) {
+            atomicAdd(&bins[cur], a0);
+            atomicAdd(&bins[numbins + cur], a1);
+            atomicAdd(&bins[2 * numbins + cur], a2);
+            atomicAdd(&bins[3 * numbins + cur], a3);
+            atomicAdd(&bins[4 * numbins + cur], a4);
+        }
+    }
+    __syncthreads();
+    const int64_t groups = (int64_t)gridDim.x * gridDim.y, g = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    for (int j = threadIdx.x; j < 5 * numbins; j += PK_NT) partials[j * groups + g] = bins[j];
+}
+
+// out[j] = sum over g of partials[j * groups + g]: one workgroup per j, each
+// thread a strided run of g, then the runs added in thread order
+__global__ __launch_bounds__(PK_NT) void k_pk_reduce(const double *__restrict__ partials, int groups,
+                                                     double *__restrict__ out)
+{
+    __shared__ double part[PK_NT / 64];
+    const double *row = partials + (int64_t)blockIdx.x * groups;
+    double s = 0.0;
+    for (int g = threadIdx.x; g < groups; g += PK_NT) s += row[g];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < PK_NT / 64; w++) t += part[w];
+        out[blockIdx.x] = t;
+    }
+}
+
+template <typename T>
+static int power_spectrum_impl(gev_ctx *c, const cx<T> *F, int numbins, int deconvolve, double *sums_host)
+{
+    const int N = c->N;
+    const int64_t cols = (int64_t)N * (N / 2 + 1);
+    const int nbx = (int)((cols + PK_NT - 1) / PK_NT);
+    // split the columns into kz segments until the launch holds about 2048
+    // threads per compute unit, keeping segments of at least 16 modes
+    int nseg = (int)std::min<int64_t>(std::max<int64_t>(1, (int64_t)c->ncu * 2048 / cols), std::max(1, N / 16));
+    const int zlen = (N + nseg - 1) / nseg;
+    nseg = (N + zlen - 1) / zlen;
+    const int nb = nbx * nseg;
+    const int nvals = 5 * numbins;
+    const size_t shm = sizeof(double) * (5 * (size_t)numbins + 2 * (size_t)N);
+    if (shm > 65536) {
+        gev_set_error("gev_power_spectrum: numbins too large for this mesh (LDS)");
+        return GEV_ERR_ARG;
+    }
+    double *buf = nullptr;
+    hipError_t e = hipMallocAsync((void **)&buf, sizeof(double) * ((size_t)nb + 1) * nvals, c->stream);
+    if (e != hipSuccess) {
+        gev_set_error(std::string("gev_power_spectrum: ") + hipGetErrorString(e));
+        return GEV_ERR_NOMEM;
+    }
+    double *out = buf + (size_t)nb * nvals;
+    hipLaunchKernelGGL((k_pk_columns<T>), dim3(nbx, nseg), dim3(PK_NT), shm, c->stream, F, N, c->P, numbins,
+                       deconvolve, zlen, buf);
+    hipLaunchKernelGGL(k_pk_reduce, dim3(nvals), dim3(PK_NT), 0, c->stream, buf, nb, out);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(sums_host, out, sizeof(double) * nvals, hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e2 = hipFreeAsync(buf, c->stream);
+    if (e == hipSuccess) e = e2;
+    const hipError_t e3 = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = e3;
+    if (e != hipSuccess) {
+        gev_set_error(std::string("gev_power_spectrum: ") + hipGetErrorString(e));
+        return GEV_ERR_HIP;
+    }
+    return GEV_OK;
+}
+
+}  // namespace gev
+
+using namespace gev;
+
+extern "C" int gev_power_spectrum(gev_ctx *c, const void *F, int numbins, int deconvolve, double *kbin,
+                                  double *power, double *kscatter, double *pscatter, int64_t *occupation)
+{
+    if (!c || !F || !kbin || !power || !kscatter || !pscatter || !occupation) {
+        gev_set_error("gev_power_spectrum: NULL argument");
+        return GEV_ERR_ARG;
+    }
+    if (numbins < 1 || numbins > PK_MAX_BINS) {
+        gev_set_error("gev_power_spectrum: numbins must be in [1, 1024]");
+        return GEV_ERR_ARG;
+    }
+    std::vector<double> s(5 * (size_t)numbins);
+    const int rc = c->prec == GEV_FP64
+                       ? power_spectrum_impl<double>(c, (const cx<double> *)F, numbins, deconvolve, s.data())
+                       : power_spectrum_impl<float>(c, (const cx<float> *)F, numbins, deconvolve, s.data());
+    if (rc) return rc;
+    // per-bin normalisation of extractPowerSpectrum (tools.hpp:300-342)
+    for (int b = 0; b < numbins; b++) {
+        const double occ = s[b], ks = s[numbins + b], k2s = s[2 * numbins + b], ps = s[3 * numbins + b],
+                     p2s = s[4 * numbins + b];
+        occupation[b] = (int64_t)llround(occ);
+        if (occ > 0) {
+            kbin[b] = ks / occ;
+            kscatter[b] = sqrt(fmax(k2s * occ - ks * ks, 0.0)) / occ;
+            power[b] = ps / occ;
+            pscatter[b] = sqrt(fmax(p2s / occ - power[b] * power[b], 0.0));
+        } else {
+            kbin[b] = power[b] = kscatter[b] = pscatter[b] = 0.0;
+        }
+    }
+    return GEV_OK;
+}

@@ -116,6 +116,7 @@ SIGNATURES = {
     "gev_pcls_save_gadget2_lightcone": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header),
                                              C.POINTER(gev_lightcone_geometry), _d, _d, _d, _d,
                                              C.POINTER(C.c_double), _i, _vp, _vp, _i, C.POINTER(C.c_int64)]),
+    "gev_power_spectrum": (_i, [_vp, _vp, _i, _i, _dp, _dp, _dp, _dp, C.POINTER(C.c_int64)]),
     "gev_adj_deposit": (_i, [_vp, _vp, _vp]),
     "gev_adj_divergence": (_i, [_vp, _vp, _vp, _d]),
     "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _d]),

@@ -299,6 +299,17 @@ int gev_pcls_save_gadget2_lightcone(gev_pcls *p, const char *path, gev_gadget2_h
                                     double dadtau, const double *vertex, int vertexcount, gev_idlog *log,
                                     const void *phi, int tracer_factor, int64_t *written);
 
+/* ---- power spectra (extractPowerSpectrum, tools.hpp:65-342, KTYPE_LINEAR;
+ * the deconvolved density spectrum of writeSpectra, output.hpp:3006-3033).
+ * F: one Fourier component in the context's layout (complex of the context's
+ * precision, pitch gev_ctx_kpitch).  numbins in [1, 1024] linear bins up to
+ * sqrt(3) * 2 pi N/2; deconvolve != 0 divides |F|^2 by the CIC window squared.
+ * Per bin (host arrays of numbins): mean k, mean of |F|^2 k^3 / W^2, their
+ * scatters, and the occupation (modes whose conjugate is not stored count
+ * twice).  Sums in FP64; returns after the result is on the host. */
+int gev_power_spectrum(gev_ctx *ctx, const void *F, int numbins, int deconvolve, double *kbin, double *power,
+                       double *kscatter, double *pscatter, int64_t *occupation);
+
 /* ---- adjoint of the Newtonian cycle (no reference counterpart; reverse mode
  *      of update_q_Newton, update_pos_Newton, the CIC mass deposit and the
  *      Poisson solve, gevolution.hpp:1051-1118, 1270-1273, 1902-1905, 806-852).

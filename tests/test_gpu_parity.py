@@ -470,21 +470,62 @@ def test_kick_drift_mixed(gp, O, N, cluster):
 
 
 # ------------------------------------------------------------------- P(k)
-@pytest.mark.parametrize("N", [16, 64])
-def test_power_spectrum_vs_oracle(gp, O, N):
-    """device P(k) estimator (extractPowerSpectrum semantics) against the oracle's"""
+@pytest.mark.parametrize("N,prec,numbins,deconv", [(16, "fp64", 24, True), (64, "fp64", 24, True),
+                                                    (64, "fp64", 7, False), (128, "fp64", 200, True),
+                                                    (64, "mixed", 24, True), (128, "mixed", 64, False)])
+def test_power_spectrum_vs_oracle(gp, O, N, prec, numbins, deconv):
+    """gev_power_spectrum (extractPowerSpectrum semantics) against the oracle's
+    on the same modes (for the mixed form: the complex64 modes, widened)"""
     from gevpm import spectra
-    ctx = gp.Context(N, gp.GEV_FP64)
-    rng = np.random.default_rng(5)
+    ctx = gp.Context(N, gp.GEV_FP64 if prec == "fp64" else gp.GEV_MIXED)
+    rng = np.random.default_rng(5 + N)
     f = rng.standard_normal(N ** 3)
     F = O.fft_r2c(f)
-    res = spectra.power_spectrum(ctx, ft_gpu(ctx, F), 24, deconvolve=True)
-    ref = O.power_spectrum(F, 24, deconvolve=True)
-    # kbin and power: sums of positive terms; the scatters are square roots of
-    # differences of such sums (cancellation amplifies the summation order)
-    for got, want, tol in zip(res[:4], ref[:4], (1e-12, 1e-12, 1e-8, 1e-8)):
-        assert relmax(got.numpy(), want) < tol
+    Fd = ft_gpu(ctx, F)
+    print("power spectrum", N, prec, numbins, deconv, flush=True)
+    res = spectra.power_spectrum(ctx, Fd, numbins, deconvolve=deconv)
+    Fo = to_np(ctx.unpad(Fd)[0]).astype(np.complex128)
+    ref = O.power_spectrum(Fo, numbins, deconvolve=deconv)
+    # kbin and power: sums of positive terms, held to 1e-12.  The scatters are
+    # square roots of differences of such sums: a bin whose modes share one |k|
+    # (or one power) leaves only rounding, of size sqrt(eps) times the bin's
+    # mean, which the order of the sums decides
+    for got, want in zip(res[:2], ref[:2]):
+        assert relmax(got.numpy(), want) < 1e-12
+    for got, want, mean in zip(res[2:4], ref[2:4], ref[:2]):
+        assert (np.abs(got.numpy() - want) <= 1e-8 * np.abs(want).max() + 4e-8 * np.abs(mean)).all()
     assert np.array_equal(res[4].numpy(), ref[4])
+    # every mode below k_max except k = 0 is counted once, its conjugate too
+    tk = (2 * np.pi * np.abs(np.fft.fftfreq(N, 1.0 / N))) ** 2
+    sk = np.sqrt((tk[:, None, None] + tk[None, :, None]) + tk[None, None, :])
+    inside = np.floor(numbins * sk / np.sqrt(3 * tk[N // 2])) < numbins
+    assert res[4].sum() == int(inside.sum()) - 1
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_power_spectrum_single_mode_and_errors(gp, prec):
+    """a single real cosine mode: one bin holds power |F|^2 k^3 / W^2 at its k,
+    the others none; bad arguments are refused"""
+    from gevpm import spectra
+    N = 32
+    ctx = gp.Context(N, gp.GEV_FP64 if prec == "fp64" else gp.GEV_MIXED)
+    F = ctx.fourier(1)
+    m = (3, 1, 2)  # (kx, ky, kz)
+    F[0, m[2], m[1], m[0]] = 2.0 + 1.0j
+    nb = 40
+    kb, pw, ks, ps, occ = spectra.power_spectrum(ctx, F, nb, deconvolve=True)
+    k = 2 * np.pi * np.sqrt(sum(x * x for x in m))
+    kmax = np.sqrt(3) * 2 * np.pi * N / 2
+    b = int(np.floor(nb * k / kmax))
+    W = np.prod([np.sin(np.pi * x / N) * N / (np.pi * x) for x in m])
+    assert np.isclose(pw[b].item() * occ[b].item(), 2 * 5.0 * k ** 3 / W ** 2, rtol=1e-6)
+    assert (pw.numpy()[np.arange(nb) != b] == 0).all()
+    with pytest.raises(gp.GevError):
+        spectra.power_spectrum(ctx, F, 0)
+    with pytest.raises(gp.GevError):
+        spectra.power_spectrum(ctx, F, 2000)
+    with pytest.raises(gp.GevError):
+        spectra.power_spectrum(ctx, F.to(torch.complex128 if prec == "mixed" else torch.complex64), 8)
 
 
 # --------------------------------------------- fused z-pass solves

@@ -43,7 +43,22 @@ def parse():
     ap.add_argument("--adj-modes", type=int, default=64)
     ap.add_argument("--cpu-mesh", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB fill and three 1 GiB copies (the reference kernels that scale the "
+                         "FETCH_SIZE / WRITE_SIZE counter readings to bytes), then exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """a 1 GiB device fill, then three 1 GiB device-to-device copies"""
+    import torch
+    x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    x.fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copy calibration: 1 GiB fill + 3 x 1 GiB copy done")
 
 
 def log(*a):
@@ -56,6 +71,9 @@ def main():
         print(f"bench.py: this build runs the PM step on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is designed but not built.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     import torch
 

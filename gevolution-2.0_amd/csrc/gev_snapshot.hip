@@ -14,9 +14,11 @@
 // q (1 + (4 - (2q^2 + a^2)/e^2) phi) / e, velocities kick by dtau_vel with
 // (2q^2 + a^2)/e times the CIC edge gradient of phi (phi optional).
 //
-// The conversion runs on the device into the container's ping-pong arrays
-// (free between cycles), one copy brings it to the host, and the host writes
-// the records in container (cell) order.
+// The selection and conversion run on the device: a count per workgroup, the
+// exclusive scan of the counts, then each selected particle written at its
+// rank into the container's ping-pong arrays (free between cycles).  Only the
+// selected particles are copied to the host, which writes the records in
+// container (cell) order.
 //
 // The express format (saveExpress / loadExpress, Particles_gevolution.hpp:
 // 2642-2900) is the raw restart state of one rank: a 120-byte express header,
@@ -38,83 +40,151 @@ namespace gev {
 constexpr double GADGET_VELOCITY_CONVERSION = 3.335640952e-6;  // Gadget velocity unit / c (metadata.hpp:117)
 static_assert(sizeof(gev_gadget2_header) == 256, "the Gadget-2 header is 256 bytes");
 
-// per particle: snapshot position (length units) and velocity, in the order of the container
-template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_gadget_convert(PV<T> v, double dtau_pos, double dtau_vel, double a,
-                                                          double boxsize, const T *__restrict__ phi, float *px,
-                                                          float *py, float *pz, float *vx, float *vy, float *vz)
+constexpr int GS_NT = 256, GS_PER = 4, GS_CHUNK = GS_NT * GS_PER;
+
+// tracers (ID a multiple of the factor) among the GS_CHUNK particles of each workgroup
+__global__ __launch_bounds__(GS_NT) void k_gadget_count(const int64_t *__restrict__ id, int64_t n, int tracer,
+                                                        int32_t *__restrict__ bcount)
 {
-    const int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
-    if (i >= v.n) return;
-    const int N = v.N;
-    int x, y, z;
-    cell_coords(v.cell[i], N, v.lg, x, y, z);
-    const T xs[3] = {v.p0[i], v.p1[i], v.p2[i]};
-    T fr[3];
-    cell_rel(xs, x, y, z, N, fr);
-    const double q[3] = {(double)v.q0[i], (double)v.q1[i], (double)v.q2[i]};
-    double phip = 0.0, g[3] = {0.0, 0.0, 0.0};
-    if (phi) {
-        // CIC value and edge gradient (differences, not yet divided by dx)
-        auto P = [&](int dx_, int dy_, int dz_) { return (double)phi[site(N, x + dx_, y + dy_, z + dz_)]; };
-        const double f0 = fr[0], f1 = fr[1], f2 = fr[2], m0 = 1.0 - f0, m1 = 1.0 - f1, m2 = 1.0 - f2;
-        phip = P(0, 0, 0) * m0 * m1 * m2 + P(1, 0, 0) * f0 * m1 * m2 + P(0, 1, 0) * m0 * f1 * m2 +
-               P(1, 1, 0) * f0 * f1 * m2 + P(0, 0, 1) * m0 * m1 * f2 + P(1, 0, 1) * f0 * m1 * f2 +
-               P(0, 1, 1) * m0 * f1 * f2 + P(1, 1, 1) * f0 * f1 * f2;
-        g[0] = m1 * m2 * (P(1, 0, 0) - P(0, 0, 0)) + f1 * m2 * (P(1, 1, 0) - P(0, 1, 0)) +
-               m1 * f2 * (P(1, 0, 1) - P(0, 0, 1)) + f1 * f2 * (P(1, 1, 1) - P(0, 1, 1));
-        g[1] = m0 * m2 * (P(0, 1, 0) - P(0, 0, 0)) + f0 * m2 * (P(1, 1, 0) - P(1, 0, 0)) +
-               m0 * f2 * (P(0, 1, 1) - P(0, 0, 1)) + f0 * f2 * (P(1, 1, 1) - P(1, 0, 1));
-        g[2] = m0 * m1 * (P(0, 0, 1) - P(0, 0, 0)) + f0 * m1 * (P(1, 0, 1) - P(1, 0, 0)) +
-               m0 * f1 * (P(0, 1, 1) - P(0, 1, 0)) + f0 * f1 * (P(1, 1, 1) - P(1, 1, 0));
-    }
-    const double q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-    const double e2 = q2 + a * a, e = sqrt(e2);
-    const double w = 2.0 * q2 + a * a;  // q^2 + e^2
-    const double vfac = 1.0 + (4.0 - w / e2) * phip;
-    const double rescale = 1.0 / sqrt(a) / GADGET_VELOCITY_CONVERSION;
-    const double cell[3] = {(double)x, (double)y, (double)z};
-    float *po[3] = {px, py, pz}, *vo[3] = {vx, vy, vz};
+    __shared__ int32_t wsum[GS_NT / 64];
+    const int64_t base = (int64_t)blockIdx.x * GS_CHUNK;
+    int c = 0;
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        // absolute position in box units, drifted, wrapped to [0, 1)
-        const double p = (cell[d] + (double)fr[d]) / N + dtau_pos * q[d] * vfac / e;
-        double wrapped = p - floor(p);
-        if (wrapped >= 1.0) wrapped = 0.0;
-        po[d][i] = (float)(wrapped * boxsize);
-        vo[d][i] = (float)((q[d] - dtau_vel * w * g[d] * N / e) * rescale / a);
+    for (int j = 0; j < GS_PER; j++) {
+        const int64_t i = base + j * GS_NT + threadIdx.x;
+        c += (i < n && id[i] % tracer == 0) ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t t = 0;
+        for (int w = 0; w < GS_NT / 64; w++) t += wsum[w];
+        bcount[blockIdx.x] = t;
     }
 }
 
+// Each selected particle's snapshot position (length units) and velocity,
+// written at its rank among the selected ones in container order: the
+// workgroup's offset from the scan of k_gadget_count, then wave ballots.
 template <typename T>
-static int gadget_convert(gev_pcls *p, double dtau_pos, double dtau_vel, double a, double boxsize, const T *phi,
-                          std::vector<float> &pos, std::vector<float> &vel, std::vector<int64_t> &id)
+__global__ __launch_bounds__(GS_NT) void k_gadget_convert(PV<T> v, int tracer, const int32_t *__restrict__ bstart,
+                                                          double dtau_pos, double dtau_vel, double a, double boxsize,
+                                                          const T *__restrict__ phi, float *px, float *py, float *pz,
+                                                          float *vx, float *vy, float *vz, int64_t *oid)
+{
+    __shared__ int32_t wc[GS_PER][GS_NT / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * GS_CHUNK;
+    bool sel[GS_PER];
+    int pre[GS_PER];
+#pragma unroll
+    for (int j = 0; j < GS_PER; j++) {
+        const int64_t i = base + j * GS_NT + threadIdx.x;
+        sel[j] = i < v.n && v.id[i] % tracer == 0;
+        const unsigned long long m = __ballot(sel[j]);
+        pre[j] = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wc[j][wv] = __popcll(m);
+    }
+    __syncthreads();
+    int32_t run = bstart[blockIdx.x];
+    const int N = v.N;
+#pragma unroll
+    for (int j = 0; j < GS_PER; j++) {
+        int32_t slot = run + pre[j];
+        for (int w2 = 0; w2 < GS_NT / 64; w2++) {
+            if (w2 < wv) slot += wc[j][w2];
+            run += wc[j][w2];
+        }
+        if (!sel[j]) continue;
+        const int64_t i = base + j * GS_NT + threadIdx.x;
+        int x, y, z;
+        cell_coords(v.cell[i], N, v.lg, x, y, z);
+        const T xs[3] = {v.p0[i], v.p1[i], v.p2[i]};
+        T fr[3];
+        cell_rel(xs, x, y, z, N, fr);
+        const double q[3] = {(double)v.q0[i], (double)v.q1[i], (double)v.q2[i]};
+        double phip = 0.0, g[3] = {0.0, 0.0, 0.0};
+        if (phi) {
+            // CIC value and edge gradient (differences, not yet divided by dx)
+            auto P = [&](int dx_, int dy_, int dz_) { return (double)phi[site(N, x + dx_, y + dy_, z + dz_)]; };
+            const double f0 = fr[0], f1 = fr[1], f2 = fr[2], m0 = 1.0 - f0, m1 = 1.0 - f1, m2 = 1.0 - f2;
+            phip = P(0, 0, 0) * m0 * m1 * m2 + P(1, 0, 0) * f0 * m1 * m2 + P(0, 1, 0) * m0 * f1 * m2 +
+                   P(1, 1, 0) * f0 * f1 * m2 + P(0, 0, 1) * m0 * m1 * f2 + P(1, 0, 1) * f0 * m1 * f2 +
+                   P(0, 1, 1) * m0 * f1 * f2 + P(1, 1, 1) * f0 * f1 * f2;
+            g[0] = m1 * m2 * (P(1, 0, 0) - P(0, 0, 0)) + f1 * m2 * (P(1, 1, 0) - P(0, 1, 0)) +
+                   m1 * f2 * (P(1, 0, 1) - P(0, 0, 1)) + f1 * f2 * (P(1, 1, 1) - P(0, 1, 1));
+            g[1] = m0 * m2 * (P(0, 1, 0) - P(0, 0, 0)) + f0 * m2 * (P(1, 1, 0) - P(1, 0, 0)) +
+                   m0 * f2 * (P(0, 1, 1) - P(0, 0, 1)) + f0 * f2 * (P(1, 1, 1) - P(1, 0, 1));
+            g[2] = m0 * m1 * (P(0, 0, 1) - P(0, 0, 0)) + f0 * m1 * (P(1, 0, 1) - P(1, 0, 0)) +
+                   m0 * f1 * (P(0, 1, 1) - P(0, 1, 0)) + f0 * f1 * (P(1, 1, 1) - P(1, 1, 0));
+        }
+        const double q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+        const double e2 = q2 + a * a, e = sqrt(e2);
+        const double w = 2.0 * q2 + a * a;  // q^2 + e^2
+        const double vfac = 1.0 + (4.0 - w / e2) * phip;
+        const double rescale = 1.0 / sqrt(a) / GADGET_VELOCITY_CONVERSION;
+        const double cell[3] = {(double)x, (double)y, (double)z};
+        float *po[3] = {px, py, pz}, *vo[3] = {vx, vy, vz};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            // absolute position in box units, drifted, wrapped to [0, 1)
+            const double p = (cell[d] + (double)fr[d]) / N + dtau_pos * q[d] * vfac / e;
+            double wrapped = p - floor(p);
+            if (wrapped >= 1.0) wrapped = 0.0;
+            po[d][slot] = (float)(wrapped * boxsize);
+            vo[d][slot] = (float)((q[d] - dtau_vel * w * g[d] * N / e) * rescale / a);
+        }
+        oid[slot] = v.id[i];
+    }
+}
+
+// the selected particles' snapshot records, interleaved on the host
+template <typename T>
+static int gadget_convert(gev_pcls *p, int tracer, double dtau_pos, double dtau_vel, double a, double boxsize,
+                          const T *phi, std::vector<float> &pos, std::vector<float> &vel, std::vector<int64_t> &id)
 {
     gev_ctx *c = p->ctx;
+    hipStream_t s = c->stream;
     const int64_t n = p->n;
-    // staging: the ping-pong position and momentum arrays hold n floats each
-    float *px = (float *)p->pos2[0], *py = (float *)p->pos2[1], *pz = (float *)p->pos2[2];
-    float *vx = (float *)p->q2[0], *vy = (float *)p->q2[1], *vz = (float *)p->q2[2];
-    if (n > 0)
-        hipLaunchKernelGGL((k_gadget_convert<T>), dim3((unsigned)((n + PT_NT - 1) / PT_NT)), dim3(PT_NT), 0, c->stream,
-                           make_pv<T>(p), dtau_pos, dtau_vel, a, boxsize, phi, px, py, pz, vx, vy, vz);
+    pos.clear(); vel.clear(); id.clear();
+    if (n == 0) return GEV_OK;
+    const int64_t nb = (n + GS_CHUNK - 1) / GS_CHUNK;
+    if (nb > c->M) {
+        gev_set_error("gev_pcls_save_gadget2: more than 1024 particles per lattice site");
+        return GEV_ERR_ARG;
+    }
+    // workgroup counts in the cell counters (left zero again by the scan), offsets in rank
+    int rc = pcls_clear_counts(p);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gadget_count, dim3((unsigned)nb), dim3(GS_NT), 0, s, p->id, n, tracer, p->cell_count);
     GEV_HIP(hipGetLastError());
-    std::vector<float> tmp((size_t)n);
-    pos.assign(3 * (size_t)n, 0.0f);
-    vel.assign(3 * (size_t)n, 0.0f);
-    id.assign((size_t)n, 0);
-    const float *src[6] = {px, py, pz, vx, vy, vz};
+    rc = scan_exclusive_counts(c, p->cell_count, nb, p->rank);
+    if (rc) return rc;
+    p->counts_zero = true;
+    int32_t total = 0;
+    GEV_HIP(hipMemcpyAsync(&total, p->rank + nb, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GEV_HIP(hipStreamSynchronize(s));
+    if (total == 0) return GEV_OK;
+    // staging: the ping-pong arrays hold n values each
+    float *st[6] = {(float *)p->pos2[0], (float *)p->pos2[1], (float *)p->pos2[2],
+                    (float *)p->q2[0],   (float *)p->q2[1],   (float *)p->q2[2]};
+    hipLaunchKernelGGL((k_gadget_convert<T>), dim3((unsigned)nb), dim3(GS_NT), 0, s, make_pv<T>(p), tracer, p->rank,
+                       dtau_pos, dtau_vel, a, boxsize, phi, st[0], st[1], st[2], st[3], st[4], st[5], p->id2);
+    GEV_HIP(hipGetLastError());
+    const size_t m = (size_t)total;
+    pos.resize(3 * m);
+    vel.resize(3 * m);
+    id.resize(m);
+    std::vector<float> tmp(m);
     for (int k = 0; k < 6; k++) {
-        if (n == 0) break;
-        GEV_HIP(hipMemcpyAsync(tmp.data(), src[k], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
-        GEV_HIP(hipStreamSynchronize(c->stream));
+        GEV_HIP(hipMemcpyAsync(tmp.data(), st[k], sizeof(float) * m, hipMemcpyDeviceToHost, s));
+        GEV_HIP(hipStreamSynchronize(s));
         std::vector<float> &dst = k < 3 ? pos : vel;
-        for (int64_t j = 0; j < n; j++) dst[3 * j + (k % 3)] = tmp[j];
+        for (size_t j = 0; j < m; j++) dst[3 * j + (k % 3)] = tmp[j];
     }
-    if (n > 0) {
-        GEV_HIP(hipMemcpyAsync(id.data(), p->id, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
-        GEV_HIP(hipStreamSynchronize(c->stream));
-    }
+    GEV_HIP(hipMemcpyAsync(id.data(), p->id2, sizeof(int64_t) * m, hipMemcpyDeviceToHost, s));
+    GEV_HIP(hipStreamSynchronize(s));
     return GEV_OK;
 }
 
@@ -186,19 +256,14 @@ int gev_pcls_save_gadget2(gev_pcls *p, const char *path, const gev_gadget2_heade
     if (!p->sorted) { gev_set_error("gev_pcls_save_gadget2: particles are not binned"); return GEV_ERR_STATE; }
     std::vector<float> pos, vel;
     std::vector<int64_t> id;
+    // tracer selection (ID multiple of the factor, Particles_gevolution.hpp:444,494) in container order
     const int rc = p->ctx->prec == GEV_FP64
-                       ? gadget_convert<double>(p, dtau_pos, dtau_vel, hdr->time, hdr->BoxSize, (const double *)phi, pos,
-                                                vel, id)
-                       : gadget_convert<float>(p, dtau_pos, dtau_vel, hdr->time, hdr->BoxSize, (const float *)phi, pos,
-                                               vel, id);
+                       ? gadget_convert<double>(p, tracer_factor, dtau_pos, dtau_vel, hdr->time, hdr->BoxSize,
+                                                (const double *)phi, pos, vel, id)
+                       : gadget_convert<float>(p, tracer_factor, dtau_pos, dtau_vel, hdr->time, hdr->BoxSize,
+                                               (const float *)phi, pos, vel, id);
     if (rc) return rc;
-    // tracer selection (ID multiple of the factor, Particles_gevolution.hpp:444,494), order kept
-    int64_t m = 0;
-    for (size_t j = 0; j < id.size(); j++) {
-        if (id[j] % tracer_factor != 0) continue;
-        for (int d = 0; d < 3; d++) { pos[3 * m + d] = pos[3 * j + d]; vel[3 * m + d] = vel[3 * j + d]; }
-        id[m++] = id[j];
-    }
+    const int64_t m = (int64_t)id.size();
     const uint64_t declared = (uint64_t)hdr->npart[1] + ((uint64_t)hdr->npartTotalHW[1] << 32);
     if (declared != (uint64_t)m || hdr->npartTotalHW[1] != 0) {
         gev_set_error("gev_pcls_save_gadget2: header npart[1] does not match the selected particles (or exceeds 2^32-1)");

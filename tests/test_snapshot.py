@@ -72,9 +72,12 @@ def test_gadget2_layout_units_and_round_trip(tmp_path, prec):
     hb, fpos, fvel, fid = read_gadget2(path)
     assert np.frombuffer(hb, np.uint32, 6, 0)[1] == n
     assert np.frombuffer(hb, np.float64, 1, 72)[0] == a and np.frombuffer(hb, np.float64, 1, 128)[0] == box
+    assert np.array_equal(fid, gid)  # container (cell) order
+    # the IDs are odd, so a tracer factor of 2 selects none: empty records
+    p.save_gadget2(str(tmp_path / "none"), header(gp, 0, a, box), tracer_factor=2)
+    assert all(len(r) == 0 for r in read_gadget2(str(tmp_path / "none"))[1:])
     o = np.argsort(gid)
     f = np.argsort(fid)
-    assert np.array_equal(fid[f], gid[o])
     # positions p * BoxSize and velocities q / (a^1.5 GVC), rounded once to float
     assert np.allclose(fpos[f], (gpos[:, o] * box).T.astype(np.float32), rtol=2e-7, atol=box * 2e-7)
     assert np.allclose(fvel[f], (gq[:, o] / (a ** 1.5 * GVC)).T.astype(np.float32), rtol=2e-7)
@@ -118,6 +121,8 @@ def test_gadget2_tracer_and_exact_redshift(tmp_path):
     path = str(tmp_path / "snap_tr")
     p.save_gadget2(path, header(gp, len(sel), a, box), tracer_factor=3, dtau_pos=dtp, dtau_vel=dtv, phi=phi)
     _, fpos, fvel, fid = read_gadget2(path)
+    cid = p.store()[2].cpu().numpy()
+    assert np.array_equal(fid, cid[cid % 3 == 0])  # the tracers, in container order
     assert np.array_equal(np.sort(fid), sel)
     o = np.argsort(fid)
     P, Q = pos[:, sel], q[:, sel]

@@ -321,7 +321,7 @@ def adjoint_leg(args):
     N, steps = args.adj_mesh, args.adj_steps
     rng = np.random.default_rng(7)
 
-    def smooth(seed, amp):
+    def smooth(seed, amp, N=N):
         r = np.random.default_rng(seed)
         F = np.fft.rfftn(r.standard_normal((N, N, N)))
         kk = np.fft.fftfreq(N, 1.0 / N)
@@ -357,6 +357,33 @@ def adjoint_leg(args):
     out.update(forward_ms=fwd_ms, backward_ms=bwd_ms, adjoint_over_forward=bwd_ms / fwd_ms,
                checkpoint_bytes=run32.checkpoint_bytes())
     del run32
+    torch.cuda.empty_cache()
+
+    # the larger adjoint workload: twice the mesh, twice the cycles, a checkpoint
+    # every 4 cycles (the backward replays up to 3 cycles per segment)
+    N5, steps5 = 2 * N, 2 * steps
+    amp5 = 0.3 / N5 ** 2
+    run5 = PMRun(N5, 2.0 * N5, 50.0, 0.0, steps5, precision=gevpm.GEV_MIXED, ckpt_every=4)
+    dev5 = run5.ctx.device
+    t5 = run5.forward(torch.from_numpy(smooth(12, amp5, N5)).to(run5.ctx.rdtype).to(dev5)).double().clone()
+    psi5 = torch.from_numpy(smooth(11, amp5, N5)).to(run5.ctx.rdtype).to(dev5).reshape(1, N5, N5, N5)
+    run5.backward(2 * (run5.forward(psi5).double() - t5))
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    e[0].record()
+    rho5 = run5.forward(psi5)
+    e[1].record()
+    run5.backward(2 * (rho5.double() - t5))
+    e[2].record()
+    torch.cuda.synchronize()
+    out["checkpointed"] = {
+        "workload": f"Newtonian PM, {N5}^3 particles on {N5}^3 mesh, L={2 * N5} Mpc/h, {steps5} cycles z=50->0, "
+                    "mixed form, a checkpoint every 4 cycles",
+        "forward_ms": e[0].elapsed_time(e[1]), "backward_ms": e[1].elapsed_time(e[2]),
+        "adjoint_over_forward": e[1].elapsed_time(e[2]) / e[0].elapsed_time(e[1]),
+        "checkpoint_bytes": run5.checkpoint_bytes()}
+    log(f"adjoint {N5}^3 checkpointed: {out['checkpointed']}")
+    del run5, rho5, t5, psi5
     torch.cuda.empty_cache()
     run64, target, g64, f64_ms, b64_ms = run_grad(gevpm.GEV_FP64)
     out.update(fp64_forward_ms=f64_ms, fp64_backward_ms=b64_ms,

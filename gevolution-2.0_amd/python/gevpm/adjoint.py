@@ -103,23 +103,29 @@ class PMRun:
         for si in range(len(starts) - 1, -1, -1):
             k0 = starts[si]
             k1 = starts[si + 1] if si + 1 < len(starts) else self.steps
-            # replay the segment from its checkpoint to get every cycle's start state
+            # replay the segment from its checkpoint to get every cycle's start
+            # state and potential (the last cycle's potential is formed below)
             states = {k0: self._ckpts[k0]}
+            phis = {}
             if k1 - k0 > 1:
                 p.restore(self._ckpts[k0])
                 for k in range(k0, k1 - 1):
                     self._cycle(k)
+                    phis[k] = self.phi.clone()
                     states[k + 1] = p.save()
             for k in range(k1 - 1, k0 - 1, -1):
                 s = self.sched[k]
                 K = s["dtau_k"] * s["a"] * self.N
                 p.restore(states[k])
-                p.project_mass(self.rho)
-                c.solve_poisson(self.rho, self.fpg / s["a"], 0.0, out=self.phi, work=self.work)
+                phi = phis.pop(k, None)
+                if phi is None:
+                    p.project_mass(self.rho)
+                    phi = c.solve_poisson(self.rho, self.fpg / s["a"], 0.0, out=self.phi, work=self.work)
                 p.adj_deposit(lq, self.u)
                 c.adj_divergence(self.u, self.lphi, K)
                 c.solve_poisson(self.lphi, self.fpg / s["a"], 0.0, out=self.lrho, work=self.work)
-                p.adj_gather(lx, lq, phi=self.phi, lrho=self.lrho, K=K, mN=self.mN, C_prev=c_of(k - 1))
+                p.adj_gather(lx, lq, phi=phi, lrho=self.lrho, K=K, mN=self.mN, C_prev=c_of(k - 1))
+                del phi
             del states
         lpsi = c.real(1)
         c.adj_ic_lattice(lx, lq, self.cx, self.cq, lpsi)

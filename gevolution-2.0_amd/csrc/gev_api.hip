@@ -180,6 +180,7 @@ int gev_pcls_create(gev_ctx *c, int64_t capacity, double mass, gev_pcls **out)
     GEV_HIP(hipMalloc(&p->cell, 4 * cap));
     GEV_HIP(hipMalloc(&p->cell2, 4 * cap));
     GEV_HIP(hipMalloc(&p->rank, 4 * cap));
+    GEV_HIP(hipMalloc(&p->perm, 4 * cap));
     GEV_HIP(hipMalloc(&p->cell_start, 4 * (size_t)(c->M + 1)));
     GEV_HIP(hipMalloc(&p->cell_count, 4 * (size_t)c->M));
     *out = p;
@@ -197,6 +198,7 @@ int gev_pcls_destroy(gev_pcls *p)
     (void)hipFree(p->id); (void)hipFree(p->id2);
     (void)hipFree(p->cell); (void)hipFree(p->cell2);
     (void)hipFree(p->rank);
+    (void)hipFree(p->perm);
     (void)hipFree(p->cell_start);
     (void)hipFree(p->cell_count);
     delete p;

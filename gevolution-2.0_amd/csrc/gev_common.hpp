@@ -123,6 +123,7 @@ struct gev_pcls {
     int64_t *id2 = nullptr;
     int32_t *cell2 = nullptr;
     int32_t *rank = nullptr;        // slot of each particle inside its cell
+    int32_t *perm = nullptr;        // re-binning: perm[new slot] = old slot
     int32_t *cell_start = nullptr;  // M+1 offsets, valid when sorted
     int32_t *cell_count = nullptr;  // M counters
     bool sorted = false;

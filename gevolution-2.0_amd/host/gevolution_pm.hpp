@@ -13,6 +13,8 @@
 //                                 gevolution.hpp free functions, same defaults
 //   Hconf / rungekutta4bg / particleHorizon
 //                                 background.hpp:200-290
+//   extractPowerSpectrum / writePowerSpectrum
+//                                 tools.hpp:339-342, 370-452 (linear bins)
 //
 // Errors: any non-zero status of the library becomes std::runtime_error with
 // the library's message (the reference throws std::runtime_error on GPU
@@ -447,6 +449,46 @@ inline void evolveFTvector(Field<Cplx> &SijFT, Field<Cplx> &BiFT, const Real a2d
 {
     check(gev_evolve_ft_vector(SijFT.lattice().ctx(), SijFT.data(), BiFT.data(), a2dtau), "evolveFTvector");
 }
+// extractPowerSpectrum (tools.hpp:339-342 over extractCrossSpectrum, :65-337): the
+// binned spectrum of one Fourier field, KTYPE_LINEAR bins only
+constexpr int KTYPE_GRID = 0, KTYPE_LINEAR = 1;  // tools.hpp:27-28
+inline void extractPowerSpectrum(Field<Cplx> &fldFT, Real *kbin, Real *power, Real *kscatter, Real *pscatter,
+                                 int *occupation, const int numbins, const bool deconvolve = true,
+                                 const int ktype = KTYPE_LINEAR)
+{
+    if (ktype != KTYPE_LINEAR) throw std::runtime_error("extractPowerSpectrum: only KTYPE_LINEAR bins are built");
+    std::vector<double> kb(numbins), pw(numbins), ks(numbins), ps(numbins);
+    std::vector<int64_t> oc(numbins);
+    check(gev_power_spectrum(fldFT.lattice().ctx(), fldFT.data(), numbins, deconvolve ? 1 : 0, kb.data(), pw.data(),
+                             ks.data(), ps.data(), oc.data()),
+          "extractPowerSpectrum");
+    for (int i = 0; i < numbins; i++) {
+        kbin[i] = (Real)kb[i];
+        power[i] = (Real)pw[i];
+        kscatter[i] = (Real)ks[i];
+        pscatter[i] = (Real)ps[i];
+        occupation[i] = (int)oc[i];
+    }
+}
+// writePowerSpectrum (tools.hpp:370-452) for the current redshift (the
+// EXACT_OUTPUT_REDSHIFTS interpolation against an earlier file is not built):
+// the same text table, one line per occupied bin
+inline void writePowerSpectrum(const Real *kbin, const Real *power, const Real *kscatter, const Real *pscatter,
+                               const int *occupation, const int numbins, const Real rescalek, const Real rescalep,
+                               const char *filename, const char *description, double a)
+{
+    FILE *f = std::fopen(filename, "w");
+    if (!f) throw std::runtime_error(std::string("writePowerSpectrum: cannot open ") + filename);
+    std::fprintf(f, "# %s\n", description);
+    std::fprintf(f, "# redshift z=%f\n", 1. / a - 1.);
+    std::fprintf(f, "# k              Pk             sigma(k)       sigma(Pk)      count\n");
+    for (int i = 0; i < numbins; i++)
+        if (occupation[i] > 0)
+            std::fprintf(f, "  %e   %e   %e   %e   %d\n", kbin[i] / rescalek, power[i] / rescalep, kscatter[i] / rescalek,
+                         pscatter[i] / rescalep / std::sqrt((double)occupation[i]), occupation[i]);
+    if (std::fclose(f) != 0) throw std::runtime_error(std::string("writePowerSpectrum: write failed on ") + filename);
+}
+
 // the mixed form's FP64 k=0 mode (no reference counterpart: FP64 mean of an FP32 field)
 inline void set_k0(Field<Cplx> &ft, double re, double im = 0.0)
 {

@@ -5,7 +5,7 @@
 //
 //   main_pm -n N -L boxsize -zin z -zend z -steps S [-newton] [-seed s]
 //           [-zrelax z] -pk mpk.txt [-out final.bin] [-gadget snap] [-express restart]
-//           [-lightcone prefix -lcvertex x,y,z -lcz z]
+//           [-lightcone prefix -lcvertex x,y,z -lcz z] [-spectra prefix [-numbins n]]
 //
 // The mPk file has the loadPowerSpectrum format (ic_basic.hpp:370-449): two
 // columns, k [h/Mpc] and P(k) [(Mpc/h)^3].  Steps are uniform in conformal
@@ -16,7 +16,9 @@
 // -lightcone writes, each cycle before the kick, the full-sky particle
 // lightcone of an observer at lcvertex (box units) and redshift lcz as
 // <prefix>_<cycle>_cdm (writeLightcones' Gadget branch, output.hpp:2094-2160,
-// and the ID backlog hand-over, :2166-2880).
+// and the ID backlog hand-over, :2166-2880).  -spectra writes, at the end, the
+// deconvolved CIC density spectrum as <prefix>_deltaN.dat in writeSpectra's
+// delta_N form (output.hpp:3017-3021, writePowerSpectrum tools.hpp:370-452).
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -38,7 +40,8 @@ struct Options {
     int N = 32, steps = 5, newton = 0;
     uint32_t seed = 42;
     double L = 320.0, zin = 100.0, zend = 50.0, zrelax = -2.0;
-    std::string pk, out, gadget, express, lightcone;
+    std::string pk, out, gadget, express, lightcone, spectra;
+    int numbins = 32;
     double lcvertex[3] = {0.5, 0.5, 0.5}, lcz = 0.0;
 };
 
@@ -64,6 +67,8 @@ Options parse(int argc, char **argv)
         else if (a == "-gadget") o.gadget = next();
         else if (a == "-express") o.express = next();
         else if (a == "-lightcone") o.lightcone = next();
+        else if (a == "-spectra") o.spectra = next();
+        else if (a == "-numbins") o.numbins = std::atoi(next());
         else if (a == "-lcvertex") {
             if (std::sscanf(next(), "%lf,%lf,%lf", &o.lcvertex[0], &o.lcvertex[1], &o.lcvertex[2]) != 3)
                 throw std::runtime_error("-lcvertex wants x,y,z");
@@ -274,6 +279,23 @@ int main(int argc, char **argv)
             std::printf("cycle %d z=%.6f a=%.8f tau=%.8f maxvel=%.6e\n", cycle, 1. / a - 1., a, tau, maxvel);
         }
         lat.sync();
+
+        if (!opt.spectra.empty()) {
+            // writeSpectra's particle branch (output.hpp:2997-3022) at the final time:
+            // the deconvolved CIC density spectrum as delta_N
+            projection_init(&source);
+            scalarProjectionCIC_project(&pcls, &source);
+            plan_source.execute(FFT_FORWARD);
+            const int nb = opt.numbins;
+            std::vector<Real> kbin(nb), power(nb), kscatter(nb), pscatter(nb);
+            std::vector<int> occupation(nb);
+            extractPowerSpectrum(scalarFT, kbin.data(), power.data(), kscatter.data(), pscatter.data(),
+                                 occupation.data(), nb, true, KTYPE_LINEAR);
+            const Real np3 = (Real)M;
+            writePowerSpectrum(kbin.data(), power.data(), kscatter.data(), pscatter.data(), occupation.data(), nb,
+                               (Real)opt.L, np3 * np3 * (Real)(2. * M_PI * M_PI * cosmo.Omega_m * cosmo.Omega_m),
+                               (opt.spectra + "_deltaN.dat").c_str(), "power spectrum of delta_N", a);
+        }
 
         if (!opt.out.empty()) {
             std::vector<part_simple> ps;

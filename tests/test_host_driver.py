@@ -66,6 +66,29 @@ def _check_files(snap, restart, pid, pos, q, L, single):
     assert er[-256:] == hb  # the Gadget-2 header closes the express file
 
 
+def _check_spectrum(path, gevpm, pid, pos, q, N, L, numbins, single):
+    """the façade's extractPowerSpectrum + writePowerSpectrum (-spectra) against
+    the Python binding's density spectrum of the final particles from -out:
+    the text table of writePowerSpectrum, k / L, P / (N^6 2 pi^2 Omega_m^2),
+    sigma(P) / sqrt(count), one line per occupied bin"""
+    from gevpm.spectra import density_power
+    lines = open(path).read().splitlines()
+    assert lines[0] == "# power spectrum of delta_N" and lines[1].startswith("# redshift z=")
+    assert lines[2].split() == ["#", "k", "Pk", "sigma(k)", "sigma(Pk)", "count"]
+    tab = np.array([[float(x) for x in ln.split()] for ln in lines[3:]])
+    ctx = gevpm.Context(N, gevpm.GEV_FP64)
+    p = gevpm.Particles(ctx, N ** 3, 0.31 / N ** 3)
+    p.load(pos, q, pid)
+    kb, pw, ks, ps, occ = (t.numpy() for t in density_power(ctx, p, numbins))
+    sel = occ > 0
+    rescalep = float(N ** 3) ** 2 * 2 * np.pi ** 2 * 0.31 ** 2
+    assert len(tab) == sel.sum() and np.array_equal(tab[:, 4].astype(np.int64), occ[sel])
+    tol = 1e-4 if single else 1e-5  # %e prints 7 digits; the mixed run deposits in FP32
+    assert np.allclose(tab[:, 0], kb[sel] / L, rtol=1e-5)
+    assert np.allclose(tab[:, 1], pw[sel] / rescalep, rtol=tol)
+    assert np.allclose(tab[:, 3], ps[sel] / rescalep / np.sqrt(occ[sel]), rtol=1e-3, atol=1e-3 * tab[:, 3].max())
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("newton,single", [(1, False), (0, False), (0, True)])
 def test_driver_matches_gev_step(tmp_path, newton, single):
@@ -82,14 +105,15 @@ def test_driver_matches_gev_step(tmp_path, newton, single):
     snap, restart = str(tmp_path / "snap_cdm"), str(tmp_path / "restart")
     exe = os.path.join(LIB, "main_pm_single" if single else "main_pm")
     cmd = [exe, "-n", str(N), "-L", str(L), "-zin", str(z_in), "-zend", str(z_end), "-steps", str(steps),
-           "-seed", str(seed), "-pk", mpk, "-out", out, "-gadget", snap, "-express", restart] + \
-        (["-newton"] if newton else [])
+           "-seed", str(seed), "-pk", mpk, "-out", out, "-gadget", snap, "-express", restart,
+           "-spectra", str(tmp_path / "pk"), "-numbins", "20"] + (["-newton"] if newton else [])
     print("running", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     pid, pos, q = _read_out(out)
     assert np.array_equal(pid, np.arange(N ** 3))
     _check_files(snap, restart + ".0", pid, pos, q, L, single)
+    _check_spectrum(str(tmp_path / "pk_deltaN.dat"), gevpm, pid, pos, q, N, L, 20, single)
 
     # the same run through gev_step (fused kick+drift, one re-binning)
     fpg = gevpm.fourpiG_of(L)

@@ -411,8 +411,8 @@ OR_API void or_prepare_source_Tij(int N, const double *phi, const double *Tij, d
 }
 
 /* ------------------------------------------------------------------------
- * Particle helpers.  LATfield2's dispatch (SURVEY call-site inference,
- * gevolution.hpp:1339,1358 and Particles_gevolution.hpp:517-528): the particle
+ * Particle helpers.  LATfield2's dispatch, as its call sites use it
+ * (gevolution.hpp:1339,1358 and Particles_gevolution.hpp:517-528): the particle
  * sits in cell c = floor(pos N) and ref_dist = (pos - c dx)/dx in [0,1).
  * ------------------------------------------------------------------------ */
 static inline void cell_of(int N, const double *pos, long np, long i, long c[3], double r[3])

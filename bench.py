@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--adj-modes", type=int, default=64)
     ap.add_argument("--cpu-mesh", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time the mixed-form adjoint workloads (no FP64 gradient, no finite differences), "
+                         "print their JSON and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB fill and three 1 GiB copies (the reference kernels that scale the "
                          "FETCH_SIZE / WRITE_SIZE counter readings to bytes), then exit")
@@ -73,6 +76,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(args, checks=False)), flush=True)
         return
     import numpy as np
     import torch
@@ -307,7 +313,7 @@ def kernel_roofline(ctx, n, phases):
             "algorithmic_bytes_per_launch": bytes_per_launch, "bytes_model": desc}
 
 
-def adjoint_leg(args):
+def adjoint_leg(args, checks=True):
     """Differentiable Newtonian run at adj-mesh^3, L = 2 adj-mesh Mpc/h, adj-steps
     cycles z=50 -> 0, loss = sum (rho_final - rho_target)^2 with the target from
     another seed.  Reports the backward / forward time ratio of the mixed form
@@ -385,6 +391,8 @@ def adjoint_leg(args):
     log(f"adjoint {N5}^3 checkpointed: {out['checkpointed']}")
     del run5, rho5, t5, psi5
     torch.cuda.empty_cache()
+    if not checks:
+        return out
     run64, target, g64, f64_ms, b64_ms = run_grad(gevpm.GEV_FP64)
     out.update(fp64_forward_ms=f64_ms, fp64_backward_ms=b64_ms,
                fp32_vs_fp64_grad_rel_l2=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)))

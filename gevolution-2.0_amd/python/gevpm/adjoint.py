@@ -112,11 +112,14 @@ class PMRun:
                 for k in range(k0, k1 - 1):
                     self._cycle(k)
                     phis[k] = self.phi.clone()
-                    states[k + 1] = p.save()
+                    if k + 1 < k1 - 1:
+                        states[k + 1] = p.save()
+                # the container now holds the start state of cycle k1 - 1
             for k in range(k1 - 1, k0 - 1, -1):
                 s = self.sched[k]
                 K = s["dtau_k"] * s["a"] * self.N
-                p.restore(states[k])
+                if k in states:  # the backward kernels leave the particles as they are
+                    p.restore(states[k])
                 phi = phis.pop(k, None)
                 if phi is None:
                     p.project_mass(self.rho)

@@ -229,19 +229,22 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(int32_t *__restrict__ cn
     }
 }
 
-// Re-binning in two passes: k_perm scatters only the old slot of each
-// particle to its new one (start of its cell + its rank), then k_gather copies
-// every particle to its new slot with coalesced stores.  Particles mostly keep
+// Re-binning in two passes: k_perm scatters only the old slot and the cell
+// key of each particle to its new slot (start of its cell + its rank), then
+// k_gather copies the rest of every particle to its new slot with coalesced
+// stores.  Particles mostly keep
 // their place, so the gathered loads stay close to sequential.  Scattering the
 // 36 or 60 bytes of each particle directly breaks the stores of a wave into
 // partial lines and measured about 1.5x slower.
 __global__ __launch_bounds__(PT_NT) void k_perm(const int32_t *__restrict__ cell, const int32_t *__restrict__ start,
                                                 const int32_t *__restrict__ rank, int32_t *__restrict__ perm,
-                                                int64_t n)
+                                                int32_t *__restrict__ cell_out, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * PT_NT + threadIdx.x;
     if (i >= n) return;
-    perm[start[cell[i]] + rank[i]] = (int32_t)i;
+    const int32_t c = cell[i], j = start[c] + rank[i];
+    perm[j] = (int32_t)i;
+    cell_out[j] = c;
 }
 
 template <typename T>
@@ -253,7 +256,6 @@ __global__ __launch_bounds__(PT_NT) void k_gather(PV<T> s, PV<T> d, const int32_
     d.p0[j] = s.p0[i]; d.p1[j] = s.p1[i]; d.p2[j] = s.p2[i];
     d.q0[j] = s.q0[i]; d.q1[j] = s.q1[i]; d.q2[j] = s.q2[i];
     d.id[j] = s.id[i];
-    d.cell[j] = s.cell[i];
 }
 
 // displace_pcls_ic_basic, ic_basic.hpp:65-96 (one field)
@@ -343,7 +345,8 @@ template <typename T> int pcls_sort_counted_impl(gev_pcls *p)
         dst.q0 = (T *)p->q2[0]; dst.q1 = (T *)p->q2[1]; dst.q2 = (T *)p->q2[2];
         dst.id = p->id2;
         dst.cell = p->cell2;
-        hipLaunchKernelGGL(k_perm, dim3(pblocks(n)), dim3(PT_NT), 0, s, p->cell, p->cell_start, p->rank, p->perm, n);
+        hipLaunchKernelGGL(k_perm, dim3(pblocks(n)), dim3(PT_NT), 0, s, p->cell, p->cell_start, p->rank, p->perm,
+                           p->cell2, n);
         hipLaunchKernelGGL((k_gather<T>), dim3(pblocks(n)), dim3(PT_NT), 0, s, src, dst, p->perm);
         for (int d = 0; d < 3; d++) {
             std::swap(p->pos[d], p->pos2[d]);

@@ -18,23 +18,38 @@
  *   - N must be a power of two.
  *
  * What pins each function (tests/test_oracle_*.py, tests/test_abi.py); every
- * check is fixed by mathematics or data independent of this file:
+ * check is fixed by mathematics or data independent of this file, and every
+ * function below is pinned (a swapped weight, corner, factor or conjugate in
+ * any of them fails at least one check):
  *   or_fft_r2c / or_fft_c2r   brute-force DFT (N=4,8), numpy.fft fixtures
  *                             (N=8,16), Parseval + round trip, c2r ignoring
  *                             Im of the kx=0 / N/2 planes
  *   or_poisson                7-point Laplacian of the result = source (+modif)
- *   or_project_ft_scalar      longitudinal / pure-trace tensors, closed forms
+ *   or_project_ft_scalar      longitudinal / pure-trace tensors, closed forms;
+ *                             on a random tensor the real-space identity
+ *                             Lap^2 chi = (3 DD S - Lap tr S) / (2 N^3) with the
+ *                             staggered double divergence DD
  *   or_project_ft_vector      inverse of (-Lap + modif) on transverse input,
  *                             D- transversality of the output
- *   or_evolve_ft_vector       PARTIALLY PINNED: trace/longitudinal inputs leave
- *                             B unchanged, increment is transverse; no closed
- *                             form fixes the transverse amplitude
- *   or_prepare_source_T00/Tij hand-evaluated stencils, phi = 0 limits
+ *   or_evolve_ft_vector       on a random tensor -Lap dB = -(2 a^2 dtau / N)
+ *                             T(div S), with the staggered divergence onto the
+ *                             links and the discrete Helmholtz projection T;
+ *                             trace/longitudinal inputs leave B unchanged
+ *   or_prepare_source_T00/Tij hand-evaluated stencils along every axis and axis
+ *                             pair, phi = 0 limits
  *   or_project_T00/T0i/Tij    sum rules (energy, momentum, stress), single
- *                             particle weights, staggering by symmetry, lattice
- *   or_update_q[_newton]      linear potential (exact force), Newton limit of
- *                             the GR kick, chi and B terms by construction
- *   or_update_pos[_newton]    free streaming and uniform-field closed forms
+ *                             particle weights per site and per link, the
+ *                             per-corner / per-link phi factors on random phi,
+ *                             linear-phi closed forms (incl. the quadratic T00
+ *                             term), and the point-particle stress tensor of
+ *                             the weak-field metric expanded to O(phi^2) (T00)
+ *                             and O(phi) (Tij)
+ *   or_update_q[_newton]      random potentials against tensor products of 1-D
+ *                             lattice stencils (central and link gradients, all
+ *                             nine B_c d_d terms), separable closed forms, the
+ *                             weak-field geodesic equation to O(phi grad phi)
+ *   or_update_pos[_newton]    free streaming, random staggered B, and the
+ *                             geodesic velocity to O(phi, chi)
  *   or_displace_particles /
  *   or_initialize_q           closed forms on single modes
  *   or_hconf / or_rk4bg /
@@ -308,7 +323,8 @@ OR_API void or_project_ft_vector(int N, const double *Sd, double *Bd, double coe
 }
 
 /* evolveFTvector, gevolution.hpp:359-429 (parabolic vector method).
- * Pinned by: transversality of the increment, pure-trace source gives zero. */
+ * Pinned by: transversality of the increment, pure-trace source gives zero,
+ * and -Lap dB = -(2 a2dtau / N) T(div S) on a random tensor (tests). */
 OR_API void or_evolve_ft_vector(int N, const double *Sd, double *Bd, double a2dtau)
 {
     const cplx *S = (const cplx *)Sd;
@@ -461,7 +477,9 @@ typedef struct { double *out; const double *phi; double a, mass; } dep_ctx;
 /* particle_T00_project host branch, gevolution.hpp:1729-1790 (GR, nfield=2)
  * and the nfield=1 branch (:1819-1829), which is also the Newtonian
  * meshprojection_project mass density (gevolution.hpp:1902-1905).
- * Pinned by: one-particle exact weights, sum rule sum(T00) = sum m'(e+...),
+ * Pinned by: one-particle exact weights and per-corner phi polynomial, sum
+ * rule sum(T00) = sum m'(e+...), linear-phi closed form of the phi^2 term, the
+ * point-particle a^3 T00 of the weak-field metric to O(phi^2),
  * uniform lattice at rest -> uniform field (tests). */
 static void dep_T00(int N, long np, long i, const double *pos, const double *q, void *vctx)
 {
@@ -506,7 +524,8 @@ OR_API void or_project_T00(int N, long np, const double *pos, const double *q, d
 
 /* particle_T0i_project host branch, gevolution.hpp:2031-2105 (12 staggered
  * deposits, component i lives on the link x + e_i/2).  Pinned by: total
- * momentum sum rule, single-particle exact weights (tests). */
+ * momentum sum rule, single-particle exact weights and per-link phi factors,
+ * the linear-phi sum rule at the link midpoints (tests). */
 static void dep_T0i(int N, long np, long i, const double *pos, const double *q, void *vctx)
 {
     dep_ctx *c = (dep_ctx *)vctx;
@@ -576,7 +595,9 @@ OR_API void or_project_T0i(int N, long np, const double *pos, const double *q, d
  * gevolution.hpp:2582-2604, 2681-2775, 2778-2812, 2957-2962: diagonal parts
  * by full CIC (24 deposits), off-diagonal ij on the edge x + (e_i+e_j)/2
  * split over the remaining direction (6 deposits).  Pinned by: trace sum rule
- * sum(Tii) = sum f m' q^2/e, single-particle exact weights (tests). */
+ * sum(Tii) = sum f m' q^2/e, single-particle exact weights with and without
+ * phi, the linear-phi sum rule, the point-particle a^3 Tij of the weak-field
+ * metric to O(phi) (tests). */
 static void dep_Tij(int N, long np, long i, const double *pos, const double *q, void *vctx)
 {
     dep_ctx *c = (dep_ctx *)vctx;
@@ -689,7 +710,8 @@ static double cic_interp(int N, const double *f, long x, long y, long z, const d
  * ANISOTROPIC_EXPANSION.  params[0]=a, params[1]=bcoeff (= a^2 N, main.cu:1227).
  * Returns the maximum over particles of the functor's return value v2/a^2.
  * Pinned by: uniform fields -> free streaming, linear potential -> exact
- * constant force, Newtonian limit (tests). */
+ * constant force, Newtonian limit, random B against the staggered stencils,
+ * the weak-field geodesic equation to O(phi grad phi) (tests). */
 OR_API double or_update_q(int N, long np, const double *pos, double *q, double dtau, const double *phi,
                           const double *chi, const double *Bi, int nfield, double a, double bcoeff)
 {
@@ -799,7 +821,9 @@ static inline double wrap_pos(double p)
 }
 
 /* update_pos (GR drift), gevolution.hpp:1162-1232.  params {a, a^2 N}.
- * Pinned by: free streaming v = q/sqrt(q^2+a^2) with zero fields (tests). */
+ * Pinned by: free streaming v = q/sqrt(q^2+a^2) with zero fields, random B
+ * against the staggered interpolation, the geodesic velocity to O(phi, chi)
+ * (tests). */
 OR_API void or_update_pos(int N, long np, double *pos, const double *q, double dtau, const double *phi,
                           const double *chi, const double *Bi, int nfield, double a, double bcoeff)
 {

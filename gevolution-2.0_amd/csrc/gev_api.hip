@@ -215,19 +215,21 @@ int gev_pcls_load(gev_pcls *p, int64_t n, const double *pos, const double *q, co
     if (n == 0) return PREC_DISPATCH(c, pcls_sort_impl<double>(p), pcls_sort_impl<float>(p));
     const double *dpos = pos, *dq = q;
     const int64_t *did = id;
-    double *tmp = nullptr;
+    // device staging for the host path, freed on every return (hipFree waits
+    // for the work queued on it)
+    struct Staging {
+        double *p = nullptr;
+        ~Staging() { if (p) (void)hipFree(p); }
+    } tmp;
     if (!on_device) {
-        GEV_HIP(hipMalloc(&tmp, sizeof(double) * 7 * n));
-        GEV_HIP(hipMemcpyAsync(tmp, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-        GEV_HIP(hipMemcpyAsync(tmp + 3 * n, q, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-        GEV_HIP(hipMemcpyAsync(tmp + 6 * n, id, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream));
-        dpos = tmp; dq = tmp + 3 * n; did = (const int64_t *)(tmp + 6 * n);
+        GEV_HIP(hipMalloc(&tmp.p, sizeof(double) * 7 * n));
+        GEV_HIP(hipMemcpyAsync(tmp.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+        GEV_HIP(hipMemcpyAsync(tmp.p + 3 * n, q, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+        GEV_HIP(hipMemcpyAsync(tmp.p + 6 * n, id, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream));
+        dpos = tmp.p; dq = tmp.p + 3 * n; did = (const int64_t *)(tmp.p + 6 * n);
     }
     int rc = PREC_DISPATCH(c, pcls_load_impl<double>(p, dpos, dq, did), pcls_load_impl<float>(p, dpos, dq, did));
-    if (tmp) {
-        GEV_HIP(hipStreamSynchronize(c->stream));
-        GEV_HIP(hipFree(tmp));
-    }
+    if (tmp.p && rc == GEV_OK) GEV_HIP(hipStreamSynchronize(c->stream));
     return rc;
 }
 
@@ -239,19 +241,23 @@ int gev_pcls_store(gev_pcls *p, double *pos, double *q, int64_t *id, int on_devi
     if (n == 0) return GEV_OK;
     double *dpos = pos, *dq = q;
     int64_t *did = id;
-    double *tmp = nullptr;
+    // host staging for the host path, freed on every return (hipFree waits for
+    // the work queued on it)
+    struct Staging {
+        double *p = nullptr;
+        ~Staging() { if (p) (void)hipFree(p); }
+    } tmp;
     if (!on_device) {
-        GEV_HIP(hipMalloc(&tmp, sizeof(double) * 7 * n));
-        dpos = tmp; dq = tmp + 3 * n; did = (int64_t *)(tmp + 6 * n);
+        GEV_HIP(hipMalloc(&tmp.p, sizeof(double) * 7 * n));
+        dpos = tmp.p; dq = tmp.p + 3 * n; did = (int64_t *)(tmp.p + 6 * n);
     }
     int rc = PREC_DISPATCH(c, pcls_store_impl<double>(p, dpos, dq, did), pcls_store_impl<float>(p, dpos, dq, did));
     if (rc) return rc;
-    if (tmp) {
+    if (tmp.p) {
         GEV_HIP(hipMemcpyAsync(pos, dpos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
         GEV_HIP(hipMemcpyAsync(q, dq, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
         GEV_HIP(hipMemcpyAsync(id, did, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
         GEV_HIP(hipStreamSynchronize(c->stream));
-        GEV_HIP(hipFree(tmp));
     }
     return GEV_OK;
 }

@@ -1,14 +1,18 @@
 // gev_particles.hip -- particle container kernels for CDNA4:
-//   * re-binning: counting sort by cell (rank via atomic count, 3-phase
-//     exclusive scan, scatter into ping-pong buffers);
-//   * CIC projections of T00 / T0i / Tij as a two-phase gather: per tile of
-//     sites, every cell of the tile plus its lower halo walks its (cell-sorted)
-//     particles and accumulates the per-corner contributions in registers,
-//     parks them in LDS, then each site sums the 8 (or 4, 2) cells that touch
-//     it and issues one plain coalesced store.  No global atomics, no zeroing
-//     pass, and the result does not depend on thread scheduling;
-//   * kick / drift / IC gathers: one thread per particle in cell order, so a
-//     wavefront's stencil reads hit the same cache lines.
+//   * load / store / cell-centred lattice of the container (FP64 absolute
+//     positions in, FP64 or FP32 cell offsets plus an int32 cell key inside);
+//   * re-binning by cell: k_count ranks each particle in its cell with one
+//     counter atomic (the fused kick/drift in gev_pupdate.hip takes the same
+//     ranks itself), a three-kernel exclusive scan turns the counts into
+//     cell_start and clears the counters, k_perm writes each particle's old
+//     slot and cell key at its new slot, and k_gather copies the rest of the
+//     particle into the ping-pong arrays with coalesced stores;
+//   * the IC displacement and velocity gathers (one thread per particle in
+//     cell order).
+// The order of particles inside a cell follows the counter atomics, so it can
+// change from run to run; IDs, cell keys and cell_start do not.  The CIC
+// deposits live in gev_deposit.hip (LDS atomics: their sums are not
+// bit-reproducible from run to run) and the kick/drift in gev_pupdate.hip.
 #include "gev_common.hpp"
 #include "gev_kernels.hpp"
 #include "gev_pcommon.hpp"

@@ -223,8 +223,12 @@ __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const
         }
         pn = locate(tid + (it + 1) * NT);
         fetch();
+        // the stand-in particle's return value is dropped, so maxvel is the
+        // maximum over real particles only (a select, no branch)
+        double rit = 0.0;
         const int32_t newc =
-            upd_particle<T, F, SX, SY>(v, u, active ? p : v.sink, cc, xk, q, fld[0], x0, y0, z0, NS, ret);
+            upd_particle<T, F, SX, SY>(v, u, active ? p : v.sink, cc, xk, q, fld[0], x0, y0, z0, NS, rit);
+        ret = active ? fmax(ret, rit) : ret;
         // re-binning rank in the new cell, one counter atomic per run of equal
         // cells; each pass stores the rank of the previous one, so the
         // atomic's round trip overlaps a pass of work

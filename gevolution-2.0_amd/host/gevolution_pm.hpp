@@ -6,11 +6,16 @@
 //   Field<Real> / Field<Cplx>     LATfield2 Field in real / Fourier layout
 //   PlanFFT<Cplx>                 LATfield2 PlanFFT (execute FFT_FORWARD / FFT_BACKWARD)
 //   perfParticles_gevolution<>    Particles_gevolution.hpp:176-211 (+ perfParticles base:
+//                                 initialize, addParticle_global, updateRowBuffers,
+//                                 projectParticles[_Async], meshprojection_project,
 //                                 updateVel, moveParticles, num_particles, sampleParticles,
 //                                 saveGadget2, loadGadget2, saveExpress, loadExpress)
-//   projection_* / prepareFTsource / solveModifiedPoissonFT /
+//   projection_* (+ _Async, _comm) / prepareFTsource / solveModifiedPoissonFT /
 //   projectFTscalar / projectFTvector / evolveFTvector
 //                                 gevolution.hpp free functions, same defaults
+//   loadHomogeneousTemplate / initializeParticlePositions
+//                                 ic_basic.hpp:205-330, 1612-1644
+//   DeviceStagingBuffer<T>        cuda_staging.hpp:18-63 (parameter arrays)
 //   Hconf / rungekutta4bg / particleHorizon
 //                                 background.hpp:200-290
 //   extractPowerSpectrum / writePowerSpectrum
@@ -29,6 +34,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -107,6 +113,11 @@ template <typename T> class Field {
     int64_t count() const { return n_; }
     Lattice &lattice() const { return *lat_; }
 
+    // Field::updateHalo (main.cu:802,902,955): the fields here have no ghost
+    // layers (the kernels index the periodic lattice directly), so there is
+    // nothing to exchange; kept so that main.cu's call sequence compiles as is
+    void updateHalo() {}
+
     void to_host(std::vector<T> &h) const
     {
         lat_->sync();
@@ -146,6 +157,30 @@ template <typename T> class PlanFFT {
     Field<Real> *r_;
     Field<T> *k_;
     int ncomp_;
+};
+
+// ---------------------------------------------------- DeviceStagingBuffer
+// cuda_staging.hpp:18-63: the reference copies small parameter arrays to the
+// device before each functor launch.  The C ABI takes FP64 scalars by value,
+// so the array stays in host memory here and data() returns it; the
+// interface (count, copy_from_host, data) is the reference's, so main.cu's
+// d_f_params.copy_from_host(f_params, 7) / d_f_params.data() compile as they are.
+template <typename T> class DeviceStagingBuffer {
+  public:
+    DeviceStagingBuffer() = default;
+    explicit DeviceStagingBuffer(size_t count) : v_(count) {}
+    DeviceStagingBuffer(const T *host_data, size_t count) : v_(count) { copy_from_host(host_data, count); }
+    T *data() { return v_.data(); }
+    const T *data() const { return v_.data(); }
+    size_t size() const { return v_.size(); }
+    void copy_from_host(const T *host_data, size_t count)
+    {
+        if (count > v_.size()) throw std::runtime_error("DeviceStagingBuffer::copy_from_host: count exceeds capacity");
+        std::copy(host_data, host_data + count, v_.begin());
+    }
+
+  private:
+    std::vector<T> v_;
 };
 
 // ------------------------------------------------------------- cosmology
@@ -239,11 +274,18 @@ inline int findIntersectingLightcones(lightcone_geometry &lightcone, double oute
     return n;
 }
 
-// functor tags: which gevolution functor updateVel / moveParticles apply
+// functor tags: which gevolution functor updateVel / moveParticles /
+// projectParticles apply
 struct update_q_functor {};         // gevolution.hpp:888-1011
 struct update_q_Newton_functor {};  // gevolution.hpp:1051-1118
 struct update_pos_functor {};       // gevolution.hpp:1162-1232
 struct update_pos_Newton_functor {};  // gevolution.hpp:1270-1273
+struct particle_T00_project_functor {};  // gevolution.hpp:1729-1854, params {a, coeff}
+struct particle_T0i_project_functor {};  // gevolution.hpp:2031-2201, params {coeff}
+struct particle_Tij_project_functor {};  // gevolution.hpp:2582-2965, params {a, coeff, hij_hom[5]}
+struct displace_pcls_ic_basic_functor {};  // ic_basic.hpp:65-109, dtau = coeff
+struct initialize_q_ic_basic_functor {};   // ic_basic.hpp:133-178, dtau = coeff
+enum { SUM = 1, MIN = 2, MAX = 4 };  // reduction ops of the outputs tail (LATfield2's)
 
 template <class part = part_simple, class part_info = part_simple_info> class perfParticles_gevolution {
   public:
@@ -255,37 +297,61 @@ template <class part = part_simple, class part_info = part_simple_info> class pe
     perfParticles_gevolution(const perfParticles_gevolution &) = delete;
     perfParticles_gevolution &operator=(const perfParticles_gevolution &) = delete;
 
-    // perfParticles::initialize (ic_basic.hpp:2206); the box is the unit box
+    // perfParticles::initialize (ic_basic.hpp:2206, unit_tests.cu:541).
+    // gevolution's particle coordinates are in box units, so boxSize must be
+    // the unit box; capacity already includes the `extra` slack the reference
+    // keeps for its row buffers (the container here re-bins in place).
+    void initialize(const part_info &info, Lattice *lat, Real boxSize[3], uint64_t capacity, uint64_t extra)
+    {
+        for (int d = 0; d < 3; d++)
+            if (boxSize[d] != (Real)1)
+                throw std::runtime_error("perfParticles::initialize: particle coordinates are in box units (boxSize = 1)");
+        if (extra > capacity) throw std::runtime_error("perfParticles::initialize: extra exceeds capacity");
+        initialize(info, lat, capacity);
+    }
     void initialize(const part_info &info, Lattice *lat, uint64_t capacity)
     {
+        if (p_) throw std::runtime_error("perfParticles::initialize: already initialised");
         lat_ = lat;
         info_ = info;
         check(gev_pcls_create(lat->ctx(), (int64_t)capacity, info.mass, &p_), "perfParticles::initialize");
     }
-    // addParticle_global for a whole set + updateRowBuffers (ic_basic.hpp:1639,2211)
-    void addParticles_global(const std::vector<part> &ps)
+    // addParticle_global (ic_basic.hpp:1639): the particle is queued on the
+    // host; updateRowBuffers() (ic_basic.hpp:2211) hands the queue to the
+    // device container and re-bins it.  A particle outside [0,1) is wrapped
+    // into the box.
+    void addParticle_global(const part &p) { pending_.push_back(p); }
+    void addParticles_global(const std::vector<part> &ps) { pending_.insert(pending_.end(), ps.begin(), ps.end()); }
+    void updateRowBuffers()
     {
-        const size_t n = ps.size();
-        std::vector<double> pos(3 * n), q(3 * n);
-        std::vector<int64_t> id(n);
-        for (size_t i = 0; i < n; i++) {
-            for (int d = 0; d < 3; d++) {
-                pos[d * n + i] = ps[i].pos[d];
-                q[d * n + i] = ps[i].vel[d];
-            }
-            id[i] = ps[i].ID;
-        }
-        check(gev_pcls_load(p_, (int64_t)n, pos.data(), q.data(), id.data(), 0), "addParticle_global");
+        if (!flush()) check(gev_pcls_sort(p_), "updateRowBuffers");
     }
-    void updateRowBuffers() { check(gev_pcls_sort(p_), "updateRowBuffers"); }
 
     int64_t num_particles()
     {
+        flush();
         gev_pcls_view v;
         check(gev_pcls_view_get(p_, &v), "num_particles");
         return v.n;
     }
-    // sampleParticles (unit_tests.cu:537): every particle, FP64, host memory
+    // sampleParticles (unit_tests.cu:537): the first n particles in container
+    // order (at most num_particles()), three values per particle in the order
+    // of part_simple's pos[3] / vel[3]; returns how many were written
+    int64_t sampleParticles(Real *pos, Real *vel, long *id, uint64_t n)
+    {
+        std::vector<part> ps;
+        sampleParticles(ps);
+        const int64_t m = std::min<int64_t>((int64_t)n, (int64_t)ps.size());
+        for (int64_t i = 0; i < m; i++) {
+            for (int d = 0; d < 3; d++) {
+                pos[3 * i + d] = (Real)ps[i].pos[d];
+                vel[3 * i + d] = (Real)ps[i].vel[d];
+            }
+            id[i] = (long)ps[i].ID;
+        }
+        return m;
+    }
+    // every particle, FP64, host memory
     void sampleParticles(std::vector<part> &out)
     {
         const int64_t n = num_particles();
@@ -302,34 +368,113 @@ template <class part = part_simple, class part_info = part_simple_info> class pe
         }
     }
 
-    // updateVel: applies the kick functor and returns sqrt of the maximum of
-    // the functor values (v^2/a^2), as main.cu:1232 then converts it to maxvel
-    Real updateVel(update_q_functor, double dtau, Field<Real> **fields, int nfield, const double *params)
+    // projectParticles / projectParticles_Async (gevolution.hpp:1898, 2244,
+    // 3027): the functor's deposit is added to fields[0]; fields[1] is phi
+    // when nfield == 2.  Both forms are ordered on the lattice's stream; the
+    // _Async form returns without waiting, as the reference's does.
+    void projectParticles(particle_T00_project_functor, Field<Real> **fields, int nfield, const double *params)
     {
+        flush();
+        check(gev_project_T00(p_, fields[0]->data(), params[0], nfield >= 2 ? fields[1]->data() : nullptr, params[1], 1),
+              "projectParticles(T00)");
+    }
+    void projectParticles(particle_T0i_project_functor, Field<Real> **fields, int nfield, const double *params)
+    {
+        flush();
+        check(gev_project_T0i(p_, fields[0]->data(), nfield >= 2 ? fields[1]->data() : nullptr, params[0], 1),
+              "projectParticles(T0i)");
+    }
+    void projectParticles(particle_Tij_project_functor, Field<Real> **fields, int nfield, const double *params)
+    {
+        flush();
+        check(gev_project_Tij(p_, fields[0]->data(), params[0], nfield >= 2 ? fields[1]->data() : nullptr, params[1], 1),
+              "projectParticles(Tij)");
+    }
+    template <class F> void projectParticles_Async(F f, Field<Real> **fields, int nfield, const double *params)
+    {
+        projectParticles(f, fields, nfield, params);
+    }
+    // meshprojection_project (gevolution.hpp:1904): adds the CIC mass density
+    void meshprojection_project(Field<Real> *field)
+    {
+        flush();
+        check(gev_project_mass(p_, field->data(), 1.0, 1), "meshprojection_project");
+    }
+
+    // updateVel: applies the kick functor and returns sqrt of the maximum of
+    // the functor values (v^2/a^2), as main.cu:1232 then converts it to
+    // maxvel.  The kick and drift functors write no outputs, so the reduction
+    // tail (outputs, ops, nout) leaves outputs as they are.
+    Real updateVel(update_q_functor, double dtau, Field<Real> **fields, int nfield, const double *params,
+                   double *outputs = nullptr, int *ops = nullptr, int nout = 0)
+    {
+        (void)outputs, (void)ops, (void)nout;
+        flush();
         double m = 0;
         check(gev_update_q(p_, dtau, fields[0]->data(), nfield >= 2 ? fields[1]->data() : nullptr,
                            nfield >= 3 ? fields[2]->data() : nullptr, nfield, params[0], params[1], &m),
               "updateVel(update_q)");
         return (Real)std::sqrt(m);
     }
-    Real updateVel(update_q_Newton_functor, double dtau, Field<Real> **fields, int nfield, const double *params)
+    Real updateVel(update_q_Newton_functor, double dtau, Field<Real> **fields, int nfield, const double *params,
+                   double *outputs = nullptr, int *ops = nullptr, int nout = 0)
     {
+        (void)outputs, (void)ops, (void)nout;
+        flush();
         double m = 0;
         check(gev_update_q_newton(p_, dtau, fields[0]->data(), nfield >= 2 ? fields[1]->data() : nullptr, nfield,
                                   params[0], &m),
               "updateVel(update_q_Newton)");
         return (Real)std::sqrt(m);
     }
-    void moveParticles(update_pos_functor, double dtau, Field<Real> **fields, int nfield, const double *params)
+    // initialize_q_ic_basic (ic_basic.hpp:133-178): q = -coeff grad(fields[0]),
+    // coeff = dtau; returns sqrt of the maximum q^2 (one potential only: the
+    // two-potential hybrid baryon form is not built)
+    Real updateVel(initialize_q_ic_basic_functor, double dtau, Field<Real> **fields, int nfield,
+                   const double *params = nullptr, double *outputs = nullptr, int *ops = nullptr, int nout = 0)
     {
+        (void)params, (void)outputs, (void)ops, (void)nout;
+        if (nfield != 1) throw std::runtime_error("updateVel(initialize_q_ic_basic): one velocity potential only");
+        flush();
+        double m = 0;
+        check(gev_initialize_q(p_, fields[0]->data(), dtau, &m), "updateVel(initialize_q_ic_basic)");
+        return (Real)std::sqrt(m);
+    }
+    Real updateVel(initialize_q_ic_basic_functor f, double dtau, Field<Real> *field, int nfield)
+    {
+        Field<Real> *fl[1] = {field};
+        return updateVel(f, dtau, fl, nfield);
+    }
+    void moveParticles(update_pos_functor, double dtau, Field<Real> **fields, int nfield, const double *params,
+                       double *outputs = nullptr, int *ops = nullptr, int nout = 0)
+    {
+        (void)outputs, (void)ops, (void)nout;
+        flush();
         check(gev_update_pos(p_, dtau, nfield >= 1 ? fields[0]->data() : nullptr,
                              nfield >= 2 ? fields[1]->data() : nullptr, nfield >= 3 ? fields[2]->data() : nullptr,
                              nfield, params[0], params[1]),
               "moveParticles(update_pos)");
     }
-    void moveParticles(update_pos_Newton_functor, double dtau, Field<Real> **, int, const double *params)
+    void moveParticles(update_pos_Newton_functor, double dtau, Field<Real> **, int, const double *params,
+                       double *outputs = nullptr, int *ops = nullptr, int nout = 0)
     {
+        (void)outputs, (void)ops, (void)nout;
+        flush();
         check(gev_update_pos_newton(p_, dtau, params[0]), "moveParticles(update_pos_Newton)");
+    }
+    // displace_pcls_ic_basic (ic_basic.hpp:65-109): pos += coeff grad(fields[0])
+    // with coeff = dtau; outputs[0] receives the maximum displacement when
+    // nout > 0 (the reference reduces it with op MAX, ic_basic.hpp:2214-2219)
+    void moveParticles(displace_pcls_ic_basic_functor, double dtau, Field<Real> **fields, int nfield,
+                       const double *params = nullptr, double *outputs = nullptr, int *ops = nullptr, int nout = 0)
+    {
+        (void)params;
+        if (nfield != 1) throw std::runtime_error("moveParticles(displace_pcls_ic_basic): one displacement field only");
+        if (nout > 0 && ops && ops[0] != MAX) throw std::runtime_error("moveParticles(displace_pcls_ic_basic): MAX only");
+        flush();
+        double m = 0;
+        check(gev_displace_particles(p_, fields[0]->data(), dtau, &m), "moveParticles(displace_pcls_ic_basic)");
+        if (nout > 0 && outputs) outputs[0] = m;
     }
 
     // Particle files (Particles_gevolution.hpp:170-186): Gadget-2 snapshot,
@@ -367,17 +512,117 @@ template <class part = part_simple, class part_info = part_simple_info> class pe
         check(gev_pcls_load_express(p_, filename.c_str(), &hdr), "loadExpress");
     }
 
-    gev_pcls *handle() const { return p_; }
+    gev_pcls *handle()
+    {
+        flush();
+        return p_;
+    }
     Lattice *lattice() const { return lat_; }
     const part_info &parts_info() const { return info_; }
 
   private:
+    // hands the queued addParticle_global particles to the device container
+    // (after the particles it already holds) and re-bins; true if it did
+    bool flush()
+    {
+        if (!p_) throw std::runtime_error("perfParticles: not initialised");
+        if (pending_.empty()) return false;
+        gev_pcls_view v;
+        check(gev_pcls_view_get(p_, &v), "addParticle_global");
+        const int64_t n0 = v.n, n = n0 + (int64_t)pending_.size();
+        std::vector<double> pos(3 * n), q(3 * n), op(3 * n0), oq(3 * n0);
+        std::vector<int64_t> id(n), oid(n0);
+        if (n0 > 0) check(gev_pcls_store(p_, op.data(), oq.data(), oid.data(), 0), "addParticle_global");
+        for (int64_t i = 0; i < n; i++) {
+            for (int d = 0; d < 3; d++) {
+                double x, u;
+                if (i < n0) {
+                    x = op[d * n0 + i];
+                    u = oq[d * n0 + i];
+                } else {
+                    x = pending_[i - n0].pos[d];
+                    u = pending_[i - n0].vel[d];
+                    x -= std::floor(x);
+                    if (x >= 1.0) x = 0.0;
+                }
+                pos[d * n + i] = x;
+                q[d * n + i] = u;
+            }
+            id[i] = i < n0 ? oid[i] : pending_[i - n0].ID;
+        }
+        pending_.clear();
+        check(gev_pcls_load(p_, n, pos.data(), q.data(), id.data(), 0), "addParticle_global");
+        return true;
+    }
+
     gev_pcls *p_ = nullptr;
     Lattice *lat_ = nullptr;
     part_info info_{};
+    std::vector<part> pending_;
 };
 
 using Particles = perfParticles_gevolution<part_simple, part_simple_info>;
+
+// loadHomogeneousTemplate (ic_basic.hpp:205-330): the type-1 particle
+// positions of a single-file Gadget-2 template (sc0/sc1/bcc/fcc_crystal.dat),
+// divided by the header's BoxSize so they lie in [0,1].  partdata is
+// allocated with malloc, as the reference does, and freed by the caller.
+// Every failure the reference aborts on throws std::runtime_error here.
+inline void loadHomogeneousTemplate(const char *filename, long &numpart, float *&partdata)
+{
+    struct File {
+        FILE *f;
+        ~File() { if (f) std::fclose(f); }
+    } tf{std::fopen(filename, "rb")};
+    auto fail = [&](const std::string &why) -> void {
+        throw std::runtime_error(std::string("loadHomogeneousTemplate: ") + why + " (" + filename + ")");
+    };
+    if (!tf.f) fail("unable to open template file");
+    int32_t b1 = 0, b2 = 0;
+    gadget2_header hdr;
+    if (std::fread(&b1, 4, 1, tf.f) != 1 || b1 != (int32_t)sizeof(hdr)) fail("header not recognized");
+    if (std::fread(&hdr, sizeof(hdr), 1, tf.f) != 1) fail("unable to read header");
+    if (std::fread(&b2, 4, 1, tf.f) != 1 || b1 != b2) fail("block size mismatch while reading header");
+    if (hdr.num_files != 1) fail("multiple input files are not supported");
+    if (!(hdr.BoxSize > 0.)) fail("BoxSize must be positive");
+    if (hdr.npart[1] == 0) fail("no particles declared");
+    if (std::fread(&b1, 4, 1, tf.f) != 1) fail("unable to read particle data");
+    if (hdr.npart[0] > 0 && std::fseek(tf.f, 3L * 4 * (long)hdr.npart[0], SEEK_CUR)) fail("unable to skip gas particles");
+    std::vector<float> buf(3 * (size_t)hdr.npart[1]);
+    if (std::fread(buf.data(), 4, buf.size(), tf.f) != buf.size()) fail("unable to read particle data");
+    for (int i = 2; i < 6; i++)
+        if (hdr.npart[i] > 0 && std::fseek(tf.f, 3L * 4 * (long)hdr.npart[i], SEEK_CUR)) fail("unable to skip particles");
+    if (std::fread(&b2, 4, 1, tf.f) != 1 || b1 != b2) fail("block size mismatch while reading particles");
+    for (float &x : buf) {
+        x = (float)(x / hdr.BoxSize);
+        if (x < 0.f || x > 1.f) fail("particle data corrupted");
+    }
+    partdata = (float *)std::malloc(sizeof(float) * buf.size());
+    if (!partdata) fail("memory error");
+    std::copy(buf.begin(), buf.end(), partdata);
+    numpart = (long)hdr.npart[1];
+}
+
+// initializeParticlePositions (ic_basic.hpp:1612-1644) on one rank: the
+// template tiled numtile times along each axis, particles at rest, ID =
+// i + numpart (xtile + numtile (ytile + numtile ztile)), added with
+// addParticle_global (call updateRowBuffers() afterwards, as the reference).
+template <typename part_handler>
+void initializeParticlePositions(const long numpart, const float *partdata, const int numtile, part_handler &pcls)
+{
+    part_simple part;
+    part.vel[0] = part.vel[1] = part.vel[2] = 0.0;
+    for (int ztile = 0; ztile < numtile; ztile++)
+        for (int ytile = 0; ytile < numtile; ytile++)
+            for (int xtile = 0; xtile < numtile; xtile++)
+                for (long i = 0; i < numpart; i++) {
+                    part.pos[0] = ((Real)xtile + partdata[3 * i]) / (Real)numtile;
+                    part.pos[1] = ((Real)ytile + partdata[3 * i + 1]) / (Real)numtile;
+                    part.pos[2] = ((Real)ztile + partdata[3 * i + 2]) / (Real)numtile;
+                    part.ID = i + numpart * ((long)xtile + (long)numtile * ((long)ytile + (long)numtile * (long)ztile));
+                    pcls.addParticle_global(part);
+                }
+}
 
 // ------------------------------------------------------------ projections
 // projection_init: zero a field before additive projections
@@ -393,10 +638,7 @@ inline void projection_T00_project(Particles *pcls, Field<Real> *T00, double a =
           "projection_T00_project");
 }
 // gevolution.hpp:1902
-inline void scalarProjectionCIC_project(Particles *pcls, Field<Real> *rho)
-{
-    check(gev_project_mass(pcls->handle(), rho->data(), 1.0, 1), "scalarProjectionCIC_project");
-}
+inline void scalarProjectionCIC_project(Particles *pcls, Field<Real> *rho) { pcls->meshprojection_project(rho); }
 // gevolution.hpp:2232 (functor :2031-2201)
 inline void projection_T0i_project(Particles *pcls, Field<Real> *T0i, Field<Real> *phi = nullptr, double coeff = 1.)
 {
@@ -410,6 +652,30 @@ inline void projection_Tij_project(Particles *pcls, Field<Real> *Tij, double a =
     check(gev_project_Tij(pcls->handle(), Tij->data(), a, phi ? phi->data() : nullptr, coeff, 1),
           "projection_Tij_project");
 }
+
+// projection_T0i_project_Async / projection_Tij_project_Async
+// (gevolution.hpp:2242, 3030; main.cu:756, 858): fields {T0i or Tij, phi},
+// nfield 1 or 2, params {coeff} (T0i) or {a, coeff, ...} (Tij); the deposit is
+// added to fields[0], ordered on the lattice's stream, and the call returns
+// without waiting.  main.cu's cudaDeviceSynchronize() after them is a plain
+// hipDeviceSynchronize() here.
+inline void projection_T0i_project_Async(Particles *pcls, Field<Real> **fields, int nfield, double *params)
+{
+    pcls->projectParticles_Async(particle_T0i_project_functor(), fields, nfield, params);
+}
+inline void projection_Tij_project_Async(Particles *pcls, Field<Real> **fields, int nfield, double *params)
+{
+    pcls->projectParticles_Async(particle_Tij_project_functor(), fields, nfield, params);
+}
+// projection_comm1 and its aliases projection_T00_comm, projection_T0i_comm,
+// projection_Tij_comm (gevolution.hpp:1543-1620, 1726, 2028, 2573-2579) fold
+// the upper ghost layers of a deposit into their owners.  The deposits here
+// are gather-form tiles over a periodic lattice without ghost layers, so every
+// site already holds its whole sum and the folds have nothing to do.
+inline void projection_comm1(Field<Real> *) {}
+inline void projection_T00_comm(Field<Real> *f) { projection_comm1(f); }
+inline void projection_T0i_comm(Field<Real> *f) { projection_comm1(f); }
+inline void projection_Tij_comm(Field<Real> *f) { projection_comm1(f); }
 
 // ----------------------------------------------------------------- sources
 // gevolution.hpp:183 (T00 form); returns the sum of source, as the reference

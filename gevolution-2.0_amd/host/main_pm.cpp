@@ -6,6 +6,7 @@
 //   main_pm -n N -L boxsize -zin z -zend z -steps S [-newton] [-seed s]
 //           [-zrelax z] -pk mpk.txt [-out final.bin] [-gadget snap] [-express restart]
 //           [-lightcone prefix -lcvertex x,y,z -lcz z] [-spectra prefix [-numbins n]]
+//   main_pm -n N -template crystal.dat -out lattice.bin
 //
 // The mPk file has the loadPowerSpectrum format (ic_basic.hpp:370-449): two
 // columns, k [h/Mpc] and P(k) [(Mpc/h)^3].  Steps are uniform in conformal
@@ -19,6 +20,11 @@
 // and the ID backlog hand-over, :2166-2880).  -spectra writes, at the end, the
 // deconvolved CIC density spectrum as <prefix>_deltaN.dat in writeSpectra's
 // delta_N form (output.hpp:3017-3021, writePowerSpectrum tools.hpp:370-452).
+// -template builds the particle lattice as generateIC_basic does before the
+// displacement (loadHomogeneousTemplate, initialize with the reference's
+// arguments, initializeParticlePositions, updateRowBuffers; ic_basic.hpp:
+// 205-330, 1612-1644, 2197-2211), writes it with -out through
+// sampleParticles(Real*, Real*, long*, n) and exits.
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -40,7 +46,7 @@ struct Options {
     int N = 32, steps = 5, newton = 0;
     uint32_t seed = 42;
     double L = 320.0, zin = 100.0, zend = 50.0, zrelax = -2.0;
-    std::string pk, out, gadget, express, lightcone, spectra;
+    std::string pk, out, gadget, express, lightcone, spectra, tmpl;
     int numbins = 32;
     double lcvertex[3] = {0.5, 0.5, 0.5}, lcz = 0.0;
 };
@@ -69,11 +75,16 @@ Options parse(int argc, char **argv)
         else if (a == "-lightcone") o.lightcone = next();
         else if (a == "-spectra") o.spectra = next();
         else if (a == "-numbins") o.numbins = std::atoi(next());
+        else if (a == "-template") o.tmpl = next();
         else if (a == "-lcvertex") {
             if (std::sscanf(next(), "%lf,%lf,%lf", &o.lcvertex[0], &o.lcvertex[1], &o.lcvertex[2]) != 3)
                 throw std::runtime_error("-lcvertex wants x,y,z");
         } else if (a == "-lcz") o.lcz = std::atof(next());
         else throw std::runtime_error("unknown option " + a);
+    }
+    if (!o.tmpl.empty()) {
+        if (o.out.empty()) throw std::runtime_error("-template needs -out");
+        return o;
     }
     if (o.pk.empty()) throw std::runtime_error("-pk <mPk file> is required");
     if (o.zrelax < -1.0) o.zrelax = o.zin;  // parser.hpp:1468 default
@@ -107,12 +118,66 @@ template <typename S> void convert_field(Lattice &src_lat, void *src, Field<Real
     dst.from_host(r);
 }
 
+void write_particles(const std::string &path, int64_t n, const std::vector<int64_t> &id, const std::vector<double> &pos,
+                     const std::vector<double> &q)
+{
+    std::ofstream f(path, std::ios::binary);
+    f.write((const char *)&n, sizeof(n));
+    f.write((const char *)id.data(), sizeof(int64_t) * n);
+    f.write((const char *)pos.data(), sizeof(double) * 3 * n);
+    f.write((const char *)q.data(), sizeof(double) * 3 * n);
+    if (!f) throw std::runtime_error("writing " + path + " failed");
+}
+
+// the particle lattice of generateIC_basic from a homogeneous template
+int template_lattice(const Options &opt)
+{
+    const int N = opt.N;
+    const int64_t M = (int64_t)N * N * N;
+    long numpart = 0;
+    float *pcldata = nullptr;
+    loadHomogeneousTemplate(opt.tmpl.c_str(), numpart, pcldata);
+    int numtile = (int)std::lround(std::cbrt((double)M / (double)numpart));
+    if ((int64_t)numpart * numtile * numtile * numtile != M) {
+        std::free(pcldata);
+        throw std::runtime_error("template does not tile the N^3 lattice");
+    }
+    Lattice lat(N);
+    Particles pcls;
+    part_simple_info info;
+    info.mass = 0.31 / (double)M;
+    Real boxSize[3] = {1., 1., 1.};
+    const uint64_t extra = 1024;  // PCL_EXTRA_CAPACITY-style slack
+    pcls.initialize(info, &lat, boxSize, (uint64_t)M + extra, extra);
+    initializeParticlePositions(numpart, pcldata, numtile, pcls);
+    pcls.updateRowBuffers();
+    std::free(pcldata);
+    const int64_t n = pcls.num_particles();
+    std::vector<Real> pos_s(3 * n), vel_s(3 * n);
+    std::vector<long> id_s(n);
+    if (pcls.sampleParticles(pos_s.data(), vel_s.data(), id_s.data(), (uint64_t)n) != n)
+        throw std::runtime_error("sampleParticles returned too few particles");
+    std::vector<int64_t> id(n);
+    std::vector<double> pos(3 * n), q(3 * n);
+    for (int64_t i = 0; i < n; i++) {
+        id[i] = id_s[i];
+        for (int d = 0; d < 3; d++) {
+            pos[d * n + i] = pos_s[3 * i + d];
+            q[d * n + i] = vel_s[3 * i + d];
+        }
+    }
+    write_particles(opt.out, n, id, pos, q);
+    std::printf("template: %ld particles tiled %d times per axis, %lld particles\n", numpart, numtile, (long long)n);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     try {
         const Options opt = parse(argc, argv);
+        if (!opt.tmpl.empty()) return template_lattice(opt);
         const int N = opt.N;
         const int64_t M = (int64_t)N * N * N;
         const double dx = 1.0 / N;
@@ -177,8 +242,16 @@ int main(int argc, char **argv)
 
         PlanFFT<Cplx> plan_source(&source, &scalarFT), plan_phi(&phi, &scalarFT), plan_chi(&chi, &scalarFT);
         PlanFFT<Cplx> plan_Sij(&Sij, &SijFT, 6), plan_Bi(&Bi, &BiFT, 3);
-        Field<Real> *update_fields[3] = {&phi, &chi, &Bi};
-        double f_params[2];
+        // main.cu:312-403: the field lists and the parameter buffer of the functor calls
+        Field<Real> *update_cdm_fields[3] = {&phi, &chi, &Bi};
+        Field<Real> *project_Tij_fields[2] = {&Sij, &phi};
+        Field<Real> *project_T0i_fields[2] = {&Bi, &phi};
+        double f_params[7] = {0., 0., 0., 0., 0., 0., 0.};
+        DeviceStagingBuffer<double> d_f_params(7);
+        const double numpts3d = (double)M;
+        const double Omega_bg = cosmo.Omega_cdm;  // cosmo.Omega_cdm + cosmo.Omega_b + bg_ncdm(a, cosmo), CDM only
+        double T00hom = 0., phi_hom = 0.;
+        (void)phi_hom;
 
         lightcone_geometry lc{};
         for (int d = 0; d < 3; d++) lc.vertex[d] = opt.lcvertex[d];
@@ -218,65 +291,106 @@ int main(int argc, char **argv)
             IDbacklog.finalize();
         };
 
+        // The cycle of main.cu:608-1317 for one CDM species (no baryons, ncdm,
+        // radiation, fluid or tensor evolution; vector method elliptic), call
+        // for call in main.cu's order.  Ghost-layer folds and halo updates
+        // are kept as calls: they have nothing to do on this lattice.
+        const int gr_flag = opt.newton ? 0 : 1;
+        Particles *pcls_cdm = &pcls;
         for (int cycle = 0; cycle < opt.steps; cycle++) {
-            if (!opt.newton) {
-                // T00 (main.cu:615-651) with the old phi
-                projection_init(&source);
-                projection_T00_project(&pcls, &source, a, &phi);
-                const double H = Hconf(a, fourpiG, cosmo);
-                if (dtau_old > 0.) {
-                    double rsum = 0;
-                    prepareFTsource(phi, chi, source, cosmo.Omega_m, source, 3. * H * dx * dx / dtau_old,
-                                    fourpiG * dx * dx / a, 3. * H * H * dx * dx, &rsum);  // main.cu:725
-                    plan_source.execute(FFT_FORWARD);                                    // main.cu:742
-                    if (kPrecision == GEV_MIXED) set_k0(scalarFT, rsum);
-                }
-                // Tij (main.cu:680-758) with the old phi
-                projection_init(&Sij);
-                projection_Tij_project(&pcls, &Sij, a, &phi);
-                if (dtau_old > 0.) {
-                    solveModifiedPoissonFT(scalarFT, scalarFT, 1. / (dx * dx), 3. * H / dtau_old);  // main.cu:769
-                    plan_phi.execute(FFT_BACKWARD);                                               // main.cu:794
-                }
+            // construct stress-energy tensor (main.cu:614-665)
+            projection_init(&source);  // thrust::fill_n(source.data(), ..., 0)
+            if (gr_flag > 0)
+                projection_T00_project(pcls_cdm, &source, a, &phi);
+            else
+                scalarProjectionCIC_project(pcls_cdm, &source);
+            projection_T00_comm(&source);
+            projection_init(&Sij);  // main.cu:678-681
+
+            // solve phi (main.cu:719-806)
+            double T00_terms = 0.;
+            if (gr_flag > 0 && dtau_old > 0.) {
+                T00hom = prepareFTsource(phi, chi, source, Omega_bg, source, 3. * Hconf(a, fourpiG, cosmo) * dx * dx / dtau_old,
+                                         fourpiG * dx * dx / a, 3. * Hconf(a, fourpiG, cosmo) * Hconf(a, fourpiG, cosmo) * dx * dx,
+                                         &T00_terms);  // main.cu:725
+                T00hom /= numpts3d;
+            }
+            if (gr_flag == 0 || dtau_old > 0.) {
+                plan_source.execute(FFT_FORWARD);  // main.cu:742
+#ifdef SINGLE
+                // the mixed form restores the FP64 mean of the source (no reference counterpart)
+                if (gr_flag > 0) set_k0(scalarFT, T00_terms);
+#endif
+            }
+            if (gr_flag > 0) {
+                f_params[0] = a;
+                f_params[1] = 1.;
+                d_f_params.copy_from_host(f_params, 7);
+                projection_Tij_project_Async(pcls_cdm, project_Tij_fields, 2, d_f_params.data());  // main.cu:756
+            }
+            if (gr_flag == 0)
+                solveModifiedPoissonFT(scalarFT, scalarFT, fourpiG / a);  // main.cu:765
+            else if (dtau_old > 0.)
+                solveModifiedPoissonFT(scalarFT, scalarFT, 1. / (dx * dx), 3. * Hconf(a, fourpiG, cosmo) / dtau_old);  // :769
+            if (gr_flag > 0) {
+                hip_check(hipDeviceSynchronize(), "projection_Tij_project_Async");  // main.cu:776
+                projection_Tij_comm(&Sij);                                         // main.cu:784
+            }
+            if (gr_flag == 0 || dtau_old > 0.) {
+                plan_phi.execute(FFT_BACKWARD);  // main.cu:794
+                phi.updateHalo();                // main.cu:802
+            }
+            // phi_hom = scalarFT(k = 0) (main.cu:808-809) is read only for the
+            // background file, which this driver does not write
+
+            if (gr_flag > 0) {
+                // solve chi (main.cu:810-906)
                 prepareFTsource(phi, Sij, Sij, 2. * fourpiG * dx * dx / a);  // main.cu:822
                 plan_Sij.execute(FFT_FORWARD);                               // main.cu:829
-                projection_init(&Bi);
-                projection_T0i_project(&pcls, &Bi, &phi);  // main.cu:858
-                projectFTscalar(SijFT, scalarFT);          // main.cu:873
-                plan_chi.execute(FFT_BACKWARD);            // main.cu:895
-                plan_Bi.execute(FFT_FORWARD);              // main.cu:918
+                projection_init(&Bi);                                        // main.cu:841
+                f_params[0] = 1.;
+                d_f_params.copy_from_host(f_params, 1);
+                projection_T0i_project_Async(pcls_cdm, project_T0i_fields, 2, d_f_params.data());  // main.cu:858
+                projectFTscalar(SijFT, scalarFT);                                                   // main.cu:873
+                hip_check(hipDeviceSynchronize(), "projection_T0i_project_Async");                 // main.cu:879
+                projection_T0i_comm(&Bi);                                                           // main.cu:887
+                plan_chi.execute(FFT_BACKWARD);                                                     // main.cu:895
+                chi.updateHalo();                                                                   // main.cu:902
+
+                // solve B (main.cu:908-957)
+                plan_Bi.execute(FFT_FORWARD);                    // main.cu:918
                 projectFTvector(BiFT, BiFT, fourpiG * dx * dx);  // main.cu:925
                 plan_Bi.execute(FFT_BACKWARD);                   // main.cu:948
-                write_lightcone(cycle);  // main.cu:1013-1131 outputs, before the particle update
-                // kick (main.cu:1227-1240), background half step, drift (:1253-1266)
-                f_params[0] = a;
-                f_params[1] = a * a * N;
-                maxvel = pcls.updateVel(update_q_functor(), (dtau + dtau_old) / 2., update_fields,
-                                        (1. / a < opt.zrelax + 1. ? 3 : 2), f_params);
-                rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau);
-                f_params[0] = a;
-                f_params[1] = a * a * N;
-                pcls.moveParticles(update_pos_functor(), dtau, update_fields, (1. / a < opt.zrelax + 1. ? 3 : 0),
-                                   f_params);
-                rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau);
-                maxvel /= std::sqrt(maxvel * maxvel + 1.0);  // main.cu:1279-1283
-            } else {
-                projection_init(&source);
-                scalarProjectionCIC_project(&pcls, &source);           // main.cu:655
-                plan_source.execute(FFT_FORWARD);                      // main.cu:742
-                solveModifiedPoissonFT(scalarFT, scalarFT, fourpiG / a);  // main.cu:765
-                plan_phi.execute(FFT_BACKWARD);                           // main.cu:794
-                write_lightcone(cycle);
-                f_params[0] = a;
-                maxvel = pcls.updateVel(update_q_Newton_functor(), (dtau + dtau_old) / 2., update_fields, 1, f_params);
-                rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau);
-                f_params[0] = a;
-                pcls.moveParticles(update_pos_Newton_functor(), dtau, nullptr, 0, f_params);
-                rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau);
+                Bi.updateHalo();                                 // main.cu:955
             }
+
+            write_lightcone(cycle);  // main.cu:1064-1070, outputs before the particle update
+
+            // cdm particle update (main.cu:1225-1283)
+            f_params[0] = a;
+            f_params[1] = a * a * N;  // a * a * sim.numpts
+            d_f_params.copy_from_host(f_params, 7);
+            if (gr_flag > 0)
+                maxvel = pcls_cdm->updateVel(update_q_functor(), (dtau + dtau_old) / 2., update_cdm_fields,
+                                             (1. / a < opt.zrelax + 1. ? 3 : 2), d_f_params.data());
+            else
+                maxvel = pcls_cdm->updateVel(update_q_Newton_functor(), (dtau + dtau_old) / 2., update_cdm_fields, 1,
+                                             d_f_params.data());
+            rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau);  // evolve background by half a time step
+            f_params[0] = a;
+            f_params[1] = a * a * N;
+            d_f_params.copy_from_host(f_params, 7);
+            if (gr_flag > 0)
+                pcls_cdm->moveParticles(update_pos_functor(), dtau, update_cdm_fields, (1. / a < opt.zrelax + 1. ? 3 : 0),
+                                        d_f_params.data());
+            else
+                pcls_cdm->moveParticles(update_pos_Newton_functor(), dtau, NULL, 0, d_f_params.data());
+            rungekutta4bg(a, fourpiG, cosmo, 0.5 * dtau);  // evolve background by half a time step
+            if (gr_flag > 0) maxvel /= std::sqrt(maxvel * maxvel + 1.0);  // main.cu:1279-1283
             tau += dtau;
             dtau_old = dtau;
-            std::printf("cycle %d z=%.6f a=%.8f tau=%.8f maxvel=%.6e\n", cycle, 1. / a - 1., a, tau, maxvel);
+            std::printf("cycle %d z=%.6f a=%.8f tau=%.8f maxvel=%.6e T00hom=%.10e\n", cycle, 1. / a - 1., a, tau, maxvel,
+                        T00hom);
         }
         lat.sync();
 
@@ -310,12 +424,7 @@ int main(int argc, char **argv)
                     q[d * n + i] = ps[i].vel[d];
                 }
             }
-            std::ofstream f(opt.out, std::ios::binary);
-            f.write((const char *)&n, sizeof(n));
-            f.write((const char *)id.data(), sizeof(int64_t) * n);
-            f.write((const char *)pos.data(), sizeof(double) * 3 * n);
-            f.write((const char *)q.data(), sizeof(double) * 3 * n);
-            if (!f) throw std::runtime_error("writing " + opt.out + " failed");
+            write_particles(opt.out, n, id, pos, q);
         }
         if (!opt.gadget.empty() || !opt.express.empty()) {
             // output.hpp:418-466 at the current time (no exact-redshift step)

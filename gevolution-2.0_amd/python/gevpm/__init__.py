@@ -22,11 +22,25 @@ from ._lib import (GEV_FP64, GEV_MIXED, GevError, check, gev_fields, gev_gadget2
 __all__ = ["Context", "Particles", "Fields", "StepState", "GEV_FP64", "GEV_MIXED", "GevError"]
 
 
-def _ptr(t):
+def _checked(t, name, dtype, numel, device, optional=False, at_least=False):
+    """device pointer of t after checking everything the kernels assume: a
+    contiguous tensor of `dtype` on `device` with exactly `numel` elements
+    (at least `numel` when at_least).  None is passed through only when
+    optional."""
     if t is None:
-        return None
-    if not t.is_cuda:
-        raise GevError("gevpm: tensors must live on the GPU")
+        if optional:
+            return None
+        raise GevError(f"{name}: a tensor is required")
+    if not isinstance(t, torch.Tensor):
+        raise GevError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise GevError(f"{name}: expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise GevError(f"{name}: the tensor must be contiguous")
+    if (t.numel() < numel) if at_least else (t.numel() != numel):
+        raise GevError(f"{name}: expected {'at least ' if at_least else ''}{numel} elements, got {t.numel()}")
+    if not t.is_cuda or (device is not None and t.device != device):
+        raise GevError(f"{name}: the tensor must live on {device or 'the GPU'} (got {t.device})")
     return C.c_void_p(t.data_ptr())
 
 
@@ -38,7 +52,10 @@ class Context:
             raise GevError("gevpm needs a GPU (HIP device) -- there is no CPU path")
         self.N = int(N)
         self.precision = precision
-        self.device = torch.device(device)
+        dev = torch.device(device)
+        if dev.type != "cuda":
+            raise GevError(f"gevpm: device must be a GPU, got {dev}")
+        self.device = dev if dev.index is not None else torch.device("cuda", torch.cuda.current_device())
         self.rdtype = torch.float64 if precision == GEV_FP64 else torch.float32
         self.cdtype = torch.complex128 if precision == GEV_FP64 else torch.complex64
         h = C.c_void_p()
@@ -55,6 +72,26 @@ class Context:
     @property
     def handle(self):
         return self._h
+
+    # -------------------------------------------------- checked pointers
+    def r(self, t, ncomp, name, optional=False):
+        """pointer to a real field of ncomp components of this context"""
+        return _checked(t, name, self.rdtype, ncomp * self.M, self.device, optional)
+
+    def k(self, t, ncomp, name, optional=False, at_least=False):
+        """pointer to a Fourier field of ncomp components of this context"""
+        return _checked(t, name, self.cdtype, ncomp * self.N * self.N * self.P, self.device, optional, at_least)
+
+    def ncomp_real(self, t, name):
+        if not isinstance(t, torch.Tensor) or t.numel() == 0 or t.numel() % self.M:
+            raise GevError(f"{name}: a real field of this context holds a multiple of {self.M} elements")
+        return t.numel() // self.M
+
+    def ncomp_fourier(self, t, name):
+        m = self.N * self.N * self.P
+        if not isinstance(t, torch.Tensor) or t.numel() == 0 or t.numel() % m:
+            raise GevError(f"{name}: a Fourier field of this context holds a multiple of {m} elements")
+        return t.numel() // m
 
     def bind_stream(self):
         """Make the library follow torch's current stream."""
@@ -91,57 +128,65 @@ class Context:
     # -------------------------------------------------------------- FFT
     def fft_r2c(self, f, out=None):
         f = f.contiguous()
-        nc = f.numel() // self.M
+        nc = self.ncomp_real(f, "fft_r2c")
         out = self.fourier(nc) if out is None else out
+        pf, po = self.r(f, nc, "fft_r2c(in)"), self.k(out, nc, "fft_r2c(out)")
         self.bind_stream()
-        check(lib.gev_fft_r2c(self._h, _ptr(f), _ptr(out), nc), "gev_fft_r2c")
+        check(lib.gev_fft_r2c(self._h, pf, po, nc), "gev_fft_r2c")
         return out
 
     def fft_c2r(self, F, out=None, preserve=True):
-        nc = F.numel() // (self.N * self.N * self.P)
+        nc = self.ncomp_fourier(F, "fft_c2r")
         src = F.clone() if preserve else F
         out = self.real(nc) if out is None else out
+        pf, po = self.k(src, nc, "fft_c2r(in)"), self.r(out, nc, "fft_c2r(out)")
         self.bind_stream()
-        check(lib.gev_fft_c2r(self._h, _ptr(src), _ptr(out), nc), "gev_fft_c2r")
+        check(lib.gev_fft_c2r(self._h, pf, po, nc), "gev_fft_c2r")
         return out
 
     # ----------------------------------------------------------- k-space
     def poisson(self, src, coeff, modif=0.0, out=None):
         out = torch.empty_like(src) if out is None else out
         self.bind_stream()
-        check(lib.gev_poisson(self._h, _ptr(src), _ptr(out), coeff, modif), "gev_poisson")
+        check(lib.gev_poisson(self._h, self.k(src, 1, "poisson(src)"), self.k(out, 1, "poisson(out)"), coeff, modif),
+              "gev_poisson")
         return out
 
     def project_ft_scalar(self, S, chi=None):
         add = chi is not None
         out = self.fourier(1) if chi is None else chi
         self.bind_stream()
-        check(lib.gev_project_ft_scalar(self._h, _ptr(S), _ptr(out), int(add)), "gev_project_ft_scalar")
+        check(lib.gev_project_ft_scalar(self._h, self.k(S, 6, "project_ft_scalar(S)"),
+                                        self.k(out, 1, "project_ft_scalar(chi)"), int(add)), "gev_project_ft_scalar")
         return out
 
     def project_ft_vector(self, S, coeff=1.0, modif=0.0, out=None):
         out = torch.empty_like(S) if out is None else out
         self.bind_stream()
-        check(lib.gev_project_ft_vector(self._h, _ptr(S), _ptr(out), coeff, modif), "gev_project_ft_vector")
+        check(lib.gev_project_ft_vector(self._h, self.k(S, 3, "project_ft_vector(S)"),
+                                        self.k(out, 3, "project_ft_vector(B)"), coeff, modif), "gev_project_ft_vector")
         return out
 
     def evolve_ft_vector(self, S, B, a2dtau):
         self.bind_stream()
-        check(lib.gev_evolve_ft_vector(self._h, _ptr(S), _ptr(B), a2dtau), "gev_evolve_ft_vector")
+        check(lib.gev_evolve_ft_vector(self._h, self.k(S, 6, "evolve_ft_vector(S)"), self.k(B, 3, "evolve_ft_vector(B)"),
+                                       a2dtau), "gev_evolve_ft_vector")
         return B
 
     def set_k0(self, F, re, im=0.0):
         self.bind_stream()
-        check(lib.gev_set_k0(self._h, _ptr(F), re, im), "gev_set_k0")
+        check(lib.gev_set_k0(self._h, self.k(F, 1, "set_k0", at_least=True), re, im), "gev_set_k0")
 
     def adj_divergence(self, u, lphi, K):
         self.bind_stream()
-        check(lib.gev_adj_divergence(self._h, _ptr(u), _ptr(lphi), K), "gev_adj_divergence")
+        check(lib.gev_adj_divergence(self._h, self.r(u, 3, "adj_divergence(u)"), self.r(lphi, 1, "adj_divergence(lphi)"),
+                                     K), "gev_adj_divergence")
         return lphi
 
     def adj_ic_lattice(self, lx, lq, cx, cq, lpsi):
         self.bind_stream()
-        check(lib.gev_adj_ic_lattice(self._h, _ptr(lx), _ptr(lq), cx, cq, _ptr(lpsi)), "gev_adj_ic_lattice")
+        check(lib.gev_adj_ic_lattice(self._h, self.r(lx, 3, "adj_ic_lattice(lx)"), self.r(lq, 3, "adj_ic_lattice(lq)"), cx,
+                                     cq, self.r(lpsi, 1, "adj_ic_lattice(lpsi)")), "gev_adj_ic_lattice")
         return lpsi
 
     # ------------------------------------ fused real -> k -> real solves
@@ -150,8 +195,9 @@ class Context:
         out = torch.empty_like(src) if out is None else out
         work = self.fourier(1) if work is None else work
         self.bind_stream()
-        check(lib.gev_solve_poisson(self._h, _ptr(src), _ptr(out), _ptr(work), coeff, modif, int(k0 is not None),
-                                    0.0 if k0 is None else float(k0)), "gev_solve_poisson")
+        check(lib.gev_solve_poisson(self._h, self.r(src, 1, "solve_poisson(src)"), self.r(out, 1, "solve_poisson(out)"),
+                                    self.k(work, 1, "solve_poisson(work)", at_least=True), coeff, modif,
+                                    int(k0 is not None), 0.0 if k0 is None else float(k0)), "gev_solve_poisson")
         return out
 
     def solve_poisson_T00(self, phi, chi, source, bg, c1, c2, c3, coeff, modif=0.0, k0set=False, out=None,
@@ -163,8 +209,11 @@ class Context:
         work = self.fourier(1) if work is None else work
         sums = (C.c_double * 2)()
         self.bind_stream()
-        check(lib.gev_solve_poisson_T00(self._h, _ptr(phi), _ptr(chi), _ptr(source), bg, c1, c2, c3, _ptr(out),
-                                        _ptr(work), coeff, modif, int(k0set), sums), "gev_solve_poisson_T00")
+        nm = "solve_poisson_T00"
+        check(lib.gev_solve_poisson_T00(self._h, self.r(phi, 1, nm + "(phi)"), self.r(chi, 1, nm + "(chi)"),
+                                        self.r(source, 1, nm + "(source)"), bg, c1, c2, c3, self.r(out, 1, nm + "(out)"),
+                                        self.k(work, 1, nm + "(work)", at_least=True), coeff, modif, int(k0set), sums),
+              "gev_solve_poisson_T00")
         return out, (sums[0], sums[1])
 
     def solve_chi(self, Sij, out=None, work6=None, work1=None):
@@ -173,7 +222,9 @@ class Context:
         work6 = self.fourier(6) if work6 is None else work6
         work1 = self.fourier(1) if work1 is None else work1
         self.bind_stream()
-        check(lib.gev_solve_chi(self._h, _ptr(Sij), _ptr(out), _ptr(work6), _ptr(work1)), "gev_solve_chi")
+        check(lib.gev_solve_chi(self._h, self.r(Sij, 6, "solve_chi(Sij)"), self.r(out, 1, "solve_chi(out)"),
+                                self.k(work6, 6, "solve_chi(work6)", at_least=True),
+                                self.k(work1, 1, "solve_chi(work1)", at_least=True)), "gev_solve_chi")
         return out
 
     def solve_chi_Tij(self, phi, Tij, coeff, out=None, work6=None, work1=None):
@@ -182,22 +233,28 @@ class Context:
         work6 = self.fourier(6) if work6 is None else work6
         work1 = self.fourier(1) if work1 is None else work1
         self.bind_stream()
-        check(lib.gev_solve_chi_Tij(self._h, _ptr(phi), _ptr(Tij), coeff, _ptr(out), _ptr(work6), _ptr(work1)),
-              "gev_solve_chi_Tij")
+        nm = "solve_chi_Tij"
+        check(lib.gev_solve_chi_Tij(self._h, self.r(phi, 1, nm + "(phi)"), self.r(Tij, 6, nm + "(Tij)"), coeff,
+                                    self.r(out, 1, nm + "(out)"), self.k(work6, 6, nm + "(work6)", at_least=True),
+                                    self.k(work1, 1, nm + "(work1)", at_least=True)), "gev_solve_chi_Tij")
         return out
 
     def solve_vector(self, Bi, coeff=1.0, modif=0.0, work=None):
         """Bi <- c2r(projectFTvector(r2c(Bi))), in place"""
         work = self.fourier(3) if work is None else work
         self.bind_stream()
-        check(lib.gev_solve_vector(self._h, _ptr(Bi), _ptr(work), coeff, modif), "gev_solve_vector")
+        check(lib.gev_solve_vector(self._h, self.r(Bi, 3, "solve_vector(Bi)"),
+                                   self.k(work, 3, "solve_vector(work)", at_least=True), coeff, modif),
+              "gev_solve_vector")
         return Bi
 
     def solve_vector_T0i_links(self, Bi, phi, coeff=1.0, modif=0.0, work=None):
         """Bi (T0i link sums) <- B: the link factor 1 + phi_a + phi_b, then solve_vector"""
         work = self.fourier(3) if work is None else work
         self.bind_stream()
-        check(lib.gev_solve_vector_T0i_links(self._h, _ptr(Bi), _ptr(phi), _ptr(work), coeff, modif),
+        nm = "solve_vector_T0i_links"
+        check(lib.gev_solve_vector_T0i_links(self._h, self.r(Bi, 3, nm + "(Bi)"), self.r(phi, 1, nm + "(phi)"),
+                                             self.k(work, 3, nm + "(work)", at_least=True), coeff, modif),
               "gev_solve_vector_T0i_links")
         return Bi
 
@@ -206,20 +263,27 @@ class Context:
         result = torch.empty_like(source) if result is None else result
         s1, s2 = C.c_double(), C.c_double()
         self.bind_stream()
-        check(lib.gev_prepare_source_T00(self._h, _ptr(phi), _ptr(chi), _ptr(source), bg, _ptr(result), c1, c2, c3,
-                                         C.byref(s1), C.byref(s2)), "gev_prepare_source_T00")
+        nm = "prepare_source_T00"
+        check(lib.gev_prepare_source_T00(self._h, self.r(phi, 1, nm + "(phi)"), self.r(chi, 1, nm + "(chi)"),
+                                         self.r(source, 1, nm + "(source)"), bg, self.r(result, 1, nm + "(result)"),
+                                         c1, c2, c3, C.byref(s1), C.byref(s2)), "gev_prepare_source_T00")
         return result, s1.value, s2.value
 
     def prepare_source_Tij(self, phi, Tij, coeff, Sij=None):
         Sij = torch.empty_like(Tij) if Sij is None else Sij
         self.bind_stream()
-        check(lib.gev_prepare_source_Tij(self._h, _ptr(phi), _ptr(Tij), _ptr(Sij), coeff), "gev_prepare_source_Tij")
+        nm = "prepare_source_Tij"
+        check(lib.gev_prepare_source_Tij(self._h, self.r(phi, 1, nm + "(phi)"), self.r(Tij, 6, nm + "(Tij)"),
+                                         self.r(Sij, 6, nm + "(Sij)"), coeff), "gev_prepare_source_Tij")
         return Sij
 
     def field_sum(self, f):
         s = C.c_double()
+        f = f.contiguous()
+        nc = self.ncomp_real(f, "field_sum")
+        pf = self.r(f, nc, "field_sum")
         self.bind_stream()
-        check(lib.gev_field_sum(self._h, _ptr(f.contiguous()), f.numel() // self.M, C.byref(s)), "gev_field_sum")
+        check(lib.gev_field_sum(self._h, pf, nc, C.byref(s)), "gev_field_sum")
         return s.value
 
     def ic_displacement_ft(self, seed, kL, amp, kerFT, coeff=0.0, ksphere=1, out=None):
@@ -230,7 +294,9 @@ class Context:
         self.bind_stream()
         check(lib.gev_ic_displacement_ft(self._h, seed, ksphere, coeff, len(kL),
                                          kL.ctypes.data_as(C.POINTER(C.c_double)),
-                                         amp.ctypes.data_as(C.POINTER(C.c_double)), _ptr(kerFT), _ptr(out)),
+                                         amp.ctypes.data_as(C.POINTER(C.c_double)),
+                                         self.k(kerFT, 1, "ic_displacement_ft(kerFT)"),
+                                         self.k(out, 1, "ic_displacement_ft(out)")),
               "gev_ic_displacement_ft")
         return out
 
@@ -260,6 +326,14 @@ class Particles:
         except Exception:
             pass
 
+    def f(self, t, ncomp, name, optional=False):
+        """pointer to a real field of ncomp components of this container's context"""
+        return self.ctx.r(t, ncomp, name, optional)
+
+    def per_particle(self, t, name):
+        """pointer to a [3][n] per-particle array in the context precision"""
+        return _checked(t, name, self.ctx.rdtype, 3 * self.n, self.ctx.device)
+
     def view(self):
         v = gev_pcls_view()
         check(lib.gev_pcls_view_get(self._h, C.byref(v)), "gev_pcls_view_get")
@@ -271,15 +345,27 @@ class Particles:
 
     def load(self, pos, q, pid):
         """pos, q: (3, n) float64; pid: (n,) int64 -- host (numpy/CPU torch) or device tensors."""
+        def shapes(pos, q, pid):
+            if tuple(pos.shape) != (3, pos.shape[-1]) or len(pos.shape) != 2 or tuple(q.shape) != tuple(pos.shape) \
+                    or tuple(pid.shape) != (pos.shape[1],):
+                raise GevError(f"load: expected pos (3, n), q (3, n), pid (n,); got {tuple(pos.shape)}, "
+                               f"{tuple(q.shape)}, {tuple(pid.shape)}")
+            return pos.shape[1]
         if isinstance(pos, torch.Tensor) and pos.is_cuda:
             pos = pos.to(torch.float64).contiguous(); q = q.to(torch.float64).contiguous()
             pid = pid.to(torch.int64).contiguous()
+            n = shapes(pos, q, pid)
+            dev = self.ctx.device
+            pp = _checked(pos, "load(pos)", torch.float64, 3 * n, dev)
+            pq = _checked(q, "load(q)", torch.float64, 3 * n, dev)
+            pi = _checked(pid, "load(pid)", torch.int64, n, dev)
             self.ctx.bind_stream()
-            check(lib.gev_pcls_load(self._h, pos.shape[1], _ptr(pos), _ptr(q), _ptr(pid), 1), "gev_pcls_load")
+            check(lib.gev_pcls_load(self._h, n, pp, pq, pi, 1), "gev_pcls_load")
         else:
             import numpy as np
             pos = np.ascontiguousarray(pos, np.float64); q = np.ascontiguousarray(q, np.float64)
             pid = np.ascontiguousarray(pid, np.int64)
+            shapes(pos, q, pid)
             self.ctx.bind_stream()
             check(lib.gev_pcls_load(self._h, pos.shape[1], C.c_void_p(pos.ctypes.data), C.c_void_p(q.ctypes.data),
                                     C.c_void_p(pid.ctypes.data), 0), "gev_pcls_load")
@@ -292,20 +378,21 @@ class Particles:
         q = torch.empty((3, n), dtype=torch.float64, device=dev)
         pid = torch.empty((n,), dtype=torch.int64, device=dev)
         self.ctx.bind_stream()
-        check(lib.gev_pcls_store(self._h, _ptr(pos), _ptr(q), _ptr(pid), 1), "gev_pcls_store")
+        check(lib.gev_pcls_store(self._h, C.c_void_p(pos.data_ptr()), C.c_void_p(q.data_ptr()),
+                                 C.c_void_p(pid.data_ptr()), 1), "gev_pcls_store")
         return pos, q, pid
 
     def cells(self):
         """cell index (x + N(y + N z)) of every particle, in container order"""
         out = torch.empty((max(self.n, 1),), dtype=torch.int32, device=self.ctx.device)
         self.ctx.bind_stream()
-        check(lib.gev_pcls_copy_index(self._h, _ptr(out), None), "gev_pcls_copy_index")
+        check(lib.gev_pcls_copy_index(self._h, C.c_void_p(out.data_ptr()), None), "gev_pcls_copy_index")
         return out[: self.n]
 
     def cell_start(self):
         out = torch.empty((self.ctx.M + 1,), dtype=torch.int32, device=self.ctx.device)
         self.ctx.bind_stream()
-        check(lib.gev_pcls_copy_index(self._h, None, _ptr(out)), "gev_pcls_copy_index")
+        check(lib.gev_pcls_copy_index(self._h, None, C.c_void_p(out.data_ptr())), "gev_pcls_copy_index")
         return out
 
     def sort(self):
@@ -319,57 +406,67 @@ class Particles:
     # ---------------------------------------------------------- deposits
     def project_T00(self, out, a=1.0, phi=None, coeff=1.0, accumulate=False):
         self.ctx.bind_stream()
-        check(lib.gev_project_T00(self._h, _ptr(out), a, _ptr(phi), coeff, int(accumulate)), "gev_project_T00")
+        check(lib.gev_project_T00(self._h, self.f(out, 1, "project_T00(out)"), a,
+                                  self.f(phi, 1, "project_T00(phi)", True), coeff, int(accumulate)), "gev_project_T00")
         return out
 
     def project_mass(self, out, coeff=1.0, accumulate=False):
         self.ctx.bind_stream()
-        check(lib.gev_project_mass(self._h, _ptr(out), coeff, int(accumulate)), "gev_project_mass")
+        check(lib.gev_project_mass(self._h, self.f(out, 1, "project_mass(out)"), coeff, int(accumulate)),
+              "gev_project_mass")
         return out
 
     def project_T0i(self, out, phi=None, coeff=1.0, accumulate=False):
         self.ctx.bind_stream()
-        check(lib.gev_project_T0i(self._h, _ptr(out), _ptr(phi), coeff, int(accumulate)), "gev_project_T0i")
+        check(lib.gev_project_T0i(self._h, self.f(out, 3, "project_T0i(out)"), self.f(phi, 1, "project_T0i(phi)", True),
+                                  coeff, int(accumulate)), "gev_project_T0i")
         return out
 
     def project_Tij(self, out, a=1.0, phi=None, coeff=1.0, accumulate=False):
         self.ctx.bind_stream()
-        check(lib.gev_project_Tij(self._h, _ptr(out), a, _ptr(phi), coeff, int(accumulate)), "gev_project_Tij")
+        check(lib.gev_project_Tij(self._h, self.f(out, 6, "project_Tij(out)"), a,
+                                  self.f(phi, 1, "project_Tij(phi)", True), coeff, int(accumulate)), "gev_project_Tij")
         return out
 
     def project_T00_Tij(self, T00, Tij, a=1.0, phi=None, coeff=1.0, accumulate=False):
         """project_T00 and project_Tij of the same particles in one pass"""
         self.ctx.bind_stream()
-        check(lib.gev_project_T00_Tij(self._h, _ptr(T00), _ptr(Tij), a, _ptr(phi), coeff, int(accumulate)),
-              "gev_project_T00_Tij")
+        nm = "project_T00_Tij"
+        check(lib.gev_project_T00_Tij(self._h, self.f(T00, 1, nm + "(T00)"), self.f(Tij, 6, nm + "(Tij)"), a,
+                                      self.f(phi, 1, nm + "(phi)", True), coeff, int(accumulate)), "gev_project_T00_Tij")
         return T00, Tij
 
     def project_T00_Tij_T0i_links(self, T00, Tij, links, a=1.0, phi=None, coeff=1.0):
         """project_T00 and project_Tij, and the T0i link sums without their
         1 + phi_a + phi_b factor (see Context.solve_vector_T0i_links)"""
         self.ctx.bind_stream()
-        check(lib.gev_project_T00_Tij_T0i_links(self._h, _ptr(T00), _ptr(Tij), _ptr(links), a, _ptr(phi), coeff),
-              "gev_project_T00_Tij_T0i_links")
+        nm = "project_T00_Tij_T0i_links"
+        check(lib.gev_project_T00_Tij_T0i_links(self._h, self.f(T00, 1, nm + "(T00)"), self.f(Tij, 6, nm + "(Tij)"),
+                                                self.f(links, 3, nm + "(links)"), a, self.f(phi, 1, nm + "(phi)", True),
+                                                coeff), "gev_project_T00_Tij_T0i_links")
         return T00, Tij, links
 
     # ------------------------------------------------------------ updates
     def update_q(self, dtau, phi, chi, Bi, nfield, a, bcoeff):
         r = C.c_double()
         self.ctx.bind_stream()
-        check(lib.gev_update_q(self._h, dtau, _ptr(phi), _ptr(chi), _ptr(Bi), nfield, a, bcoeff, C.byref(r)),
-              "gev_update_q")
+        check(lib.gev_update_q(self._h, dtau, self.f(phi, 1, "update_q(phi)"), self.f(chi, 1, "update_q(chi)", True),
+                               self.f(Bi, 3, "update_q(Bi)", True), nfield, a, bcoeff, C.byref(r)), "gev_update_q")
         return r.value
 
     def update_q_newton(self, dtau, psi, a, chi=None, nfield=1):
         r = C.c_double()
         self.ctx.bind_stream()
-        check(lib.gev_update_q_newton(self._h, dtau, _ptr(psi), _ptr(chi), nfield, a, C.byref(r)),
+        check(lib.gev_update_q_newton(self._h, dtau, self.f(psi, 1, "update_q_newton(psi)"),
+                                      self.f(chi, 1, "update_q_newton(chi)", True), nfield, a, C.byref(r)),
               "gev_update_q_newton")
         return r.value
 
     def update_pos(self, dtau, phi, chi, Bi, nfield, a, bcoeff):
         self.ctx.bind_stream()
-        check(lib.gev_update_pos(self._h, dtau, _ptr(phi), _ptr(chi), _ptr(Bi), nfield, a, bcoeff), "gev_update_pos")
+        check(lib.gev_update_pos(self._h, dtau, self.f(phi, 1, "update_pos(phi)", True),
+                                 self.f(chi, 1, "update_pos(chi)", True), self.f(Bi, 3, "update_pos(Bi)", True), nfield,
+                                 a, bcoeff), "gev_update_pos")
 
     def update_pos_newton(self, dtau, a):
         self.ctx.bind_stream()
@@ -379,20 +476,23 @@ class Particles:
                    bc_drift):
         r = C.c_double()
         self.ctx.bind_stream()
-        check(lib.gev_kick_drift(self._h, int(gr), dtau_kick, dtau_drift, _ptr(phi), _ptr(chi), _ptr(Bi), nf_kick,
+        nm = "kick_drift"
+        check(lib.gev_kick_drift(self._h, int(gr), dtau_kick, dtau_drift, self.f(phi, 1, nm + "(phi)"),
+                                 self.f(chi, 1, nm + "(chi)", True), self.f(Bi, 3, nm + "(Bi)", True), nf_kick,
                                  nf_drift, a_kick, a_drift, bc_kick, bc_drift, C.byref(r), 1), "gev_kick_drift")
         return r.value
 
     def displace(self, xi, coeff=1.0):
         r = C.c_double()
         self.ctx.bind_stream()
-        check(lib.gev_displace_particles(self._h, _ptr(xi), coeff, C.byref(r)), "gev_displace_particles")
+        check(lib.gev_displace_particles(self._h, self.f(xi, 1, "displace(xi)"), coeff, C.byref(r)),
+              "gev_displace_particles")
         return r.value
 
     def initialize_q(self, pot, coeff):
         r = C.c_double()
         self.ctx.bind_stream()
-        check(lib.gev_initialize_q(self._h, _ptr(pot), coeff, C.byref(r)), "gev_initialize_q")
+        check(lib.gev_initialize_q(self._h, self.f(pot, 1, "initialize_q(pot)"), coeff, C.byref(r)), "gev_initialize_q")
         return r.value
 
     # ------------------------------------------------------ Gadget-2 files
@@ -401,7 +501,7 @@ class Particles:
         gev_gadget2_header whose npart[1] equals their number"""
         self.ctx.bind_stream()
         check(lib.gev_pcls_save_gadget2(self._h, os.fsencode(path), C.byref(header), int(tracer_factor), dtau_pos,
-                                        dtau_vel, _ptr(phi)), "gev_pcls_save_gadget2")
+                                        dtau_vel, self.f(phi, 1, "save_gadget2(phi)", True)), "gev_pcls_save_gadget2")
 
     def load_gadget2(self, path):
         """loadGadget2: replaces the particles with the snapshot's; returns its header"""
@@ -432,7 +532,8 @@ class Particles:
         check(lib.gev_pcls_save_gadget2_lightcone(
             self._h, os.fsencode(path), C.byref(header), C.byref(geo), float(dist), float(dtau), float(dtau_old),
             float(dadtau), v.ctypes.data_as(C.POINTER(C.c_double)), len(v), None if idlog is None else idlog._h,
-            _ptr(phi), int(tracer_factor), C.byref(written)), "gev_pcls_save_gadget2_lightcone")
+            self.f(phi, 1, "save_gadget2_lightcone(phi)", True), int(tracer_factor), C.byref(written)),
+            "gev_pcls_save_gadget2_lightcone")
         return written.value
 
     def load_express(self, path):
@@ -452,24 +553,27 @@ class Particles:
         """copy the sorted container state into a device byte tensor"""
         nb = self.state_bytes()
         buf = torch.empty(nb, dtype=torch.uint8, device=self.ctx.device) if buf is None else buf
-        assert buf.numel() >= nb
+        pb = _checked(buf, "save(buf)", torch.uint8, nb, self.ctx.device, at_least=True)
         self.ctx.bind_stream()
-        check(lib.gev_pcls_save(self._h, _ptr(buf)), "gev_pcls_save")
+        check(lib.gev_pcls_save(self._h, pb), "gev_pcls_save")
         return buf
 
     def restore(self, buf):
+        pb = _checked(buf, "restore(buf)", torch.uint8, self.state_bytes(), self.ctx.device, at_least=True)
         self.ctx.bind_stream()
-        check(lib.gev_pcls_restore(self._h, _ptr(buf)), "gev_pcls_restore")
+        check(lib.gev_pcls_restore(self._h, pb), "gev_pcls_restore")
 
     # ------------------------------------------------ adjoint (Newtonian)
     def adj_deposit(self, lq, u):
         self.ctx.bind_stream()
-        check(lib.gev_adj_deposit(self._h, _ptr(lq), _ptr(u)), "gev_adj_deposit")
+        check(lib.gev_adj_deposit(self._h, self.per_particle(lq, "adj_deposit(lq)"), self.f(u, 3, "adj_deposit(u)")),
+              "gev_adj_deposit")
 
     def adj_gather(self, lx, lq, phi=None, lrho=None, K=0.0, mN=0.0, C_prev=0.0):
         self.ctx.bind_stream()
-        check(lib.gev_adj_gather(self._h, _ptr(phi), _ptr(lrho), _ptr(lq), _ptr(lx), K, mN, C_prev),
-              "gev_adj_gather")
+        check(lib.gev_adj_gather(self._h, self.f(phi, 1, "adj_gather(phi)", True), self.f(lrho, 1, "adj_gather(lrho)", True),
+                                 self.per_particle(lq, "adj_gather(lq)"), self.per_particle(lx, "adj_gather(lx)"), K, mN,
+                                 C_prev), "gev_adj_gather")
 
 
 class Fields:

@@ -72,7 +72,7 @@ class PMRun:
         p.kick_drift(0, s["dtau_k"], s["dtau"], self.phi, None, None, 1, 0, s["a"], s["a_d"], 1.0, 1.0)
 
     def forward(self, psi):
-        psi = psi.reshape(1, self.N, self.N, self.N).contiguous()
+        psi = psi.reshape(1, self.N, self.N, self.N).to(self.ctx.rdtype).contiguous()
         self._ic(psi)
         self._ckpts = {}
         for n in range(self.steps):

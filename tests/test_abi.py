@@ -65,3 +65,26 @@ def test_bbks_table_normalisation():
     k2, P2 = cosmo.bbks_table(z=100.0)
     D = cosmo.growth_factor(1 / 101.0, 0.31, 0.69)
     assert np.allclose(P2, P * D * D, rtol=1e-12)
+
+
+def test_checked_pointers_refuse_mismatches():
+    """the Python layer checks dtype, contiguity, element count and device of
+    every tensor before its pointer reaches a kernel"""
+    import torch
+    from gevpm import GevError, _checked
+    good = torch.zeros(2, 8, dtype=torch.float32)
+    with pytest.raises(GevError, match="expected torch.float32"):
+        _checked(good.double(), "f", torch.float32, 16, None)
+    with pytest.raises(GevError, match="contiguous"):
+        _checked(good.t(), "f", torch.float32, 16, None)
+    with pytest.raises(GevError, match="expected 24 elements"):
+        _checked(good, "f", torch.float32, 24, None)
+    with pytest.raises(GevError, match="at least 24"):
+        _checked(good, "f", torch.float32, 24, None, at_least=True)
+    with pytest.raises(GevError, match="a tensor is required"):
+        _checked(None, "f", torch.float32, 16, None)
+    with pytest.raises(GevError, match="torch tensor"):
+        _checked(np.zeros(16, np.float32), "f", torch.float32, 16, None)
+    with pytest.raises(GevError, match="must live on"):
+        _checked(good, "f", torch.float32, 16, None)  # right in every way but on the host
+    assert _checked(None, "f", torch.float32, 16, None, optional=True) is None

@@ -400,6 +400,13 @@ def test_steps_vs_oracle(gp, O, gr, N, steps):
         assert relmax(real_np(f.chi)[0], ost["chi"]) < 1e-7
         assert relmax(real_np(f.Bi), ost["Bi"]) < 1e-7
     assert np.isclose(st.maxvel, ost["maxvel"], rtol=1e-9)
+    # cell keys per ID across the two implementations: equal except within eps of a face
+    eps = 1e-9
+    xa, xb = gpos * N, ost["pos"] * N
+    mism = np.any(np.floor(xa) != np.floor(xb), axis=0)
+    near = np.any(np.abs(xa - np.rint(xa)) < eps, axis=0) | np.any(np.abs(xb - np.rint(xb)) < eps, axis=0)
+    print(f"  cell keys: {int(mism.sum())} differ, {int(near.sum())} particles within {eps} cells of a face")
+    assert np.all(near[mism])
 
 
 @pytest.mark.parametrize("gr", [0, 1])

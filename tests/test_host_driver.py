@@ -23,6 +23,15 @@ def test_driver_rejects_bad_options(tmp_path):
     assert r.returncode == 1 and "unknown option" in r.stderr
     r = subprocess.run([os.path.join(LIB, "main_pm"), "-n", "16"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "-pk" in r.stderr
+    # loadHomogeneousTemplate's failures are reported before any GPU work
+    r = subprocess.run([os.path.join(LIB, "main_pm"), "-n", "16", "-template", str(tmp_path / "none.dat"), "-out",
+                        str(tmp_path / "x.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "unable to open template file" in r.stderr
+    bad = tmp_path / "bad.dat"
+    bad.write_bytes(b"\x10\x00\x00\x00" + bytes(16))
+    r = subprocess.run([os.path.join(LIB, "main_pm"), "-n", "16", "-template", str(bad), "-out",
+                        str(tmp_path / "x.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "header not recognized" in r.stderr
 
 
 def _read_out(path):
@@ -207,3 +216,45 @@ def test_driver_lightcone_output(tmp_path):
         ids.append(set(pid.tolist()))
     for a_, b_ in zip(ids, ids[1:]):
         assert len(a_ & b_) <= 0.05 * max(len(b_), 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("single", [False, True])
+def test_driver_template_lattice(tmp_path, single):
+    """main_pm -template: loadHomogeneousTemplate, initialize(info, lat, boxSize,
+    capacity, extra), initializeParticlePositions (addParticle_global per
+    particle), updateRowBuffers and sampleParticles(Real*, Real*, long*, n) of
+    the façade on the reference's sc1 template: the reference's IDs, particles
+    at rest, and positions equal, particle for particle, to the cell-centred
+    lattice gev_pcls_init_lattice builds"""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gevpm
+    N = 16
+    out = str(tmp_path / "lattice.bin")
+    exe = os.path.join(LIB, "main_pm_single" if single else "main_pm")
+    tmpl = os.path.join(ROOT, "tests", "golden", "sc1_crystal.dat")
+    cmd = [exe, "-n", str(N), "-template", tmpl, "-out", out]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    pid, pos, q = _read_out(out)
+    nt = N // 4
+    assert np.array_equal(pid, np.arange(N ** 3))
+    assert np.all(q == 0.0)
+    # the reference's ID: i + 64 (xtile + nt (ytile + nt ztile)), i the template index
+    i = pid % 64
+    tile = pid // 64
+    xt, yt, zt = tile % nt, (tile // nt) % nt, tile // (nt * nt)
+    t = np.fromfile(tmpl, np.float32, 3 * 64, offset=268).reshape(64, 3).astype(np.float64)
+    expect = np.stack([(xt + t[i, 0]) / nt, (yt + t[i, 1]) / nt, (zt + t[i, 2]) / nt])
+    assert np.array_equal(pos, expect)
+    ctx = gevpm.Context(N, gevpm.GEV_MIXED if single else gevpm.GEV_FP64)
+    p = gevpm.Particles(ctx, N ** 3, 1.0)
+    p.init_lattice()
+    lpos, _, lid = (x.cpu().numpy() for x in p.store())
+    lattice = lpos[:, np.argsort(lid)]
+    cells = np.floor(pos * N).astype(np.int64)
+    key = cells[0] + N * (cells[1] + N * cells[2])
+    assert np.array_equal(pos, lattice[:, key])  # the same point set, bit for bit

@@ -3,8 +3,8 @@
 // with the CIC window divided out (the deconvolved density spectrum of
 // output.hpp:3006-3033), sums in FP64.
 //
-// One thread owns one segment of a (kx, ky) column of the half spectrum and
-// walks it along the slowest index kz, so the lanes of a wavefront read
+// One thread owns one segment of a (kx, ky) column of the half spectrum at a
+// time and walks it along the slowest index kz, so the lanes of a wavefront read
 // consecutive kx of the same row (coalesced).  Along the column |k| rises for
 // kz = 0..N/2 and falls after, so the bin changes only a few times per
 // segment: the thread keeps the five sums of its current bin in registers and
@@ -15,6 +15,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,7 +29,8 @@ constexpr int PK_MAX_BINS = 1024;
 
 // Per bin: occupation, sum k, sum k^2, sum P k^3 / W^2, sum (P k^3 / W^2)^2,
 // with the weight 2 of a mode whose conjugate is not stored (tools.hpp:180-290).
-// grid: x over columns (PK_NT per workgroup), y over kz segments of zlen;
+// grid: x over columns (PK_NT per workgroup; a grid narrower than the columns
+// strides over them), y over kz segments of zlen;
 // partials[j * groups + g], g = blockIdx.y * gridDim.x + blockIdx.x
 template <typename T>
 __global__ __launch_bounds__(PK_NT) void k_pk_columns(const cx<T> *__restrict__ F, int N, int P, int numbins,
@@ -48,8 +50,8 @@ __global__ __launch_bounds__(PK_NT) void k_pk_columns(const cx<T> *__restrict__ 
     }
     __syncthreads();
     const double kmax = sqrt(3.0 * tk2[H]), inv_kmax = 1.0 / kmax;
-    const int64_t col = (int64_t)blockIdx.x * PK_NT + threadIdx.x;
-    if (col < (int64_t)N * (H + 1)) {
+    const int64_t ncols = (int64_t)N * (H + 1);
+    for (int64_t col = (int64_t)blockIdx.x * PK_NT + threadIdx.x; col < ncols; col += (int64_t)gridDim.x * PK_NT) {
         const int kx = (int)(col % (H + 1)), ky = (int)(col / (H + 1));
         const double kxy2 = tk2[kx] + tk2[ky], ixy = isn[kx] * isn[ky];
         const double w = (kx == 0 || kx == H) ? 1.0 : 2.0;
@@ -124,40 +126,41 @@ __global__ __launch_bounds__(PK_NT) void k_pk_reduce(const double *__restrict__ 
     }
 }
 
-template <typename T>
-static int power_spectrum_impl(gev_ctx *c, const cx<T> *F, int numbins, int deconvolve, double *sums_host)
+// Launch shape of one call: nbx workgroups across the columns, nseg kz
+// segments; every workgroup writes all 5 numbins of its partials.
+struct PkShape {
+    int nbx, nseg, zlen;
+    int64_t groups() const { return (int64_t)nbx * nseg; }
+};
+
+static PkShape pk_shape(const gev_ctx *c)
 {
     const int N = c->N;
     const int64_t cols = (int64_t)N * (N / 2 + 1);
-    const int nbx = (int)((cols + PK_NT - 1) / PK_NT);
+    PkShape s;
+    s.nbx = (int)((cols + PK_NT - 1) / PK_NT);
     // split the columns into kz segments until the launch holds about 2048
     // threads per compute unit, keeping segments of at least 16 modes
     int nseg = (int)std::min<int64_t>(std::max<int64_t>(1, (int64_t)c->ncu * 2048 / cols), std::max(1, N / 16));
-    const int zlen = (N + nseg - 1) / nseg;
-    nseg = (N + zlen - 1) / zlen;
-    const int nb = nbx * nseg;
+    s.zlen = (N + nseg - 1) / nseg;
+    s.nseg = (N + s.zlen - 1) / s.zlen;
+    return s;
+}
+
+// buf: groups * 5 numbins partials followed by the 5 numbins sums
+template <typename T>
+static int power_spectrum_launch(gev_ctx *c, const cx<T> *F, int numbins, int deconvolve, const PkShape &sh,
+                                 double *buf, double *sums_host)
+{
     const int nvals = 5 * numbins;
-    const size_t shm = sizeof(double) * (5 * (size_t)numbins + 2 * (size_t)N);
-    if (shm > 65536) {
-        gev_set_error("gev_power_spectrum: numbins too large for this mesh (LDS)");
-        return GEV_ERR_ARG;
-    }
-    double *buf = nullptr;
-    hipError_t e = hipMallocAsync((void **)&buf, sizeof(double) * ((size_t)nb + 1) * nvals, c->stream);
-    if (e != hipSuccess) {
-        gev_set_error(std::string("gev_power_spectrum: ") + hipGetErrorString(e));
-        return GEV_ERR_NOMEM;
-    }
+    const int64_t nb = sh.groups();
     double *out = buf + (size_t)nb * nvals;
-    hipLaunchKernelGGL((k_pk_columns<T>), dim3(nbx, nseg), dim3(PK_NT), shm, c->stream, F, N, c->P, numbins,
-                       deconvolve, zlen, buf);
-    hipLaunchKernelGGL(k_pk_reduce, dim3(nvals), dim3(PK_NT), 0, c->stream, buf, nb, out);
-    e = hipGetLastError();
+    const size_t shm = sizeof(double) * (5 * (size_t)numbins + 2 * (size_t)c->N);
+    hipLaunchKernelGGL((k_pk_columns<T>), dim3(sh.nbx, sh.nseg), dim3(PK_NT), shm, c->stream, F, c->N, c->P, numbins,
+                       deconvolve, sh.zlen, buf);
+    hipLaunchKernelGGL(k_pk_reduce, dim3(nvals), dim3(PK_NT), 0, c->stream, buf, (int)nb, out);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(sums_host, out, sizeof(double) * nvals, hipMemcpyDeviceToHost, c->stream);
-    const hipError_t e2 = hipFreeAsync(buf, c->stream);
-    if (e == hipSuccess) e = e2;
-    const hipError_t e3 = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = e3;
     if (e != hipSuccess) {
         gev_set_error(std::string("gev_power_spectrum: ") + hipGetErrorString(e));
         return GEV_ERR_HIP;
@@ -165,27 +168,119 @@ static int power_spectrum_impl(gev_ctx *c, const cx<T> *F, int numbins, int deco
     return GEV_OK;
 }
 
+static int pk_lds_check(const gev_ctx *c, int numbins)
+{
+    const size_t shm = sizeof(double) * (5 * (size_t)numbins + 2 * (size_t)c->N);
+    if (shm > 65536) {
+        gev_set_error("gev_power_spectrum: numbins too large for this mesh (LDS)");
+        return GEV_ERR_ARG;
+    }
+    return GEV_OK;
+}
+
+template <typename T>
+static int power_spectrum_impl(gev_ctx *c, const cx<T> *F, int numbins, int deconvolve, double *sums_host)
+{
+    if (int rc = pk_lds_check(c, numbins)) return rc;
+    const PkShape sh = pk_shape(c);
+    const int nvals = 5 * numbins;
+    double *buf = nullptr;
+    hipError_t e = hipMallocAsync((void **)&buf, sizeof(double) * ((size_t)sh.groups() + 1) * nvals, c->stream);
+    if (e != hipSuccess) {
+        gev_set_error(std::string("gev_power_spectrum: ") + hipGetErrorString(e));
+        return GEV_ERR_NOMEM;
+    }
+    int rc = power_spectrum_launch<T>(c, F, numbins, deconvolve, sh, buf, sums_host);
+    e = hipFreeAsync(buf, c->stream);
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = e2;
+    if (rc == GEV_OK && e != hipSuccess) {
+        gev_set_error(std::string("gev_power_spectrum: ") + hipGetErrorString(e));
+        rc = GEV_ERR_HIP;
+    }
+    return rc;
+}
+
+// ------------------------------------------------ allocation-free variant --
+// Workspace of gev_power_spectrum_noalloc: the per-workgroup bin partials and
+// the reduced sums, PK_SCRATCH_DOUBLES doubles (4 MiB) in every device's
+// memory from the library's load until the process ends.  Host code reaches
+// it only through the address that hipGetSymbolAddress returns for the
+// current device (scratch_on below); the symbol itself is a host-side stand-in.
+constexpr size_t PK_SCRATCH_DOUBLES = (size_t)1 << 19;
+static __device__ __align__(256) double scratch[PK_SCRATCH_DOUBLES];
+
+constexpr int PK_MAX_DEVICES = 64;
+static double *g_scratch_ptr[PK_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller's current device is restored)
+static int scratch_on(int device, double **out)
+{
+    if (device < 0 || device >= PK_MAX_DEVICES) {
+        gev_set_error("gev_power_spectrum_noalloc: device ordinal out of range");
+        return GEV_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int prev = -1;
+        GEV_HIP(hipGetDevice(&prev));
+        if (prev != device) GEV_HIP(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != device) GEV_HIP(hipSetDevice(prev));
+        if (e != hipSuccess || !p) {
+            gev_set_error(std::string("gev_power_spectrum_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return GEV_ERR_HIP;
+        }
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return GEV_OK;
+}
+
+// Same sums as power_spectrum_impl, with the partials in the device's static
+// workspace: the column grid is narrowed (the kernel strides over the
+// columns) until the partials of all workgroups and the sums fit.  Every
+// partial the reduction reads is written by the same call, so nothing in the
+// workspace needs clearing; two calls on one device that overlap in time
+// share the workspace and race.
+template <typename T>
+static int power_spectrum_noalloc_impl(gev_ctx *c, const cx<T> *F, int numbins, int deconvolve, double *sums_host)
+{
+    if (int rc = pk_lds_check(c, numbins)) return rc;
+    const int64_t nvals = 5 * (int64_t)numbins;
+    const int64_t max_groups = (int64_t)(PK_SCRATCH_DOUBLES / (size_t)nvals) - 1;
+    if (max_groups < 1) {
+        gev_set_error("gev_power_spectrum_noalloc: 5 * numbins * 2 doubles exceed the 4 MiB workspace");
+        return GEV_ERR_ARG;
+    }
+    PkShape sh = pk_shape(c);
+    if (sh.groups() > max_groups) {
+        sh.nseg = 1;
+        sh.zlen = c->N;
+        sh.nbx = (int)std::min<int64_t>(sh.nbx, max_groups);
+    }
+    double *buf = nullptr;
+    if (int rc = scratch_on(c->device, &buf)) return rc;
+    int rc = power_spectrum_launch<T>(c, F, numbins, deconvolve, sh, buf, sums_host);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (rc == GEV_OK && e != hipSuccess) {
+        gev_set_error(std::string("gev_power_spectrum_noalloc: ") + hipGetErrorString(e));
+        rc = GEV_ERR_HIP;
+    }
+    return rc;
+}
+
 }  // namespace gev
 
 using namespace gev;
 
-extern "C" int gev_power_spectrum(gev_ctx *c, const void *F, int numbins, int deconvolve, double *kbin,
-                                  double *power, double *kscatter, double *pscatter, int64_t *occupation)
+// per-bin normalisation of extractPowerSpectrum (tools.hpp:300-342)
+static void pk_normalise(const double *s, int numbins, double *kbin, double *power, double *kscatter,
+                         double *pscatter, int64_t *occupation)
 {
-    if (!c || !F || !kbin || !power || !kscatter || !pscatter || !occupation) {
-        gev_set_error("gev_power_spectrum: NULL argument");
-        return GEV_ERR_ARG;
-    }
-    if (numbins < 1 || numbins > PK_MAX_BINS) {
-        gev_set_error("gev_power_spectrum: numbins must be in [1, 1024]");
-        return GEV_ERR_ARG;
-    }
-    std::vector<double> s(5 * (size_t)numbins);
-    const int rc = c->prec == GEV_FP64
-                       ? power_spectrum_impl<double>(c, (const cx<double> *)F, numbins, deconvolve, s.data())
-                       : power_spectrum_impl<float>(c, (const cx<float> *)F, numbins, deconvolve, s.data());
-    if (rc) return rc;
-    // per-bin normalisation of extractPowerSpectrum (tools.hpp:300-342)
     for (int b = 0; b < numbins; b++) {
         const double occ = s[b], ks = s[numbins + b], k2s = s[2 * numbins + b], ps = s[3 * numbins + b],
                      p2s = s[4 * numbins + b];
@@ -199,5 +294,45 @@ extern "C" int gev_power_spectrum(gev_ctx *c, const void *F, int numbins, int de
             kbin[b] = power[b] = kscatter[b] = pscatter[b] = 0.0;
         }
     }
+}
+
+static int pk_args(const char *fn, gev_ctx *c, const void *F, int numbins, const double *kbin, const double *power,
+                   const double *kscatter, const double *pscatter, const int64_t *occupation)
+{
+    if (!c || !F || !kbin || !power || !kscatter || !pscatter || !occupation) {
+        gev_set_error(std::string(fn) + ": NULL argument");
+        return GEV_ERR_ARG;
+    }
+    if (numbins < 1 || numbins > PK_MAX_BINS) {
+        gev_set_error(std::string(fn) + ": numbins must be in [1, 1024]");
+        return GEV_ERR_ARG;
+    }
+    return GEV_OK;
+}
+
+extern "C" int gev_power_spectrum(gev_ctx *c, const void *F, int numbins, int deconvolve, double *kbin,
+                                  double *power, double *kscatter, double *pscatter, int64_t *occupation)
+{
+    if (int rc = pk_args("gev_power_spectrum", c, F, numbins, kbin, power, kscatter, pscatter, occupation)) return rc;
+    std::vector<double> s(5 * (size_t)numbins);
+    const int rc = c->prec == GEV_FP64
+                       ? power_spectrum_impl<double>(c, (const cx<double> *)F, numbins, deconvolve, s.data())
+                       : power_spectrum_impl<float>(c, (const cx<float> *)F, numbins, deconvolve, s.data());
+    if (rc) return rc;
+    pk_normalise(s.data(), numbins, kbin, power, kscatter, pscatter, occupation);
+    return GEV_OK;
+}
+
+extern "C" int gev_power_spectrum_noalloc(gev_ctx *c, const void *F, int numbins, int deconvolve, double *kbin,
+                                          double *power, double *kscatter, double *pscatter, int64_t *occupation)
+{
+    if (int rc = pk_args("gev_power_spectrum_noalloc", c, F, numbins, kbin, power, kscatter, pscatter, occupation))
+        return rc;
+    double s[5 * PK_MAX_BINS];  // the sums come back into the host stack, not a heap buffer
+    const int rc = c->prec == GEV_FP64
+                       ? power_spectrum_noalloc_impl<double>(c, (const cx<double> *)F, numbins, deconvolve, s)
+                       : power_spectrum_noalloc_impl<float>(c, (const cx<float> *)F, numbins, deconvolve, s);
+    if (rc) return rc;
+    pk_normalise(s, numbins, kbin, power, kscatter, pscatter, occupation);
     return GEV_OK;
 }

@@ -11,10 +11,12 @@ import torch
 from ._lib import GevError, check, lib
 
 
-def power_spectrum(ctx, F, numbins, deconvolve=True):
+def power_spectrum(ctx, F, numbins, deconvolve=True, noalloc=False):
     """F: (N, N, P) or (1, N, N, P) complex device tensor in the library layout.
     Returns (kbin, power, kscatter, pscatter, occupation) as float64/int64 CPU
-    tensors, power = <|F|^2 k^3 / W^2> per bin as extractPowerSpectrum defines it."""
+    tensors, power = <|F|^2 k^3 / W^2> per bin as extractPowerSpectrum defines it.
+    noalloc=True calls gev_power_spectrum_noalloc (the library's static device
+    workspace instead of a per-call allocation)."""
     N = ctx.N
     want = torch.complex128 if ctx.rdtype == torch.float64 else torch.complex64
     if F.dtype != want or not F.is_cuda or not F.is_contiguous() or F.numel() != N * N * ctx.P:
@@ -23,8 +25,9 @@ def power_spectrum(ctx, F, numbins, deconvolve=True):
     occ = np.zeros(numbins, np.int64)
     dp = [o.ctypes.data_as(C.POINTER(C.c_double)) for o in out]
     ctx.bind_stream()
-    check(lib.gev_power_spectrum(ctx._h, C.c_void_p(F.data_ptr()), int(numbins), int(bool(deconvolve)), *dp,
-                                 occ.ctypes.data_as(C.POINTER(C.c_int64))), "gev_power_spectrum")
+    name = "gev_power_spectrum_noalloc" if noalloc else "gev_power_spectrum"
+    check(getattr(lib, name)(ctx._h, C.c_void_p(F.data_ptr()), int(numbins), int(bool(deconvolve)), *dp,
+                             occ.ctypes.data_as(C.POINTER(C.c_int64))), name)
     return tuple(torch.from_numpy(o) for o in out) + (torch.from_numpy(occ),)
 
 

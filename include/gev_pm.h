@@ -310,6 +310,17 @@ int gev_pcls_save_gadget2_lightcone(gev_pcls *p, const char *path, gev_gadget2_h
 int gev_power_spectrum(gev_ctx *ctx, const void *F, int numbins, int deconvolve, double *kbin, double *power,
                        double *kscatter, double *pscatter, int64_t *occupation);
 
+/* gev_power_spectrum without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results; the workgroup partials live in a static 4 MiB
+ * device workspace that every device holds until the process ends (the
+ * column grid narrows until 5 numbins (groups + 1) doubles fit; numbins as
+ * above, larger requests are refused).  Calls that overlap in time on one
+ * device share that workspace and race. */
+int gev_power_spectrum_noalloc(gev_ctx *ctx, const void *F, int numbins, int deconvolve, double *kbin,
+                               double *power, double *kscatter, double *pscatter, int64_t *occupation);
+
 /* ---- adjoint of the Newtonian cycle (no reference counterpart; reverse mode
  *      of update_q_Newton, update_pos_Newton, the CIC mass deposit and the
  *      Poisson solve, gevolution.hpp:1051-1118, 1270-1273, 1902-1905, 806-852).

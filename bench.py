@@ -41,8 +41,8 @@ def parse():
     ap.add_argument("--adj-mesh", type=int, default=128)
     ap.add_argument("--adj-steps", type=int, default=20)
     ap.add_argument("--adj-modes", type=int, default=64)
-    ap.add_argument("--cpu-mesh", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-mesh", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the mixed-form adjoint workloads (no FP64 gradient, no finite differences), "
                          "print their JSON and exit")
@@ -120,36 +120,51 @@ def main():
         setattr(stm, key, getattr(st64, key))
 
     def run(ctx, pcls, fields, st, label):
+        """warmup cycles, the timed cycles, then the rest of the workload's
+        cycles; each segment is bracketed by events on the library stream's
+        device (synchronised), so the whole run's average is reported beside
+        the timed window's.  Phase times come from the library's own events
+        on its streams, every cycle."""
         gevpm.step_timing(ctx, True)
-        for _ in range(args.warmup):
-            st.dtau = dtau
-            gevpm.step(pcls, fields, st)
-        phases = {}
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        w0 = time.time()
-        for _ in range(args.steps):
-            st.dtau = dtau
-            gevpm.step(pcls, fields, st)
-            for kname, v in gevpm.step_timings(ctx).items():
-                phases[kname] = phases.get(kname, 0.0) + v
-        e1.record()
-        torch.cuda.synchronize()
-        wall = time.time() - w0
-        ms = e0.elapsed_time(e1) / args.steps
-        for _ in range(total - args.warmup - args.steps):
-            st.dtau = dtau
-            gevpm.step(pcls, fields, st)
-        torch.cuda.synchronize()
-        log(f"{label}: {ms:.2f} ms/cycle (wall {1e3 * wall / args.steps:.2f}), z_final={1 / st.a - 1:.4f}")
-        return ms, {kk: v / args.steps for kk, v in phases.items()}
+        phases, phases_all = {}, {}
 
-    ms_m, ph_m = run(cm, pm, fm, stm, "mixed")
+        def segment(ncyc, acc):
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            w0 = time.time()
+            for _ in range(ncyc):
+                st.dtau = dtau
+                gevpm.step(pcls, fields, st)
+                for kname, v in gevpm.step_timings(ctx).items():
+                    for a_ in acc:
+                        a_[kname] = a_.get(kname, 0.0) + v
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) if ncyc else 0.0, time.time() - w0
+
+        z0 = 1.0 / st.a - 1.0
+        ms_w, _ = segment(args.warmup, [phases_all])
+        z1 = 1.0 / st.a - 1.0
+        ms_t, wall = segment(args.steps, [phases, phases_all])
+        z2 = 1.0 / st.a - 1.0
+        ms_r, _ = segment(total - args.warmup - args.steps, [phases_all])
+        ms = ms_t / args.steps
+        log(f"{label}: {ms:.2f} ms/cycle over cycles {args.warmup + 1}-{args.warmup + args.steps} "
+            f"(wall {1e3 * wall / args.steps:.2f}), all {total} cycles {(ms_w + ms_t + ms_r) / total:.2f} ms/cycle, "
+            f"z_final={1 / st.a - 1:.4f}")
+        window = {"timed_cycles": [args.warmup + 1, args.warmup + args.steps], "timed_z": [z1, z2],
+                  "all_cycles": {"cycles": total, "z": [z0, 1.0 / st.a - 1.0],
+                                 "ms_per_step": (ms_w + ms_t + ms_r) / total,
+                                 "value": N ** 3 * 1e3 * total / (ms_w + ms_t + ms_r),
+                                 "phases_ms": {kk: v / total for kk, v in phases_all.items()}}}
+        return ms, {kk: v / args.steps for kk, v in phases.items()}, window
+
+    ms_m, ph_m, win_m = run(cm, pm, fm, stm, "mixed")
     result = {"ic_fp64_s": ic_s, "rows_next": rows_next(cm, pm, fm, stm, dtau, L, om, ol, fpg)}
     if not args.no_fp64:
-        ms_64, ph_64 = run(c64, p64, f64, st64, "fp64")
+        ms_64, ph_64, win_64 = run(c64, p64, f64, st64, "fp64")
         # ---------------------------------------------- errors of the mixed form
         nb = 64
         kb, pw64, _, _, occ = density_power(c64, p64, nb, work=f64.source, F=f64.scalarFT)
@@ -174,13 +189,17 @@ def main():
         d = torch.remainder(pb[:, ob] - pa[:, oa] + 0.5, 1.0) - 0.5
         pos_err = {"rms_cells": float(d.pow(2).sum(0).mean().sqrt() * N), "max_cells": float(d.abs().max() * N)}
         del pa, pb, ia, ib, d
-        result.update(fp64={"value": N ** 3 * 1e3 / ms_64, "ms_per_step": ms_64, "phases_ms": ph_64},
-                      mixed_over_fp64_speedup=ms_64 / ms_m, pk_rel_err_max=pk_err,
+        result.update(fp64={"value": N ** 3 * 1e3 / ms_64, "ms_per_step": ms_64, "phases_ms": ph_64,
+                            "all_cycles": win_64["all_cycles"]},
+                      mixed_over_fp64_speedup=ms_64 / ms_m,
+                      mixed_over_fp64_speedup_all_cycles=win_64["all_cycles"]["ms_per_step"] /
+                      win_m["all_cycles"]["ms_per_step"],
+                      pk_rel_err_max=pk_err,
                       pk_bins_compared=int(sel.sum()), field_errors=fields_err, position_errors=pos_err)
     torch.cuda.empty_cache()
 
     # ----------------------------------------------------- dominant kernel
-    roof = kernel_roofline(cm, pm.n, ph_m)
+    roof = kernel_roofline(cm, pm.n, win_m["all_cycles"]["phases_ms"], total)
     cpu = None if args.no_cpu else cpu_baseline(args, om, ol, k, P)
     if not args.no_adjoint:
         torch.cuda.empty_cache()
@@ -203,7 +222,9 @@ def main():
                                f"L={L:g} Mpc/h, BBKS Zel'dovich IC z={args.z_in:g}, {total} cycles to "
                                f"z={args.z_end:g}, mixed precision (FP32 fields/FFT/deposit, FP64 k=0, "
                                "chi combination, sums)",
-                   "mesh": N, "particles": N ** 3, "boxsize_mpc_h": L, "cycles": total},
+                   "mesh": N, "particles": N ** 3, "boxsize_mpc_h": L, "cycles": total,
+                   "timed_cycles": win_m["timed_cycles"], "timed_z": win_m["timed_z"]},
+        "all_cycles": win_m["all_cycles"],
         "phases_ms": ph_m,
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -269,7 +290,7 @@ def _bytes_models(M, n, R=4, links=True):
     # links: the GR step sums T0i on the links in the T00+Tij pass (128 <= N <= 1024)
     nc = 10 if links else 7
     return {
-        "kick_drift": (n * (7 * R + 7 * R + 8) + M * 5 * R,
+        "kick_drift": (n * (7 * R + 8 * R + 8) + M * 5 * R,
                        "k_update_tiled (fused GR kick + drift): per particle 3 offsets + 3 momenta + cell read, "
                        "3 offsets + 3 momenta + cell + rank written, cell counter read + written; per cell phi, "
                        "chi, B0..B2 read"),
@@ -298,19 +319,31 @@ def measured_traffic(kernel):
     return None
 
 
-def kernel_roofline(ctx, n, phases):
-    """Roofline of the kernel with the largest share of the timed cycle.  Its
-    launch duration is the library's own HIP-event timing of that phase on the
-    library stream, averaged over the timed cycles (the phase brackets exactly
-    that one kernel)."""
+# the library kernel each single-kernel phase brackets (mixed form)
+# (the names a rocprofv3 kernel trace shows)
+_PHASE_KERNEL = {
+    "deposit_T00_Tij": "gev::k_deposit<float, 5, 16, 4, 4>",
+    "kick_drift": "gev::k_update_tiled<float, 16, 8, 8, 5, 512>",
+    "deposit_T0i": "gev::k_deposit<float, 1, 32, 4, 4>",
+}
+
+
+def kernel_roofline(ctx, n, phases, cycles):
+    """Roofline of the single kernel with the largest time over all cycles of
+    the workload.  Its launch duration is the library's own HIP-event timing
+    of that phase on the library stream (the phase brackets that one kernel
+    and a one-word fill), averaged over every cycle of the run, so it can be
+    compared with the kernel's average duration in a rocprofv3 kernel trace
+    of the same command."""
     models = _bytes_models(ctx.M, n, links=128 <= ctx.N <= 1024)
     name = max(models, key=lambda kk: phases.get(kk, 0.0))
     bytes_per_launch, desc = models[name]
     ms = phases[name]
     gbs = bytes_per_launch / (ms * 1e-3) / 1e9
-    return {"kernel": name, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic(name), "avg_launch_ms": ms,
-            "algorithmic_bytes_per_launch": bytes_per_launch, "bytes_model": desc}
+    return {"kernel": name, "hip_kernel": _PHASE_KERNEL.get(name), "bound": "hbm", "achieved": gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic(name),
+            "avg_launch_ms": ms, "launches_averaged": cycles, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "bytes_model": desc}
 
 
 def adjoint_leg(args, checks=True):
@@ -429,9 +462,9 @@ def adjoint_leg(args, checks=True):
 
 def cpu_baseline(args, om, ol, k, P):
     """The FP64 oracle (oracle/, a C + OpenMP restatement of the reference
-    step) timed on this host: GR cycles on a cpu-mesh^3 sample from the same
-    IC recipe.  The reference CPU build (LATfield2 + FFTW + MPI) cannot be
-    built here."""
+    step) timed on this host: cpu-steps GR cycles of the cpu-mesh^3 workload
+    (by default the benchmark's own 512^3 mesh) from the same IC.  The
+    reference CPU build (LATfield2 + FFTW + MPI) cannot be built here."""
     import numpy as np
     import torch
     import gevpm
@@ -462,9 +495,8 @@ def cpu_baseline(args, om, ol, k, P):
         O.step(N, ost, 1, om / M, fpg, om, ol, args.z_in)
     dt = time.time() - t0
     return {"value": M * args.cpu_steps / dt, "unit": "particle-steps/s", "cores": O.get_threads(),
-            "kind": "port", "sample": f"{args.cpu_steps} GR cycles, {N}^3 particles on {N}^3 mesh "
-                                      f"(L scaled to {L:g} Mpc/h, same mass resolution), FP64 oracle, "
-                                      f"{dt:.1f}s"}
+            "kind": "port", "sample": f"{args.cpu_steps} GR cycles from the z={args.z_in:g} IC, {N}^3 particles "
+                                      f"on {N}^3 mesh, L={L:g} Mpc/h, FP64 oracle, {dt:.1f}s"}
 
 
 if __name__ == "__main__":

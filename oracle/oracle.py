@@ -82,6 +82,9 @@ def lib():
                                _dp, _dp, C.POINTER(C.c_double)]),
         "or_step": (V, [I, Lg, I, D, D, D, D, D, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_power_spectrum": (V, [I, _cp, I, I, _dp, _dp, _dp, _dp, _i32p]),
+        "or_tlm_deposit_mass": (V, [I, Lg, _dp, _dp, D, _dp]),
+        "or_tlm_kick_newton": (V, [I, Lg, _dp, _dp, _dp, _dp, _dp, D]),
+        "or_newton_run_tlm": (D, [I, _dp, vp, _dp, I, _dp, D, D, D, D, vp, vp, vp, C.POINTER(C.c_double)]),
         "or_lightcone_vertices": (I, [_dp, _dp, D, D, D, _dp, _dp, I]),
         "or_lightcone_select": (Lg, [I, Lg, _dp, _dp, _ip, vp, I, _dp, _dp, D, D, D, D, D, D, D, _dp, I, vp, Lg,
                                      _f32p, _f32p, _ip, _u8p, _dp]),
@@ -291,3 +294,42 @@ def lightcone_select(N, pos, q, pid, phi, tracer_factor, lc_vertex, direction, o
                                   float(boxsize), vertices, len(vertices), _ptr(bl), 0 if bl is None else len(bl),
                                   po, vo, io, pl, margin)
     return po[:n].copy(), vo[:n].copy(), io[:n].copy(), pl[:n].astype(bool), margin[:np_].copy()
+
+
+# ------------------------------------------------ tangent-linear Newton run
+def newton_schedule(N, boxsize, z_in, z_end, steps, om=0.31, ol=0.69):
+    """(sched [steps, 4] = a, a_d, dtau_k, dtau per cycle; 4 pi G; cx; cq) of a
+    Newtonian run uniform in conformal time from z_in to z_end, the lattice
+    Zel'dovich map x = x_lat + cx N grad1(psi), q = cq N grad1(psi) with
+    cx = 1 and cq = a f(a) H(a), f = Omega_m(a)^0.55"""
+    fpg = 1.5 * boxsize * boxsize / 2997.92458 / 2997.92458
+    a = 1.0 / (1.0 + z_in)
+    tau0 = particle_horizon(a, fpg, om, ol)
+    dtau = (particle_horizon(1.0 / (1.0 + z_end), fpg, om, ol) - tau0) / steps
+    om_a = om / (om + ol * a ** 3)
+    cq = a * om_a ** 0.55 * hconf(a, fpg, om, ol)
+    sched = np.zeros((steps, 4))
+    dtau_old = 0.0
+    for k in range(steps):
+        a_d = rk4bg(a, fpg, om, ol, 0.5 * dtau)
+        sched[k] = (a, a_d, 0.5 * (dtau + dtau_old), dtau)
+        a = rk4bg(a_d, fpg, om, ol, 0.5 * dtau)
+        dtau_old = dtau
+    return sched, fpg, 1.0, cq
+
+
+def newton_run_tlm(N, psi, target, sched, fpg, cx, cq, v=None, om=0.31, want_state=False):
+    """L (and dL = <grad L, v> when v is given) of the Newtonian run; with
+    want_state also the final rho and particle pos, q indexed by ID"""
+    M = N ** 3
+    psi = _c(np.asarray(psi).reshape(M))
+    target = _c(np.asarray(target).reshape(M))
+    vv = None if v is None else _c(np.asarray(v).reshape(M))
+    rho = np.zeros(M) if want_state else None
+    pos = np.zeros((3, M)) if want_state else None
+    q = np.zeros((3, M)) if want_state else None
+    dL = C.c_double(0.0)
+    L = lib().or_newton_run_tlm(N, psi, _ptr(vv), target, len(sched), _c(sched), fpg, om, cx, cq, _ptr(rho),
+                                _ptr(pos), _ptr(q), C.byref(dL))
+    out = (L, dL.value if v is not None else None)
+    return out + ((rho, pos, q) if want_state else ())

@@ -1,0 +1,576 @@
+// gev_adjoint_gr.hip -- reverse-mode kernels of the weak-field GR cycle
+// (main.cu:608-1317 with the elliptic B, as gev_step runs it):
+//
+//   T00, Tij, L = deposits(x, q; phi_old)           projection_T00 / Tij / T0i functors
+//                                                   (gevolution.hpp:1729-1854, 2582-2965, 2031-2201;
+//                                                   L = T0i link sums without 1 + phi_a + phi_b)
+//   src  = prepareFTsource T00 form(T00, phi_old, chi_old)      (gevolution.hpp:160-218)
+//   phi  = P src                                    (solveModifiedPoissonFT, :806-852)
+//   Sij  = prepareFTsource Tij form(phi, Tij)       (:64-138)
+//   chi  = C Sij                                    (projectFTscalar, :251-340)
+//   B    = V (L (1 + phi_a + phi_b))                (projectFTvector, :449-515, elliptic)
+//   q'   = update_q(x, q; phi, chi, B)              (:888-1011)
+//   x'   = x + dtau update_pos rate(x, q'; phi, chi, B)   (:1162-1232)
+//
+// P, C and V are FFT-diagonal: P and V are real symmetric (self-adjoint) as
+// real-space maps, and C's adjoint is the same transform pair with the
+// conjugate per-mode weights (k_adj_project_ft_scalar).  The per-particle
+// Jacobians come from evaluating the forward functor arithmetic on dual
+// numbers (gev_dual.hpp) seeded in the particle's cell offsets r and momenta
+// q; the field adjoints are scattered with global atomics (their order
+// follows atomic arbitration).  Per-particle adjoints lx, lq are [3][n]
+// arrays indexed by particle ID, as in the Newtonian adjoint; positions in box
+// units (d/dx = N d/dr).
+#include "gev_common.hpp"
+#include "gev_dual.hpp"
+#include "gev_kernels.hpp"
+#include "gev_kops.hpp"
+#include "gev_pcommon.hpp"
+
+namespace gev {
+
+using D6 = Dual<double, 6>;
+
+template <typename T> __device__ __forceinline__ void gadd(T *p, double v) { atomicAdd(p, (T)v); }
+
+// CIC weight of corner b and its derivatives in r
+__device__ __forceinline__ double cw(int b, const double r[3])
+{
+    return (((b >> 2) & 1) ? r[0] : 1.0 - r[0]) * (((b >> 1) & 1) ? r[1] : 1.0 - r[1]) * ((b & 1) ? r[2] : 1.0 - r[2]);
+}
+__device__ __forceinline__ void cw_grad(int b, const double r[3], double g[3])
+{
+    const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+    const double wx = bx ? r[0] : 1.0 - r[0], wy = by ? r[1] : 1.0 - r[1], wz = bz ? r[2] : 1.0 - r[2];
+    g[0] = (bx ? 1.0 : -1.0) * wy * wz;
+    g[1] = wx * (by ? 1.0 : -1.0) * wz;
+    g[2] = wx * wy * (bz ? 1.0 : -1.0);
+}
+
+// field accessor in FP64 around the particle's cell (fields of either precision)
+template <typename T> struct FAt {
+    const T *f;
+    int N, x, y, z;
+    __device__ __forceinline__ double operator()(int dx, int dy, int dz) const
+    {
+        return (double)f[site(N, x + dx, y + dy, z + dz)];
+    }
+};
+
+// update_pos rate (gevolution.hpp:1162-1232) over a generic real type: the
+// displacement per unit dtau, v + b / bcoeff
+template <typename R, class FA>
+__device__ __forceinline__ void drift_rate(const R r[3], const R q[3], const FA &Fp, const FA &Fc, const FA &B0,
+                                           const FA &B1, const FA &B2, int nfield, double a, double bcoeff, R out[3])
+{
+    R v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const R e2 = v2 + a * a;
+    R ph = (R)0, ch = (R)0;
+    if (nfield >= 1) ph = cic<R>(Fp, r);
+    if (nfield >= 2) ch = cic<R>(Fc, r);
+    const R ie = frsqrt(e2);
+    v2 = ((R)1 + ((R)3 - v2 * ie * ie) * ph - ch) * ie;
+#pragma unroll
+    for (int l = 0; l < 3; l++) out[l] = q[l] * v2;
+    if (nfield >= 3) {
+        const R o = (R)1;
+        R b[3];
+        b[0] = B0(0, 0, 0) * (o - r[1]) * (o - r[2]);
+        b[1] = B1(0, 0, 0) * (o - r[0]) * (o - r[2]);
+        b[2] = B2(0, 0, 0) * (o - r[0]) * (o - r[1]);
+        b[1] += B1(1, 0, 0) * r[0] * (o - r[2]);
+        b[2] += B2(1, 0, 0) * r[0] * (o - r[1]);
+        b[0] += B0(0, 1, 0) * r[1] * (o - r[2]);
+        b[2] += B2(0, 1, 0) * (o - r[0]) * r[1];
+        b[0] += B0(0, 0, 1) * (o - r[1]) * r[2];
+        b[1] += B1(0, 0, 1) * (o - r[0]) * r[2];
+        b[1] += B1(1, 0, 1) * r[0] * r[2];
+        b[0] += B0(0, 1, 1) * r[1] * r[2];
+        b[2] += B2(1, 1, 0) * r[0] * r[1];
+        const double ibc = 1.0 / bcoeff;
+#pragma unroll
+        for (int l = 0; l < 3; l++) out[l] += b[l] * ibc;
+    }
+}
+
+struct AdjKD {
+    int nf_k, nf_d;
+    double a_k, a_d, dtau_k, dtau_d, bc_k, bc_d;
+};
+
+// Reverse of the fused kick + drift of one particle (state before the kick).
+// In: lx, lq = adjoints after the drift (by ID).  Out: lx, lq before the kick;
+// lphi, lchi, lB (3 components) accumulate the fields' adjoints.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_gr_kick_drift(PV<T> v, const T *__restrict__ phi, const T *__restrict__ chi,
+                                                           const T *__restrict__ Bi, T *lx, T *lq, T *lphi, T *lchi,
+                                                           T *lB, int64_t M, AdjKD s)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= v.n) return;
+    const int N = v.N;
+    const int64_t n = v.n, id = v.id[i];
+    int x, y, z;
+    cell_coords(v.cell[i], N, v.lg, x, y, z);
+    T rt[3];
+    ref_dist(v, i, x, y, z, rt);
+    const double r[3] = {(double)rt[0], (double)rt[1], (double)rt[2]};
+    const double q[3] = {(double)v.q0[i], (double)v.q1[i], (double)v.q2[i]};
+    const FAt<T> Fp{phi, N, x, y, z}, Fc{chi, N, x, y, z}, FB0{Bi, N, x, y, z}, FB1{Bi + M, N, x, y, z},
+        FB2{Bi + 2 * M, N, x, y, z};
+    const double Nd = (double)N;
+
+    // kick on duals seeded in (r, q)
+    D6 rD[3], qk[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) { rD[j] = D6::seed(r[j], j); qk[j] = D6::seed(q[j], 3 + j); }
+    phys_update_q<D6>(rD, qk, Fp, Fc, FB0, FB1, FB2, s.nf_k, D6(s.a_k), D6(s.dtau_k), D6(s.bc_k), D6(Nd));
+    // drift rate on duals seeded in (r, q')
+    D6 qp[3], rate[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) qp[j] = D6::seed(qk[j].v, 3 + j);
+    drift_rate<D6>(rD, qp, Fp, Fc, FB0, FB1, FB2, s.nf_d, s.a_d, s.bc_d, rate);
+
+    double LX[3], LQ[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) { LX[j] = (double)lx[j * n + id]; LQ[j] = (double)lq[j * n + id]; }
+    // adjoint of the kicked momentum: its own plus the drift's use of it
+    double lqt[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        double acc = LQ[j];
+#pragma unroll
+        for (int l = 0; l < 3; l++) acc += s.dtau_d * LX[l] * rate[l].d[3 + j];
+        lqt[j] = acc;
+    }
+    double nx[3], nq[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        double ar = 0.0, aq = 0.0;
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+            ar += s.dtau_d * LX[l] * rate[l].d[j] + lqt[l] * qk[l].d[j];
+            aq += lqt[l] * qk[l].d[3 + j];
+        }
+        nx[j] = LX[j] + Nd * ar;
+        nq[j] = aq;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) { lx[j * n + id] = (T)nx[j]; lq[j * n + id] = (T)nq[j]; }
+
+    // ---- field adjoints of the kick (pre-kick q), lambda = lqt
+    const double K = s.dtau_k * Nd;
+    const double v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const double e2 = v2 + s.a_k * s.a_k, e = sqrt(e2);
+    auto S = [&](T *f, int dx, int dy, int dz, double val) { gadd(f + site(N, x + dx, y + dy, z + dz), val); };
+    {
+        // g = Phibar grad2(phi) - grad1(chi) + pgradB / (bcoeff e); q' = q - K e g
+        const double Phi = cic<double>(Fp, r);
+        const double ph = Phi / e2;
+        const double Phibar = 4.0 * ph * v2 + (1.0 + s.a_k * s.a_k * ph) * (v2 + e2) / e2;
+        const double dPhibar = (4.0 * v2 + s.a_k * s.a_k * (v2 + e2) / e2) / e2;
+        double G2[3];
+        grad2<double>(Fp, r, G2);
+        const double lG2[3] = {-K * e * Phibar * lqt[0], -K * e * Phibar * lqt[1], -K * e * Phibar * lqt[2]};
+        const double lPhi = -K * e * dPhibar * (G2[0] * lqt[0] + G2[1] * lqt[1] + G2[2] * lqt[2]);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+            const double w = cw(b, r);
+            S(lphi, bx, by, bz, w * lPhi);
+            const double hw = 0.5 * w;
+            S(lphi, bx + 1, by, bz, hw * lG2[0]);
+            S(lphi, bx - 1, by, bz, -hw * lG2[0]);
+            S(lphi, bx, by + 1, bz, hw * lG2[1]);
+            S(lphi, bx, by - 1, bz, -hw * lG2[1]);
+            S(lphi, bx, by, bz + 1, hw * lG2[2]);
+            S(lphi, bx, by, bz - 1, -hw * lG2[2]);
+        }
+        if (s.nf_k >= 2) {
+            const double lG1[3] = {K * e * lqt[0], K * e * lqt[1], K * e * lqt[2]};
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                double gw[3];
+                cw_grad(b, r, gw);
+                S(lchi, (b >> 2) & 1, (b >> 1) & 1, b & 1, gw[0] * lG1[0] + gw[1] * lG1[1] + gw[2] * lG1[2]);
+            }
+        }
+        if (s.nf_k >= 3) {
+            // pgradB (gevolution.hpp:968-994) is linear in B: its transpose, term by term
+            const double c = -K / s.bc_k;
+            const double l0 = c * lqt[0], l1 = c * lqt[1], l2 = c * lqt[2];
+            T *L0 = lB, *L1 = lB + M, *L2 = lB + 2 * M;
+            const double o = 1.0;
+            // pg[0]
+            double t = l0 * q[1];
+            S(L1, 1, 0, 0, t * (o - r[2])); S(L1, 0, 0, 0, -t * (o - r[2]));
+            S(L1, 1, 0, 1, t * r[2]); S(L1, 0, 0, 1, -t * r[2]);
+            t = l0 * q[2];
+            S(L2, 1, 0, 0, t * (o - r[1])); S(L2, 0, 0, 0, -t * (o - r[1]));
+            S(L2, 1, 1, 0, t * r[1]); S(L2, 0, 1, 0, -t * r[1]);
+            t = l0 * q[0];
+#pragma unroll
+            for (int by = 0; by < 2; by++)
+#pragma unroll
+                for (int bz = 0; bz < 2; bz++) {
+                    const double w = (by ? r[1] : o - r[1]) * (bz ? r[2] : o - r[2]) * t;
+                    S(L0, -1, by, bz, w * (r[0] - o)); S(L0, 0, by, bz, w * (o - 2.0 * r[0])); S(L0, 1, by, bz, w * r[0]);
+                }
+            // pg[1]
+            t = l1 * q[2];
+            S(L2, 0, 1, 0, t * (o - r[0])); S(L2, 0, 0, 0, -t * (o - r[0]));
+            S(L2, 1, 1, 0, t * r[0]); S(L2, 1, 0, 0, -t * r[0]);
+            t = l1 * q[0];
+            S(L0, 0, 1, 0, t * (o - r[2])); S(L0, 0, 0, 0, -t * (o - r[2]));
+            S(L0, 0, 1, 1, t * r[2]); S(L0, 0, 0, 1, -t * r[2]);
+            t = l1 * q[1];
+#pragma unroll
+            for (int bx = 0; bx < 2; bx++)
+#pragma unroll
+                for (int bz = 0; bz < 2; bz++) {
+                    const double w = (bx ? r[0] : o - r[0]) * (bz ? r[2] : o - r[2]) * t;
+                    S(L1, bx, -1, bz, w * (r[1] - o)); S(L1, bx, 0, bz, w * (o - 2.0 * r[1])); S(L1, bx, 1, bz, w * r[1]);
+                }
+            // pg[2]
+            t = l2 * q[0];
+            S(L0, 0, 0, 1, t * (o - r[1])); S(L0, 0, 0, 0, -t * (o - r[1]));
+            S(L0, 0, 1, 1, t * r[1]); S(L0, 0, 1, 0, -t * r[1]);
+            t = l2 * q[1];
+            S(L1, 0, 0, 1, t * (o - r[0])); S(L1, 0, 0, 0, -t * (o - r[0]));
+            S(L1, 1, 0, 1, t * r[0]); S(L1, 1, 0, 0, -t * r[0]);
+            t = l2 * q[2];
+#pragma unroll
+            for (int bx = 0; bx < 2; bx++)
+#pragma unroll
+                for (int by = 0; by < 2; by++) {
+                    const double w = (bx ? r[0] : o - r[0]) * (by ? r[1] : o - r[1]) * t;
+                    S(L2, bx, by, -1, w * (r[2] - o)); S(L2, bx, by, 0, w * (o - 2.0 * r[2])); S(L2, bx, by, 1, w * r[2]);
+                }
+        }
+    }
+    // ---- field adjoints of the drift (kicked q'), lambda = LX
+    {
+        const double qq[3] = {qk[0].v, qk[1].v, qk[2].v};
+        const double w2 = qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2];
+        const double f2 = w2 + s.a_d * s.a_d, ie = 1.0 / sqrt(f2);
+        const double qL = qq[0] * LX[0] + qq[1] * LX[1] + qq[2] * LX[2];
+        const double lph = s.dtau_d * qL * (3.0 - w2 / f2) * ie, lch = -s.dtau_d * qL * ie;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const double w = cw(b, r);
+            const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+            if (s.nf_d >= 1) S(lphi, bx, by, bz, w * lph);
+            if (s.nf_d >= 2) S(lchi, bx, by, bz, w * lch);
+        }
+        if (s.nf_d >= 3) {
+            const double c = s.dtau_d / s.bc_d, o = 1.0;
+            const double b0 = c * LX[0], b1 = c * LX[1], b2 = c * LX[2];
+            T *L0 = lB, *L1 = lB + M, *L2 = lB + 2 * M;
+            S(L0, 0, 0, 0, b0 * (o - r[1]) * (o - r[2])); S(L0, 0, 1, 0, b0 * r[1] * (o - r[2]));
+            S(L0, 0, 0, 1, b0 * (o - r[1]) * r[2]);       S(L0, 0, 1, 1, b0 * r[1] * r[2]);
+            S(L1, 0, 0, 0, b1 * (o - r[0]) * (o - r[2])); S(L1, 1, 0, 0, b1 * r[0] * (o - r[2]));
+            S(L1, 0, 0, 1, b1 * (o - r[0]) * r[2]);       S(L1, 1, 0, 1, b1 * r[0] * r[2]);
+            S(L2, 0, 0, 0, b2 * (o - r[0]) * (o - r[1])); S(L2, 1, 0, 0, b2 * r[0] * (o - r[1]));
+            S(L2, 0, 1, 0, b2 * (o - r[0]) * r[1]);       S(L2, 1, 1, 0, b2 * r[0] * r[1]);
+        }
+    }
+}
+
+// Reverse of the T00 + Tij + T0i-link deposits of one particle (phi_old the
+// deposits' phi; masses per unit of the reference's coeff, as
+// gev_project_T00_Tij_T0i_links scales them).  In: lT00, lTij (6), lL (3)
+// field adjoints.  Out: lx, lq accumulate; lphi accumulates phi_old's adjoint.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_gr_deposit(PV<T> v, const T *__restrict__ phi, const T *__restrict__ lT00,
+                                                        const T *__restrict__ lTij, const T *__restrict__ lL, T *lx,
+                                                        T *lq, T *lphi, int64_t M, double m00, double mij, double m0i,
+                                                        double a)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= v.n) return;
+    const int N = v.N;
+    const int64_t n = v.n, id = v.id[i];
+    int x, y, z;
+    cell_coords(v.cell[i], N, v.lg, x, y, z);
+    T rt[3];
+    ref_dist(v, i, x, y, z, rt);
+    const double r[3] = {(double)rt[0], (double)rt[1], (double)rt[2]};
+    const double q[3] = {(double)v.q0[i], (double)v.q1[i], (double)v.q2[i]};
+    auto F = [&](const T *f, int dx, int dy, int dz) { return (double)f[site(N, x + dx, y + dy, z + dz)]; };
+    D6 rD[3], qD[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) { rD[j] = D6::seed(r[j], j); qD[j] = D6::seed(q[j], 3 + j); }
+    const D6 o(1.0);
+    D6 W[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+        W[b] = (((b >> 2) & 1) ? rD[0] : o - rD[0]) * (((b >> 1) & 1) ? rD[1] : o - rD[1]) * ((b & 1) ? rD[2] : o - rD[2]);
+    double ph[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) ph[b] = phi ? F(phi, (b >> 2) & 1, (b >> 1) & 1, b & 1) : 0.0;
+    // T00 (gevolution.hpp:1729-1854)
+    const D6 q2 = qD[0] * qD[0] + qD[1] * qD[1] + qD[2] * qD[2];
+    const D6 e = fsqrt(q2 + a * a), ie = frcp(e);
+    const D6 f = 3.0 * e + q2 * ie;
+    const D6 f2 = (0.5 * f * f + q2 * frcp(o + q2 * (1.0 / (a * a)))) * ie;
+    D6 Ssum(0.0);
+    double lt[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        lt[b] = m00 * F(lT00, (b >> 2) & 1, (b >> 1) & 1, b & 1);
+        Ssum += W[b] * (e + f * ph[b] + f2 * (ph[b] * ph[b])) * lt[b];
+    }
+    // Tij (gevolution.hpp:2582-2965, no CIC_PROJECT_TIJ) without its factor fT
+    const D6 e2 = q2 + a * a, iet = frsqrt(e2);
+    D6 pint(0.0);
+#pragma unroll
+    for (int b = 0; b < 8; b++) pint += W[b] * ph[b];
+    const D6 cf = 4.0 + (a * a) * frcp(e2);
+    const D6 fT = o + cf * pint;
+    D6 Sij(0.0);
+    const int dc[3] = {0, 3, 5};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        D6 acc(0.0);
+#pragma unroll
+        for (int b = 0; b < 8; b++) acc += W[b] * F(lTij + dc[d] * M, (b >> 2) & 1, (b >> 1) & 1, b & 1);
+        Sij += acc * qD[d] * qD[d] * iet;
+    }
+    Sij += qD[0] * qD[1] * iet * ((o - rD[2]) * F(lTij + M, 0, 0, 0) + rD[2] * F(lTij + M, 0, 0, 1));
+    Sij += qD[0] * qD[2] * iet * ((o - rD[1]) * F(lTij + 2 * M, 0, 0, 0) + rD[1] * F(lTij + 2 * M, 0, 1, 0));
+    Sij += qD[1] * qD[2] * iet * ((o - rD[0]) * F(lTij + 4 * M, 0, 0, 0) + rD[0] * F(lTij + 4 * M, 1, 0, 0));
+    Sij = Sij * mij;
+    Ssum += fT * Sij;
+    // T0i links (gevolution.hpp:2031-2201) without the link factor
+    if (lL) {
+        const T *L0 = lL, *L1 = lL + M, *L2 = lL + 2 * M;
+        D6 Sl = (o - rD[1]) * (o - rD[2]) * qD[0] * F(L0, 0, 0, 0) + (o - rD[0]) * (o - rD[2]) * qD[1] * F(L1, 0, 0, 0) +
+                (o - rD[0]) * (o - rD[1]) * qD[2] * F(L2, 0, 0, 0);
+        Sl += rD[0] * (o - rD[2]) * qD[1] * F(L1, 1, 0, 0) + rD[0] * (o - rD[1]) * qD[2] * F(L2, 1, 0, 0);
+        Sl += rD[1] * (o - rD[2]) * qD[0] * F(L0, 0, 1, 0) + rD[1] * (o - rD[0]) * qD[2] * F(L2, 0, 1, 0);
+        Sl += rD[2] * (o - rD[1]) * qD[0] * F(L0, 0, 0, 1) + rD[2] * (o - rD[0]) * qD[1] * F(L1, 0, 0, 1);
+        Sl += rD[1] * rD[2] * qD[0] * F(L0, 0, 1, 1) + rD[0] * rD[2] * qD[1] * F(L1, 1, 0, 1) +
+              rD[0] * rD[1] * qD[2] * F(L2, 1, 1, 0);
+        Ssum += Sl * m0i;
+    }
+    const double Nd = (double)N;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        lx[j * n + id] += (T)(Nd * Ssum.d[j]);
+        lq[j * n + id] += (T)Ssum.d[3 + j];
+    }
+    if (phi && lphi) {
+        // d/dphi_b: T00's e + f phi + f2 phi^2 per corner, Tij's fT through the CIC of phi
+        const double tt = Sij.v * cf.v;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const double val = W[b].v * (lt[b] * (f.v + 2.0 * f2.v * ph[b]) + tt);
+            gadd(lphi + site(N, x + ((b >> 2) & 1), y + ((b >> 1) & 1), z + (b & 1)), val);
+        }
+    }
+}
+
+// Reverse of prepareFTsource, T00 form (gevolution.hpp:160-218):
+//   src = c2 (T00 - bg)(1 - 2 phi (1 - phi)) + 1/8 sum_i (phi(s-e_i) - phi(s+e_i))^2
+//         + (c3 (1 - 3 phi) - c1) phi - c3 chi
+// lT00 = d src / dT00 lsrc; lphi += ...; lchi += -c3 lsrc
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_source_T00(const T *__restrict__ phi, const T *__restrict__ T00,
+                                                        const T *__restrict__ lsrc, T *lT00, T *lphi, T *lchi, int N,
+                                                        int64_t M, double bg, double c1, double c2, double c3)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= M) return;
+    const int x = (int)(s % N), y = (int)((s / N) % N), z = (int)(s / ((int64_t)N * N));
+    auto P = [&](int dx, int dy, int dz) { return (double)phi[site(N, x + dx, y + dy, z + dz)]; };
+    auto Ls = [&](int dx, int dy, int dz) { return (double)lsrc[site(N, x + dx, y + dy, z + dz)]; };
+    const double p = P(0, 0, 0), l = Ls(0, 0, 0), t = (double)T00[s];
+    double acc = l * (c2 * (t - bg) * (-2.0 + 4.0 * p) + c3 * (1.0 - 6.0 * p) - c1);
+    // 1/8 (phi(s'-e) - phi(s'+e))^2 at s' = s + e and s' = s - e
+    acc += 0.25 * (Ls(1, 0, 0) * (p - P(2, 0, 0)) - Ls(-1, 0, 0) * (P(-2, 0, 0) - p));
+    acc += 0.25 * (Ls(0, 1, 0) * (p - P(0, 2, 0)) - Ls(0, -1, 0) * (P(0, -2, 0) - p));
+    acc += 0.25 * (Ls(0, 0, 1) * (p - P(0, 0, 2)) - Ls(0, 0, -1) * (P(0, 0, -2) - p));
+    lphi[s] = (T)((double)lphi[s] + acc);
+    lT00[s] = (T)(c2 * (1.0 - 2.0 * p * (1.0 - p)) * l);
+    lchi[s] = (T)((double)lchi[s] - c3 * l);
+}
+
+// Reverse of prepareFTsource, Tij form (gevolution.hpp:64-138):
+//   S_ii = 1/2 (phi(s+e_i) - phi(s-e_i))^2 + c T_ii
+//   S_ij = phi_i phi_j - phi phi_ij + 1/2 (phi^2 - phi_i^2 - phi_j^2 + phi_ij^2) + c T_ij
+// (phi_i = phi(s+e_i), phi_ij = phi(s+e_i+e_j)); lphi += the stencil's
+// transpose applied to lS, gathered per site
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_source_Tij_phi(const T *__restrict__ phi, const T *__restrict__ lS, T *lphi,
+                                                            int N, int64_t M)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= M) return;
+    const int x = (int)(s % N), y = (int)((s / N) % N), z = (int)(s / ((int64_t)N * N));
+    auto P = [&](int dx, int dy, int dz) { return (double)phi[site(N, x + dx, y + dy, z + dz)]; };
+    auto L = [&](int c, int dx, int dy, int dz) { return (double)lS[c * M + site(N, x + dx, y + dy, z + dz)]; };
+    const int dcomp[3] = {0, 3, 5};
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int ex = i == 0, ey = i == 1, ez = i == 2;
+        // S_ii at s - e_i uses phi(s) as its +e_i point, at s + e_i as its -e_i point
+        acc += L(dcomp[i], -ex, -ey, -ez) * (P(0, 0, 0) - P(-2 * ex, -2 * ey, -2 * ez));
+        acc -= L(dcomp[i], ex, ey, ez) * (P(2 * ex, 2 * ey, 2 * ez) - P(0, 0, 0));
+    }
+    const int oc[3] = {1, 2, 4};           // (01, 02, 12)
+    const int oi[3] = {0, 0, 1}, oj[3] = {1, 2, 2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int ix = oi[c] == 0, iy = oi[c] == 1, iz = oi[c] == 2;
+        const int jx = oj[c] == 0, jy = oj[c] == 1, jz = oj[c] == 2;
+        const int k = oc[c];
+        const double p = P(0, 0, 0);
+        // s' = s - e_i: d/dphi(s'+e_i) = phi(s'+e_j) - phi(s'+e_i)
+        acc += L(k, -ix, -iy, -iz) * (P(jx - ix, jy - iy, jz - iz) - p);
+        // s' = s - e_j: d/dphi(s'+e_j) = phi(s'+e_i) - phi(s'+e_j)
+        acc += L(k, -jx, -jy, -jz) * (P(ix - jx, iy - jy, iz - jz) - p);
+        // s' = s: d/dphi(s') = phi(s') - phi(s'+e_i+e_j)
+        acc += L(k, 0, 0, 0) * (p - P(ix + jx, iy + jy, iz + jz));
+        // s' = s - e_i - e_j: d/dphi(s'+e_i+e_j) = phi(s'+e_i+e_j) - phi(s')
+        acc += L(k, -ix - jx, -iy - jy, -iz - jz) * (p - P(-ix - jx, -iy - jy, -iz - jz));
+    }
+    lphi[s] = (T)((double)lphi[s] + acc);
+}
+
+// Reverse of the link factor of the B solve's input, L_i (1 + phi(s) + phi(s+e_i)):
+// u = V^T lB (given); lL_i(s) = (1 + phi(s) + phi(s+e_i)) u_i(s) (written over u);
+// lphi(s) += sum_i L_i(s) u_i(s) + L_i(s-e_i) u_i(s-e_i)
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_link_factor_phi(const T *__restrict__ phi, const T *__restrict__ L,
+                                                             const T *__restrict__ u, T *lphi, int N, int64_t M)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= M) return;
+    const int x = (int)(s % N), y = (int)((s / N) % N), z = (int)(s / ((int64_t)N * N));
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int64_t sm = site(N, x - (i == 0), y - (i == 1), z - (i == 2));
+        acc += (double)L[i * M + s] * (double)u[i * M + s] + (double)L[i * M + sm] * (double)u[i * M + sm];
+    }
+    lphi[s] = (T)((double)lphi[s] + acc);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_link_factor_L(const T *__restrict__ phi, T *u, int N, int64_t M)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= M) return;
+    const int x = (int)(s % N), y = (int)((s / N) % N), z = (int)(s / ((int64_t)N * N));
+    const double p = (double)phi[s];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double pe = (double)phi[site(N, x + (i == 0), y + (i == 1), z + (i == 2))];
+        u[i * M + s] = (T)((1.0 + p + pe) * (double)u[i * M + s]);
+    }
+}
+
+// adjoint of projectFTscalar's per-mode map chi = sum_c w_c S_c / (2 k^4 N):
+// lS_c = conj(w_c) lchi / (2 k^4 N), k = 0 -> 0 (6 components out)
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_project_ft_scalar(const cx<T> *__restrict__ lchi, cx<T> *__restrict__ lS,
+                                                               int N, int P, int64_t NK, const double *__restrict__ gk2,
+                                                               const cx<double> *__restrict__ ks)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= NK) return;
+    const int kx = (int)(e % P);
+    const int64_t rr = e / P;
+    const int ky = (int)(rr % N), kz = (int)(rr / N);
+    if (kx > N / 2) return;
+    const cx<T> lv = lchi[e];
+    const cx<double> l = mk((double)lv.x, (double)lv.y);
+    const double g0 = gk2[kx], g1 = gk2[ky], g2 = gk2[kz];
+    const bool zero = kx == 0 && ky == 0 && kz == 0;
+    const double nrm = zero ? 0.0 : kop_scalar_norm(g0 + g1 + g2, N);
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const cx<double> w = conj(kop_scalar_weight(c, g0, g1, g2, ks[kx], ks[ky], ks[kz]));
+        const cx<double> o = (w * l) * nrm;
+        lS[c * NK + e] = mk((T)o.x, (T)o.y);
+    }
+}
+
+static inline unsigned nbl(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+template <typename T>
+int adj_gr_kick_drift_impl(gev_pcls *p, const T *phi, const T *chi, const T *Bi, T *lx, T *lq, T *lphi, T *lchi,
+                           T *lB, int nf_k, int nf_d, double a_k, double a_d, double dtau_k, double dtau_d)
+{
+    gev_ctx *c = p->ctx;
+    AdjKD s{nf_k, nf_d, a_k, a_d, dtau_k, dtau_d, a_k * a_k * (double)c->N, a_d * a_d * (double)c->N};
+    if (p->n > 0)
+        hipLaunchKernelGGL((k_adj_gr_kick_drift<T>), dim3(nbl(p->n)), dim3(256), 0, c->stream, make_pv<T>(p), phi, chi,
+                           Bi, lx, lq, lphi, lchi, lB, c->M, s);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+template <typename T>
+int adj_gr_deposit_impl(gev_pcls *p, const T *phi, const T *lT00, const T *lTij, const T *lL, T *lx, T *lq, T *lphi,
+                        double a)
+{
+    gev_ctx *c = p->ctx;
+    const double dx = 1.0 / (double)c->N, m = p->mass / (dx * dx * dx);
+    if (p->n > 0)
+        hipLaunchKernelGGL((k_adj_gr_deposit<T>), dim3(nbl(p->n)), dim3(256), 0, c->stream, make_pv<T>(p), phi, lT00,
+                           lTij, lL, lx, lq, lphi, c->M, phi ? m / a : m, m / a, m, a);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+template <typename T>
+int adj_source_T00_impl(gev_ctx *c, const T *phi, const T *T00, const T *lsrc, T *lT00, T *lphi, T *lchi, double bg,
+                        double c1, double c2, double c3)
+{
+    hipLaunchKernelGGL((k_adj_source_T00<T>), dim3(nbl(c->M)), dim3(256), 0, c->stream, phi, T00, lsrc, lT00, lphi,
+                       lchi, c->N, c->M, bg, c1, c2, c3);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+template <typename T> int adj_source_Tij_phi_impl(gev_ctx *c, const T *phi, const T *lS, T *lphi)
+{
+    hipLaunchKernelGGL((k_adj_source_Tij_phi<T>), dim3(nbl(c->M)), dim3(256), 0, c->stream, phi, lS, lphi, c->N,
+                       c->M);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+template <typename T> int adj_link_factor_impl(gev_ctx *c, const T *phi, const T *L, T *u, T *lphi)
+{
+    hipLaunchKernelGGL((k_adj_link_factor_phi<T>), dim3(nbl(c->M)), dim3(256), 0, c->stream, phi, L, u, lphi, c->N,
+                       c->M);
+    hipLaunchKernelGGL((k_adj_link_factor_L<T>), dim3(nbl(c->M)), dim3(256), 0, c->stream, phi, u, c->N, c->M);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+template <typename T> int adj_project_ft_scalar_impl(gev_ctx *c, const cx<T> *lchi, cx<T> *lS)
+{
+    hipLaunchKernelGGL((k_adj_project_ft_scalar<T>), dim3(nbl(c->NK)), dim3(256), 0, c->stream, lchi, lS, c->N, c->P,
+                       c->NK, c->gk2, c->kshift);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
+}
+
+#define INST(T)                                                                                                     \
+    template int adj_gr_kick_drift_impl<T>(gev_pcls *, const T *, const T *, const T *, T *, T *, T *, T *, T *, int, \
+                                           int, double, double, double, double);                                 \
+    template int adj_gr_deposit_impl<T>(gev_pcls *, const T *, const T *, const T *, const T *, T *, T *, T *,      \
+                                        double);                                                                   \
+    template int adj_source_T00_impl<T>(gev_ctx *, const T *, const T *, const T *, T *, T *, T *, double, double, \
+                                        double, double);                                                           \
+    template int adj_source_Tij_phi_impl<T>(gev_ctx *, const T *, const T *, T *);                                 \
+    template int adj_link_factor_impl<T>(gev_ctx *, const T *, const T *, T *, T *);                               \
+    template int adj_project_ft_scalar_impl<T>(gev_ctx *, const cx<T> *, cx<T> *);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace gev

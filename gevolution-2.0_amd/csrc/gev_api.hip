@@ -525,6 +525,75 @@ int gev_adj_ic_lattice(gev_ctx *c, const void *lx, const void *lq, double cx, do
                          adj_ic_lattice_impl<float>(c, (const float *)lx, (const float *)lq, cx, cq, (float *)lpsi));
 }
 
+// ---- adjoint of the weak-field GR cycle (gev_adjoint_gr.hip)
+int gev_adj_gr_kick_drift(gev_pcls *p, const void *phi, const void *chi, const void *Bi, void *lx, void *lq,
+                          void *lphi, void *lchi, void *lB, int nfield_kick, int nfield_drift, double a_kick,
+                          double a_drift, double dtau_kick, double dtau_drift)
+{
+    CHECK(p && phi && chi && Bi && lx && lq && lphi && lchi && lB, "gev_adj_gr_kick_drift: NULL argument");
+    CHECK(nfield_kick >= 1 && nfield_kick <= 3 && nfield_drift >= 0 && nfield_drift <= 3,
+          "gev_adj_gr_kick_drift: nfield out of range");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx,
+                         adj_gr_kick_drift_impl<double>(p, (const double *)phi, (const double *)chi,
+                                                        (const double *)Bi, (double *)lx, (double *)lq,
+                                                        (double *)lphi, (double *)lchi, (double *)lB, nfield_kick,
+                                                        nfield_drift, a_kick, a_drift, dtau_kick, dtau_drift),
+                         adj_gr_kick_drift_impl<float>(p, (const float *)phi, (const float *)chi, (const float *)Bi,
+                                                       (float *)lx, (float *)lq, (float *)lphi, (float *)lchi,
+                                                       (float *)lB, nfield_kick, nfield_drift, a_kick, a_drift,
+                                                       dtau_kick, dtau_drift));
+}
+
+int gev_adj_gr_deposit(gev_pcls *p, const void *phi, const void *lT00, const void *lTij, const void *lL, void *lx,
+                       void *lq, void *lphi, double a)
+{
+    CHECK(p && lT00 && lTij && lx && lq && (!phi || lphi), "gev_adj_gr_deposit: NULL argument");
+    NEED_SORTED(p);
+    return PREC_DISPATCH(p->ctx,
+                         adj_gr_deposit_impl<double>(p, (const double *)phi, (const double *)lT00,
+                                                     (const double *)lTij, (const double *)lL, (double *)lx,
+                                                     (double *)lq, (double *)lphi, a),
+                         adj_gr_deposit_impl<float>(p, (const float *)phi, (const float *)lT00, (const float *)lTij,
+                                                    (const float *)lL, (float *)lx, (float *)lq, (float *)lphi, a));
+}
+
+int gev_adj_source_T00(gev_ctx *c, const void *phi, const void *T00, const void *lsrc, void *lT00, void *lphi,
+                       void *lchi, double bg, double c1, double c2, double c3)
+{
+    CHECK(c && phi && T00 && lsrc && lT00 && lphi && lchi, "gev_adj_source_T00: NULL argument");
+    CHECK(lT00 != lsrc, "gev_adj_source_T00: lT00 must not alias lsrc");
+    return PREC_DISPATCH(c,
+                         adj_source_T00_impl<double>(c, (const double *)phi, (const double *)T00, (const double *)lsrc,
+                                                     (double *)lT00, (double *)lphi, (double *)lchi, bg, c1, c2, c3),
+                         adj_source_T00_impl<float>(c, (const float *)phi, (const float *)T00, (const float *)lsrc,
+                                                    (float *)lT00, (float *)lphi, (float *)lchi, bg, c1, c2, c3));
+}
+
+int gev_adj_source_Tij(gev_ctx *c, const void *phi, const void *lS, void *lphi)
+{
+    CHECK(c && phi && lS && lphi, "gev_adj_source_Tij: NULL argument");
+    return PREC_DISPATCH(c, adj_source_Tij_phi_impl<double>(c, (const double *)phi, (const double *)lS, (double *)lphi),
+                         adj_source_Tij_phi_impl<float>(c, (const float *)phi, (const float *)lS, (float *)lphi));
+}
+
+int gev_adj_link_factor(gev_ctx *c, const void *phi, const void *L, void *u, void *lphi)
+{
+    CHECK(c && phi && L && u && lphi, "gev_adj_link_factor: NULL argument");
+    return PREC_DISPATCH(c,
+                         adj_link_factor_impl<double>(c, (const double *)phi, (const double *)L, (double *)u,
+                                                      (double *)lphi),
+                         adj_link_factor_impl<float>(c, (const float *)phi, (const float *)L, (float *)u,
+                                                     (float *)lphi));
+}
+
+int gev_adj_project_ft_scalar(gev_ctx *c, const void *lchiFT, void *lSijFT)
+{
+    CHECK(c && lchiFT && lSijFT, "gev_adj_project_ft_scalar: NULL argument");
+    return PREC_DISPATCH(c, adj_project_ft_scalar_impl<double>(c, (const cx<double> *)lchiFT, (cx<double> *)lSijFT),
+                         adj_project_ft_scalar_impl<float>(c, (const cx<float> *)lchiFT, (cx<float> *)lSijFT));
+}
+
 // ---- fused real -> k-space -> real solves (the operator runs inside the z pass)
 int gev_solve_poisson(gev_ctx *c, const void *src, void *pot, void *workFT, double coeff, double modif, int set_k0,
                       double k0_re)

@@ -53,6 +53,18 @@ template <typename T> int field_fill_impl(gev_ctx *ctx, T *f, int64_t count, T v
 
 // copy the last kick's maximum (FP64 slot) to the host; synchronises the stream
 int read_max_slot(gev_ctx *c, double *out);
+template <typename T>
+int adj_gr_kick_drift_impl(gev_pcls *p, const T *phi, const T *chi, const T *Bi, T *lx, T *lq, T *lphi, T *lchi,
+                           T *lB, int nf_k, int nf_d, double a_k, double a_d, double dtau_k, double dtau_d);
+template <typename T>
+int adj_gr_deposit_impl(gev_pcls *p, const T *phi, const T *lT00, const T *lTij, const T *lL, T *lx, T *lq, T *lphi,
+                        double a);
+template <typename T>
+int adj_source_T00_impl(gev_ctx *c, const T *phi, const T *T00, const T *lsrc, T *lT00, T *lphi, T *lchi, double bg,
+                        double c1, double c2, double c3);
+template <typename T> int adj_source_Tij_phi_impl(gev_ctx *c, const T *phi, const T *lS, T *lphi);
+template <typename T> int adj_link_factor_impl(gev_ctx *c, const T *phi, const T *L, T *u, T *lphi);
+template <typename T> int adj_project_ft_scalar_impl(gev_ctx *c, const cx<T> *lchi, cx<T> *lS);
 
 // zero the cell counters unless the last scan already left them zero
 int pcls_clear_counts(gev_pcls *p);

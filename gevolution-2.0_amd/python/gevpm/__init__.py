@@ -189,6 +189,43 @@ class Context:
                                      cq, self.r(lpsi, 1, "adj_ic_lattice(lpsi)")), "gev_adj_ic_lattice")
         return lpsi
 
+    # ------------------------------------------ adjoint of the GR cycle
+    def adj_source_T00(self, phi, T00, lsrc, lT00, lphi, lchi, bg, c1, c2, c3):
+        """reverse of prepare_source_T00: lT00 = , lphi += , lchi += """
+        self.bind_stream()
+        nm = "adj_source_T00"
+        check(lib.gev_adj_source_T00(self._h, self.r(phi, 1, nm + "(phi)"), self.r(T00, 1, nm + "(T00)"),
+                                     self.r(lsrc, 1, nm + "(lsrc)"), self.r(lT00, 1, nm + "(lT00)"),
+                                     self.r(lphi, 1, nm + "(lphi)"), self.r(lchi, 1, nm + "(lchi)"), bg, c1, c2, c3),
+              "gev_adj_source_T00")
+        return lT00
+
+    def adj_source_Tij(self, phi, lS, lphi):
+        """lphi += the transpose of prepare_source_Tij's phi stencil applied to lS (6)"""
+        self.bind_stream()
+        check(lib.gev_adj_source_Tij(self._h, self.r(phi, 1, "adj_source_Tij(phi)"), self.r(lS, 6, "adj_source_Tij(lS)"),
+                                     self.r(lphi, 1, "adj_source_Tij(lphi)")), "gev_adj_source_Tij")
+        return lphi
+
+    def adj_link_factor(self, phi, L, u, lphi):
+        """u (= the B solve of lB) <- lL; lphi += the link factor's phi adjoint"""
+        self.bind_stream()
+        check(lib.gev_adj_link_factor(self._h, self.r(phi, 1, "adj_link_factor(phi)"), self.r(L, 3, "adj_link_factor(L)"),
+                                      self.r(u, 3, "adj_link_factor(u)"), self.r(lphi, 1, "adj_link_factor(lphi)")),
+              "gev_adj_link_factor")
+        return u
+
+    def adj_solve_chi(self, lchi, out=None, work1=None, work6=None):
+        """the adjoint of solve_chi: lS (6) = c2r(conj(weights) r2c(lchi) / (2 k^4 N))"""
+        out = self.real(6) if out is None else out
+        work1 = self.fourier(1) if work1 is None else work1
+        work6 = self.fourier(6) if work6 is None else work6
+        self.fft_r2c(lchi, out=work1)
+        self.bind_stream()
+        check(lib.gev_adj_project_ft_scalar(self._h, self.k(work1, 1, "adj_solve_chi(work1)"),
+                                            self.k(work6, 6, "adj_solve_chi(work6)")), "gev_adj_project_ft_scalar")
+        return self.fft_c2r(work6, out=out, preserve=False)
+
     # ------------------------------------ fused real -> k -> real solves
     def solve_poisson(self, src, coeff, modif=0.0, k0=None, out=None, work=None):
         """c2r(poisson(r2c(src))), optionally with the k=0 mode set to k0 first"""
@@ -568,6 +605,28 @@ class Particles:
         self.ctx.bind_stream()
         check(lib.gev_adj_deposit(self._h, self.per_particle(lq, "adj_deposit(lq)"), self.f(u, 3, "adj_deposit(u)")),
               "gev_adj_deposit")
+
+    # ------------------------------------------------ adjoint (GR cycle)
+    def adj_gr_kick_drift(self, phi, chi, Bi, lx, lq, lphi, lchi, lB, nf_kick, nf_drift, a_kick, a_drift, dtau_kick,
+                          dtau_drift):
+        """reverse of kick_drift(gr=1) from the pre-kick state: lx, lq (after the
+        drift) become the adjoints before the kick; lphi, lchi, lB += """
+        self.ctx.bind_stream()
+        nm = "adj_gr_kick_drift"
+        check(lib.gev_adj_gr_kick_drift(self._h, self.f(phi, 1, nm + "(phi)"), self.f(chi, 1, nm + "(chi)"),
+                                        self.f(Bi, 3, nm + "(Bi)"), self.per_particle(lx, nm + "(lx)"),
+                                        self.per_particle(lq, nm + "(lq)"), self.f(lphi, 1, nm + "(lphi)"),
+                                        self.f(lchi, 1, nm + "(lchi)"), self.f(lB, 3, nm + "(lB)"), int(nf_kick),
+                                        int(nf_drift), a_kick, a_drift, dtau_kick, dtau_drift), "gev_adj_gr_kick_drift")
+
+    def adj_gr_deposit(self, phi, lT00, lTij, lL, lx, lq, lphi, a):
+        """reverse of project_T00_Tij_T0i_links (coeff 1): lx, lq += , lphi += """
+        self.ctx.bind_stream()
+        nm = "adj_gr_deposit"
+        check(lib.gev_adj_gr_deposit(self._h, self.f(phi, 1, nm + "(phi)", True), self.f(lT00, 1, nm + "(lT00)"),
+                                     self.f(lTij, 6, nm + "(lTij)"), self.f(lL, 3, nm + "(lL)", True),
+                                     self.per_particle(lx, nm + "(lx)"), self.per_particle(lq, nm + "(lq)"),
+                                     self.f(lphi, 1, nm + "(lphi)", True), a), "gev_adj_gr_deposit")
 
     def adj_gather(self, lx, lq, phi=None, lrho=None, K=0.0, mN=0.0, C_prev=0.0):
         self.ctx.bind_stream()

@@ -122,6 +122,12 @@ SIGNATURES = {
     "gev_adj_divergence": (_i, [_vp, _vp, _vp, _d]),
     "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "gev_adj_ic_lattice": (_i, [_vp, _vp, _vp, _d, _d, _vp]),
+    "gev_adj_gr_kick_drift": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d]),
+    "gev_adj_gr_deposit": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d]),
+    "gev_adj_source_T00": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d]),
+    "gev_adj_source_Tij": (_i, [_vp, _vp, _vp, _vp]),
+    "gev_adj_link_factor": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "gev_adj_project_ft_scalar": (_i, [_vp, _vp, _vp]),
     "gev_ic_displacement_ft": (_i, [_vp, _u32, _i, _d, _i, _dp, _dp, _vp, _vp]),
     "gev_displace_particles": (_i, [_vp, _vp, _d, _dp]),
     "gev_initialize_q": (_i, [_vp, _vp, _d, _dp]),

@@ -57,6 +57,7 @@ class PMRun:
         self.mN = omega_m * N  # particle mass / dx^3 * N
         self._ckpts = {}
         self._final = None
+        self.mem_probe = None  # optional callable, run where device memory use peaks
 
     # ------------------------------------------------------------ forward
     def _ic(self, psi):
@@ -82,6 +83,8 @@ class PMRun:
         self._final = self.pcls.save()
         out = self.ctx.real(1)
         self.pcls.project_mass(out)
+        if self.mem_probe:
+            self.mem_probe()
         return out
 
     def checkpoint_bytes(self):
@@ -115,6 +118,8 @@ class PMRun:
                     if k + 1 < k1 - 1:
                         states[k + 1] = p.save()
                 # the container now holds the start state of cycle k1 - 1
+            if self.mem_probe:
+                self.mem_probe()
             for k in range(k1 - 1, k0 - 1, -1):
                 s = self.sched[k]
                 K = s["dtau_k"] * s["a"] * self.N
@@ -148,4 +153,179 @@ class _PMFunction(torch.autograd.Function):
 
 def pm_density(psi, run):
     """final CIC mass density of `run` from the displacement potential psi, differentiable in psi"""
+    return _PMFunction.apply(psi, run)
+
+
+class PMRunGR:
+    """Differentiable weak-field GR run: the lattice Zel'dovich map of PMRun,
+    then `steps` cycles of main.cu's GR loop with the elliptic B
+    (main.cu:608-1317, in gev_step's order), then the final CIC mass density:
+
+        T00, Tij, L = deposits(x, q; phi_old)   (L: T0i link sums)
+        phi = Poisson_modified(prepareFTsource_T00(T00, phi_old, chi_old))
+        chi = projectFTscalar(prepareFTsource_Tij(phi, Tij))
+        B   = projectFTvector(L (1 + phi_a + phi_b))
+        kick (update_q, nfield 3) and drift (update_pos, nfield 3)
+
+    The run starts from phi = chi = 0; its first cycle takes dtau_old = dtau
+    for the modified Poisson solve (modif 3 H / dtau_old) and half a kick.
+    The backward replays every cycle's fields from a per-cycle checkpoint of
+    the particles and of phi and chi (segments of ckpt_every cycles, each
+    replayed once from its first checkpoint) and runs the reverse kernels
+    (gev_adj_gr_*, gev_adj_source_*, gev_adj_link_factor, the adjoint chi
+    solve; the Poisson and B solves are their own adjoints)."""
+
+    def __init__(self, N, boxsize, z_in, z_end, steps, precision=GEV_FP64, ckpt_every=1, omega_m=0.31,
+                 omega_lambda=0.69, device="cuda"):
+        self.N, self.steps, self.ckpt_every = N, int(steps), max(1, int(ckpt_every))
+        self.om, self.ol = omega_m, omega_lambda
+        self.ctx = Context(N, precision, device)
+        self.M = N ** 3
+        self.pcls = Particles(self.ctx, self.M, omega_m / self.M)
+        self.fpg = fourpiG_of(boxsize)
+        c = self.ctx
+        self.dx = 1.0 / N
+        self.phi, self.chi = c.real(1), c.real(1)
+        self.T00, self.Tij, self.L = c.real(1), c.real(6), c.real(3)
+        self.src, self.Bi = c.real(1), c.real(3)
+        self.work1, self.work3, self.work6 = c.fourier(1), c.fourier(3), c.fourier(6)
+        a = 1.0 / (1.0 + z_in)
+        tau0 = particle_horizon(a, self.fpg, omega_m, omega_lambda)
+        dtau = (particle_horizon(1.0 / (1.0 + z_end), self.fpg, omega_m, omega_lambda) - tau0) / self.steps
+        H = hconf(a, self.fpg, omega_m, omega_lambda)
+        om_a = omega_m / (omega_m + omega_lambda * a ** 3)
+        self.cx = 1.0
+        self.cq = a * om_a ** 0.55 * H * self.cx
+        self.sched = []
+        dtau_prev = 0.0
+        for _ in range(self.steps):
+            a_d = rk4bg(a, self.fpg, omega_m, omega_lambda, 0.5 * dtau)
+            self.sched.append(dict(a=a, a_d=a_d, dtau_k=0.5 * (dtau + dtau_prev), dtau=dtau,
+                                   dtau_old=dtau_prev if dtau_prev > 0 else dtau,
+                                   H=hconf(a, self.fpg, omega_m, omega_lambda)))
+            a = rk4bg(a_d, self.fpg, omega_m, omega_lambda, 0.5 * dtau)
+            dtau_prev = dtau
+        self.a_final = a
+        self.mN = omega_m * N
+        self._ckpts = {}
+        self._final = None
+        self.mem_probe = None
+
+    def _coef(self, k):
+        s, dx = self.sched[k], self.dx
+        return dict(c1=3.0 * s["H"] * dx * dx / s["dtau_old"], c2=self.fpg * dx * dx / s["a"],
+                    c3=3.0 * s["H"] ** 2 * dx * dx, pcoeff=1.0 / (dx * dx), modif=3.0 * s["H"] / s["dtau_old"],
+                    ccoeff=2.0 * self.fpg * dx * dx / s["a"], bcoeff=self.fpg * dx * dx)
+
+    def _fields(self, k, phi_old, chi_old):
+        """the cycle's deposits and metric solves from the container's state;
+        returns (phi, chi, B) of the cycle (new tensors) and fills self.T00,
+        self.Tij, self.L"""
+        s, c, p, k_ = self.sched[k], self.ctx, self.pcls, self._coef(k)
+        p.project_T00_Tij_T0i_links(self.T00, self.Tij, self.L, a=s["a"], phi=phi_old, coeff=1.0)
+        c.prepare_source_T00(phi_old, chi_old, self.T00, self.om, k_["c1"], k_["c2"], k_["c3"], result=self.src)
+        phi = c.solve_poisson(self.src, k_["pcoeff"], k_["modif"], work=self.work1)
+        Sij = c.prepare_source_Tij(phi, self.Tij, k_["ccoeff"])
+        chi = c.solve_chi(Sij, work6=self.work6, work1=self.work1)
+        B = self.L.clone()
+        c.solve_vector_T0i_links(B, phi, k_["bcoeff"], 0.0, work=self.work3)
+        return phi, chi, B
+
+    def _cycle(self, k):
+        s, p = self.sched[k], self.pcls
+        phi, chi, B = self._fields(k, self.phi, self.chi)
+        N = self.N
+        p.kick_drift(1, s["dtau_k"], s["dtau"], phi, chi, B, 3, 3, s["a"], s["a_d"], s["a"] ** 2 * N,
+                     s["a_d"] ** 2 * N)
+        self.phi, self.chi = phi, chi
+
+    def _ic(self, psi):
+        p = self.pcls
+        p.init_lattice()
+        p.initialize_q(psi, -self.cq)
+        p.displace(psi, self.cx)
+        self.phi.zero_()
+        self.chi.zero_()
+
+    def _save(self):
+        return (self.pcls.save(), self.phi.clone(), self.chi.clone())
+
+    def _restore(self, st):
+        self.pcls.restore(st[0])
+        self.phi, self.chi = st[1].clone(), st[2].clone()
+
+    def forward(self, psi):
+        psi = psi.reshape(1, self.N, self.N, self.N).to(self.ctx.rdtype).contiguous()
+        self._ic(psi)
+        self._ckpts = {}
+        for k in range(self.steps):
+            if k % self.ckpt_every == 0:
+                self._ckpts[k] = self._save()
+            self._cycle(k)
+        self._final = self.pcls.save()
+        out = self.ctx.real(1)
+        self.pcls.project_mass(out)
+        if self.mem_probe:
+            self.mem_probe()
+        return out
+
+    def checkpoint_bytes(self):
+        nb = sum(t[0].numel() + t[1].numel() * t[1].element_size() + t[2].numel() * t[2].element_size()
+                 for t in self._ckpts.values())
+        return nb + (self._final.numel() if self._final is not None else 0)
+
+    def _reverse_cycle(self, k, phi_old, chi_old, lx, lq, lphi, lchi):
+        """the adjoint of cycle k from its start state (in the container, with
+        phi_old, chi_old); lphi, lchi: adjoints of the cycle's phi and chi
+        (consumed).  Returns the adjoints of phi_old and chi_old."""
+        s, c, p, k_ = self.sched[k], self.ctx, self.pcls, self._coef(k)
+        N = self.N
+        phi, chi, B = self._fields(k, phi_old, chi_old)
+        lB = c.real(3)
+        p.adj_gr_kick_drift(phi, chi, B, lx, lq, lphi, lchi, lB, 3, 3, s["a"], s["a_d"], s["dtau_k"], s["dtau"])
+        # B = V (L (1 + phi_a + phi_b)), V self-adjoint
+        c.solve_vector(lB, k_["bcoeff"], 0.0, work=self.work3)
+        c.adj_link_factor(phi, self.L, lB, lphi)           # lB now holds lL
+        # chi = C Sij, Sij = stencil(phi) + ccoeff Tij
+        lS = c.adj_solve_chi(lchi, work1=self.work1, work6=self.work6)
+        c.adj_source_Tij(phi, lS, lphi)
+        lTij = lS.mul_(k_["ccoeff"])
+        # phi = P src, P self-adjoint
+        lsrc = c.solve_poisson(lphi, k_["pcoeff"], k_["modif"], work=self.work1)
+        lphi_old, lchi_old, lT00 = c.real(1), c.real(1), c.real(1)
+        c.adj_source_T00(phi_old, self.T00, lsrc, lT00, lphi_old, lchi_old, self.om, k_["c1"], k_["c2"], k_["c3"])
+        p.adj_gr_deposit(phi_old, lT00, lTij, lB, lx, lq, lphi_old, s["a"])
+        return lphi_old, lchi_old
+
+    def backward(self, g_rho):
+        p, c, n = self.pcls, self.ctx, self.M
+        g_rho = g_rho.reshape(1, self.N, self.N, self.N).to(c.rdtype).contiguous()
+        lx = torch.zeros(3, n, dtype=c.rdtype, device=c.device)
+        lq = torch.zeros_like(lx)
+        p.restore(self._final)
+        p.adj_gather(lx, lq, lrho=g_rho, mN=self.mN)
+        lphi, lchi = c.real(1), c.real(1)
+        starts = sorted(self._ckpts)
+        for si in range(len(starts) - 1, -1, -1):
+            k0 = starts[si]
+            k1 = starts[si + 1] if si + 1 < len(starts) else self.steps
+            states = {k0: self._ckpts[k0]}
+            if k1 - k0 > 1:
+                self._restore(self._ckpts[k0])
+                for k in range(k0, k1 - 1):
+                    self._cycle(k)
+                    states[k + 1] = self._save()
+            if self.mem_probe:
+                self.mem_probe()
+            for k in range(k1 - 1, k0 - 1, -1):
+                self._restore(states[k])
+                lphi, lchi = self._reverse_cycle(k, self.phi, self.chi, lx, lq, lphi, lchi)
+            del states
+        lpsi = c.real(1)
+        c.adj_ic_lattice(lx, lq, self.cx, self.cq, lpsi)
+        return lpsi
+
+
+def pm_density_gr(psi, run):
+    """final CIC mass density of the GR run `run` (PMRunGR), differentiable in psi"""
     return _PMFunction.apply(psi, run)

@@ -343,6 +343,42 @@ int gev_adj_gather(gev_pcls *p, const void *phi, const void *lrho, void *lq, voi
                    double C_prev);
 int gev_adj_ic_lattice(gev_ctx *ctx, const void *lx, const void *lq, double cx, double cq, void *lpsi);
 
+/* ---- adjoint of the weak-field GR cycle (no reference counterpart; reverse
+ *      mode of the T00 / Tij / T0i projections, prepareFTsource in both forms,
+ *      solveModifiedPoissonFT, projectFTscalar, projectFTvector with the
+ *      elliptic B, update_q and update_pos: gevolution.hpp:1729-2965, 64-218,
+ *      806-852, 251-340, 449-515, 888-1232).  lx, lq are [3][n] per particle
+ *      ID; field adjoints are in the context precision and accumulate (+=)
+ *      unless stated.
+ *   gev_adj_gr_kick_drift: the fused kick (nfield_kick, a_kick, dtau_kick,
+ *      bcoeff a_kick^2 N) and drift (nfield_drift, a_drift, dtau_drift) of the
+ *      particles in their pre-kick state: lx, lq (after the drift) are
+ *      replaced by the adjoints before the kick; lphi, lchi, lB (3) += the
+ *      fields' adjoints
+ *   gev_adj_gr_deposit: the T00 + Tij + T0i-link deposits (gev_project_T00_Tij_
+ *      T0i_links with coeff 1) given their adjoints lT00, lTij (6), lL (3, may
+ *      be NULL): lx, lq += ; lphi += phi's adjoint (phi may be NULL: no phi)
+ *   gev_adj_source_T00: prepareFTsource T00 form (bg, c1, c2, c3 as in
+ *      gev_prepare_source_T00) given lsrc: lT00 = (overwritten), lphi +=,
+ *      lchi +=
+ *   gev_adj_source_Tij: the phi stencil of prepareFTsource Tij form given lS
+ *      (6): lphi += (the Tij part is lS times the coefficient)
+ *   gev_adj_link_factor: given u = the B solve applied to lB (it is its own
+ *      adjoint) and the link sums L (3): lphi += sum over links of L u at
+ *      both ends; u is replaced by lL = (1 + phi_a + phi_b) u
+ *   gev_adj_project_ft_scalar: k-space adjoint of projectFTscalar, lSijFT (6
+ *      components) = conj(weights) lchiFT / (2 k^4 N) (overwritten) */
+int gev_adj_gr_kick_drift(gev_pcls *p, const void *phi, const void *chi, const void *Bi, void *lx, void *lq,
+                          void *lphi, void *lchi, void *lB, int nfield_kick, int nfield_drift, double a_kick,
+                          double a_drift, double dtau_kick, double dtau_drift);
+int gev_adj_gr_deposit(gev_pcls *p, const void *phi, const void *lT00, const void *lTij, const void *lL, void *lx,
+                       void *lq, void *lphi, double a);
+int gev_adj_source_T00(gev_ctx *ctx, const void *phi, const void *T00, const void *lsrc, void *lT00, void *lphi,
+                       void *lchi, double bg, double c1, double c2, double c3);
+int gev_adj_source_Tij(gev_ctx *ctx, const void *phi, const void *lS, void *lphi);
+int gev_adj_link_factor(gev_ctx *ctx, const void *phi, const void *L, void *u, void *lphi);
+int gev_adj_project_ft_scalar(gev_ctx *ctx, const void *lchiFT, void *lSijFT);
+
 /* ---- background (background.hpp:200-290), Omega_rad/ncdm/fld as given */
 double gev_hconf(double a, double fourpiG, double Omega_m, double Omega_Lambda, double Omega_rad);
 double gev_rk4bg(double a, double fourpiG, double Omega_m, double Omega_Lambda, double Omega_rad, double dtau);

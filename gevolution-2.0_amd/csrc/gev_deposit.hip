@@ -13,8 +13,15 @@
 // tile then issues one plain store per site and component.  The particles of
 // the halo layer are evaluated by both neighbouring tiles, which trades
 // (TX+1)(TY+1)(TZ+1)/(TX TY TZ) particle reads for having no global atomics
-// and no zeroing pass.  The summation order inside a site follows LDS atomic
-// arbitration, as the reference's atomicAdd projections do.
+// and no zeroing pass.  FP64 tiles sum in FP64 with LDS atomics, whose order
+// inside a site follows atomic arbitration, as the reference's atomicAdd
+// projections do.  FP32 tiles sum in 32-bit fixed point (integer LDS
+// atomics, which this part executes several times faster than FP32 or FP64
+// ones, and whose sums do not depend on the order): each tile bounds its
+// per-site sums by (largest particle count of the 8 cells around one of its
+// sites) x (largest per-particle value, from the tile's particles and |phi|,
+// separately for the T00, Tij and T0i components, whose sizes differ by the
+// ratio of kinetic to rest energy) and scales so that bound is 2^30.
 #include "gev_common.hpp"
 #include "gev_kernels.hpp"
 #include "gev_pcommon.hpp"
@@ -36,12 +43,23 @@ template <int KIND> struct DepC {
                                                                     : (KIND == DEP_TIJ ? 6 : (KIND == DEP_T00TIJ ? 7 : 10)));
 };
 
-// FP64 LDS accumulators in both precisions: ds_add_f64 is the fast LDS atomic
-// on this part, and the per-site sums carry the full FP64 mantissa.
+// LDS accumulation: FP64 sums (ds_add_f64) in the FP64 form; in the FP32 form
+// 32-bit fixed point (ds_add_u32) of values already multiplied by the tile's
+// scale, rounded to nearest.
 __device__ __forceinline__ void lds_add(double *a, double v)
 {
     __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+__device__ __forceinline__ void lds_add(int32_t *a, float v)
+{
+    __hip_atomic_fetch_add(a, __float2int_rn(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// accumulator of a kind and precision: fixed point for the FP32 form of the
+// physical projections (their values are bounded by the particles' energies);
+// the adjoint's vector deposit keeps FP64
+template <typename T, int KIND> struct DepAcc { using type = double; };
+template <int KIND> struct DepAcc<float, KIND> { using type = int32_t; };
+template <> struct DepAcc<float, 4> { using type = double; };  // DEP_VEC3
 
 #ifndef DEP_CHUNK_X4
 #define DEP_CHUNK_X4 5  // FP32 single-particle kinds: as DEP_SPREAD_X4, per-lane chunks
@@ -63,7 +81,12 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     constexpr int PX = TX + 2, PY = TY + 2, PZ = TZ + 2;  // phi box x0-1 .. x0+TX
     constexpr int NPH = PX * PY * PZ;
     static_assert(NSEG <= PT_NT, "one run per thread in the scan");
-    __shared__ double acc[C][NS];
+    using AccT = typename DepAcc<T, KIND>::type;
+    constexpr bool FIX = sizeof(AccT) == 4;
+    constexpr int NCB = (TX + 1) * (TY + 1) * (TZ + 1);  // cells of the halo box
+    __shared__ AccT acc[C][NS];
+    __shared__ int32_t ccount[FIX ? NCB : 1];
+    __shared__ float wmax[3][PT_NT / 64];
     __shared__ T phs[NPH];
     __shared__ int32_t rbeg[NSEG + 1];
     __shared__ int32_t rst[NSEG];
@@ -77,7 +100,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
 
     for (int e = t; e < NS; e += PT_NT) {
 #pragma unroll
-        for (int c = 0; c < C; c++) acc[c][e] = 0.0;
+        for (int c = 0; c < C; c++) acc[c][e] = (AccT)0;
     }
     if (use_phi) {
         for (int e = t; e < NPH; e += PT_NT) {
@@ -112,6 +135,85 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     __syncthreads();
     const int32_t total = rbeg[NSEG];
     const T a = (T)a_d, o = (T)1, ia2 = (T)(1.0 / (a_d * a_d));
+    const bool need_q_ = KIND != DEP_T00 || use_phi;
+    // flattened particle index -> container slot
+    auto locate = [&](int32_t i, int &run) -> int64_t {
+        int lo = 0, hi = NSEG - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (rbeg[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        run = lo;
+        return (int64_t)rst[lo] + (i - rbeg[lo]);
+    };
+    // fixed-point scale of the FP32 form (see the top of the file): S maps the
+    // bound of every per-site sum of the tile to 2^30
+    // S0, S1, S2: scales of the T00, Tij and T0i accumulators (1 in FP64)
+    float S0 = 1.0f, S1 = 1.0f, S2 = 1.0f;
+    if constexpr (FIX) {
+        __shared__ float wmax2[2][PT_NT / 64];
+        for (int e = t; e < NCB; e += PT_NT) {
+            const int cx = e % (TX + 1), cy = (e / (TX + 1)) % (TY + 1), cz = e / ((TX + 1) * (TY + 1));
+            const uint32_t g = ((uint32_t)(x0 - 1 + cx) & msk) +
+                               (uint32_t)N * (((uint32_t)(y0 - 1 + cy) & msk) + (uint32_t)N * ((uint32_t)(z0 - 1 + cz) & msk));
+            ccount[e] = v.start[g + 1] - v.start[g];
+        }
+        __syncthreads();
+        // site (sx,sy,sz) takes particles from the halo-box cells (sx+1-bx, ...)
+        float cmax = 0.0f, emax = 0.0f, pmax = 0.0f, tmax = 0.0f, vmax = 0.0f;
+        for (int e = t; e < NS; e += PT_NT) {
+            const int sx = e % TX, sy = (e / TX) % TY, sz = e / (TX * TY);
+            int c8 = 0;
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+                c8 += ccount[(sx + 1 - ((b >> 2) & 1)) + (TX + 1) * ((sy + 1 - ((b >> 1) & 1)) + (TY + 1) * (sz + 1 - (b & 1)))];
+            cmax = fmaxf(cmax, (float)c8);
+        }
+        if (use_phi)  // the phi box is staged (the scan's barriers ordered it)
+            for (int e = t; e < NPH; e += PT_NT) pmax = fmaxf(pmax, fabsf((float)phs[e]));
+        if (need_q_) {
+            const float a2 = (float)(a_d * a_d);
+            for (int32_t i = t; i < total; i += PT_NT) {
+                int run;
+                const int64_t p = locate(i, run);
+                const float q0 = (float)v.q0[p], q1 = (float)v.q1[p], q2 = (float)v.q2[p];
+                const float qq = q0 * q0 + q1 * q1 + q2 * q2, ee = sqrtf(qq + a2);
+                emax = fmaxf(emax, ee);
+                tmax = fmaxf(tmax, qq / ee);
+                vmax = fmaxf(vmax, fmaxf(fabsf(q0), fmaxf(fabsf(q1), fabsf(q2))));
+            }
+        }
+        for (int o_ = 32; o_ > 0; o_ >>= 1) {
+            cmax = fmaxf(cmax, __shfl_xor(cmax, o_, 64));
+            emax = fmaxf(emax, __shfl_xor(emax, o_, 64));
+            pmax = fmaxf(pmax, __shfl_xor(pmax, o_, 64));
+            tmax = fmaxf(tmax, __shfl_xor(tmax, o_, 64));
+            vmax = fmaxf(vmax, __shfl_xor(vmax, o_, 64));
+        }
+        if (lane == 0) {
+            wmax[0][wv] = cmax; wmax[1][wv] = emax; wmax[2][wv] = pmax; wmax2[0][wv] = tmax; wmax2[1][wv] = vmax;
+        }
+        __syncthreads();
+        for (int j = 0; j < PT_NT / 64; j++) {
+            cmax = fmaxf(cmax, wmax[0][j]); emax = fmaxf(emax, wmax[1][j]); pmax = fmaxf(pmax, wmax[2][j]);
+            tmax = fmaxf(tmax, wmax2[0][j]); vmax = fmaxf(vmax, wmax2[1][j]);
+        }
+        // largest |value| one particle adds to one site, per accumulator group:
+        // T00's e + f phi + f2 phi^2 <= e (1 + 4|phi| + 9 phi^2) (f <= 4e, f2 <= 9e;
+        // 1 for the mass deposit), Tij's f q_i q_j / e <= (1 + 5|phi|) q^2 / e,
+        // T0i's (1 + phi_a + phi_b) q_i <= (1 + 2|phi|) max|q_i|; each rounded up
+        // by 1 % against the FP32 rounding of the values.  The tile's per-site
+        // sums stay below cmax times these.
+        if (cmax > 0.0f) {
+            const double pm = (double)pmax, c = (double)cmax, top = 1073741824.0;
+            const double B0 = need_q_ ? (double)emax * (1.0 + 4.0 * pm + 9.0 * pm * pm) * 1.01 : 1.0;
+            const double B1 = (double)tmax * (1.0 + 5.0 * pm) * 1.01, B2 = (double)vmax * (1.0 + 2.0 * pm) * 1.01;
+            S0 = (float)(top / (c * B0));
+            if (B1 > 0.0) S1 = (float)(top / (c * B1));
+            if (B2 > 0.0) S2 = (float)(top / (c * B2));
+        }
+    }
 
     // U particles per thread and pass: their loads are issued together
     constexpr bool T00TIJ = KIND == DEP_T00TIJ || KIND == DEP_T00TIJL;
@@ -209,22 +311,24 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
                 const T e = fsqrt(f2 + a * a), ie = frcp(e);
                 const T f = (T)3 * e + f2 * ie;
                 f2 = ((T)0.5 * f * f + f2 * frcp(o + f2 * ia2)) * ie;
+                const T es = FIX ? e * (T)S0 : e, fs = FIX ? f * (T)S0 : f, f2s = FIX ? f2 * (T)S0 : f2;
 #pragma unroll
                 for (int b = 0; b < 8; b++) {
                     const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
                     if (own(bx, by, bz)) {
                         const T ph = phs[pb + bx + PX * (by + PY * bz)];
                         const T w = (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]);
-                        lds_add(&acc[0][sidx(bx, by, bz)], w * (e + f * ph + f2 * ph * ph));
+                        lds_add(&acc[0][sidx(bx, by, bz)], w * (es + fs * ph + f2s * ph * ph));
                     }
                 }
             } else {
+                const T sc = FIX ? (T)S0 : o;
 #pragma unroll
                 for (int b = 0; b < 8; b++) {
                     const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
                     if (own(bx, by, bz))
                         lds_add(&acc[0][sidx(bx, by, bz)],
-                                (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
+                                sc * (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
                 }
             }
         }
@@ -233,7 +337,8 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
             // each weighted by (1 + phi_a + phi_b) of its two end points (DEP_T0I;
             // DEP_T00TIJL leaves the factor to the B solve)
             constexpr int L0 = KIND == DEP_T0I ? 0 : 7;  // first T0i accumulator
-            const T v0 = qu[u][0], v1 = qu[u][1], v2 = qu[u][2];
+            const T sq = FIX ? (T)S2 : o;
+            const T v0 = qu[u][0] * sq, v1 = qu[u][1] * sq, v2 = qu[u][2] * sq;
             T g[12] = {(o - rd[1]) * (o - rd[2]) * v0, (o - rd[0]) * (o - rd[2]) * v1, (o - rd[0]) * (o - rd[1]) * v2,
                        rd[0] * (o - rd[2]) * v1,       rd[0] * (o - rd[1]) * v2,       rd[1] * (o - rd[2]) * v0,
                        rd[1] * (o - rd[0]) * v2,       rd[2] * (o - rd[1]) * v0,       rd[2] * (o - rd[0]) * v1,
@@ -277,6 +382,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
             } else {
                 f = o;
             }
+            if constexpr (FIX) f *= (T)S1;
             constexpr int dcomp[3] = {TB + 0, TB + 3, TB + 5};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -312,7 +418,11 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
             double ms = mass_d;
             if (T00TIJ && c > 0 && c < 7) { oc = out2 + (c - 1) * M; ms = mass2_d; }
             if (KIND == DEP_T00TIJL && c >= 7) { oc = out3 + (c - 7) * M; ms = mass3_d; }
-            T val = (T)(acc[c][s] * ms);
+            // component groups: T00 (and the mass deposit), Tij, T0i
+            const int grp = KIND == DEP_T00 ? 0 : KIND == DEP_TIJ ? 1 : KIND == DEP_T0I ? 2
+                                                                 : (c == 0 ? 0 : (c < 7 ? 1 : 2));
+            const double Sc = grp == 0 ? (double)S0 : grp == 1 ? (double)S1 : (double)S2;
+            T val = FIX ? (T)((double)acc[c][s] * (ms / Sc)) : (T)(acc[c][s] * ms);
             if (accumulate) val += oc[gs];
             oc[gs] = val;
         }
@@ -341,6 +451,12 @@ static int dispatch_dep(gev_pcls *p, T *out, const T *phi, double mass, double a
     constexpr bool big = KIND != DEP_TIJ && KIND != DEP_T00TIJ && KIND != DEP_T00TIJL;
     if (KIND == DEP_T0I && N >= 32) return launch_dep<T, KIND, 32, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
     if (big && N >= 16) return launch_dep<T, KIND, 16, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
+#ifndef DEP_F32_TY
+#define DEP_F32_TY 4
+#endif
+    // the FP32 form's fixed-point accumulators take half the LDS of FP64 ones
+    if ((KIND == DEP_T00TIJ || KIND == DEP_T00TIJL) && N >= 16 && sizeof(T) == 4 && DEP_F32_TY != 4)
+        return launch_dep<T, KIND, 16, DEP_F32_TY, 4>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
     if ((KIND == DEP_T00TIJ || KIND == DEP_T00TIJL) && N >= 16)
         return launch_dep<T, KIND, 16, 4, 4>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);
     if (N >= 8) return launch_dep<T, KIND, 8, 8, 8>(p, out, out2, phi, mass, mass2, a, acc, aux, out3, mass3);

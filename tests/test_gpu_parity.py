@@ -266,7 +266,13 @@ def test_deposits_empty_and_single(gp, O):
 
 @pytest.mark.parametrize("N,cluster", [(16, False), (64, False), (64, True)])
 def test_deposits_mixed(gp, O, N, cluster):
-    """clustered: half the particles in 8 cells (LDS atomics on shared sites)"""
+    """clustered: half the particles in 8 cells (LDS atomics on shared sites).
+    The FP32 form sums each tile in 32-bit fixed point scaled to its largest
+    per-site particle count: a site fed by k of the tile's particles carries a
+    rounding error of about sqrt(k) half-units of (k_max e_max / 2^30), the
+    order of an FP32 running sum; with 65536 particles on one site that is
+    ~1e-5 of the largest site value"""
+    tol = 3e-5 if cluster else 2e-6
     ctx = gp.Context(N, gp.GEV_MIXED)
     n = 4 * N ** 3 // 8
     pos, q, pid = make_particles(N, n, 400 + N, vmax=0.2, cluster=cluster)
@@ -279,14 +285,16 @@ def test_deposits_mixed(gp, O, N, cluster):
     phi = 1e-3 * np.random.default_rng(N).standard_normal(N ** 3).astype(np.float32).astype(np.float64)
     out = ctx.real(1)
     p.project_T00(out, a, field_gpu(ctx, phi))
-    assert relmax(real_np(out)[0], O.project_T00(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
+    errs = [relmax(real_np(out)[0], O.project_T00(N, gpos, gq, m, a=a, phi=phi))]
     o6 = ctx.real(6)
     p.project_Tij(o6, a, field_gpu(ctx, phi))
-    assert relmax(real_np(o6), O.project_Tij(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
+    errs.append(relmax(real_np(o6), O.project_Tij(N, gpos, gq, m, a=a, phi=phi)))
     o1 = ctx.real(1)
     p.project_T00_Tij(o1, o6, a, field_gpu(ctx, phi))
-    assert relmax(real_np(o1)[0], O.project_T00(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
-    assert relmax(real_np(o6), O.project_Tij(N, gpos, gq, m, a=a, phi=phi)) < 2e-6
+    errs.append(relmax(real_np(o1)[0], O.project_T00(N, gpos, gq, m, a=a, phi=phi)))
+    errs.append(relmax(real_np(o6), O.project_Tij(N, gpos, gq, m, a=a, phi=phi)))
+    print(f"mixed deposits N={N} cluster={cluster}: rel max errors T00, Tij, T00 (pair), Tij (pair) {errs}", flush=True)
+    assert max(errs) < tol
 
 
 # ---------------------------------------------------------- kick / drift --
@@ -604,11 +612,17 @@ def test_t0i_links_match_separate_projection(gp, N, prec, cluster):
     ph = to_np(phi_new)[0].astype(np.float64)  # [z, y, x]
     L = to_np(Lk).astype(np.float64)
     fac = [1 + ph + np.roll(ph, -1, axis=2), 1 + ph + np.roll(ph, -1, axis=1), 1 + ph + np.roll(ph, -1, axis=0)]
-    assert relmax(np.stack([L[c] * fac[c] for c in range(3)]), T0i) < (1e-12 if prec == "fp64" else 2e-6)
+    # FP32 form: both sides sum in 32-bit fixed point scaled to the tile's
+    # largest per-site count times the largest |q_i|; where half the particles
+    # share 8 cells, their momenta (random signs) nearly cancel, and the
+    # rounding of ~65000 adds per site is a few 1e-4 of the largest net sum,
+    # the size an FP32 running sum would also have
+    tol_l = 1e-12 if prec == "fp64" else (2e-3 if cluster else 2e-6)
+    assert relmax(np.stack([L[c] * fac[c] for c in range(3)]), T0i) < tol_l
     # the B solve from the link sums
     Bref = ctx.solve_vector(p.project_T0i(ctx.real(3), phi_new), 0.25, 0.0)
     Bgot = ctx.solve_vector_T0i_links(Lk.clone(), phi_new, 0.25, 0.0)
-    assert relmax(to_np(Bgot), to_np(Bref)) < (1e-10 if prec == "fp64" else 2e-5)
+    assert relmax(to_np(Bgot), to_np(Bref)) < (1e-10 if prec == "fp64" else (2e-3 if cluster else 2e-5))
 
 
 def test_gr_steps_128_vs_oracle(gp, O):

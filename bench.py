@@ -78,7 +78,9 @@ def main():
         copy_calibration()
         return
     if args.adjoint_only:
-        print(json.dumps(adjoint_leg(args, checks=False)), flush=True)
+        out = adjoint_leg(args, checks=False)
+        out["gr"] = adjoint_gr_leg(args, checks=False)
+        print(json.dumps(out), flush=True)
         return
     import numpy as np
     import torch
@@ -204,6 +206,8 @@ def main():
     if not args.no_adjoint:
         torch.cuda.empty_cache()
         result["adjoint"] = adjoint_leg(args)
+        torch.cuda.empty_cache()
+        result["adjoint"]["gr"] = adjoint_gr_leg(args)
     value = N ** 3 * 1e3 / ms_m
     line = {
         "metric": METRIC,
@@ -346,6 +350,84 @@ def kernel_roofline(ctx, n, phases, cycles):
             "bytes_model": desc}
 
 
+def adjoint_gr_leg(args, checks=True):
+    """The same two differentiable workloads with the weak-field GR cycle
+    (gevpm.adjoint.PMRunGR: T00/Tij/T0i deposits, phi/chi/B solves, GR kick
+    and drift): adj-mesh^3 for adj-steps cycles (ckpt every cycle) and twice
+    the mesh for twice the cycles with a checkpoint every 4 cycles, both in the
+    mixed form; backward / forward time ratios, peak device memory of the
+    checkpointed run, and its gradient against the FP64 form's on the eight
+    half-size crops."""
+    import numpy as np
+    import torch
+    import gevpm
+    from gevpm.adjoint import PMRunGR
+
+    def smooth(seed, amp, N):
+        r = np.random.default_rng(seed)
+        F = np.fft.rfftn(r.standard_normal((N, N, N)))
+        kk = np.fft.fftfreq(N, 1.0 / N)
+        kz, ky, kx = np.meshgrid(kk, kk, np.arange(N // 2 + 1), indexing="ij")
+        F *= np.exp(-(kx ** 2 + ky ** 2 + kz ** 2) / (2 * (N / 16.0) ** 2))
+        f = np.fft.irfftn(F, s=(N, N, N), axes=(0, 1, 2))
+        return amp * f / f.std()
+
+    def used_bytes():
+        free, total = torch.cuda.mem_get_info()
+        return total - free
+
+    def timed(N, steps, prec, ckpt, peak=None):
+        amp = 0.3 / N ** 2
+        run = PMRunGR(N, 2.0 * N, 50.0, 0.0, steps, precision=prec, ckpt_every=ckpt)
+        dev = run.ctx.device
+        t = run.forward(torch.from_numpy(smooth(12, amp, N)).to(dev)).double().clone()
+        psi = torch.from_numpy(smooth(11, amp, N)).to(dev).reshape(1, N, N, N)
+        run.backward(2 * (run.forward(psi).double() - t))  # warm-up
+        torch.cuda.synchronize()
+        if peak is not None:
+            run.mem_probe = lambda: peak.__setitem__(0, max(peak[0], used_bytes()))
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        rho = run.forward(psi)
+        e[1].record()
+        g = run.backward(2 * (rho.double() - t))
+        e[2].record()
+        torch.cuda.synchronize()
+        res = {"forward_ms": e[0].elapsed_time(e[1]), "backward_ms": e[1].elapsed_time(e[2]),
+               "adjoint_over_forward": e[1].elapsed_time(e[2]) / e[0].elapsed_time(e[1]),
+               "checkpoint_bytes": run.checkpoint_bytes()}
+        g = g.double().cpu().numpy().reshape(N, N, N)
+        del run, t, psi, rho
+        torch.cuda.empty_cache()
+        return res, g
+
+    N, steps = args.adj_mesh, args.adj_steps
+    out = {"workload": f"weak-field GR PM (elliptic B), {N}^3 particles on {N}^3 mesh, L={2 * N} Mpc/h, {steps} "
+                       "cycles z=50->0, mixed form, loss sum (rho_final - rho_target)^2, gradient w.r.t. the "
+                       "initial displacement potential"}
+    res, _ = timed(N, steps, gevpm.GEV_MIXED, 1)
+    out.update(res)
+    N5, steps5 = 2 * N, 2 * steps
+    torch.cuda.empty_cache()
+    base = used_bytes()
+    peak = [0]
+    res5, g5 = timed(N5, steps5, gevpm.GEV_MIXED, 4, peak)
+    res5.update(workload=f"weak-field GR PM, {N5}^3 particles on {N5}^3 mesh, L={2 * N5} Mpc/h, {steps5} cycles "
+                         "z=50->0, mixed form, a checkpoint every 4 cycles",
+                peak_device_bytes=peak[0] - base)
+    if checks:
+        _, g5d = timed(N5, steps5, gevpm.GEV_FP64, 4)
+        h = N5 // 2
+        crops = [float(np.linalg.norm(g5[z:z + h, y:y + h, x:x + h] - g5d[z:z + h, y:y + h, x:x + h]) /
+                       np.linalg.norm(g5d[z:z + h, y:y + h, x:x + h]))
+                 for z in (0, h) for y in (0, h) for x in (0, h)]
+        res5.update(mixed_vs_fp64_grad_rel_l2=float(np.linalg.norm(g5 - g5d) / np.linalg.norm(g5d)), crop_size=h,
+                    crops_rel_l2_max=max(crops), crops_rel_l2_median=float(np.median(crops)))
+    out["checkpointed"] = res5
+    log(f"GR adjoint: {out}")
+    return out
+
+
 def adjoint_leg(args, checks=True):
     """Differentiable Newtonian run at adj-mesh^3, L = 2 adj-mesh Mpc/h, adj-steps
     cycles z=50 -> 0, loss = sum (rho_final - rho_target)^2 with the target from
@@ -402,6 +484,12 @@ def adjoint_leg(args, checks=True):
     # every 4 cycles (the backward replays up to 3 cycles per segment)
     N5, steps5 = 2 * N, 2 * steps
     amp5 = 0.3 / N5 ** 2
+    def used_bytes():
+        free, total = torch.cuda.mem_get_info()
+        return total - free
+
+    torch.cuda.empty_cache()
+    base5 = used_bytes()
     run5 = PMRun(N5, 2.0 * N5, 50.0, 0.0, steps5, precision=gevpm.GEV_MIXED, ckpt_every=4)
     dev5 = run5.ctx.device
     t5 = run5.forward(torch.from_numpy(smooth(12, amp5, N5)).to(run5.ctx.rdtype).to(dev5)).double().clone()
@@ -409,54 +497,91 @@ def adjoint_leg(args, checks=True):
     run5.backward(2 * (run5.forward(psi5).double() - t5))
     torch.cuda.synchronize()
     e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    peak = [0]
+    run5.mem_probe = lambda: peak.__setitem__(0, max(peak[0], used_bytes()))
     e[0].record()
     rho5 = run5.forward(psi5)
     e[1].record()
-    run5.backward(2 * (rho5.double() - t5))
+    g5 = run5.backward(2 * (rho5.double() - t5))
     e[2].record()
     torch.cuda.synchronize()
+    g5 = g5.double().cpu().numpy().reshape(N5, N5, N5)
     out["checkpointed"] = {
         "workload": f"Newtonian PM, {N5}^3 particles on {N5}^3 mesh, L={2 * N5} Mpc/h, {steps5} cycles z=50->0, "
                     "mixed form, a checkpoint every 4 cycles",
         "forward_ms": e[0].elapsed_time(e[1]), "backward_ms": e[1].elapsed_time(e[2]),
         "adjoint_over_forward": e[1].elapsed_time(e[2]) / e[0].elapsed_time(e[1]),
-        "checkpoint_bytes": run5.checkpoint_bytes()}
+        "checkpoint_bytes": run5.checkpoint_bytes(),
+        "peak_device_bytes": peak[0] - base5,
+        "peak_note": "device memory in use (hipMemGetInfo) above what was in use before the run was built, sampled "
+                     "after the forward and after each backward segment's replay"}
     log(f"adjoint {N5}^3 checkpointed: {out['checkpointed']}")
-    del run5, rho5, t5, psi5
+    del run5, rho5, t5
     torch.cuda.empty_cache()
+    if checks:
+        # the same gradient from the FP64 form, compared on the eight 128^3 crops
+        run5d = PMRun(N5, 2.0 * N5, 50.0, 0.0, steps5, precision=gevpm.GEV_FP64, ckpt_every=4)
+        t5d = run5d.forward(torch.from_numpy(smooth(12, amp5, N5)).to(dev5)).double().clone()
+        g5d = run5d.backward(2 * (run5d.forward(psi5.double()).double() - t5d)).double().cpu().numpy()
+        g5d = g5d.reshape(N5, N5, N5)
+        h = N5 // 2
+        crops = []
+        for cz in (0, h):
+            for cy in (0, h):
+                for cx_ in (0, h):
+                    a_ = g5[cz:cz + h, cy:cy + h, cx_:cx_ + h]
+                    b_ = g5d[cz:cz + h, cy:cy + h, cx_:cx_ + h]
+                    crops.append(float(np.linalg.norm(a_ - b_) / np.linalg.norm(b_)))
+        out["checkpointed"].update(mixed_vs_fp64_grad_rel_l2=float(np.linalg.norm(g5 - g5d) / np.linalg.norm(g5d)),
+                                   crop_size=h, crops_rel_l2_max=max(crops),
+                                   crops_rel_l2_median=float(np.median(crops)))
+        del run5d, t5d, g5d
+        torch.cuda.empty_cache()
+    del psi5, g5
     if not checks:
         return out
     run64, target, g64, f64_ms, b64_ms = run_grad(gevpm.GEV_FP64)
     out.update(fp64_forward_ms=f64_ms, fp64_backward_ms=b64_ms,
                fp32_vs_fp64_grad_rel_l2=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)))
-    # central differences on random single modes
+    # central differences on random single modes, at three step sizes; for
+    # each, the particles whose final cell differs between the + and - runs
+    # (a lower bound on the cell-face crossings that put kinks of the CIC loss
+    # inside the difference interval)
     x = np.arange(N)
-    errs = []
     dev = run64.ctx.device
 
     def loss(p):
         rho = run64.forward(torch.from_numpy(p).to(dev).reshape(1, N, N, N)).double()
-        return float(((rho - target) ** 2).sum())
+        pos, _, pid = run64.pcls.store()
+        cells = torch.floor(pos[:, torch.argsort(pid)] * N)
+        return float(((rho - target) ** 2).sum()), cells
 
-    errs_small = []
+    eps_list = (1e-3, 1e-6, 1e-9)
+    errs = {r: [] for r in eps_list}
+    cross = {r: [] for r in eps_list}
     for _ in range(args.adj_modes):
         kv = rng.integers(-4, 5, size=3)
         ph = rng.uniform(0, 2 * np.pi)
         mode = np.cos(2 * np.pi * (kv[0] * x[None, None, :] + kv[1] * x[None, :, None] + kv[2] * x[:, None, None]) / N
                       + ph)
         ad = float((g64 * mode).sum())
-        for rel, lst in ((1e-3, errs), (1e-6, errs_small)):
+        for rel in eps_list:
             eps = rel * amp
-            fd = (loss(psi_np + eps * mode) - loss(psi_np - eps * mode)) / (2 * eps)
-            lst.append(abs(ad - fd) / max(abs(fd), 1e-300))
-    errs, errs_small = np.array(errs), np.array(errs_small)
-    # eps = 1e-3 of the field scale moves ~1e-4 of the particles across a cell face,
-    # where the CIC loss has a kink; eps = 1e-6 stays on the smooth pieces
-    out.update(fd_modes=int(args.adj_modes), fd_eps_rel=1e-3, fd_rel_err_median=float(np.median(errs)),
-               fd_rel_err_max=float(errs.max()), fd_frac_within_1e5=float((errs <= 1e-5).mean()),
-               fd_small_eps_rel=1e-6, fd_small_rel_err_median=float(np.median(errs_small)),
-               fd_small_rel_err_max=float(errs_small.max()),
-               fd_small_frac_within_1e5=float((errs_small <= 1e-5).mean()))
+            (lp, cp), (lm, cm_) = loss(psi_np + eps * mode), loss(psi_np - eps * mode)
+            fd = (lp - lm) / (2 * eps)
+            errs[rel].append(abs(ad - fd) / max(abs(fd), 1e-300))
+            cross[rel].append(int((cp != cm_).any(0).sum()))
+    out.update(fd_modes=int(args.adj_modes), fd={
+        f"{rel:g}": {"rel_err_median": float(np.median(errs[rel])), "rel_err_max": float(np.max(errs[rel])),
+                     "frac_within_1e5": float(np.mean(np.array(errs[rel]) <= 1e-5)),
+                     "final_cell_changes_median": float(np.median(cross[rel])),
+                     "modes_without_final_cell_change": int(np.sum(np.array(cross[rel]) == 0)),
+                     "rel_err_median_without_cell_change": (float(np.median(
+                         [e_ for e_, c_ in zip(errs[rel], cross[rel]) if c_ == 0]))
+                         if any(c_ == 0 for c_ in cross[rel]) else None)}
+        for rel in eps_list},
+        fd_note="eps is relative to the rms of psi; the exact check of the gradient is the dot-product test "
+                "against the oracle's forward-mode tangent (tests/test_pm_adjoint_dot.py)")
     return out
 
 

@@ -183,6 +183,10 @@ int gev_pcls_create(gev_ctx *c, int64_t capacity, double mass, gev_pcls **out)
     GEV_HIP(hipMalloc(&p->perm, 4 * cap));
     GEV_HIP(hipMalloc(&p->cell_start, 4 * (size_t)(c->M + 1)));
     GEV_HIP(hipMalloc(&p->cell_count, 4 * (size_t)c->M));
+    if (c->prec == GEV_MIXED && c->N >= 16) {
+        const size_t nb = (size_t)(c->N >> QB_LG);
+        GEV_HIP(hipMalloc(&p->qbound, 4 * nb * nb * nb));
+    }
     *out = p;
     return GEV_OK;
 }
@@ -201,6 +205,7 @@ int gev_pcls_destroy(gev_pcls *p)
     (void)hipFree(p->perm);
     (void)hipFree(p->cell_start);
     (void)hipFree(p->cell_count);
+    (void)hipFree(p->qbound);
     delete p;
     return GEV_OK;
 }
@@ -212,6 +217,7 @@ int gev_pcls_load(gev_pcls *p, int64_t n, const double *pos, const double *q, co
     gev_ctx *c = p->ctx;
     p->n = n;
     p->sorted = false;
+    p->qbound_valid = false;
     if (n == 0) return PREC_DISPATCH(c, pcls_sort_impl<double>(p), pcls_sort_impl<float>(p));
     const double *dpos = pos, *dq = q;
     const int64_t *did = id;
@@ -288,6 +294,7 @@ int gev_pcls_copy_index(gev_pcls *p, int32_t *cell, int32_t *cell_start)
 int gev_pcls_sort(gev_pcls *p)
 {
     CHECK(p, "gev_pcls_sort: NULL argument");
+    p->qbound_valid = false;  // cells are recomputed from the positions, which a caller may have written
     return PREC_DISPATCH(p->ctx, pcls_sort_impl<double>(p), pcls_sort_impl<float>(p));
 }
 
@@ -296,6 +303,7 @@ int gev_pcls_init_lattice(gev_pcls *p)
     CHECK(p, "gev_pcls_init_lattice: NULL argument");
     CHECK(p->cap >= p->ctx->M, "gev_pcls_init_lattice: capacity below N^3");
     p->n = p->ctx->M;
+    p->qbound_valid = false;
     return PREC_DISPATCH(p->ctx, pcls_lattice_impl<double>(p), pcls_lattice_impl<float>(p));
 }
 
@@ -471,6 +479,7 @@ int gev_pcls_save(gev_pcls *p, void *dst)
 int gev_pcls_restore(gev_pcls *p, const void *src)
 {
     CHECK(p && src, "gev_pcls_restore: NULL argument");
+    p->qbound_valid = false;
     hipStream_t s = p->ctx->stream;
     int64_t n = 0;
     GEV_HIP(hipMemcpyAsync(&n, src, 8, hipMemcpyDeviceToHost, s));
@@ -828,6 +837,7 @@ int gev_displace_particles(gev_pcls *p, const void *xi, double coeff, double *ma
 {
     CHECK(p && xi, "gev_displace_particles: NULL argument");
     NEED_SORTED(p);
+    p->qbound_valid = false;
     return PREC_DISPATCH(p->ctx, displace_impl<double>(p, (const double *)xi, coeff, max_disp),
                          displace_impl<float>(p, (const float *)xi, coeff, max_disp));
 }
@@ -836,6 +846,7 @@ int gev_initialize_q(gev_pcls *p, const void *pot, double coeff, double *max_ret
 {
     CHECK(p && pot, "gev_initialize_q: NULL argument");
     NEED_SORTED(p);
+    p->qbound_valid = false;
     return PREC_DISPATCH(p->ctx, initialize_q_impl<double>(p, (const double *)pot, coeff, max_ret),
                          initialize_q_impl<float>(p, (const float *)pot, coeff, max_ret));
 }

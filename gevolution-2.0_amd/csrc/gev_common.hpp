@@ -128,7 +128,17 @@ struct gev_pcls {
     int32_t *cell_count = nullptr;  // M counters
     bool sorted = false;
     bool counts_zero = false;       // cell_count is all zero (the scan clears what it consumes)
+    // mixed form, N >= 16: the largest |q|^2 (float bits) of the particles in
+    // each block of 8^3 cells, written by the particle updates and read by the
+    // fixed-point deposits as their per-tile bound; valid from an update until
+    // anything else writes positions or momenta
+    uint32_t *qbound = nullptr;
+    bool qbound_valid = false;
 };
+
+namespace gev {
+constexpr int QB_LG = 3;  // log2 of the block edge of gev_pcls::qbound
+}
 
 #define GEV_HIP(call)                                                                        \
     do {                                                                                     \

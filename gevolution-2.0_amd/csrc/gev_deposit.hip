@@ -72,7 +72,7 @@ template <typename T, int KIND, int TX, int TY, int TZ>
 __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *out3, int64_t M,
                                                    const T *__restrict__ phi, double mass_d, double mass2_d,
                                                    double mass3_d, double a_d, int accumulate,
-                                                   const T *__restrict__ aux)
+                                                   const T *__restrict__ aux, const uint32_t *__restrict__ qb)
 {
     constexpr int C = DepC<KIND>::C;
     constexpr int NS = TX * TY * TZ;                    // owned sites
@@ -97,6 +97,23 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     int x0, y0, z0;
     tile_origin<TX, TY, TZ>(N, x0, y0, z0);
     const bool use_phi = phi != nullptr;
+    // the particle updates' largest |q|^2 over the blocks of 8^3 cells that
+    // the halo box touches (per dimension the block of the halo layer and
+    // those of the tile), loaded first so the loads overlap the staging below
+    float qqb = 0.0f;
+    if (FIX && qb && (KIND != DEP_T00 || use_phi)) {
+        const int nbq = N >> QB_LG;
+        const int bx0 = x0 >> QB_LG, by0 = y0 >> QB_LG, bz0 = z0 >> QB_LG;
+        const int nx = ((x0 + TX - 1) >> QB_LG) - bx0 + 2, ny = ((y0 + TY - 1) >> QB_LG) - by0 + 2,
+                  nz = ((z0 + TZ - 1) >> QB_LG) - bz0 + 2;
+        for (int e = t; e < nx * ny * nz; e += PT_NT) {
+            const int ix = e % nx, iy = (e / nx) % ny, iz = e / (nx * ny);
+            const int bx = ix == 0 ? (((x0 - 1) & (int)msk) >> QB_LG) : bx0 + ix - 1;
+            const int by = iy == 0 ? (((y0 - 1) & (int)msk) >> QB_LG) : by0 + iy - 1;
+            const int bz = iz == 0 ? (((z0 - 1) & (int)msk) >> QB_LG) : bz0 + iz - 1;
+            qqb = fmaxf(qqb, __uint_as_float(qb[bx + nbq * (by + nbq * bz)]));
+        }
+    }
 
     for (int e = t; e < NS; e += PT_NT) {
 #pragma unroll
@@ -172,7 +189,15 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
         }
         if (use_phi)  // the phi box is staged (the scan's barriers ordered it)
             for (int e = t; e < NPH; e += PT_NT) pmax = fmaxf(pmax, fabsf((float)phs[e]));
-        if (need_q_) {
+        if (need_q_ && qb) {
+            // bounds from this thread's share of the block maxima (the
+            // reduction below takes the tile's): a particle's e, q^2/e and
+            // |q_i| all increase with |q|^2
+            const float a2 = (float)(a_d * a_d);
+            emax = sqrtf(qqb + a2);
+            tmax = qqb / emax;
+            vmax = sqrtf(qqb);
+        } else if (need_q_) {
             const float a2 = (float)(a_d * a_d);
             for (int32_t i = t; i < total; i += PT_NT) {
                 int run;
@@ -436,8 +461,12 @@ static int launch_dep(gev_pcls *p, T *out, T *out2, const T *phi, double mass, d
     gev_ctx *c = p->ctx;
     const int N = c->N;
     const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
+    // the updates' |q|^2 block bounds replace the per-particle bound pass
+    // of the fixed-point tiles while they are valid (not for the adjoint's
+    // vector deposit, whose values are not momenta)
+    const uint32_t *qb = (sizeof(T) == 4 && KIND != DEP_VEC3 && p->qbound_valid) ? p->qbound : nullptr;
     hipLaunchKernelGGL((k_deposit<T, KIND, TX, TY, TZ>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), out, out2,
-                       out3, c->M, phi, mass, mass2, mass3, a, acc, aux);
+                       out3, c->M, phi, mass, mass2, mass3, a, acc, aux, qb);
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }

@@ -518,8 +518,13 @@ struct ZParams {
     const double *k0dev;  // when set, the real part of that mode is read here instead
 };
 
+// FP32 vector projection from L = 128 up: 4 waves per SIMD (two workgroups
+// per CU) at the price of a spill of up to 80 bytes, 4.5 -> 4.1 ms for the
+// 512^3 B solve; the scalar projection and FP64 measured no better that way
+template <typename T, int L, int OP> constexpr int zs_waves() { return (OP == 2 && sizeof(T) == 4 && L >= 128) ? 4 : 1; }
+
 template <typename T, int L, int TX, int OP>
-__global__ __launch_bounds__(TX * L / 8) void k_zsolve(const cx<T> *in, cx<T> *out, int P, int64_t NK,
+__global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(const cx<T> *in, cx<T> *out, int P, int64_t NK,
                                                        const cx<T> *__restrict__ tw, ZParams zp)
 {
     constexpr int NT = TX * L / 8, LDL = L + 1;

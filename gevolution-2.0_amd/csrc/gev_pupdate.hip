@@ -143,7 +143,8 @@ template <typename T, int TX, int TY, int TZ, int F, int NT = UpdShape<T>::NT>
 __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const T *__restrict__ phi,
                                                         const T *__restrict__ chi, const T *__restrict__ B0g,
                                                         const T *__restrict__ B1g, const T *__restrict__ B2g,
-                                                        unsigned long long *maxslot, int32_t *count, int32_t *rank)
+                                                        unsigned long long *maxslot, int32_t *count, int32_t *rank,
+                                                        uint32_t *__restrict__ qb)
 {
     constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3, NS = SX * SY * SZ, NROW = TY * TZ;
     static_assert(NROW <= 64 && SX <= 64 && (NROW & (NROW - 1)) == 0, "tile shape unsupported by the wave-level staging");
@@ -184,6 +185,19 @@ __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const
             }
         }
     }
+    // qb (FP32 form, N >= 16): the largest |q|^2 of the updated particles per
+    // block of 8^3 cells (gev_pcls::qbound), for the fixed-point deposits.
+    // The blocks of cells [x0-1, x0+TX] x ... (where every particle lands
+    // that moved at most one cell) are gathered in LDS with integer max on
+    // the float bits (monotone for values >= 0) and merged into qb at the
+    // end; a particle that landed elsewhere goes to qb directly.
+    constexpr int QWX = TX / 8 + 2, QWY = TY / 8 + 2, QWZ = TZ / 8 + 2, QWN = QWX * QWY * QWZ;
+    __shared__ uint32_t qwin[QWN];
+    const bool qbon = sizeof(T) == 4 && qb != nullptr;
+    const int qx0 = (x0 - 1 + N) >> QB_LG, qy0 = (y0 - 1 + N) >> QB_LG, qz0 = (z0 - 1 + N) >> QB_LG;
+    const int nbq = N >> QB_LG;
+    if (qbon)
+        for (int e = tid; e < QWN; e += NT) qwin[e] = 0u;
     // particle ranges of the tile's TY*TZ rows, computed by every wave for
     // itself (no barrier before the first particle loads)
     int32_t rb0, incl;
@@ -229,6 +243,24 @@ __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const
         const int32_t newc =
             upd_particle<T, F, SX, SY>(v, u, active ? p : v.sink, cc, xk, q, fld[0], x0, y0, z0, NS, rit);
         ret = active ? fmax(ret, rit) : ret;
+        if (qbon && active) {
+            const int32_t cq = drift ? newc : cc;
+            const uint32_t qq = __float_as_uint((float)(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]));
+            int cx, cy, cz;
+            cell_coords(cq, N, v.lg, cx, cy, cz);
+            const uint32_t rx = (uint32_t)(cx - x0 + 1) & msk, ry = (uint32_t)(cy - y0 + 1) & msk,
+                           rz = (uint32_t)(cz - z0 + 1) & msk;
+            if (rx <= (uint32_t)TX + 1 && ry <= (uint32_t)TY + 1 && rz <= (uint32_t)TZ + 1) {
+                const int wx = ((x0 - 1 + (int)rx + N) >> QB_LG) - qx0, wy = ((y0 - 1 + (int)ry + N) >> QB_LG) - qy0,
+                          wz = ((z0 - 1 + (int)rz + N) >> QB_LG) - qz0;
+                // values only grow: the atomic is needed only above what the
+                // window holds (most lanes of a wave share one block)
+                uint32_t *w = &qwin[wx + QWX * (wy + QWY * wz)];
+                if (qq > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMax(w, qq);
+            } else {
+                atomicMax(&qb[(cx >> QB_LG) + nbq * ((cy >> QB_LG) + nbq * (cz >> QB_LG))], qq);
+            }
+        }
         // re-binning rank in the new cell, one counter atomic per run of equal
         // cells; each pass stores the rank of the previous one, so the
         // atomic's round trip overlaps a pass of work
@@ -246,6 +278,16 @@ __global__ __launch_bounds__(NT) void k_update_tiled(PV<T> v, UpdParams u, const
         rank[ppend >= 0 ? ppend : v.sink] = rk;
     }
     if (kick) block_max_to<NT>(ret, maxslot);
+    if (qbon) {
+        __syncthreads();  // every lane's window updates
+        for (int e = tid; e < QWN; e += NT) {
+            const uint32_t val = qwin[e];
+            const int wx = e % QWX, wy = (e / QWX) % QWY, wz = e / (QWX * QWY);
+            if (val)
+                atomicMax(&qb[((qx0 + wx) & (nbq - 1)) + nbq * (((qy0 + wy) & (nbq - 1)) + nbq * ((qz0 + wz) & (nbq - 1)))],
+                          val);
+        }
+    }
 }
 
 
@@ -261,7 +303,8 @@ static void launch_tile(gev_pcls *p, const UpdParams &u, const T *phi, const T *
     const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
     if constexpr (!(sizeof(T) == 8 && TX == 16))  // the large tile exists for FP32 only
         hipLaunchKernelGGL((k_update_tiled<T, TX, TY, TZ, F>), dim3(nb), dim3(UpdShape<T>::NT), 0, c->stream,
-                           make_pv<T>(p), u, phi, chi, B0, B1, B2, slot, p->cell_count, p->rank);
+                           make_pv<T>(p), u, phi, chi, B0, B1, B2, slot, p->cell_count, p->rank,
+                           sizeof(T) == 4 ? p->qbound : nullptr);
 }
 
 template <typename T, int TX, int TY, int TZ>
@@ -294,6 +337,13 @@ template <typename T> static int launch_update(gev_pcls *p, const UpdParams &u, 
         int rc = pcls_clear_counts(p);
         if (rc) return rc;
     }
+    // the |q|^2 block bounds are rebuilt by every FP32 update (N >= 16)
+    p->qbound_valid = false;
+    if (sizeof(T) == 4 && p->qbound) {
+        const int64_t nb = (int64_t)(N >> QB_LG);
+        int rc = field_fill_impl<int32_t>(c, (int32_t *)p->qbound, nb * nb * nb, 0);
+        if (rc) return rc;
+    }
     unsigned long long *slot = (unsigned long long *)(c->red + 8);
     const T *B0 = Bi, *B1 = Bi ? Bi + c->M : nullptr, *B2 = Bi ? Bi + 2 * c->M : nullptr;
     // FP32 fits the 16x8x8 tile (5 staged fields, 46 KB of LDS); FP64 keeps 8x8x4
@@ -307,6 +357,7 @@ template <typename T> static int launch_update(gev_pcls *p, const UpdParams &u, 
         launch_mode<T, 4, 4, 4>(p, u, phi, chi, B0, B1, B2, slot);
     }
     GEV_HIP(hipGetLastError());
+    p->qbound_valid = sizeof(T) == 4 && p->qbound != nullptr;
     return GEV_OK;
 }
 

@@ -297,6 +297,50 @@ def test_deposits_mixed(gp, O, N, cluster):
     assert max(errs) < tol
 
 
+@pytest.mark.parametrize("N,cluster", [(32, False), (64, True), (128, False)])
+def test_deposits_mixed_after_update(gp, O, N, cluster):
+    """After a particle update, the fixed-point deposits bound each tile by
+    the update's largest |q|^2 over the 8^3-cell blocks the tile's halo box
+    touches, instead of a pass over the tile's particles.  The drift moves
+    particles up to ~2.5 cells at N = 128, so some land outside their tile's
+    window of blocks.  Both bounds must give T00, Tij and the T0i link sums
+    within the fixed-point tolerance of the FP64 oracle on the updated
+    particles; a particle above its block bound would overflow its tile's
+    32-bit accumulators and fail by orders of magnitude."""
+    ctx = gp.Context(N, gp.GEV_MIXED)
+    n = N ** 3 // 2
+    pos, q, pid = make_particles(N, n, 2100 + N, vmax=0.2, cluster=cluster)
+    rng = np.random.default_rng(N + 11)
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    phi, chi = f32(1e-3 * rng.standard_normal(N ** 3)), f32(1e-4 * rng.standard_normal(N ** 3))
+    B = f32(1e-2 * rng.standard_normal((3, N ** 3)))
+    m = 0.31 / n
+    p = gpu_pcls(gp, ctx, pos, q, pid, m)
+    a_k, a_d = 0.2, 0.21
+    p.kick_drift(1, 0.013, 0.02, field_gpu(ctx, phi), field_gpu(ctx, chi), field_gpu(ctx, B, 3), 3, 3, a_k, a_d,
+                 a_k ** 2 * N, a_d ** 2 * N)
+    gpos, gq, gid = by_id(p)
+    moved = np.abs(((gpos - pos[:, np.argsort(pid)]) + 0.5) % 1.0 - 0.5).max() * N
+    a = 0.05
+    ref00 = O.project_T00(N, gpos, gq, m, a=a, phi=phi)
+    refij = O.project_Tij(N, gpos, gq, m, a=a, phi=phi)
+    ref0i = O.project_T0i(N, gpos, gq, m)
+    tol = 3e-5 if cluster else 2e-6
+    tol_l = 2e-3 if cluster else 2e-6
+    got = {}
+    for form in ("block bounds", "particle pass"):
+        if form == "particle pass":
+            p.sort()  # particles may have been written: the next deposits bound their tiles themselves
+        T00, Tij, Lk = p.project_T00_Tij_T0i_links(ctx.real(1), ctx.real(6), ctx.real(3), a, field_gpu(ctx, phi))
+        got[form] = (to_np(T00), to_np(Tij), to_np(Lk))
+        errs = (relmax(real_np(T00)[0], ref00), relmax(real_np(Tij), refij), relmax(real_np(Lk), ref0i))
+        print(f"mixed deposits after an update N={N} cluster={cluster} ({form}, largest move {moved:.2f} cells): "
+              f"rel max errors T00, Tij, T0i links {errs}", flush=True)
+        assert errs[0] < tol and errs[1] < tol and errs[2] < tol_l
+    # the two bounds differ, so the fixed-point roundings do too
+    assert any(not np.array_equal(x, y) for x, y in zip(got["block bounds"], got["particle pass"]))
+
+
 # ---------------------------------------------------------- kick / drift --
 @pytest.mark.parametrize("N", [8, 32])
 def test_kick_drift_fp64(gp, O, N):

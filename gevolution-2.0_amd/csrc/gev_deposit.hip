@@ -19,9 +19,11 @@
 // atomics, which this part executes several times faster than FP32 or FP64
 // ones, and whose sums do not depend on the order): each tile bounds its
 // per-site sums by (largest particle count of the 8 cells around one of its
-// sites) x (largest per-particle value, from the tile's particles and |phi|,
-// separately for the T00, Tij and T0i components, whose sizes differ by the
-// ratio of kinetic to rest energy) and scales so that bound is 2^30.
+// sites) x (largest per-particle value, from |q| and |phi|, separately for
+// the T00, Tij and T0i components, whose sizes differ by the ratio of
+// kinetic to rest energy) and scales so that bound is 2^30.  |q| is bounded
+// by the largest |q|^2 per 8^3-cell block that the last particle update
+// recorded (gev_pcls::qbound), or else by a pass over the tile's particles.
 #include "gev_common.hpp"
 #include "gev_kernels.hpp"
 #include "gev_pcommon.hpp"

@@ -44,7 +44,10 @@ enum { GEV_FP64 = 0, GEV_MIXED = 1 };
 typedef struct gev_ctx gev_ctx;
 typedef struct gev_pcls gev_pcls;
 
-/* raw device view of a particle container (valid until the next re-binning) */
+/* raw device view of a particle container (valid until the next re-binning).
+ * For reading: a caller that writes positions or momenta through it calls
+ * gev_pcls_sort before the next deposit (the mixed form's deposits otherwise
+ * bound their fixed-point sums by the momenta of the last particle update) */
 typedef struct {
     int64_t n;
     int precision;

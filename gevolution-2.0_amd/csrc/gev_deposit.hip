@@ -54,7 +54,11 @@ __device__ __forceinline__ void lds_add(double *a, double v)
 }
 __device__ __forceinline__ void lds_add(int32_t *a, float v)
 {
-    __hip_atomic_fetch_add(a, __float2int_rn(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // floor(v + 1/2) in one instruction (v_cvt_rpi_i32_f32; the compiler emits
+    // two for round-to-nearest): the deposit is bound by its VALU issue rate
+    int32_t r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    __hip_atomic_fetch_add(a, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 // accumulator of a kind and precision: fixed point for the FP32 form of the
 // physical projections (their values are bounded by the particles' energies);
@@ -232,13 +236,15 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
         // T0i's (1 + phi_a + phi_b) q_i <= (1 + 2|phi|) max|q_i|; each rounded up
         // by 1 % against the FP32 rounding of the values.  The tile's per-site
         // sums stay below cmax times these.
+        // (in FP32 with the hardware reciprocal: any scale at or below the
+        // bound is exact to use, since the stores divide by the same value)
         if (cmax > 0.0f) {
-            const double pm = (double)pmax, c = (double)cmax, top = 1073741824.0;
-            const double B0 = need_q_ ? (double)emax * (1.0 + 4.0 * pm + 9.0 * pm * pm) * 1.01 : 1.0;
-            const double B1 = (double)tmax * (1.0 + 5.0 * pm) * 1.01, B2 = (double)vmax * (1.0 + 2.0 * pm) * 1.01;
-            S0 = (float)(top / (c * B0));
-            if (B1 > 0.0) S1 = (float)(top / (c * B1));
-            if (B2 > 0.0) S2 = (float)(top / (c * B2));
+            const float pm = pmax, top = 1073741824.0f;
+            const float B0 = need_q_ ? emax * (1.0f + 4.0f * pm + 9.0f * pm * pm) * 1.01f : 1.0f;
+            const float B1 = tmax * (1.0f + 5.0f * pm) * 1.01f, B2 = vmax * (1.0f + 2.0f * pm) * 1.01f;
+            S0 = top * __builtin_amdgcn_rcpf(cmax * B0);
+            if (B1 > 0.0f) S1 = top * __builtin_amdgcn_rcpf(cmax * B1);
+            if (B2 > 0.0f) S2 = top * __builtin_amdgcn_rcpf(cmax * B2);
         }
     }
 
@@ -306,6 +312,24 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
         const int gx = (x0 - 1 + lx) & (int)msk, gy = (y0 - 1 + ly) & (int)msk, gz = (z0 - 1 + lz) & (int)msk;
         T rd[3];
         cell_rel(xu[u], gx, gy, gz, N, rd);
+        // CIC weight of corner b = 4 bx + 2 by + bz.  FP32 forms the eight
+        // once from the (y,z) pair products and shares them between the
+        // components (its kernel is bound by the VALU issue rate); FP64 forms
+        // each product where it is used (eight more live FP64 values would
+        // cost it a wave per SIMD)
+        constexpr bool SHARE = sizeof(T) == 4;
+        const T ox = o - rd[0], oy = o - rd[1], oz = o - rd[2];
+        T pyz[4], W8[8];
+        if constexpr (SHARE) {
+            pyz[0] = oy * oz; pyz[1] = oy * rd[2]; pyz[2] = rd[1] * oz; pyz[3] = rd[1] * rd[2];
+#pragma unroll
+            for (int b = 0; b < 8; b++) W8[b] = ((b >> 2) ? rd[0] : ox) * pyz[b & 3];
+        }
+        // v times the weight of corner b
+        auto Wv = [&](int b, T v) -> T {
+            if constexpr (SHARE) return v * W8[b];
+            else return ((b >> 2) ? rd[0] : ox) * (((b >> 1) & 1) ? rd[1] : oy) * ((b & 1) ? rd[2] : oz) * v;
+        };
         // site of corner (bx,by,bz) in tile coordinates and whether the tile owns it
         const int sx = lx - 1, sy = ly - 1, sz = lz - 1;
         const bool inx0 = sx >= 0, inx1 = sx + 1 < TX;
@@ -322,11 +346,10 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
             for (int b = 0; b < 8; b++) {
                 const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
                 if (own(bx, by, bz)) {
-                    const T w = (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]);
                     const int si = sidx(bx, by, bz);
-                    lds_add(&acc[0][si], w * qu[u][0]);
-                    lds_add(&acc[1][si], w * qu[u][1]);
-                    lds_add(&acc[2][si], w * qu[u][2]);
+                    lds_add(&acc[0][si], Wv(b, qu[u][0]));
+                    lds_add(&acc[1][si], Wv(b, qu[u][1]));
+                    lds_add(&acc[2][si], Wv(b, qu[u][2]));
                 }
             }
         }
@@ -344,8 +367,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
                     const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
                     if (own(bx, by, bz)) {
                         const T ph = phs[pb + bx + PX * (by + PY * bz)];
-                        const T w = (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]);
-                        lds_add(&acc[0][sidx(bx, by, bz)], w * (es + fs * ph + f2s * ph * ph));
+                        lds_add(&acc[0][sidx(bx, by, bz)], Wv(b, es + fs * ph + f2s * ph * ph));
                     }
                 }
             } else {
@@ -353,9 +375,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
 #pragma unroll
                 for (int b = 0; b < 8; b++) {
                     const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-                    if (own(bx, by, bz))
-                        lds_add(&acc[0][sidx(bx, by, bz)],
-                                sc * (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
+                    if (own(bx, by, bz)) lds_add(&acc[0][sidx(bx, by, bz)], Wv(b, sc));
                 }
             }
         }
@@ -366,10 +386,22 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
             constexpr int L0 = KIND == DEP_T0I ? 0 : 7;  // first T0i accumulator
             const T sq = FIX ? (T)S2 : o;
             const T v0 = qu[u][0] * sq, v1 = qu[u][1] * sq, v2 = qu[u][2] * sq;
-            T g[12] = {(o - rd[1]) * (o - rd[2]) * v0, (o - rd[0]) * (o - rd[2]) * v1, (o - rd[0]) * (o - rd[1]) * v2,
-                       rd[0] * (o - rd[2]) * v1,       rd[0] * (o - rd[1]) * v2,       rd[1] * (o - rd[2]) * v0,
-                       rd[1] * (o - rd[0]) * v2,       rd[2] * (o - rd[1]) * v0,       rd[2] * (o - rd[0]) * v1,
-                       rd[1] * rd[2] * v0,             rd[0] * rd[2] * v1,             rd[0] * rd[1] * v2};
+            // link k: the pair product of the two axes across it times v
+            T g[12];
+            if constexpr (SHARE) {
+                const T pxz[4] = {ox * oz, ox * rd[2], rd[0] * oz, rd[0] * rd[2]};
+                const T pxy[4] = {ox * oy, ox * rd[1], rd[0] * oy, rd[0] * rd[1]};
+                const T gs[12] = {pyz[0] * v0, pxz[0] * v1, pxy[0] * v2, pxz[2] * v1, pxy[2] * v2, pyz[2] * v0,
+                                  pxy[1] * v2, pyz[1] * v0, pxz[1] * v1, pyz[3] * v0, pxz[3] * v1, pxy[3] * v2};
+#pragma unroll
+                for (int k = 0; k < 12; k++) g[k] = gs[k];
+            } else {
+                const T gs[12] = {oy * oz * v0,       ox * oz * v1,       ox * oy * v2,       rd[0] * oz * v1,
+                                  rd[0] * oy * v2,    rd[1] * oz * v0,    rd[1] * ox * v2,    rd[2] * oy * v0,
+                                  rd[2] * ox * v1,    rd[1] * rd[2] * v0, rd[0] * rd[2] * v1, rd[0] * rd[1] * v2};
+#pragma unroll
+                for (int k = 0; k < 12; k++) g[k] = gs[k];
+            }
             if (KIND == DEP_T0I && use_phi) {
                 T ph[8];
 #pragma unroll
@@ -401,10 +433,16 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
             if (use_phi) {
                 const T *P = phs + pb;
                 f = (T)4 + a * a * frcp(e2);
-                f *= (o - rd[0]) * ((o - rd[1]) * ((o - rd[2]) * P[0] + rd[2] * P[PX * PY]) +
-                                    rd[1] * ((o - rd[2]) * P[PX] + rd[2] * P[PX + PX * PY])) +
-                     rd[0] * ((o - rd[1]) * ((o - rd[2]) * P[1] + rd[2] * P[1 + PX * PY]) +
-                              rd[1] * ((o - rd[2]) * P[1 + PX] + rd[2] * P[1 + PX + PX * PY]));
+                T pint = (T)0;  // phi at the particle, CIC-interpolated
+                if constexpr (SHARE) {
+#pragma unroll
+                    for (int b = 0; b < 8; b++) pint += Wv(b, P[((b >> 2) & 1) + PX * (((b >> 1) & 1) + PY * (b & 1))]);
+                } else {
+                    pint = ox * (oy * (oz * P[0] + rd[2] * P[PX * PY]) + rd[1] * (oz * P[PX] + rd[2] * P[PX + PX * PY])) +
+                           rd[0] * (oy * (oz * P[1] + rd[2] * P[1 + PX * PY]) +
+                                    rd[1] * (oz * P[1 + PX] + rd[2] * P[1 + PX + PX * PY]));
+                }
+                f *= pint;
                 f += o;
             } else {
                 f = o;
@@ -417,9 +455,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
 #pragma unroll
                 for (int b = 0; b < 8; b++) {
                     const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-                    if (own(bx, by, bz))
-                        lds_add(&acc[dcomp[d]][sidx(bx, by, bz)],
-                                w * (bx ? rd[0] : o - rd[0]) * (by ? rd[1] : o - rd[1]) * (bz ? rd[2] : o - rd[2]));
+                    if (own(bx, by, bz)) lds_add(&acc[dcomp[d]][sidx(bx, by, bz)], Wv(b, w));
                 }
             }
             T w = f * vv[0] * vv[1] * ie;
@@ -436,6 +472,15 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     }
     __syncthreads();
 
+    // per component group (T00 and the mass deposit, Tij, T0i): the mass over
+    // the group's fixed-point scale, divided once
+    double fac[3] = {mass_d, mass_d, mass_d};
+    if constexpr (FIX) {
+        const double m1 = T00TIJ ? mass2_d : mass_d, m2 = KIND == DEP_T00TIJL ? mass3_d : mass_d;
+        fac[0] = mass_d / (double)S0;
+        fac[1] = m1 / (double)S1;
+        fac[2] = m2 / (double)S2;
+    }
     for (int s = t; s < NS; s += PT_NT) {
         const int sx = s % TX, sy = (s / TX) % TY, sz = s / (TX * TY);
         const uint32_t gs = (uint32_t)(x0 + sx) + (uint32_t)N * ((uint32_t)(y0 + sy) + (uint32_t)N * (uint32_t)(z0 + sz));
@@ -445,11 +490,9 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
             double ms = mass_d;
             if (T00TIJ && c > 0 && c < 7) { oc = out2 + (c - 1) * M; ms = mass2_d; }
             if (KIND == DEP_T00TIJL && c >= 7) { oc = out3 + (c - 7) * M; ms = mass3_d; }
-            // component groups: T00 (and the mass deposit), Tij, T0i
             const int grp = KIND == DEP_T00 ? 0 : KIND == DEP_TIJ ? 1 : KIND == DEP_T0I ? 2
                                                                  : (c == 0 ? 0 : (c < 7 ? 1 : 2));
-            const double Sc = grp == 0 ? (double)S0 : grp == 1 ? (double)S1 : (double)S2;
-            T val = FIX ? (T)((double)acc[c][s] * (ms / Sc)) : (T)(acc[c][s] * ms);
+            T val = FIX ? (T)((double)acc[c][s] * fac[grp]) : (T)(acc[c][s] * ms);
             if (accumulate) val += oc[gs];
             oc[gs] = val;
         }

@@ -507,7 +507,9 @@ int fft_c2r_impl(gev_ctx *ctx, cx<T> *in, T *out, int ncomp, const cx<T> *tw)
 // every mode is read once and written once instead of three times each.
 // In place (in == out) is allowed: a workgroup reads all its columns before
 // writing them and no two workgroups share a column.
-enum { ZOP_POISSON = 0, ZOP_SCALAR = 1, ZOP_VECTOR = 2 };
+// ZOP_SCALAR3: projectFTscalar from the three groups A, B, C that k_col_comb
+// forms in the y pass (below) instead of the six Sij components
+enum { ZOP_POISSON = 0, ZOP_SCALAR = 1, ZOP_VECTOR = 2, ZOP_SCALAR3 = 3 };
 
 struct ZParams {
     const double *gk2;
@@ -517,6 +519,111 @@ struct ZParams {
     int k0set;
     const double *k0dev;  // when set, the real part of that mode is read here instead
 };
+
+// ============================ y pass of Sij forming the groups of the z solve ==
+// The y transforms of the six Sij components (x-transformed, in place at
+// data + c NK) of TX adjacent kx columns and one z, each output mode added
+// with its (kx, ky) weight of projectFTscalar into the three groups that the
+// z solve transforms (gevolution.hpp:317-323 with g = gridk2, k = kshift):
+//   A = (g1 - 2 g0) S_xx - 6 k0 k1 S_xy + (g0 - 2 g1) S_yy + (g0 + g1) S_zz
+//   B = S_xx + S_yy - 2 S_zz,   C = -6 (k0 S_xz + k1 S_yz)
+// written over components 0..2 (every column of the workgroup is read before
+// any is written; no other workgroup touches them).  Padding columns
+// (kx > N/2) get 0.  Replaces six y passes and the z solve's reads of six
+// components by reads of three.
+template <typename T, int L, int TX>
+__global__ __launch_bounds__(TX * L / 8) void k_col_comb(cx<T> *data, int P, int64_t NK, const cx<T> *__restrict__ tw,
+                                                         const double *__restrict__ gk2, const cx<double> *__restrict__ ks)
+{
+    constexpr int NT = TX * L / 8;
+    __shared__ cx<T> lds[TX * (L + 1)];
+    __shared__ cx<T> twl[L];
+    // the k tables in the transform's precision (the weights are applied in it)
+    __shared__ T gkl[L];
+    __shared__ cx<T> ksl[L];
+    for (int m = threadIdx.x; m < L; m += NT) {
+        twl[m] = tw[m];
+        gkl[m] = (T)gk2[m];
+        ksl[m] = mk((T)ks[m].x, (T)ks[m].y);
+    }
+    const int ntiles = P / TX;
+    const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int ct = (int)(b % ntiles);
+    const int64_t z = b / ntiles;
+    cx<T> *base = data + (int64_t)ct * TX + (int64_t)P * L * z;
+    // byte offset of (row, column) inside a component's (kx, y) plane of this
+    // z: 32 bits, so every access takes the workgroup's base from scalar
+    // registers instead of a 64-bit address per element
+    auto boff = [&](int row, int ln) -> uint32_t { return ((uint32_t)row * (uint32_t)P + (uint32_t)ln) * (uint32_t)sizeof(cx<T>); };
+    const int line = threadIdx.x % TX;  // the column every slot of this thread lands in
+    const int kx = ct * TX + line;
+    const bool valid = kx <= L / 2;
+    const T g0 = valid ? (T)gk2[kx] : (T)0;
+    const cx<T> k0 = valid ? mk((T)ks[kx].x, (T)ks[kx].y) : mk((T)0, (T)0);
+    const cx<T> k0m = k0 * (T)-6;
+    cx<T> A[8], B[8], C[8], v[8];
+    int rows[8];
+    auto comp = [&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        const cx<T> *src = base + c * NK;
+        auto ld = [&](int ln, int row) { return *reinterpret_cast<const cx<T> *>(reinterpret_cast<const char *>(src) + boff(row, ln)); };
+        auto st = [&](int, int row, cx<T> val, int slot) {
+            if constexpr (c == 0) rows[slot] = row;
+            v[slot] = val;
+        };
+        __syncthreads();  // tables loaded; the previous transform is done with lds
+        reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, -1, ld, st);
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int row = rows[s];
+            if constexpr (c == 0) {
+                A[s] = v[s] * (gkl[row] - (T)2 * g0);
+                B[s] = v[s];
+            } else if constexpr (c == 1) {
+                A[s] = A[s] + (k0m * ksl[row]) * v[s];
+            } else if constexpr (c == 2) {
+                C[s] = k0m * v[s];
+            } else if constexpr (c == 3) {
+                A[s] = A[s] + v[s] * (g0 - (T)2 * gkl[row]);
+                B[s] = B[s] + v[s];
+            } else if constexpr (c == 4) {
+                C[s] = C[s] + (ksl[row] * (T)-6) * v[s];
+            } else {
+                A[s] = A[s] + v[s] * (g0 + gkl[row]);
+                B[s] = B[s] - v[s] * (T)2;
+            }
+        }
+    };
+    comp(std::integral_constant<int, 0>{});
+    comp(std::integral_constant<int, 1>{});
+    comp(std::integral_constant<int, 2>{});
+    comp(std::integral_constant<int, 3>{});
+    comp(std::integral_constant<int, 4>{});
+    comp(std::integral_constant<int, 5>{});
+    // (selects on the scalar parts: a select of whole structs becomes a select
+    // of their addresses, which keeps the arrays out of registers)
+    auto put = [&](cx<T> *dst, int s, cx<T> val) {
+        *reinterpret_cast<cx<T> *>(reinterpret_cast<char *>(dst) + boff(rows[s], line)) =
+            mk(valid ? val.x : (T)0, valid ? val.y : (T)0);
+    };
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        put(base, s, A[s]);
+        put(base + NK, s, B[s]);
+        put(base + 2 * NK, s, C[s]);
+    }
+}
+
+template <typename T, int L> constexpr int comb_tx() { return 4096 / L < 8 ? 4096 / L : 8; }
+
+template <typename T, int N>
+static void launch_col_comb(cx<T> *data, int P, int64_t NK, const cx<T> *tw, const ZParams &zp, hipStream_t s)
+{
+    constexpr int TX = comb_tx<T, N>();
+    const int64_t nb = (int64_t)(P / TX) * N;
+    hipLaunchKernelGGL((k_col_comb<T, N, TX>), dim3((unsigned)nb), dim3(TX * N / 8), 0, s, data, P, NK, tw, zp.gk2,
+                       zp.ks);
+}
 
 // FP32 vector projection from L = 128 up: 4 waves per SIMD (two workgroups
 // per CU) at the price of a spill of up to 80 bytes, 4.5 -> 4.1 ms for the
@@ -620,11 +727,17 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const int64_t e = (int64_t)(jt + r * NB) * stride;
-            const cx<T> s0 = src[e], s1 = src[NK + e], s2 = src[2 * NK + e], s3 = src[3 * NK + e],
-                        s4 = src[4 * NK + e], s5 = src[5 * NK + e];
-            A[r] = s0 * wa0 + wa1 * s1 + s3 * wa3 + s5 * wa5;
-            B[r] = s0 + s3 - s5 * (T)2;
-            C[r] = wc2 * s2 + wc4 * s4;
+            if constexpr (OP == ZOP_SCALAR3) {
+                A[r] = src[e];
+                B[r] = src[NK + e];
+                C[r] = src[2 * NK + e];
+            } else {
+                const cx<T> s0 = src[e], s1 = src[NK + e], s2 = src[2 * NK + e], s3 = src[3 * NK + e],
+                            s4 = src[4 * NK + e], s5 = src[5 * NK + e];
+                A[r] = s0 * wa0 + wa1 * s1 + s3 * wa3 + s5 * wa5;
+                B[r] = s0 + s3 - s5 * (T)2;
+                C[r] = wc2 * s2 + wc4 * s4;
+            }
         }
         auto fft = [&](const cx<T> *pre, auto kind) {
             constexpr int K = decltype(kind)::value;
@@ -659,7 +772,7 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
                 if (!(zero_mode && zp.modif == 0.0)) r = kop_poisson(sv, kk, zp.coeff, zp.modif);
             }
             vin[0][s] = mk((T)r.x, (T)r.y);
-        } else if constexpr (OP == ZOP_SCALAR) {
+        } else if constexpr (OP == ZOP_SCALAR || OP == ZOP_SCALAR3) {
             cx<double> r = mk(0.0, 0.0);
             if (!pad && !zero_mode) r = acc[s] * kop_scalar_norm(kk, L);
             vin[0][s] = mk((T)r.x, (T)r.y);
@@ -699,7 +812,7 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
 // (three workgroups per CU instead of one, 4 % faster at L = 512)
 template <typename T, int L, int OP> constexpr int zs_tx()
 {
-    constexpr bool narrow = OP == ZOP_VECTOR || OP == ZOP_SCALAR || L <= 512;
+    constexpr bool narrow = OP != ZOP_POISSON || L <= 512;
     return (narrow && col_tx<T, L>() * L / 8 > 512) ? 4096 / L : col_tx<T, L>();
 }
 
@@ -734,10 +847,19 @@ template <typename T, int N>
 static void zsolve_chain(gev_ctx *c, int op, const T *rin, int nin, cx<T> *ftin, cx<T> *ftout, T *rout, int nout,
                          const ZParams &zp)
 {
-    for (int i = 0; i < nin; i++) fwd_xy<T, N>(c, rin + i * c->M, ftin + i * c->NK);
-    if (op == ZOP_POISSON) launch_zsolve<T, N, ZOP_POISSON>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
-    else if (op == ZOP_SCALAR) launch_zsolve<T, N, ZOP_SCALAR>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
-    else launch_zsolve<T, N, ZOP_VECTOR>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
+    if (op == ZOP_SCALAR) {
+        // x passes, then the y pass that forms the three groups of projectFTscalar
+        for (int i = 0; i < nin; i++)
+            launch_r2c<T, N>(rin + i * c->M, ftin + i * c->NK, c->P, (int64_t)N * N, twiddles_of<T>(c), c->stream);
+        launch_col_comb<T, N>(ftin, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
+        launch_zsolve<T, N, ZOP_SCALAR3>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
+    } else {
+        for (int i = 0; i < nin; i++) fwd_xy<T, N>(c, rin + i * c->M, ftin + i * c->NK);
+        if (op == ZOP_POISSON)
+            launch_zsolve<T, N, ZOP_POISSON>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
+        else
+            launch_zsolve<T, N, ZOP_VECTOR>(ftin, ftout, c->P, c->NK, twiddles_of<T>(c), zp, c->stream);
+    }
     for (int j = 0; j < nout; j++) inv_yx<T, N>(c, ftout + j * c->NK, rout + j * c->M);
 }
 
@@ -820,19 +942,17 @@ static void chi_from_Tij_chain(gev_ctx *c, const T *phi, const T *Tij, T coeff, 
 {
     const int64_t nl = (int64_t)N * N;
     const cx<T> *tw = twiddles_of<T>(c);
-    auto xy = [&](const auto &src, int i) {
-        launch_r2c_src<T, N>(src, SijFT + i * c->NK, c->P, nl, tw, c->stream);
-        launch_col<T, N>(SijFT + i * c->NK, c->P, 1, -1, tw, c->stream);
-    };
+    auto x = [&](const auto &src, int i) { launch_r2c_src<T, N>(src, SijFT + i * c->NK, c->P, nl, tw, c->stream); };
     const int64_t M = c->M;
-    xy(SrcSij<T, N, 0>{phi, Tij, coeff}, 0);
-    xy(SrcSij<T, N, 1>{phi, Tij + M, coeff}, 1);
-    xy(SrcSij<T, N, 2>{phi, Tij + 2 * M, coeff}, 2);
-    xy(SrcSij<T, N, 3>{phi, Tij + 3 * M, coeff}, 3);
-    xy(SrcSij<T, N, 4>{phi, Tij + 4 * M, coeff}, 4);
-    xy(SrcSij<T, N, 5>{phi, Tij + 5 * M, coeff}, 5);
+    x(SrcSij<T, N, 0>{phi, Tij, coeff}, 0);
+    x(SrcSij<T, N, 1>{phi, Tij + M, coeff}, 1);
+    x(SrcSij<T, N, 2>{phi, Tij + 2 * M, coeff}, 2);
+    x(SrcSij<T, N, 3>{phi, Tij + 3 * M, coeff}, 3);
+    x(SrcSij<T, N, 4>{phi, Tij + 4 * M, coeff}, 4);
+    x(SrcSij<T, N, 5>{phi, Tij + 5 * M, coeff}, 5);
     ZParams zp{c->gk2, c->kshift, 0.0, 0.0, 0.0, 0.0, 0, nullptr};
-    launch_zsolve<T, N, ZOP_SCALAR>(SijFT, chiFT, c->P, c->NK, tw, zp, c->stream);
+    launch_col_comb<T, N>(SijFT, c->P, c->NK, tw, zp, c->stream);
+    launch_zsolve<T, N, ZOP_SCALAR3>(SijFT, chiFT, c->P, c->NK, tw, zp, c->stream);
     inv_yx<T, N>(c, chiFT, chi);
 }
 

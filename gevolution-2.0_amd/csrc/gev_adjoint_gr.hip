@@ -21,6 +21,8 @@
 // follows atomic arbitration).  Per-particle adjoints lx, lq are [3][n]
 // arrays indexed by particle ID, as in the Newtonian adjoint; positions in box
 // units (d/dx = N d/dr).
+#include <type_traits>
+
 #include "gev_common.hpp"
 #include "gev_dual.hpp"
 #include "gev_kernels.hpp"
@@ -34,27 +36,26 @@ using D6 = Dual<double, 6>;
 template <typename T> __device__ __forceinline__ void gadd(T *p, double v) { atomicAdd(p, (T)v); }
 
 // CIC weight of corner b and its derivatives in r
-__device__ __forceinline__ double cw(int b, const double r[3])
+template <typename R> __device__ __forceinline__ R cw(int b, const R r[3])
 {
-    return (((b >> 2) & 1) ? r[0] : 1.0 - r[0]) * (((b >> 1) & 1) ? r[1] : 1.0 - r[1]) * ((b & 1) ? r[2] : 1.0 - r[2]);
+    const R o = (R)1;
+    return (((b >> 2) & 1) ? r[0] : o - r[0]) * (((b >> 1) & 1) ? r[1] : o - r[1]) * ((b & 1) ? r[2] : o - r[2]);
 }
-__device__ __forceinline__ void cw_grad(int b, const double r[3], double g[3])
+template <typename R> __device__ __forceinline__ void cw_grad(int b, const R r[3], R g[3])
 {
     const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-    const double wx = bx ? r[0] : 1.0 - r[0], wy = by ? r[1] : 1.0 - r[1], wz = bz ? r[2] : 1.0 - r[2];
-    g[0] = (bx ? 1.0 : -1.0) * wy * wz;
-    g[1] = wx * (by ? 1.0 : -1.0) * wz;
-    g[2] = wx * wy * (bz ? 1.0 : -1.0);
+    const R o = (R)1;
+    const R wx = bx ? r[0] : o - r[0], wy = by ? r[1] : o - r[1], wz = bz ? r[2] : o - r[2];
+    g[0] = (bx ? o : -o) * wy * wz;
+    g[1] = wx * (by ? o : -o) * wz;
+    g[2] = wx * wy * (bz ? o : -o);
 }
 
-// field accessor in FP64 around the particle's cell (fields of either precision)
+// field accessor around the particle's cell, in the field's precision
 template <typename T> struct FAt {
     const T *f;
     int N, x, y, z;
-    __device__ __forceinline__ double operator()(int dx, int dy, int dz) const
-    {
-        return (double)f[site(N, x + dx, y + dy, z + dz)];
-    }
+    __device__ __forceinline__ T operator()(int dx, int dy, int dz) const { return f[site(N, x + dx, y + dy, z + dz)]; }
 };
 
 // update_pos rate (gevolution.hpp:1162-1232) over a generic real type: the
@@ -63,8 +64,9 @@ template <typename R, class FA>
 __device__ __forceinline__ void drift_rate(const R r[3], const R q[3], const FA &Fp, const FA &Fc, const FA &B0,
                                            const FA &B1, const FA &B2, int nfield, double a, double bcoeff, R out[3])
 {
+    using B = typename DualBase<R>::type;  // the real type under R
     R v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-    const R e2 = v2 + a * a;
+    const R e2 = v2 + (B)(a * a);
     R ph = (R)0, ch = (R)0;
     if (nfield >= 1) ph = cic<R>(Fp, r);
     if (nfield >= 2) ch = cic<R>(Fc, r);
@@ -87,7 +89,7 @@ __device__ __forceinline__ void drift_rate(const R r[3], const R q[3], const FA 
         b[1] += B1(1, 0, 1) * r[0] * r[2];
         b[0] += B0(0, 1, 1) * r[1] * r[2];
         b[2] += B2(1, 1, 0) * r[0] * r[1];
-        const double ibc = 1.0 / bcoeff;
+        const B ibc = (B)(1.0 / bcoeff);
 #pragma unroll
         for (int l = 0; l < 3; l++) out[l] += b[l] * ibc;
     }
@@ -98,55 +100,47 @@ struct AdjKD {
     double a_k, a_d, dtau_k, dtau_d, bc_k, bc_d;
 };
 
-// Reverse of the fused kick + drift of one particle (state before the kick).
-// In: lx, lq = adjoints after the drift (by ID).  Out: lx, lq before the kick;
-// lphi, lchi, lB (3 components) accumulate the fields' adjoints.
-template <typename T>
-__global__ __launch_bounds__(256) void k_adj_gr_kick_drift(PV<T> v, const T *__restrict__ phi, const T *__restrict__ chi,
-                                                           const T *__restrict__ Bi, T *lx, T *lq, T *lphi, T *lchi,
-                                                           T *lB, int64_t M, AdjKD s)
+// Reverse of the fused kick + drift of one particle (state before the kick),
+// at cell offsets r and momenta q, with the adjoints LX, LQ after the drift:
+// nx, nq the adjoints before the kick; the fields' adjoints go to
+// sc(f, dx, dy, dz, value) with f = 0 phi, 1 chi, 2..4 B_x..B_z and the
+// site's offset from the particle's cell.  Fields are read through the
+// accessors Fp, Fc, FB0..FB2 (offsets from the particle's cell).
+template <typename T, class FA, class SC>
+__device__ __forceinline__ void adj_gr_particle(const T rt[3], const T qt[3], const T LX[3], const T LQ[3], const FA &Fp,
+                                                const FA &Fc, const FA &FB0, const FA &FB1, const FA &FB2, int N,
+                                                const AdjKD &s, T nx[3], T nq[3], SC &sc)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= v.n) return;
-    const int N = v.N;
-    const int64_t n = v.n, id = v.id[i];
-    int x, y, z;
-    cell_coords(v.cell[i], N, v.lg, x, y, z);
-    T rt[3];
-    ref_dist(v, i, x, y, z, rt);
-    const double r[3] = {(double)rt[0], (double)rt[1], (double)rt[2]};
-    const double q[3] = {(double)v.q0[i], (double)v.q1[i], (double)v.q2[i]};
-    const FAt<T> Fp{phi, N, x, y, z}, Fc{chi, N, x, y, z}, FB0{Bi, N, x, y, z}, FB1{Bi + M, N, x, y, z},
-        FB2{Bi + 2 * M, N, x, y, z};
-    const double Nd = (double)N;
-
+    // the per-particle Jacobians and adjoints in the container's precision
+    // (FP32 for the mixed form: half the registers of FP64 dual numbers)
+    using R = T;
+    using DT = Dual<R, 6>;
+    const R r[3] = {rt[0], rt[1], rt[2]};
+    const R q[3] = {qt[0], qt[1], qt[2]};
+    const R Nd = (R)N;
     // kick on duals seeded in (r, q)
-    D6 rD[3], qk[3];
+    DT rD[3], qk[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) { rD[j] = D6::seed(r[j], j); qk[j] = D6::seed(q[j], 3 + j); }
-    phys_update_q<D6>(rD, qk, Fp, Fc, FB0, FB1, FB2, s.nf_k, D6(s.a_k), D6(s.dtau_k), D6(s.bc_k), D6(Nd));
+    for (int j = 0; j < 3; j++) { rD[j] = DT::seed(r[j], j); qk[j] = DT::seed(q[j], 3 + j); }
+    phys_update_q<DT>(rD, qk, Fp, Fc, FB0, FB1, FB2, s.nf_k, DT(s.a_k), DT(s.dtau_k), DT(s.bc_k), DT(Nd));
     // drift rate on duals seeded in (r, q')
-    D6 qp[3], rate[3];
+    DT qp[3], rate[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) qp[j] = D6::seed(qk[j].v, 3 + j);
-    drift_rate<D6>(rD, qp, Fp, Fc, FB0, FB1, FB2, s.nf_d, s.a_d, s.bc_d, rate);
+    for (int j = 0; j < 3; j++) qp[j] = DT::seed(qk[j].v, 3 + j);
+    drift_rate<DT>(rD, qp, Fp, Fc, FB0, FB1, FB2, s.nf_d, s.a_d, s.bc_d, rate);
 
-    double LX[3], LQ[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) { LX[j] = (double)lx[j * n + id]; LQ[j] = (double)lq[j * n + id]; }
     // adjoint of the kicked momentum: its own plus the drift's use of it
-    double lqt[3];
+    R lqt[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        double acc = LQ[j];
+        R acc = LQ[j];
 #pragma unroll
         for (int l = 0; l < 3; l++) acc += s.dtau_d * LX[l] * rate[l].d[3 + j];
         lqt[j] = acc;
     }
-    double nx[3], nq[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        double ar = 0.0, aq = 0.0;
+        R ar = 0.0, aq = 0.0;
 #pragma unroll
         for (int l = 0; l < 3; l++) {
             ar += s.dtau_d * LX[l] * rate[l].d[j] + lqt[l] * qk[l].d[j];
@@ -155,123 +149,194 @@ __global__ __launch_bounds__(256) void k_adj_gr_kick_drift(PV<T> v, const T *__r
         nx[j] = LX[j] + Nd * ar;
         nq[j] = aq;
     }
-#pragma unroll
-    for (int j = 0; j < 3; j++) { lx[j * n + id] = (T)nx[j]; lq[j * n + id] = (T)nq[j]; }
 
     // ---- field adjoints of the kick (pre-kick q), lambda = lqt
-    const double K = s.dtau_k * Nd;
-    const double v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-    const double e2 = v2 + s.a_k * s.a_k, e = sqrt(e2);
-    auto S = [&](T *f, int dx, int dy, int dz, double val) { gadd(f + site(N, x + dx, y + dy, z + dz), val); };
+    const R K = s.dtau_k * Nd;
+    const R v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const R e2 = v2 + s.a_k * s.a_k, e = sqrt(e2);
     {
         // g = Phibar grad2(phi) - grad1(chi) + pgradB / (bcoeff e); q' = q - K e g
-        const double Phi = cic<double>(Fp, r);
-        const double ph = Phi / e2;
-        const double Phibar = 4.0 * ph * v2 + (1.0 + s.a_k * s.a_k * ph) * (v2 + e2) / e2;
-        const double dPhibar = (4.0 * v2 + s.a_k * s.a_k * (v2 + e2) / e2) / e2;
-        double G2[3];
-        grad2<double>(Fp, r, G2);
-        const double lG2[3] = {-K * e * Phibar * lqt[0], -K * e * Phibar * lqt[1], -K * e * Phibar * lqt[2]};
-        const double lPhi = -K * e * dPhibar * (G2[0] * lqt[0] + G2[1] * lqt[1] + G2[2] * lqt[2]);
+        const R Phi = cic<R>(Fp, r);
+        const R ph = Phi / e2;
+        const R Phibar = 4.0 * ph * v2 + (1.0 + s.a_k * s.a_k * ph) * (v2 + e2) / e2;
+        const R dPhibar = (4.0 * v2 + s.a_k * s.a_k * (v2 + e2) / e2) / e2;
+        R G2[3];
+        grad2<R>(Fp, r, G2);
+        const R lG2[3] = {-K * e * Phibar * lqt[0], -K * e * Phibar * lqt[1], -K * e * Phibar * lqt[2]};
+        const R lPhi = -K * e * dPhibar * (G2[0] * lqt[0] + G2[1] * lqt[1] + G2[2] * lqt[2]);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
             const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-            const double w = cw(b, r);
-            S(lphi, bx, by, bz, w * lPhi);
-            const double hw = 0.5 * w;
-            S(lphi, bx + 1, by, bz, hw * lG2[0]);
-            S(lphi, bx - 1, by, bz, -hw * lG2[0]);
-            S(lphi, bx, by + 1, bz, hw * lG2[1]);
-            S(lphi, bx, by - 1, bz, -hw * lG2[1]);
-            S(lphi, bx, by, bz + 1, hw * lG2[2]);
-            S(lphi, bx, by, bz - 1, -hw * lG2[2]);
+            const R w = cw(b, r);
+            sc(0, bx, by, bz, w * lPhi);
+            const R hw = 0.5 * w;
+            sc(0, bx + 1, by, bz, hw * lG2[0]);
+            sc(0, bx - 1, by, bz, -hw * lG2[0]);
+            sc(0, bx, by + 1, bz, hw * lG2[1]);
+            sc(0, bx, by - 1, bz, -hw * lG2[1]);
+            sc(0, bx, by, bz + 1, hw * lG2[2]);
+            sc(0, bx, by, bz - 1, -hw * lG2[2]);
         }
         if (s.nf_k >= 2) {
-            const double lG1[3] = {K * e * lqt[0], K * e * lqt[1], K * e * lqt[2]};
+            const R lG1[3] = {K * e * lqt[0], K * e * lqt[1], K * e * lqt[2]};
 #pragma unroll
             for (int b = 0; b < 8; b++) {
-                double gw[3];
+                R gw[3];
                 cw_grad(b, r, gw);
-                S(lchi, (b >> 2) & 1, (b >> 1) & 1, b & 1, gw[0] * lG1[0] + gw[1] * lG1[1] + gw[2] * lG1[2]);
+                sc(1, (b >> 2) & 1, (b >> 1) & 1, b & 1, gw[0] * lG1[0] + gw[1] * lG1[1] + gw[2] * lG1[2]);
             }
         }
         if (s.nf_k >= 3) {
             // pgradB (gevolution.hpp:968-994) is linear in B: its transpose, term by term
-            const double c = -K / s.bc_k;
-            const double l0 = c * lqt[0], l1 = c * lqt[1], l2 = c * lqt[2];
-            T *L0 = lB, *L1 = lB + M, *L2 = lB + 2 * M;
-            const double o = 1.0;
+            const R c = -K / s.bc_k;
+            const R l0 = c * lqt[0], l1 = c * lqt[1], l2 = c * lqt[2];
+                        const R o = 1.0;
             // pg[0]
-            double t = l0 * q[1];
-            S(L1, 1, 0, 0, t * (o - r[2])); S(L1, 0, 0, 0, -t * (o - r[2]));
-            S(L1, 1, 0, 1, t * r[2]); S(L1, 0, 0, 1, -t * r[2]);
+            R t = l0 * q[1];
+            sc(3, 1, 0, 0, t * (o - r[2])); sc(3, 0, 0, 0, -t * (o - r[2]));
+            sc(3, 1, 0, 1, t * r[2]); sc(3, 0, 0, 1, -t * r[2]);
             t = l0 * q[2];
-            S(L2, 1, 0, 0, t * (o - r[1])); S(L2, 0, 0, 0, -t * (o - r[1]));
-            S(L2, 1, 1, 0, t * r[1]); S(L2, 0, 1, 0, -t * r[1]);
+            sc(4, 1, 0, 0, t * (o - r[1])); sc(4, 0, 0, 0, -t * (o - r[1]));
+            sc(4, 1, 1, 0, t * r[1]); sc(4, 0, 1, 0, -t * r[1]);
             t = l0 * q[0];
 #pragma unroll
             for (int by = 0; by < 2; by++)
 #pragma unroll
                 for (int bz = 0; bz < 2; bz++) {
-                    const double w = (by ? r[1] : o - r[1]) * (bz ? r[2] : o - r[2]) * t;
-                    S(L0, -1, by, bz, w * (r[0] - o)); S(L0, 0, by, bz, w * (o - 2.0 * r[0])); S(L0, 1, by, bz, w * r[0]);
+                    const R w = (by ? r[1] : o - r[1]) * (bz ? r[2] : o - r[2]) * t;
+                    sc(2, -1, by, bz, w * (r[0] - o)); sc(2, 0, by, bz, w * (o - 2.0 * r[0])); sc(2, 1, by, bz, w * r[0]);
                 }
             // pg[1]
             t = l1 * q[2];
-            S(L2, 0, 1, 0, t * (o - r[0])); S(L2, 0, 0, 0, -t * (o - r[0]));
-            S(L2, 1, 1, 0, t * r[0]); S(L2, 1, 0, 0, -t * r[0]);
+            sc(4, 0, 1, 0, t * (o - r[0])); sc(4, 0, 0, 0, -t * (o - r[0]));
+            sc(4, 1, 1, 0, t * r[0]); sc(4, 1, 0, 0, -t * r[0]);
             t = l1 * q[0];
-            S(L0, 0, 1, 0, t * (o - r[2])); S(L0, 0, 0, 0, -t * (o - r[2]));
-            S(L0, 0, 1, 1, t * r[2]); S(L0, 0, 0, 1, -t * r[2]);
+            sc(2, 0, 1, 0, t * (o - r[2])); sc(2, 0, 0, 0, -t * (o - r[2]));
+            sc(2, 0, 1, 1, t * r[2]); sc(2, 0, 0, 1, -t * r[2]);
             t = l1 * q[1];
 #pragma unroll
             for (int bx = 0; bx < 2; bx++)
 #pragma unroll
                 for (int bz = 0; bz < 2; bz++) {
-                    const double w = (bx ? r[0] : o - r[0]) * (bz ? r[2] : o - r[2]) * t;
-                    S(L1, bx, -1, bz, w * (r[1] - o)); S(L1, bx, 0, bz, w * (o - 2.0 * r[1])); S(L1, bx, 1, bz, w * r[1]);
+                    const R w = (bx ? r[0] : o - r[0]) * (bz ? r[2] : o - r[2]) * t;
+                    sc(3, bx, -1, bz, w * (r[1] - o)); sc(3, bx, 0, bz, w * (o - 2.0 * r[1])); sc(3, bx, 1, bz, w * r[1]);
                 }
             // pg[2]
             t = l2 * q[0];
-            S(L0, 0, 0, 1, t * (o - r[1])); S(L0, 0, 0, 0, -t * (o - r[1]));
-            S(L0, 0, 1, 1, t * r[1]); S(L0, 0, 1, 0, -t * r[1]);
+            sc(2, 0, 0, 1, t * (o - r[1])); sc(2, 0, 0, 0, -t * (o - r[1]));
+            sc(2, 0, 1, 1, t * r[1]); sc(2, 0, 1, 0, -t * r[1]);
             t = l2 * q[1];
-            S(L1, 0, 0, 1, t * (o - r[0])); S(L1, 0, 0, 0, -t * (o - r[0]));
-            S(L1, 1, 0, 1, t * r[0]); S(L1, 1, 0, 0, -t * r[0]);
+            sc(3, 0, 0, 1, t * (o - r[0])); sc(3, 0, 0, 0, -t * (o - r[0]));
+            sc(3, 1, 0, 1, t * r[0]); sc(3, 1, 0, 0, -t * r[0]);
             t = l2 * q[2];
 #pragma unroll
             for (int bx = 0; bx < 2; bx++)
 #pragma unroll
                 for (int by = 0; by < 2; by++) {
-                    const double w = (bx ? r[0] : o - r[0]) * (by ? r[1] : o - r[1]) * t;
-                    S(L2, bx, by, -1, w * (r[2] - o)); S(L2, bx, by, 0, w * (o - 2.0 * r[2])); S(L2, bx, by, 1, w * r[2]);
+                    const R w = (bx ? r[0] : o - r[0]) * (by ? r[1] : o - r[1]) * t;
+                    sc(4, bx, by, -1, w * (r[2] - o)); sc(4, bx, by, 0, w * (o - 2.0 * r[2])); sc(4, bx, by, 1, w * r[2]);
                 }
         }
     }
     // ---- field adjoints of the drift (kicked q'), lambda = LX
     {
-        const double qq[3] = {qk[0].v, qk[1].v, qk[2].v};
-        const double w2 = qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2];
-        const double f2 = w2 + s.a_d * s.a_d, ie = 1.0 / sqrt(f2);
-        const double qL = qq[0] * LX[0] + qq[1] * LX[1] + qq[2] * LX[2];
-        const double lph = s.dtau_d * qL * (3.0 - w2 / f2) * ie, lch = -s.dtau_d * qL * ie;
+        const R qq[3] = {qk[0].v, qk[1].v, qk[2].v};
+        const R w2 = qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2];
+        const R f2 = w2 + s.a_d * s.a_d, ie = 1.0 / sqrt(f2);
+        const R qL = qq[0] * LX[0] + qq[1] * LX[1] + qq[2] * LX[2];
+        const R lph = s.dtau_d * qL * (3.0 - w2 / f2) * ie, lch = -s.dtau_d * qL * ie;
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            const double w = cw(b, r);
+            const R w = cw(b, r);
             const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-            if (s.nf_d >= 1) S(lphi, bx, by, bz, w * lph);
-            if (s.nf_d >= 2) S(lchi, bx, by, bz, w * lch);
+            if (s.nf_d >= 1) sc(0, bx, by, bz, w * lph);
+            if (s.nf_d >= 2) sc(1, bx, by, bz, w * lch);
         }
         if (s.nf_d >= 3) {
-            const double c = s.dtau_d / s.bc_d, o = 1.0;
-            const double b0 = c * LX[0], b1 = c * LX[1], b2 = c * LX[2];
-            T *L0 = lB, *L1 = lB + M, *L2 = lB + 2 * M;
-            S(L0, 0, 0, 0, b0 * (o - r[1]) * (o - r[2])); S(L0, 0, 1, 0, b0 * r[1] * (o - r[2]));
-            S(L0, 0, 0, 1, b0 * (o - r[1]) * r[2]);       S(L0, 0, 1, 1, b0 * r[1] * r[2]);
-            S(L1, 0, 0, 0, b1 * (o - r[0]) * (o - r[2])); S(L1, 1, 0, 0, b1 * r[0] * (o - r[2]));
-            S(L1, 0, 0, 1, b1 * (o - r[0]) * r[2]);       S(L1, 1, 0, 1, b1 * r[0] * r[2]);
-            S(L2, 0, 0, 0, b2 * (o - r[0]) * (o - r[1])); S(L2, 1, 0, 0, b2 * r[0] * (o - r[1]));
-            S(L2, 0, 1, 0, b2 * (o - r[0]) * r[1]);       S(L2, 1, 1, 0, b2 * r[0] * r[1]);
+            const R c = s.dtau_d / s.bc_d, o = 1.0;
+            const R b0 = c * LX[0], b1 = c * LX[1], b2 = c * LX[2];
+                        sc(2, 0, 0, 0, b0 * (o - r[1]) * (o - r[2])); sc(2, 0, 1, 0, b0 * r[1] * (o - r[2]));
+            sc(2, 0, 0, 1, b0 * (o - r[1]) * r[2]);       sc(2, 0, 1, 1, b0 * r[1] * r[2]);
+            sc(3, 0, 0, 0, b1 * (o - r[0]) * (o - r[2])); sc(3, 1, 0, 0, b1 * r[0] * (o - r[2]));
+            sc(3, 0, 0, 1, b1 * (o - r[0]) * r[2]);       sc(3, 1, 0, 1, b1 * r[0] * r[2]);
+            sc(4, 0, 0, 0, b2 * (o - r[0]) * (o - r[1])); sc(4, 1, 0, 0, b2 * r[0] * (o - r[1]));
+            sc(4, 0, 1, 0, b2 * (o - r[0]) * r[1]);       sc(4, 1, 1, 0, b2 * r[0] * r[1]);
+        }
+    }
+}
+
+// The tiled reverse kick + drift: one workgroup per tile of TX x TY x TZ cells
+// reads the fields of the stencil box [x0-1, x0+TX+1] x ... from LDS, as the
+// forward update does, and sums the field adjoints of its particles in FP64
+// LDS accumulators over the same box (the scatter stencils stay inside it),
+// which go to the global arrays with one atomic per site and field at the
+// end.  A crowded cell then costs LDS atomics, not global ones.
+template <typename T, int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) void k_adj_gr_kick_drift(PV<T> v, const T *__restrict__ phi, const T *__restrict__ chi,
+                                                          const T *__restrict__ Bi, T *lx, T *lq, T *lphi, T *lchi,
+                                                          T *lB, int64_t M, AdjKD s)
+{
+    constexpr int SX = TX + 3, SY = TY + 3, SZ = TZ + 3, NS = SX * SY * SZ, NROW = TY * TZ;
+    __shared__ T fld[5][NS];
+    __shared__ double acc[5][NS];
+    const int N = v.N;
+    const uint32_t msk = (uint32_t)N - 1;
+    const int tid = threadIdx.x, lane = tid & 63;
+    int x0, y0, z0;
+    tile_origin<TX, TY, TZ>(N, x0, y0, z0);
+    const int nfmax = s.nf_k > s.nf_d ? s.nf_k : s.nf_d;
+    for (int e = tid; e < NS; e += NT) {
+        const int lx0 = e % SX, ly = (e / SX) % SY, lz = e / (SX * SY);
+        const uint32_t g = ((uint32_t)(x0 - 1 + lx0) & msk) +
+                           (uint32_t)N * (((uint32_t)(y0 - 1 + ly) & msk) + (uint32_t)N * ((uint32_t)(z0 - 1 + lz) & msk));
+        fld[0][e] = phi[g];
+        fld[1][e] = nfmax >= 2 ? chi[g] : (T)0;
+        fld[2][e] = nfmax >= 3 ? Bi[g] : (T)0;
+        fld[3][e] = nfmax >= 3 ? Bi[M + g] : (T)0;
+        fld[4][e] = nfmax >= 3 ? Bi[2 * M + g] : (T)0;
+#pragma unroll
+        for (int f = 0; f < 5; f++) acc[f][e] = 0.0;
+    }
+    int32_t rb0, incl;
+    tile_rows_load<TX, TY, NROW>(v.start, N, x0, y0, z0, lane, rb0, incl);
+    incl = wave_row_scan(incl, lane);
+    const int total = __shfl(incl, NROW - 1, 64);
+    const int niter = (total + NT - 1) / NT;  // the same for every lane of the workgroup
+    __syncthreads();
+    const int64_t n = v.n;
+    for (int it = 0; it < niter; it++) {
+        const int64_t p = tile_locate<NROW>(rb0, incl, tid + it * NT);
+        if (p < 0) continue;
+        const int64_t id = v.id[p];
+        int x, y, z;
+        cell_coords(v.cell[p], N, v.lg, x, y, z);
+        T rt[3];
+        ref_dist(v, p, x, y, z, rt);
+        const T qt[3] = {v.q0[p], v.q1[p], v.q2[p]};
+        T LX[3], LQ[3], nx[3], nq[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) { LX[j] = lx[j * n + id]; LQ[j] = lq[j * n + id]; }
+        const int o = (x - x0 + 1) + SX * ((y - y0 + 1) + SY * (z - z0 + 1));
+        const FieldLds<T> Fp{fld[0], o, SX, SX * SY}, Fc{fld[1], o, SX, SX * SY}, FB0{fld[2], o, SX, SX * SY},
+            FB1{fld[3], o, SX, SX * SY}, FB2{fld[4], o, SX, SX * SY};
+        auto sc = [&](int f, int dx, int dy, int dz, T val) {
+            __hip_atomic_fetch_add(&acc[f][o + dx + SX * dy + SX * SY * dz], (double)val, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        adj_gr_particle<T>(rt, qt, LX, LQ, Fp, Fc, FB0, FB1, FB2, N, s, nx, nq, sc);
+#pragma unroll
+        for (int j = 0; j < 3; j++) { lx[j * n + id] = nx[j]; lq[j * n + id] = nq[j]; }
+    }
+    __syncthreads();
+    T *dst[5] = {lphi, lchi, lB, lB + M, lB + 2 * M};
+    for (int e = tid; e < NS; e += NT) {
+        const int lx0 = e % SX, ly = (e / SX) % SY, lz = e / (SX * SY);
+        const uint32_t g = ((uint32_t)(x0 - 1 + lx0) & msk) +
+                           (uint32_t)N * (((uint32_t)(y0 - 1 + ly) & msk) + (uint32_t)N * ((uint32_t)(z0 - 1 + lz) & msk));
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            const double a = acc[f][e];
+            if (a != 0.0) atomicAdd(dst[f] + g, (T)a);
         }
     }
 }
@@ -504,9 +569,32 @@ int adj_gr_kick_drift_impl(gev_pcls *p, const T *phi, const T *chi, const T *Bi,
 {
     gev_ctx *c = p->ctx;
     AdjKD s{nf_k, nf_d, a_k, a_d, dtau_k, dtau_d, a_k * a_k * (double)c->N, a_d * a_d * (double)c->N};
-    if (p->n > 0)
-        hipLaunchKernelGGL((k_adj_gr_kick_drift<T>), dim3(nbl(p->n)), dim3(256), 0, c->stream, make_pv<T>(p), phi, chi,
-                           Bi, lx, lq, lphi, lchi, lB, c->M, s);
+    const int N = c->N;
+    // tiles as the forward update's where the FP64 accumulators fit the LDS
+    auto launch = [&](auto tx, auto ty, auto tz, auto nt) {
+        constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value, TZ = decltype(tz)::value;
+        constexpr int NT = decltype(nt)::value;
+        const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
+        hipLaunchKernelGGL((k_adj_gr_kick_drift<T, TX, TY, TZ, NT>), dim3(nb), dim3(NT), 0, c->stream, make_pv<T>(p),
+                           phi, chi, Bi, lx, lq, lphi, lchi, lB, c->M, s);
+    };
+    using I = std::integral_constant<int, 4>;
+    if (p->n > 0) {
+        bool done = false;
+        if constexpr (sizeof(T) == 4) {
+            if (N >= 16) {
+                launch(std::integral_constant<int, 16>{}, std::integral_constant<int, 8>{},
+                       std::integral_constant<int, 8>{}, std::integral_constant<int, 512>{});
+                done = true;
+            }
+        }
+        if (done) {
+        } else if (N >= 8)
+            launch(std::integral_constant<int, 8>{}, std::integral_constant<int, 8>{}, I{},
+                   std::integral_constant<int, 256>{});
+        else
+            launch(I{}, I{}, I{}, std::integral_constant<int, 256>{});
+    }
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }

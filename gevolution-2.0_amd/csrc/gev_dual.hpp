@@ -51,6 +51,10 @@ template <typename R, int K> struct Dual {
     }
 };
 
+// the real type under a dual number (the type itself for a plain real)
+template <typename R> struct DualBase { using type = R; };
+template <typename R, int K> struct DualBase<Dual<R, K>> { using type = R; };
+
 template <typename R, int K> __device__ __forceinline__ Dual<R, K> operator+(Dual<R, K> a, const Dual<R, K> &b) { return a += b; }
 template <typename R, int K> __device__ __forceinline__ Dual<R, K> operator-(Dual<R, K> a, const Dual<R, K> &b) { return a -= b; }
 template <typename R, int K> __device__ __forceinline__ Dual<R, K> operator*(Dual<R, K> a, const Dual<R, K> &b) { return a *= b; }

@@ -495,4 +495,46 @@ template <int NT = PT_NT> __device__ __forceinline__ void block_max_to(double v,
     }
 }
 
+// ---- particles of a tile in cell order (the tiled particle kernels)
+// row prefix of a tile: lane r < NROW gets the first container index of row r
+// and the inclusive prefix of the row lengths (after wave_row_scan)
+template <int TX, int TY, int NROW>
+__device__ __forceinline__ void tile_rows_load(const int32_t *start, int N, int x0, int y0, int z0, int lane,
+                                               int32_t &rb0, int32_t &len)
+{
+    rb0 = 0;
+    len = 0;
+    if (lane < NROW) {
+        const int ry = lane % TY, rz = lane / TY;
+        const uint32_t rb = (uint32_t)x0 + (uint32_t)N * ((uint32_t)(y0 + ry) + (uint32_t)N * (uint32_t)(z0 + rz));
+        rb0 = start[rb];
+        len = start[rb + TX] - rb0;
+    }
+}
+__device__ __forceinline__ int32_t wave_row_scan(int32_t len, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(len, o, 64);
+        if (lane >= o) len += y;
+    }
+    return len;
+}
+// container index of flattened particle f of a tile with row prefix (rb0,
+// incl); every lane runs the same steps (the shuffles need all lanes), lanes
+// with f beyond the tile's particles get -1
+template <int NROW> __device__ __forceinline__ int64_t tile_locate(int32_t rb0, int32_t incl, int f)
+{
+    const int32_t total = __shfl(incl, NROW - 1, 64);
+    int lo = 0;
+#pragma unroll
+    for (int step = NROW / 2; step >= 1; step >>= 1) {
+        const int j = lo + step < NROW ? lo + step : NROW - 1;
+        const int32_t rsj = __shfl(incl, j - 1, 64);  // first flattened index of row j
+        if (lo + step < NROW && rsj <= f) lo = j;
+    }
+    const int32_t rsl = __shfl(incl, lo > 0 ? lo - 1 : 0, 64);
+    const int32_t rbl = __shfl(rb0, lo, 64);
+    return f < total ? (int64_t)rbl + (f - (lo > 0 ? rsl : 0)) : (int64_t)-1;
+}
+
 }  // namespace gev

@@ -343,94 +343,155 @@ __global__ __launch_bounds__(NT) void k_adj_gr_kick_drift(PV<T> v, const T *__re
 
 // Reverse of the T00 + Tij + T0i-link deposits of one particle (phi_old the
 // deposits' phi; masses per unit of the reference's coeff, as
-// gev_project_T00_Tij_T0i_links scales them).  In: lT00, lTij (6), lL (3)
-// field adjoints.  Out: lx, lq accumulate; lphi accumulates phi_old's adjoint.
-template <typename T>
-__global__ __launch_bounds__(256) void k_adj_gr_deposit(PV<T> v, const T *__restrict__ phi, const T *__restrict__ lT00,
-                                                        const T *__restrict__ lTij, const T *__restrict__ lL, T *lx,
-                                                        T *lq, T *lphi, int64_t M, double m00, double mij, double m0i,
-                                                        double a)
+// gev_project_T00_Tij_T0i_links scales them), at cell offsets r and momenta
+// q.  F(f, dx, dy, dz) reads field f at an offset from the particle's cell:
+// f = 0 lT00, 1..6 lTij, 7..9 lL (the field adjoints), 10 phi_old.  Out:
+// gx, gq the adjoint contributions to the particle (d/dx in box units), and
+// sc(b, value) phi_old's adjoint at corner b (when has_phi).
+template <typename T, class FA, class SC>
+__device__ __forceinline__ void adj_gr_deposit_particle(const T rt[3], const T qt[3], const FA &F, bool has_phi,
+                                                        bool has_links, int N, double m00, double mij, double m0i,
+                                                        double a_d, T gx[3], T gq[3], SC &sc)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= v.n) return;
-    const int N = v.N;
-    const int64_t n = v.n, id = v.id[i];
-    int x, y, z;
-    cell_coords(v.cell[i], N, v.lg, x, y, z);
-    T rt[3];
-    ref_dist(v, i, x, y, z, rt);
-    const double r[3] = {(double)rt[0], (double)rt[1], (double)rt[2]};
-    const double q[3] = {(double)v.q0[i], (double)v.q1[i], (double)v.q2[i]};
-    auto F = [&](const T *f, int dx, int dy, int dz) { return (double)f[site(N, x + dx, y + dy, z + dz)]; };
-    D6 rD[3], qD[3];
+    using R = T;  // the container's precision (FP32 duals for the mixed form)
+    using DT = Dual<R, 6>;
+    const R a = (R)a_d;
+    DT rD[3], qD[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) { rD[j] = D6::seed(r[j], j); qD[j] = D6::seed(q[j], 3 + j); }
-    const D6 o(1.0);
-    D6 W[8];
+    for (int j = 0; j < 3; j++) { rD[j] = DT::seed(rt[j], j); qD[j] = DT::seed(qt[j], 3 + j); }
+    const DT o((R)1);
+    DT W[8];
 #pragma unroll
     for (int b = 0; b < 8; b++)
         W[b] = (((b >> 2) & 1) ? rD[0] : o - rD[0]) * (((b >> 1) & 1) ? rD[1] : o - rD[1]) * ((b & 1) ? rD[2] : o - rD[2]);
-    double ph[8];
+    R ph[8];
 #pragma unroll
-    for (int b = 0; b < 8; b++) ph[b] = phi ? F(phi, (b >> 2) & 1, (b >> 1) & 1, b & 1) : 0.0;
+    for (int b = 0; b < 8; b++) ph[b] = has_phi ? F(10, (b >> 2) & 1, (b >> 1) & 1, b & 1) : (R)0;
     // T00 (gevolution.hpp:1729-1854)
-    const D6 q2 = qD[0] * qD[0] + qD[1] * qD[1] + qD[2] * qD[2];
-    const D6 e = fsqrt(q2 + a * a), ie = frcp(e);
-    const D6 f = 3.0 * e + q2 * ie;
-    const D6 f2 = (0.5 * f * f + q2 * frcp(o + q2 * (1.0 / (a * a)))) * ie;
-    D6 Ssum(0.0);
-    double lt[8];
+    const DT q2 = qD[0] * qD[0] + qD[1] * qD[1] + qD[2] * qD[2];
+    const DT e = fsqrt(q2 + a * a), ie = frcp(e);
+    const DT f = (R)3 * e + q2 * ie;
+    const DT f2 = ((R)0.5 * f * f + q2 * frcp(o + q2 * (R)(1.0 / (a_d * a_d)))) * ie;
+    DT Ssum((R)0);
+    R lt[8];
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-        lt[b] = m00 * F(lT00, (b >> 2) & 1, (b >> 1) & 1, b & 1);
+        lt[b] = (R)m00 * F(0, (b >> 2) & 1, (b >> 1) & 1, b & 1);
         Ssum += W[b] * (e + f * ph[b] + f2 * (ph[b] * ph[b])) * lt[b];
     }
     // Tij (gevolution.hpp:2582-2965, no CIC_PROJECT_TIJ) without its factor fT
-    const D6 e2 = q2 + a * a, iet = frsqrt(e2);
-    D6 pint(0.0);
+    const DT e2 = q2 + a * a, iet = frsqrt(e2);
+    DT pint((R)0);
 #pragma unroll
     for (int b = 0; b < 8; b++) pint += W[b] * ph[b];
-    const D6 cf = 4.0 + (a * a) * frcp(e2);
-    const D6 fT = o + cf * pint;
-    D6 Sij(0.0);
+    const DT cf = (R)4 + (a * a) * frcp(e2);
+    const DT fT = o + cf * pint;
+    DT Sij((R)0);
     const int dc[3] = {0, 3, 5};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        D6 acc(0.0);
+        DT acc((R)0);
 #pragma unroll
-        for (int b = 0; b < 8; b++) acc += W[b] * F(lTij + dc[d] * M, (b >> 2) & 1, (b >> 1) & 1, b & 1);
+        for (int b = 0; b < 8; b++) acc += W[b] * F(1 + dc[d], (b >> 2) & 1, (b >> 1) & 1, b & 1);
         Sij += acc * qD[d] * qD[d] * iet;
     }
-    Sij += qD[0] * qD[1] * iet * ((o - rD[2]) * F(lTij + M, 0, 0, 0) + rD[2] * F(lTij + M, 0, 0, 1));
-    Sij += qD[0] * qD[2] * iet * ((o - rD[1]) * F(lTij + 2 * M, 0, 0, 0) + rD[1] * F(lTij + 2 * M, 0, 1, 0));
-    Sij += qD[1] * qD[2] * iet * ((o - rD[0]) * F(lTij + 4 * M, 0, 0, 0) + rD[0] * F(lTij + 4 * M, 1, 0, 0));
-    Sij = Sij * mij;
+    Sij += qD[0] * qD[1] * iet * ((o - rD[2]) * F(2, 0, 0, 0) + rD[2] * F(2, 0, 0, 1));
+    Sij += qD[0] * qD[2] * iet * ((o - rD[1]) * F(3, 0, 0, 0) + rD[1] * F(3, 0, 1, 0));
+    Sij += qD[1] * qD[2] * iet * ((o - rD[0]) * F(5, 0, 0, 0) + rD[0] * F(5, 1, 0, 0));
+    Sij = Sij * (R)mij;
     Ssum += fT * Sij;
     // T0i links (gevolution.hpp:2031-2201) without the link factor
-    if (lL) {
-        const T *L0 = lL, *L1 = lL + M, *L2 = lL + 2 * M;
-        D6 Sl = (o - rD[1]) * (o - rD[2]) * qD[0] * F(L0, 0, 0, 0) + (o - rD[0]) * (o - rD[2]) * qD[1] * F(L1, 0, 0, 0) +
-                (o - rD[0]) * (o - rD[1]) * qD[2] * F(L2, 0, 0, 0);
-        Sl += rD[0] * (o - rD[2]) * qD[1] * F(L1, 1, 0, 0) + rD[0] * (o - rD[1]) * qD[2] * F(L2, 1, 0, 0);
-        Sl += rD[1] * (o - rD[2]) * qD[0] * F(L0, 0, 1, 0) + rD[1] * (o - rD[0]) * qD[2] * F(L2, 0, 1, 0);
-        Sl += rD[2] * (o - rD[1]) * qD[0] * F(L0, 0, 0, 1) + rD[2] * (o - rD[0]) * qD[1] * F(L1, 0, 0, 1);
-        Sl += rD[1] * rD[2] * qD[0] * F(L0, 0, 1, 1) + rD[0] * rD[2] * qD[1] * F(L1, 1, 0, 1) +
-              rD[0] * rD[1] * qD[2] * F(L2, 1, 1, 0);
-        Ssum += Sl * m0i;
+    if (has_links) {
+        DT Sl = (o - rD[1]) * (o - rD[2]) * qD[0] * F(7, 0, 0, 0) + (o - rD[0]) * (o - rD[2]) * qD[1] * F(8, 0, 0, 0) +
+                (o - rD[0]) * (o - rD[1]) * qD[2] * F(9, 0, 0, 0);
+        Sl += rD[0] * (o - rD[2]) * qD[1] * F(8, 1, 0, 0) + rD[0] * (o - rD[1]) * qD[2] * F(9, 1, 0, 0);
+        Sl += rD[1] * (o - rD[2]) * qD[0] * F(7, 0, 1, 0) + rD[1] * (o - rD[0]) * qD[2] * F(9, 0, 1, 0);
+        Sl += rD[2] * (o - rD[1]) * qD[0] * F(7, 0, 0, 1) + rD[2] * (o - rD[0]) * qD[1] * F(8, 0, 0, 1);
+        Sl += rD[1] * rD[2] * qD[0] * F(7, 0, 1, 1) + rD[0] * rD[2] * qD[1] * F(8, 1, 0, 1) +
+              rD[0] * rD[1] * qD[2] * F(9, 1, 1, 0);
+        Ssum += Sl * (R)m0i;
     }
-    const double Nd = (double)N;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        lx[j * n + id] += (T)(Nd * Ssum.d[j]);
-        lq[j * n + id] += (T)Ssum.d[3 + j];
+        gx[j] = (R)N * Ssum.d[j];
+        gq[j] = Ssum.d[3 + j];
     }
-    if (phi && lphi) {
+    if (has_phi) {
         // d/dphi_b: T00's e + f phi + f2 phi^2 per corner, Tij's fT through the CIC of phi
-        const double tt = Sij.v * cf.v;
+        const R tt = Sij.v * cf.v;
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const double val = W[b].v * (lt[b] * (f.v + 2.0 * f2.v * ph[b]) + tt);
-            gadd(lphi + site(N, x + ((b >> 2) & 1), y + ((b >> 1) & 1), z + (b & 1)), val);
+        for (int b = 0; b < 8; b++) sc(b, W[b].v * (lt[b] * (f.v + (R)2 * f2.v * ph[b]) + tt));
+    }
+}
+
+// The tiled reverse deposit: one workgroup per tile of TX x TY x TZ cells
+// stages the eleven fields its particles read (the sites [x0, x0+TX] x ...)
+// in LDS and sums phi_old's adjoint there in FP64, added to lphi once per
+// site at the end.
+template <typename T, int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) void k_adj_gr_deposit(PV<T> v, const T *__restrict__ phi, const T *__restrict__ lT00,
+                                                       const T *__restrict__ lTij, const T *__restrict__ lL, T *lx,
+                                                       T *lq, T *lphi, int64_t M, double m00, double mij, double m0i,
+                                                       double a)
+{
+    constexpr int SX = TX + 1, SY = TY + 1, SZ = TZ + 1, NS = SX * SY * SZ, NROW = TY * TZ;
+    __shared__ T fld[11][NS];
+    __shared__ double acc[NS];
+    const int N = v.N;
+    const uint32_t msk = (uint32_t)N - 1;
+    const int tid = threadIdx.x, lane = tid & 63;
+    int x0, y0, z0;
+    tile_origin<TX, TY, TZ>(N, x0, y0, z0);
+    const bool has_phi = phi != nullptr, has_links = lL != nullptr, out_phi = has_phi && lphi != nullptr;
+    for (int e = tid; e < NS; e += NT) {
+        const int ex = e % SX, ey = (e / SX) % SY, ez = e / (SX * SY);
+        const uint32_t g = ((uint32_t)(x0 + ex) & msk) +
+                           (uint32_t)N * (((uint32_t)(y0 + ey) & msk) + (uint32_t)N * ((uint32_t)(z0 + ez) & msk));
+        fld[0][e] = lT00[g];
+#pragma unroll
+        for (int c = 0; c < 6; c++) fld[1 + c][e] = lTij[c * M + g];
+#pragma unroll
+        for (int c = 0; c < 3; c++) fld[7 + c][e] = has_links ? lL[c * M + g] : (T)0;
+        fld[10][e] = has_phi ? phi[g] : (T)0;
+        acc[e] = 0.0;
+    }
+    int32_t rb0, incl;
+    tile_rows_load<TX, TY, NROW>(v.start, N, x0, y0, z0, lane, rb0, incl);
+    incl = wave_row_scan(incl, lane);
+    const int total = __shfl(incl, NROW - 1, 64);
+    const int niter = (total + NT - 1) / NT;  // the same for every lane of the workgroup
+    __syncthreads();
+    const int64_t n = v.n;
+    for (int it = 0; it < niter; it++) {
+        const int64_t p = tile_locate<NROW>(rb0, incl, tid + it * NT);
+        if (p < 0) continue;
+        const int64_t id = v.id[p];
+        int x, y, z;
+        cell_coords(v.cell[p], N, v.lg, x, y, z);
+        T rt[3];
+        ref_dist(v, p, x, y, z, rt);
+        const T qt[3] = {v.q0[p], v.q1[p], v.q2[p]};
+        const int o = (x - x0) + SX * ((y - y0) + SY * (z - z0));
+        auto F = [&](int f, int dx, int dy, int dz) -> T { return fld[f][o + dx + SX * dy + SX * SY * dz]; };
+        auto sc = [&](int b, T val) {
+            __hip_atomic_fetch_add(&acc[o + ((b >> 2) & 1) + SX * ((b >> 1) & 1) + SX * SY * (b & 1)], (double)val,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        T gx[3], gq[3];
+        adj_gr_deposit_particle<T>(rt, qt, F, has_phi, has_links, N, m00, mij, m0i, a, gx, gq, sc);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            lx[j * n + id] += gx[j];
+            lq[j * n + id] += gq[j];
+        }
+    }
+    if (out_phi) {
+        __syncthreads();
+        for (int e = tid; e < NS; e += NT) {
+            const int ex = e % SX, ey = (e / SX) % SY, ez = e / (SX * SY);
+            const uint32_t g = ((uint32_t)(x0 + ex) & msk) +
+                               (uint32_t)N * (((uint32_t)(y0 + ey) & msk) + (uint32_t)N * ((uint32_t)(z0 + ez) & msk));
+            if (acc[e] != 0.0) atomicAdd(lphi + g, (T)acc[e]);
         }
     }
 }
@@ -605,9 +666,28 @@ int adj_gr_deposit_impl(gev_pcls *p, const T *phi, const T *lT00, const T *lTij,
 {
     gev_ctx *c = p->ctx;
     const double dx = 1.0 / (double)c->N, m = p->mass / (dx * dx * dx);
-    if (p->n > 0)
-        hipLaunchKernelGGL((k_adj_gr_deposit<T>), dim3(nbl(p->n)), dim3(256), 0, c->stream, make_pv<T>(p), phi, lT00,
-                           lTij, lL, lx, lq, lphi, c->M, phi ? m / a : m, m / a, m, a);
+    const int N = c->N;
+    auto launch = [&](auto tx, auto ty, auto tz, auto nt) {
+        constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value, TZ = decltype(tz)::value;
+        constexpr int NT = decltype(nt)::value;
+        const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
+        hipLaunchKernelGGL((k_adj_gr_deposit<T, TX, TY, TZ, NT>), dim3(nb), dim3(NT), 0, c->stream, make_pv<T>(p), phi,
+                           lT00, lTij, lL, lx, lq, lphi, c->M, phi ? m / a : m, m / a, m, a);
+    };
+    using I4 = std::integral_constant<int, 4>;
+    using I8 = std::integral_constant<int, 8>;
+    if (p->n > 0) {
+        bool done = false;
+        if constexpr (sizeof(T) == 4) {  // FP64 fields take twice the LDS: the smaller tile
+            if (N >= 16) {
+                launch(std::integral_constant<int, 16>{}, I8{}, I8{}, std::integral_constant<int, 256>{});
+                done = true;
+            }
+        }
+        if (done) {
+        } else if (N >= 8) launch(I8{}, I8{}, I4{}, std::integral_constant<int, 256>{});
+        else launch(I4{}, I4{}, I4{}, std::integral_constant<int, 256>{});
+    }
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }

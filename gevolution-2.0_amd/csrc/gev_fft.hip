@@ -357,6 +357,72 @@ __global__ __launch_bounds__(NL * N / 16) void k_r2c_x_reg(Src src, cx<T> *__res
     }
 }
 
+// x pass r2c of NC components of the same lines in one workgroup (out + c NK
+// for component c): the sources of all components read the same phi rows,
+// which the later components then find in the caches.  Src::get<C>(L, n)
+// gives component C's pair n of line L.
+template <typename T, int N, int NL, int NC, typename Src>
+__global__ __launch_bounds__(NL * N / 16) void k_r2c_x_multi(Src src, cx<T> *__restrict__ out, int P, int64_t NK,
+                                                             int64_t nlines, const cx<T> *__restrict__ tw)
+{
+    constexpr int H = N / 2, NT = NL * H / 8, LDL = H + 1;
+    __shared__ cx<T> lds[NL * LDL];
+    __shared__ cx<T> twl[N];
+    for (int m = threadIdx.x; m < N; m += NT) twl[m] = tw[m];
+    const int64_t line0 = (int64_t)blockIdx.x * NL;
+    auto comp = [&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        auto ld = [&](int line, int n) {
+            return (line0 + line < nlines) ? src.template get<C>(line0 + line, n) : mk((T)0, (T)0);
+        };
+        auto st = [&](int line, int k, cx<T> v, int) { lds[line * LDL + k] = v; };
+        __syncthreads();  // twiddles loaded; the previous component's stores have read lds
+        reg_fft<T, H, NL, false, 2, true, 1>(lds, twl, -1, ld, st);
+        __syncthreads();
+        cx<T> *dst = out + C * NK + line0 * P;
+        for (int e = threadIdx.x; e < NL * P; e += NT) {
+            int l = e / P, k = e % P;
+            if (line0 + l >= nlines) continue;
+            cx<T> X = mk((T)0, (T)0);
+            if (k <= H) {
+                cx<T> Zk = lds[l * LDL + (k % H)];
+                cx<T> Zm = conj(lds[l * LDL + ((H - k) % H)]);
+                cx<T> Fe = (Zk + Zm) * (T)0.5;
+                cx<T> Fo = mul_i(Zk - Zm, -1) * (T)0.5;
+                X = Fe + twl[k] * Fo;
+            }
+            dst[e] = X;
+        }
+    };
+    comp(std::integral_constant<int, 0>{});
+    if constexpr (NC > 1) comp(std::integral_constant<int, 1>{});
+    if constexpr (NC > 2) comp(std::integral_constant<int, 2>{});
+    if constexpr (NC > 3) comp(std::integral_constant<int, 3>{});
+    if constexpr (NC > 4) comp(std::integral_constant<int, 4>{});
+    if constexpr (NC > 5) comp(std::integral_constant<int, 5>{});
+}
+
+// the six Sij components (Tij at Tij + C M)
+template <typename T, int N> struct SrcSij6 {
+    const T *phi, *Tij;
+    int64_t M;
+    T coeff;
+    template <int C> __device__ __forceinline__ cx<T> get(int64_t L, int n) const
+    {
+        return SrcSij<T, N, C>{phi, Tij + C * M, coeff}(L, n);
+    }
+};
+
+template <typename T, int N, int NC, typename Src>
+static void launch_r2c_multi(const Src &src, cx<T> *out, int P, int64_t NK, int64_t nlines, const cx<T> *tw,
+                             hipStream_t s)
+{
+    constexpr int NL = xlines<T, N>();
+    int64_t nb = (nlines + NL - 1) / NL;
+    hipLaunchKernelGGL((k_r2c_x_multi<T, N, NL, NC, Src>), dim3((unsigned)nb), dim3(NL * N / 16), 0, s, src, out, P, NK,
+                       nlines, tw);
+}
+
 // x pass c2r: N/2+1 complex (pitch P) -> N reals; DC and Nyquist taken as real.
 // The lines are staged in LDS with one coalesced read per element, the first
 // stage pairs X[k] with X[H-k] from there, and the last stage's outputs go
@@ -942,14 +1008,7 @@ static void chi_from_Tij_chain(gev_ctx *c, const T *phi, const T *Tij, T coeff, 
 {
     const int64_t nl = (int64_t)N * N;
     const cx<T> *tw = twiddles_of<T>(c);
-    auto x = [&](const auto &src, int i) { launch_r2c_src<T, N>(src, SijFT + i * c->NK, c->P, nl, tw, c->stream); };
-    const int64_t M = c->M;
-    x(SrcSij<T, N, 0>{phi, Tij, coeff}, 0);
-    x(SrcSij<T, N, 1>{phi, Tij + M, coeff}, 1);
-    x(SrcSij<T, N, 2>{phi, Tij + 2 * M, coeff}, 2);
-    x(SrcSij<T, N, 3>{phi, Tij + 3 * M, coeff}, 3);
-    x(SrcSij<T, N, 4>{phi, Tij + 4 * M, coeff}, 4);
-    x(SrcSij<T, N, 5>{phi, Tij + 5 * M, coeff}, 5);
+    launch_r2c_multi<T, N, 6>(SrcSij6<T, N>{phi, Tij, c->M, coeff}, SijFT, c->P, c->NK, nl, tw, c->stream);
     ZParams zp{c->gk2, c->kshift, 0.0, 0.0, 0.0, 0.0, 0, nullptr};
     launch_col_comb<T, N>(SijFT, c->P, c->NK, tw, zp, c->stream);
     launch_zsolve<T, N, ZOP_SCALAR3>(SijFT, chiFT, c->P, c->NK, tw, zp, c->stream);

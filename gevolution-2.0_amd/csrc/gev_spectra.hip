@@ -1,7 +1,8 @@
-// gev_spectra.hip -- binned power spectrum of a Fourier field on the device:
-// extractPowerSpectrum (tools.hpp:65-342) with KTYPE_LINEAR bins, optionally
-// with the CIC window divided out (the deconvolved density spectrum of
-// output.hpp:3006-3033), sums in FP64.
+// gev_spectra.hip -- binned power and cross spectra of Fourier fields on the
+// device: extractCrossSpectrum / extractPowerSpectrum (tools.hpp:65-342) with
+// KTYPE_GRID or KTYPE_LINEAR bins, scalar, vector, symmetric-tensor or
+// single-component products, optionally with the CIC window divided out (the
+// deconvolved density spectrum of output.hpp:3006-3033), sums in FP64.
 //
 // One thread owns one segment of a (kx, ky) column of the half spectrum at a
 // time and walks it along the slowest index kz, so the lanes of a wavefront read
@@ -29,22 +30,36 @@ constexpr int PK_MAX_BINS = 1024;
 
 // Per bin: occupation, sum k, sum k^2, sum P k^3 / W^2, sum (P k^3 / W^2)^2,
 // with the weight 2 of a mode whose conjugate is not stored (tools.hpp:180-290).
+// The product of a mode (tools.hpp:200-225): Re(F1 conj F2) of components c1,
+// c2 when both are valid, else the symmetric contraction of 6 components
+// (xx xy xz yy yz zz, off-diagonals twice) when sym, else the sum over the
+// ncomp components; components lie NK modes apart.  ktype 0 KTYPE_GRID,
+// 1 KTYPE_LINEAR (tools.hpp:96-116).
+struct PkArgs {
+    const void *F1, *F2;
+    int64_t NK;
+    int ncomp, sym, c1, c2, ktype;
+};
 // grid: x over columns (PK_NT per workgroup; a grid narrower than the columns
 // strides over them), y over kz segments of zlen;
 // partials[j * groups + g], g = blockIdx.y * gridDim.x + blockIdx.x
 template <typename T>
-__global__ __launch_bounds__(PK_NT) void k_pk_columns(const cx<T> *__restrict__ F, int N, int P, int numbins,
-                                                      int deconvolve, int zlen, double *__restrict__ partials)
+__global__ __launch_bounds__(PK_NT) void k_pk_columns(PkArgs pa, int N, int P, int numbins, int deconvolve, int zlen,
+                                                      double *__restrict__ partials)
 {
+    const cx<T> *__restrict__ F1 = (const cx<T> *)pa.F1;
+    const cx<T> *__restrict__ F2 = (const cx<T> *)pa.F2;
+    const int64_t NK = pa.NK;
+    const bool pick = pa.c1 >= 0 && pa.c2 >= 0 && pa.c1 < pa.ncomp && pa.c2 < pa.ncomp;
     extern __shared__ double sh[];
     double *bins = sh;               // [5][numbins]
-    double *tk2 = sh + 5 * numbins;  // (2 pi |n|)^2 per index
+    double *tk2 = sh + 5 * numbins;  // |k|^2 per index: (2N sin(pi n / N))^2 or (2 pi |n|)^2
     double *isn = tk2 + N;           // pi |n| / (N sin(pi |n| / N)), or 1
     const int H = N / 2;
     for (int j = threadIdx.x; j < 5 * numbins; j += PK_NT) bins[j] = 0.0;
     for (int i = threadIdx.x; i < N; i += PK_NT) {
         const int lin = i <= H ? i : N - i;
-        double k = 2.0 * M_PI * lin;
+        const double k = pa.ktype == 0 ? 2.0 * (double)N * sin(M_PI * (double)i / (double)N) : 2.0 * M_PI * lin;
         tk2[i] = k * k;
         isn[i] = (deconvolve && lin > 0) ? M_PI * (double)lin / (sin(M_PI * (double)lin / (double)N) * (double)N) : 1.0;
     }
@@ -55,14 +70,28 @@ __global__ __launch_bounds__(PK_NT) void k_pk_columns(const cx<T> *__restrict__ 
         const int kx = (int)(col % (H + 1)), ky = (int)(col / (H + 1));
         const double kxy2 = tk2[kx] + tk2[ky], ixy = isn[kx] * isn[ky];
         const double w = (kx == 0 || kx == H) ? 1.0 : 2.0;
-        const cx<T> *f = F + (int64_t)ky * P + kx;
+        const int64_t fo = (int64_t)ky * P + kx;
         const int64_t zstride = (int64_t)N * P;
         int cur = -1;
         double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
         const int z0 = blockIdx.y * zlen, z1 = min(N, z0 + zlen);
         for (int kz = z0; kz < z1; kz++) {
-            const cx<T> v = f[kz * zstride];
             if (kz == 0 && kx == 0 && ky == 0) continue;
+            const int64_t m = fo + kz * zstride;
+            auto rc = [&](int a, int b) {
+                const cx<T> u = F1[a * NK + m], w = F2[b * NK + m];
+                return (double)u.x * (double)w.x + (double)u.y * (double)w.y;
+            };
+            double p;
+            if (pick) {
+                p = rc(pa.c1, pa.c2);
+            } else if (pa.sym) {
+                p = 2.0 * (rc(1, 1) + rc(2, 2) + rc(4, 4));
+                p += rc(0, 0) + rc(3, 3) + rc(5, 5);
+            } else {
+                p = 0.0;
+                for (int c = 0; c < pa.ncomp; c++) p += rc(c, c);
+            }
             const double k2 = kxy2 + tk2[kz];
             const double sk = sqrt(k2);
             // the bin is floor(numbins sk / kmax) with a correctly rounded
@@ -86,7 +115,6 @@ __global__ __launch_bounds__(PK_NT) void k_pk_columns(const cx<T> *__restrict__ 
             }
             double is = ixy * isn[kz];
             is *= is;
-            const double p = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
             const double pk = p * k2 * sk * is;
             a0 += w;
             a1 += w * sk;
@@ -149,14 +177,14 @@ static PkShape pk_shape(const gev_ctx *c)
 
 // buf: groups * 5 numbins partials followed by the 5 numbins sums
 template <typename T>
-static int power_spectrum_launch(gev_ctx *c, const cx<T> *F, int numbins, int deconvolve, const PkShape &sh,
+static int power_spectrum_launch(gev_ctx *c, const PkArgs &pa, int numbins, int deconvolve, const PkShape &sh,
                                  double *buf, double *sums_host)
 {
     const int nvals = 5 * numbins;
     const int64_t nb = sh.groups();
     double *out = buf + (size_t)nb * nvals;
     const size_t shm = sizeof(double) * (5 * (size_t)numbins + 2 * (size_t)c->N);
-    hipLaunchKernelGGL((k_pk_columns<T>), dim3(sh.nbx, sh.nseg), dim3(PK_NT), shm, c->stream, F, c->N, c->P, numbins,
+    hipLaunchKernelGGL((k_pk_columns<T>), dim3(sh.nbx, sh.nseg), dim3(PK_NT), shm, c->stream, pa, c->N, c->P, numbins,
                        deconvolve, sh.zlen, buf);
     hipLaunchKernelGGL(k_pk_reduce, dim3(nvals), dim3(PK_NT), 0, c->stream, buf, (int)nb, out);
     hipError_t e = hipGetLastError();
@@ -179,7 +207,7 @@ static int pk_lds_check(const gev_ctx *c, int numbins)
 }
 
 template <typename T>
-static int power_spectrum_impl(gev_ctx *c, const cx<T> *F, int numbins, int deconvolve, double *sums_host)
+static int power_spectrum_impl(gev_ctx *c, const PkArgs &pa, int numbins, int deconvolve, double *sums_host)
 {
     if (int rc = pk_lds_check(c, numbins)) return rc;
     const PkShape sh = pk_shape(c);
@@ -190,7 +218,7 @@ static int power_spectrum_impl(gev_ctx *c, const cx<T> *F, int numbins, int deco
         gev_set_error(std::string("gev_power_spectrum: ") + hipGetErrorString(e));
         return GEV_ERR_NOMEM;
     }
-    int rc = power_spectrum_launch<T>(c, F, numbins, deconvolve, sh, buf, sums_host);
+    int rc = power_spectrum_launch<T>(c, pa, numbins, deconvolve, sh, buf, sums_host);
     e = hipFreeAsync(buf, c->stream);
     const hipError_t e2 = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = e2;
@@ -247,7 +275,7 @@ static int scratch_on(int device, double **out)
 // workspace needs clearing; two calls on one device that overlap in time
 // share the workspace and race.
 template <typename T>
-static int power_spectrum_noalloc_impl(gev_ctx *c, const cx<T> *F, int numbins, int deconvolve, double *sums_host)
+static int power_spectrum_noalloc_impl(gev_ctx *c, const PkArgs &pa, int numbins, int deconvolve, double *sums_host)
 {
     if (int rc = pk_lds_check(c, numbins)) return rc;
     const int64_t nvals = 5 * (int64_t)numbins;
@@ -264,7 +292,7 @@ static int power_spectrum_noalloc_impl(gev_ctx *c, const cx<T> *F, int numbins, 
     }
     double *buf = nullptr;
     if (int rc = scratch_on(c->device, &buf)) return rc;
-    int rc = power_spectrum_launch<T>(c, F, numbins, deconvolve, sh, buf, sums_host);
+    int rc = power_spectrum_launch<T>(c, pa, numbins, deconvolve, sh, buf, sums_host);
     const hipError_t e = hipStreamSynchronize(c->stream);
     if (rc == GEV_OK && e != hipSuccess) {
         gev_set_error(std::string("gev_power_spectrum_noalloc: ") + hipGetErrorString(e));
@@ -310,14 +338,42 @@ static int pk_args(const char *fn, gev_ctx *c, const void *F, int numbins, const
     return GEV_OK;
 }
 
+// one scalar field with itself, KTYPE_LINEAR (extractPowerSpectrum's default)
+static PkArgs pk_scalar(const gev_ctx *c, const void *F) { return PkArgs{F, F, c->NK, 1, 0, -1, -1, 1}; }
+
 extern "C" int gev_power_spectrum(gev_ctx *c, const void *F, int numbins, int deconvolve, double *kbin,
                                   double *power, double *kscatter, double *pscatter, int64_t *occupation)
 {
     if (int rc = pk_args("gev_power_spectrum", c, F, numbins, kbin, power, kscatter, pscatter, occupation)) return rc;
     std::vector<double> s(5 * (size_t)numbins);
-    const int rc = c->prec == GEV_FP64
-                       ? power_spectrum_impl<double>(c, (const cx<double> *)F, numbins, deconvolve, s.data())
-                       : power_spectrum_impl<float>(c, (const cx<float> *)F, numbins, deconvolve, s.data());
+    const PkArgs pa = pk_scalar(c, F);
+    const int rc = c->prec == GEV_FP64 ? power_spectrum_impl<double>(c, pa, numbins, deconvolve, s.data())
+                                       : power_spectrum_impl<float>(c, pa, numbins, deconvolve, s.data());
+    if (rc) return rc;
+    pk_normalise(s.data(), numbins, kbin, power, kscatter, pscatter, occupation);
+    return GEV_OK;
+}
+
+extern "C" int gev_cross_spectrum(gev_ctx *c, const void *F1, const void *F2, int ncomp, int symmetric, int comp1,
+                                  int comp2, int numbins, int deconvolve, int ktype, double *kbin, double *power,
+                                  double *kscatter, double *pscatter, int64_t *occupation)
+{
+    const char *fn = "gev_cross_spectrum";
+    if (int rc = pk_args(fn, c, F1, numbins, kbin, power, kscatter, pscatter, occupation)) return rc;
+    if (!F2) { gev_set_error(std::string(fn) + ": NULL argument"); return GEV_ERR_ARG; }
+    if (ncomp < 1) { gev_set_error(std::string(fn) + ": ncomp must be >= 1"); return GEV_ERR_ARG; }
+    if (symmetric && ncomp != 6) {
+        gev_set_error(std::string(fn) + ": a symmetric tensor field has 6 components");
+        return GEV_ERR_ARG;
+    }
+    if (ktype != 0 && ktype != 1) {
+        gev_set_error(std::string(fn) + ": ktype must be 0 (KTYPE_GRID) or 1 (KTYPE_LINEAR)");
+        return GEV_ERR_ARG;
+    }
+    std::vector<double> s(5 * (size_t)numbins);
+    const PkArgs pa{F1, F2, c->NK, ncomp, symmetric ? 1 : 0, comp1, comp2, ktype};
+    const int rc = c->prec == GEV_FP64 ? power_spectrum_impl<double>(c, pa, numbins, deconvolve, s.data())
+                                       : power_spectrum_impl<float>(c, pa, numbins, deconvolve, s.data());
     if (rc) return rc;
     pk_normalise(s.data(), numbins, kbin, power, kscatter, pscatter, occupation);
     return GEV_OK;
@@ -329,9 +385,9 @@ extern "C" int gev_power_spectrum_noalloc(gev_ctx *c, const void *F, int numbins
     if (int rc = pk_args("gev_power_spectrum_noalloc", c, F, numbins, kbin, power, kscatter, pscatter, occupation))
         return rc;
     double s[5 * PK_MAX_BINS];  // the sums come back into the host stack, not a heap buffer
-    const int rc = c->prec == GEV_FP64
-                       ? power_spectrum_noalloc_impl<double>(c, (const cx<double> *)F, numbins, deconvolve, s)
-                       : power_spectrum_noalloc_impl<float>(c, (const cx<float> *)F, numbins, deconvolve, s);
+    const PkArgs pa = pk_scalar(c, F);
+    const int rc = c->prec == GEV_FP64 ? power_spectrum_noalloc_impl<double>(c, pa, numbins, deconvolve, s)
+                                       : power_spectrum_noalloc_impl<float>(c, pa, numbins, deconvolve, s);
     if (rc) return rc;
     pk_normalise(s, numbins, kbin, power, kscatter, pscatter, occupation);
     return GEV_OK;

@@ -118,6 +118,8 @@ SIGNATURES = {
                                              C.POINTER(C.c_double), _i, _vp, _vp, _i, C.POINTER(C.c_int64)]),
     "gev_power_spectrum": (_i, [_vp, _vp, _i, _i, _dp, _dp, _dp, _dp, C.POINTER(C.c_int64)]),
     "gev_power_spectrum_noalloc": (_i, [_vp, _vp, _i, _i, _dp, _dp, _dp, _dp, C.POINTER(C.c_int64)]),
+    "gev_cross_spectrum": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _dp, _dp, _dp, _dp,
+                                C.POINTER(C.c_int64)]),
     "gev_adj_deposit": (_i, [_vp, _vp, _vp]),
     "gev_adj_divergence": (_i, [_vp, _vp, _vp, _d]),
     "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _d]),

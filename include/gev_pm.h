@@ -312,6 +312,17 @@ int gev_pcls_save_gadget2_lightcone(gev_pcls *p, const char *path, gev_gadget2_h
  * twice).  Sums in FP64; returns after the result is on the host. */
 int gev_power_spectrum(gev_ctx *ctx, const void *F, int numbins, int deconvolve, double *kbin, double *power,
                        double *kscatter, double *pscatter, int64_t *occupation);
+/* extractCrossSpectrum / extractPowerSpectrum (tools.hpp:65-342, the latter
+ * with F1 = F2) of fields of ncomp components each (component c NK modes after
+ * component 0, the library's Fourier layout): the product per mode is
+ * Re(F1 conj F2) of components comp1, comp2 when both are in [0, ncomp), else
+ * the symmetric-tensor contraction when symmetric (ncomp = 6, components
+ * xx xy xz yy yz zz, the off-diagonals counted twice), else the sum over
+ * components.  ktype 0 = KTYPE_GRID, 1 = KTYPE_LINEAR (tools.hpp:27-28).
+ * Outputs as gev_power_spectrum. */
+int gev_cross_spectrum(gev_ctx *ctx, const void *F1, const void *F2, int ncomp, int symmetric, int comp1, int comp2,
+                       int numbins, int deconvolve, int ktype, double *kbin, double *power, double *kscatter,
+                       double *pscatter, int64_t *occupation);
 
 /* gev_power_spectrum without any allocation: no counterpart in the reference.
  * For callers that cannot allocate device memory (code under HIP graph

@@ -367,6 +367,85 @@ OR_API void or_step(int N, long np, int gr, double pmass, double fourpiG, double
 }
 
 /* ------------------------------------------------------------------------
+ * extractCrossSpectrum (tools.hpp:65-330): f1FT, f2FT hold ncomp components of
+ * N*N*(N/2+1) modes each (component c at c*N*N*(N/2+1)).  ktype 0 KTYPE_GRID
+ * (|k|^2 = sum (2N sin(pi n/N))^2), 1 KTYPE_LINEAR (sum (2 pi |n|)^2).  The
+ * mode's power p is Re(f1 conj f2) of components comp1, comp2 when both are
+ * valid, else the symmetric-tensor contraction (6 components xx xy xz yy yz
+ * zz: the off-diagonals twice) when symmetric, else the sum over components.
+ * Pinned by: cross(F, F) equals or_power_spectrum; the polarization identity
+ * cross(F, G) = (P(F + G) - P(F) - P(G)) / 2 on power and occupation; a
+ * single GRID mode lands in the closed-form bin; the symmetric contraction
+ * equals the 9-component sum over a full symmetric tensor (tests).
+ * ------------------------------------------------------------------------ */
+OR_API void or_cross_spectrum(int N, int ncomp, const double *f1FT, const double *f2FT, int symmetric, int comp1,
+                              int comp2, int numbins, int deconvolve, int ktype, double *kbin, double *power,
+                              double *kscatter, double *pscatter, int *occupation)
+{
+    const long P = N / 2 + 1, NK = (long)N * N * P;
+    double *typek2 = (double *)malloc(sizeof(double) * N), *sinc = (double *)malloc(sizeof(double) * N);
+    if (ktype == 0) {
+        for (int i = 0; i < N; i++) { typek2[i] = 2. * (double)N * sin(M_PI * (double)i / (double)N); typek2[i] *= typek2[i]; }
+    } else {
+        for (int i = 0; i <= N / 2; i++) { typek2[i] = 2. * M_PI * i; typek2[i] *= typek2[i]; }
+        for (int i = N / 2 + 1; i < N; i++) { typek2[i] = 2. * M_PI * (N - i); typek2[i] *= typek2[i]; }
+    }
+    sinc[0] = 1.;
+    for (int i = 1; i <= N / 2; i++)
+        sinc[i] = deconvolve ? sin(M_PI * (double)i / (double)N) * (double)N / (M_PI * (double)i) : 1.0;
+    for (int i = N / 2 + 1; i < N; i++) sinc[i] = sinc[N - i];
+    const double kmax = sqrt(3. * typek2[N / 2]);
+    double *acc = (double *)calloc(4 * numbins, sizeof(double));
+    long *occ = (long *)calloc(numbins, sizeof(long));
+    for (long kz = 0; kz < N; kz++)
+        for (long ky = 0; ky < N; ky++)
+            for (long kx = 0; kx < P; kx++) {
+                if (kx == 0 && ky == 0 && kz == 0) continue;
+                const int weight = (kx == 0 || kx == N / 2) ? 1 : 2;
+                const double k2 = typek2[kx] + typek2[ky] + typek2[kz];
+                double s = sinc[kx] * sinc[ky] * sinc[kz];
+                s *= s;
+                const long i = kx + P * (ky + N * kz);
+#define OR_RCP(a, b) (f1FT[2 * ((a) * NK + i)] * f2FT[2 * ((b) * NK + i)] + f1FT[2 * ((a) * NK + i) + 1] * f2FT[2 * ((b) * NK + i) + 1])
+                double p;
+                if (comp1 >= 0 && comp2 >= 0 && comp1 < ncomp && comp2 < ncomp) {
+                    p = OR_RCP(comp1, comp2);
+                } else if (symmetric) {
+                    p = 2. * (OR_RCP(1, 1) + OR_RCP(2, 2) + OR_RCP(4, 4));
+                    p += OR_RCP(0, 0) + OR_RCP(3, 3) + OR_RCP(5, 5);
+                } else {
+                    p = 0.;
+                    for (int c = 0; c < ncomp; c++) p += OR_RCP(c, c);
+                }
+#undef OR_RCP
+                const double sk = sqrt(k2);
+                const int bin = (int)floor((double)numbins * sk / kmax);
+                if (bin < numbins) {
+                    acc[bin] += weight * sk;
+                    acc[numbins + bin] += weight * k2;
+                    acc[2 * numbins + bin] += weight * p * k2 * sk / s;
+                    acc[3 * numbins + bin] += weight * p * p * k2 * k2 * k2 / s / s;
+                    occ[bin] += weight;
+                }
+            }
+    for (int b = 0; b < numbins; b++) {
+        occupation[b] = (int)occ[b];
+        if (occ[b] > 0) {
+            kbin[b] = acc[b];
+            kscatter[b] = sqrt(acc[numbins + b] * occ[b] - kbin[b] * kbin[b]) / occ[b];
+            if (!isfinite(kscatter[b])) kscatter[b] = 0.;
+            kbin[b] /= occ[b];
+            power[b] = acc[2 * numbins + b] / occ[b];
+            pscatter[b] = sqrt(acc[3 * numbins + b] / occ[b] - power[b] * power[b]);
+            if (!isfinite(pscatter[b])) pscatter[b] = 0.;
+        } else {
+            kbin[b] = power[b] = kscatter[b] = pscatter[b] = 0.;
+        }
+    }
+    free(typek2); free(sinc); free(acc); free(occ);
+}
+
+/* ------------------------------------------------------------------------
  * extractPowerSpectrum (tools.hpp:65-342), KTYPE_LINEAR, one component.
  * Pinned by: single-mode field lands in the expected bin with the expected
  * weight (tests).

@@ -82,6 +82,7 @@ def lib():
                                _dp, _dp, C.POINTER(C.c_double)]),
         "or_step": (V, [I, Lg, I, D, D, D, D, D, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_power_spectrum": (V, [I, _cp, I, I, _dp, _dp, _dp, _dp, _i32p]),
+        "or_cross_spectrum": (V, [I, I, _cp, _cp, I, I, I, I, I, I, _dp, _dp, _dp, _dp, _i32p]),
         "or_tlm_deposit_mass": (V, [I, Lg, _dp, _dp, D, _dp]),
         "or_tlm_kick_newton": (V, [I, Lg, _dp, _dp, _dp, _dp, _dp, D]),
         "or_newton_run_tlm": (D, [I, _dp, vp, _dp, I, _dp, D, D, D, D, vp, vp, vp, C.POINTER(C.c_double)]),
@@ -259,6 +260,21 @@ def power_spectrum(FT, numbins, deconvolve=True):
     kb, pw, ks, ps = (np.zeros(numbins) for _ in range(4))
     occ = np.zeros(numbins, np.int32)
     lib().or_power_spectrum(N, np.ascontiguousarray(FT), numbins, int(deconvolve), kb, pw, ks, ps, occ)
+    return kb, pw, ks, ps, occ
+
+
+def cross_spectrum(F1, F2, numbins, deconvolve=True, ktype=1, comp1=-1, comp2=-1, symmetric=False):
+    """extractCrossSpectrum (tools.hpp:65-330).  F1, F2: (ncomp, N, N, N/2+1) or
+    (N, N, N/2+1) complex arrays in the [kz, ky, kx] layout; ktype 0 grid, 1 linear."""
+    F1 = np.ascontiguousarray(F1, np.complex128)
+    F2 = np.ascontiguousarray(F2, np.complex128)
+    if F1.ndim == 3:
+        F1, F2 = F1[None], F2[None]
+    ncomp, N = F1.shape[0], F1.shape[1]
+    kb, pw, ks, ps = (np.zeros(numbins) for _ in range(4))
+    occ = np.zeros(numbins, np.int32)
+    lib().or_cross_spectrum(N, ncomp, F1, F2, int(symmetric), comp1, comp2, numbins, int(deconvolve), ktype, kb, pw,
+                            ks, ps, occ)
     return kb, pw, ks, ps, occ
 
 

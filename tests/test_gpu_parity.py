@@ -561,6 +561,36 @@ def test_power_spectrum_vs_oracle(gp, O, N, prec, numbins, deconv):
     assert res[4].sum() == int(inside.sum()) - 1
 
 
+@pytest.mark.parametrize("N,prec,case", [(32, "fp64", "cross"), (32, "fp64", "grid"), (32, "fp64", "vector"),
+                                         (32, "fp64", "symmetric"), (32, "fp64", "pair"), (64, "mixed", "cross"),
+                                         (64, "mixed", "symmetric")])
+def test_cross_spectrum_vs_oracle(gp, O, N, prec, case):
+    """gev_cross_spectrum (extractCrossSpectrum, tools.hpp:65-330) against the
+    oracle on the same modes: two scalar fields; KTYPE_GRID bins; a 3-vector
+    with itself; a symmetric tensor (xx xy xz yy yz zz) against another; one
+    component pair.  Products of two different fields can cancel in a bin, so
+    power is held to 1e-12 of the bin's largest |power|."""
+    from gevpm import spectra
+    ctx = gp.Context(N, gp.GEV_FP64 if prec == "fp64" else gp.GEV_MIXED)
+    rng = np.random.default_rng(41 + N)
+    ncomp = {"cross": 1, "grid": 1, "vector": 3, "symmetric": 6, "pair": 3}[case]
+    F1 = np.stack([O.fft_r2c(rng.standard_normal(N ** 3)) for _ in range(ncomp)])
+    F2 = F1 if case in ("grid", "vector") else np.stack([O.fft_r2c(rng.standard_normal(N ** 3)) for _ in range(ncomp)])
+    D1, D2 = ft_gpu(ctx, F1), ft_gpu(ctx, F2)
+    kw = dict(deconvolve=True, ktype=0 if case == "grid" else 1, symmetric=case == "symmetric")
+    if case == "pair":
+        kw.update(comp1=2, comp2=0)
+    print("cross spectrum", N, prec, case, flush=True)
+    res = spectra.cross_spectrum(ctx, D1, D2, 20, **kw)
+    ref = O.cross_spectrum(ft_np(ctx, D1), ft_np(ctx, D2), 20, **kw)
+    assert np.array_equal(res[4].numpy(), ref[4])
+    assert relmax(res[0].numpy(), ref[0]) < 1e-12
+    assert np.abs(res[1].numpy() - ref[1]).max() <= 1e-12 * np.abs(ref[1]).max()
+    assert np.abs(res[3].numpy() - ref[3]).max() <= 1e-7 * np.abs(ref[3]).max()
+    with pytest.raises(gp.GevError):
+        spectra.cross_spectrum(ctx, D1, D2, 20, ktype=2)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_power_spectrum_single_mode_and_errors(gp, prec):
     """a single real cosine mode: one bin holds power |F|^2 k^3 / W^2 at its k,

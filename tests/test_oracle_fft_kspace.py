@@ -208,3 +208,87 @@ def test_power_spectrum_single_mode(O, mode):
     assert abs(pw[jb] - expect) <= 1e-10 * expect
     others = np.delete(pw, jb)
     assert np.abs(others).max() <= 1e-20 * expect
+
+
+def _rand_ft(O, rng, N, ncomp):
+    return np.stack([O.fft_r2c(rng.standard_normal(N ** 3)).reshape(N, N, N // 2 + 1) for _ in range(ncomp)])
+
+
+@pytest.mark.parametrize("deconv", [True, False])
+def test_cross_spectrum_pins(O, deconv):
+    """extractCrossSpectrum (tools.hpp:65-330): the cross spectrum of a field with
+    itself is its power spectrum; power is linear in Re(F conj G), so
+    cross(F, G) = (P(F + G) - P(F) - P(G)) / 2 bin by bin (polarization), with
+    the same bins and occupation; comp1/comp2 pick one component of each
+    field."""
+    N, nb = 16, 10
+    rng = np.random.default_rng(7)
+    F, G = _rand_ft(O, rng, N, 1)[0], _rand_ft(O, rng, N, 1)[0]
+    a = O.power_spectrum(F, nb, deconvolve=deconv)
+    b = O.cross_spectrum(F, F, nb, deconvolve=deconv)
+    for x, y in zip(a, b):
+        assert np.allclose(x, y, rtol=1e-13, atol=0)
+    c = O.cross_spectrum(F, G, nb, deconvolve=deconv)
+    s = O.power_spectrum(F + G, nb, deconvolve=deconv)
+    assert np.array_equal(c[4], s[4]) and np.allclose(c[0], s[0], rtol=1e-14)
+    pol = 0.5 * (s[1] - a[1] - O.power_spectrum(G, nb, deconvolve=deconv)[1])
+    assert np.allclose(c[1], pol, rtol=1e-10, atol=1e-10 * np.abs(s[1]).max())
+    # a component pair of two 3-component fields
+    F3, G3 = _rand_ft(O, rng, N, 3), _rand_ft(O, rng, N, 3)
+    d = O.cross_spectrum(F3, G3, nb, deconvolve=deconv, comp1=1, comp2=2)
+    e = O.cross_spectrum(F3[1], G3[2], nb, deconvolve=deconv)
+    for x, y in zip(d, e):
+        assert np.allclose(x, y, rtol=1e-13, atol=0)
+    # a vector field sums its components
+    v = O.cross_spectrum(F3, F3, nb, deconvolve=deconv)
+    w = sum(np.asarray(O.power_spectrum(F3[i], nb, deconvolve=deconv)[1]) for i in range(3))
+    assert np.allclose(v[1], w, rtol=1e-12)
+
+
+def test_cross_spectrum_symmetric_tensor(O):
+    """A symmetric tensor (xx xy xz yy yz zz) contracts as the full 3x3 tensor:
+    the same spectrum as the 9 components summed without symmetry."""
+    N, nb = 8, 6
+    rng = np.random.default_rng(11)
+    S1, S2 = _rand_ft(O, rng, N, 6), _rand_ft(O, rng, N, 6)
+    idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    full1 = np.stack([S1[idx[a][b]] for a in range(3) for b in range(3)])
+    full2 = np.stack([S2[idx[a][b]] for a in range(3) for b in range(3)])
+    s = O.cross_spectrum(S1, S2, nb, symmetric=True)
+    f = O.cross_spectrum(full1, full2, nb)
+    for x, y in zip(s, f):
+        assert np.allclose(x, y, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mode", [(3, 1, 2), (0, 5, 0), (8, 0, 0)])
+def test_cross_spectrum_grid_ktype(O, mode):
+    """KTYPE_GRID: |k|^2 = sum (2N sin(pi n / N))^2 (tools.hpp:96-103) and
+    kmax^2 = 3 (2N)^2; one real mode's power lands in the bin of that |k| with
+    weight |F|^2 |k|^3 / sinc^2 / occupation, the occupations and mean |k|
+    counted by brute force over the full frequency grid."""
+    N, nb, A = 16, 12, 0.7
+    nx, ny, nz = mode
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    f = A * np.cos(2 * np.pi * (nx * x + ny * y + nz * z) / N)
+    F = O.fft_r2c(f.ravel())
+    kb, pw, ks, ps, occ = O.cross_spectrum(F, F, nb, deconvolve=True, ktype=0)
+    idx = np.arange(N)
+    g = (2 * N * np.sin(np.pi * idx / N)) ** 2
+    KZ, KY, KX = np.meshgrid(idx, idx, idx, indexing="ij")
+    kk = np.sqrt(g[KX] + g[KY] + g[KZ])
+    kmax = np.sqrt(3.0) * 2 * N
+    b = np.floor(nb * kk / kmax).astype(int)
+    b[0, 0, 0] = -1
+    for j in range(nb):
+        sel = b == j
+        assert occ[j] == sel.sum()
+        if sel.sum():
+            assert abs(kb[j] - kk[sel].mean()) <= 1e-12 * kk[sel].mean()
+    k = np.sqrt(g[nx] + g[ny] + g[nz])
+    jb = int(np.floor(nb * k / kmax))
+    sinc = np.prod([np.sin(np.pi * n / N) * N / (np.pi * n) if n else 1.0 for n in mode])
+    selfconj = all(n in (0, N // 2) for n in mode)
+    full = (A * N ** 3) ** 2 if selfconj else 2 * (A * N ** 3 / 2) ** 2
+    expect = full * k ** 3 / sinc ** 2 / occ[jb]
+    assert abs(pw[jb] - expect) <= 1e-10 * expect
+    assert np.abs(np.delete(pw, jb)).max() <= 1e-20 * expect

@@ -18,8 +18,8 @@
 //   DeviceStagingBuffer<T>        cuda_staging.hpp:18-63 (parameter arrays)
 //   Hconf / rungekutta4bg / particleHorizon
 //                                 background.hpp:200-290
-//   extractPowerSpectrum / writePowerSpectrum
-//                                 tools.hpp:339-342, 370-452 (linear bins)
+//   extractCrossSpectrum / extractPowerSpectrum / writePowerSpectrum
+//                                 tools.hpp:65-342, 370-452
 //
 // Errors: any non-zero status of the library becomes std::runtime_error with
 // the library's message (the reference throws std::runtime_error on GPU
@@ -94,10 +94,17 @@ class Lattice {
 // -------------------------------------------------------------------- field
 // Field<Real>: components x N^3 reals; Field<Cplx>: components x N x N x P complex.
 // Device memory owned by the field (RAII).
+// symmetric: the six-component layout (11, 12, 13, 22, 23, 33) of LATfield2's
+// symmetric rank-2 fields (initialize(lat, 3, 3, symmetric)); read by the
+// spectra, which contract such fields over all nine components
+enum { unsymmetric = 0, symmetric = 1 };
 template <typename T> class Field {
   public:
-    Field(Lattice &lat, int components = 1) : lat_(&lat), ncomp_(components)
+    Field(Lattice &lat, int components = 1, int symmetry = unsymmetric)
+        : lat_(&lat), ncomp_(components), sym_(symmetry)
     {
+        if (symmetry == symmetric && components != 6)
+            throw std::runtime_error("Field: a symmetric field has 6 components");
         n_ = (int64_t)components * (is_complex() ? lat.ksites() : lat.sites());
         hip_check(hipMalloc(&d_, sizeof(T) * (size_t)n_), "hipMalloc(Field)");
         hip_check(hipMemset(d_, 0, sizeof(T) * (size_t)n_), "hipMemset(Field)");
@@ -110,6 +117,7 @@ template <typename T> class Field {
     T *data() { return d_; }
     const T *data() const { return d_; }
     int components() const { return ncomp_; }
+    int symmetry() const { return sym_; }
     int64_t count() const { return n_; }
     Lattice &lattice() const { return *lat_; }
 
@@ -133,7 +141,7 @@ template <typename T> class Field {
 
   private:
     Lattice *lat_;
-    int ncomp_;
+    int ncomp_, sym_;
     int64_t n_ = 0;
     T *d_ = nullptr;
 };
@@ -715,19 +723,41 @@ inline void evolveFTvector(Field<Cplx> &SijFT, Field<Cplx> &BiFT, const Real a2d
 {
     check(gev_evolve_ft_vector(SijFT.lattice().ctx(), SijFT.data(), BiFT.data(), a2dtau), "evolveFTvector");
 }
-// extractPowerSpectrum (tools.hpp:339-342 over extractCrossSpectrum, :65-337): the
-// binned spectrum of one Fourier field, KTYPE_LINEAR bins only
+// extractCrossSpectrum (tools.hpp:65-337): the binned cross spectrum
+// Re(F1 conj F2) of two Fourier fields of the same lattice.  As in the
+// reference, the layout is fld1FT's: comp1 < fld1FT.components() and
+// comp2 < fld2FT.components() (both >= 0) pick one component of each;
+// otherwise a symmetric fld1FT is contracted with fld2FT over all nine
+// components (off-diagonal pairs twice), and any other fld1FT is summed over
+// its components against the same components of fld2FT (e.g. a scalar field
+// against the first component of a tensor field, output.hpp:3088).  ktype
+// KTYPE_GRID bins by the lattice momentum 2 N sin(pi n / N), KTYPE_LINEAR by
+// 2 pi n.
 constexpr int KTYPE_GRID = 0, KTYPE_LINEAR = 1;  // tools.hpp:27-28
-inline void extractPowerSpectrum(Field<Cplx> &fldFT, Real *kbin, Real *power, Real *kscatter, Real *pscatter,
-                                 int *occupation, const int numbins, const bool deconvolve = true,
-                                 const int ktype = KTYPE_LINEAR)
+inline void extractCrossSpectrum(Field<Cplx> &fld1FT, Field<Cplx> &fld2FT, Real *kbin, Real *power, Real *kscatter,
+                                 Real *pscatter, int *occupation, const int numbins, const bool deconvolve = true,
+                                 const int ktype = KTYPE_LINEAR, const int comp1 = -1, const int comp2 = -1)
 {
-    if (ktype != KTYPE_LINEAR) throw std::runtime_error("extractPowerSpectrum: only KTYPE_LINEAR bins are built");
+    Lattice &lat = fld1FT.lattice();
+    if (&lat != &fld2FT.lattice()) throw std::runtime_error("extractCrossSpectrum: the fields live on different lattices");
+    const Cplx *F1 = fld1FT.data(), *F2 = fld2FT.data();
+    int ncomp = fld1FT.components(), sym = 0;
+    if (comp1 >= 0 && comp2 >= 0 && comp1 < fld1FT.components() && comp2 < fld2FT.components()) {
+        F1 += (size_t)comp1 * (size_t)lat.ksites();  // one component of each
+        F2 += (size_t)comp2 * (size_t)lat.ksites();
+        ncomp = 1;
+    } else if (fld1FT.symmetry() == symmetric) {
+        if (fld2FT.symmetry() != symmetric)
+            throw std::runtime_error("extractCrossSpectrum: a symmetric field needs a symmetric partner");
+        sym = 1;
+    } else if (fld2FT.components() < ncomp) {
+        throw std::runtime_error("extractCrossSpectrum: the second field has fewer components than the first");
+    }
     std::vector<double> kb(numbins), pw(numbins), ks(numbins), ps(numbins);
     std::vector<int64_t> oc(numbins);
-    check(gev_power_spectrum(fldFT.lattice().ctx(), fldFT.data(), numbins, deconvolve ? 1 : 0, kb.data(), pw.data(),
-                             ks.data(), ps.data(), oc.data()),
-          "extractPowerSpectrum");
+    check(gev_cross_spectrum(lat.ctx(), F1, F2, ncomp, sym, -1, -1, numbins, deconvolve ? 1 : 0, ktype, kb.data(),
+                             pw.data(), ks.data(), ps.data(), oc.data()),
+          "extractCrossSpectrum");
     for (int i = 0; i < numbins; i++) {
         kbin[i] = (Real)kb[i];
         power[i] = (Real)pw[i];
@@ -736,23 +766,74 @@ inline void extractPowerSpectrum(Field<Cplx> &fldFT, Real *kbin, Real *power, Re
         occupation[i] = (int)oc[i];
     }
 }
-// writePowerSpectrum (tools.hpp:370-452) for the current redshift (the
-// EXACT_OUTPUT_REDSHIFTS interpolation against an earlier file is not built):
-// the same text table, one line per occupied bin
+// extractPowerSpectrum (tools.hpp:339-342): the cross spectrum of a field with itself
+inline void extractPowerSpectrum(Field<Cplx> &fldFT, Real *kbin, Real *power, Real *kscatter, Real *pscatter,
+                                 int *occupation, const int numbins, const bool deconvolve = true,
+                                 const int ktype = KTYPE_LINEAR)
+{
+    extractCrossSpectrum(fldFT, fldFT, kbin, power, kscatter, pscatter, occupation, numbins, deconvolve, ktype);
+}
+
+// writePowerSpectrum (tools.hpp:370-452): the text table, one line per
+// occupied bin.  Built with EXACT_OUTPUT_REDSHIFTS, a spectrum taken after the
+// target redshift z_target was crossed (1/a < z_target + 1) is interpolated
+// linearly in redshift against the one the previous call wrote to the same
+// file, and written as the spectrum at z_target:
+//   w = (z_prev - z_target) / (z_prev - z),  P = (1 - w) P_prev + w P,
+// P_prev read from the file's occupied bins in order, w = 1 when the header
+// cannot be parsed, and nothing interpolated when there is no such file.
+// The other columns are this call's.  Failures to open or parse a file are
+// reported on stdout and the call carries on, as the reference does.
 inline void writePowerSpectrum(const Real *kbin, const Real *power, const Real *kscatter, const Real *pscatter,
                                const int *occupation, const int numbins, const Real rescalek, const Real rescalep,
-                               const char *filename, const char *description, double a)
+                               const char *filename, const char *description, double a, const double z_target = -1)
 {
+    std::vector<Real> power2((size_t)numbins);
+    for (int i = 0; i < numbins; i++) power2[i] = power[i] / rescalep;
+#ifdef EXACT_OUTPUT_REDSHIFTS
+    if (1. / a < z_target + 1.) {
+        if (FILE *in = std::fopen(filename, "r")) {
+            double zprev = 0., weight = 1.;
+            char line[512];
+            if (!std::fgets(line, sizeof(line), in) || !std::fgets(line, sizeof(line), in) ||
+                std::sscanf(line, "# redshift z=%lf", &zprev) != 1) {
+                std::printf(" error parsing power spectrum file header for interpolation (EXACT_OUTPUT_REDSHIFTS)\n");
+            } else {
+                weight = (zprev - z_target) / (1. + zprev - 1. / a);
+                if (!std::fgets(line, sizeof(line), in))
+                    std::printf(" error parsing power spectrum file header for interpolation (EXACT_OUTPUT_REDSHIFTS)\n");
+                for (int i = 0; i < numbins; i++) {
+                    if (occupation[i] <= 0) continue;
+                    double k, p;
+                    if (std::fscanf(in, " %le %le %*e %*e %*d", &k, &p) != 2) {
+                        std::printf(" error parsing power spectrum file data %d for interpolation "
+                                    "(EXACT_OUTPUT_REDSHIFTS)\n", i);
+                        break;
+                    }
+                    power2[i] = (Real)p;
+                }
+            }
+            std::fclose(in);
+            for (int i = 0; i < numbins; i++) power2[i] = (Real)((1. - weight) * power2[i] + weight * power[i] / rescalep);
+            a = 1. / (z_target + 1.);
+        }
+    }
+#else
+    (void)z_target;
+#endif
     FILE *f = std::fopen(filename, "w");
-    if (!f) throw std::runtime_error(std::string("writePowerSpectrum: cannot open ") + filename);
+    if (!f) {
+        std::printf(" error opening file for power spectrum output!\n");
+        return;
+    }
     std::fprintf(f, "# %s\n", description);
     std::fprintf(f, "# redshift z=%f\n", 1. / a - 1.);
     std::fprintf(f, "# k              Pk             sigma(k)       sigma(Pk)      count\n");
     for (int i = 0; i < numbins; i++)
         if (occupation[i] > 0)
-            std::fprintf(f, "  %e   %e   %e   %e   %d\n", kbin[i] / rescalek, power[i] / rescalep, kscatter[i] / rescalek,
+            std::fprintf(f, "  %e   %e   %e   %e   %d\n", kbin[i] / rescalek, power2[i], kscatter[i] / rescalek,
                          pscatter[i] / rescalep / std::sqrt((double)occupation[i]), occupation[i]);
-    if (std::fclose(f) != 0) throw std::runtime_error(std::string("writePowerSpectrum: write failed on ") + filename);
+    std::fclose(f);
 }
 
 // the mixed form's FP64 k=0 mode (no reference counterpart: FP64 mean of an FP32 field)

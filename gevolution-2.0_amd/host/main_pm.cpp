@@ -5,7 +5,7 @@
 //
 //   main_pm -n N -L boxsize -zin z -zend z -steps S [-newton] [-seed s]
 //           [-zrelax z] -pk mpk.txt [-out final.bin] [-gadget snap] [-express restart]
-//           [-lightcone prefix -lcvertex x,y,z -lcz z] [-spectra prefix [-numbins n]]
+//           [-lightcone prefix -lcvertex x,y,z -lcz z] [-spectra prefix [-numbins n] [-zpk z]]
 //   main_pm -n N -template crystal.dat -out lattice.bin
 //
 // The mPk file has the loadPowerSpectrum format (ic_basic.hpp:370-449): two
@@ -19,7 +19,14 @@
 // <prefix>_<cycle>_cdm (writeLightcones' Gadget branch, output.hpp:2094-2160,
 // and the ID backlog hand-over, :2166-2880).  -spectra writes, at the end, the
 // deconvolved CIC density spectrum as <prefix>_deltaN.dat in writeSpectra's
-// delta_N form (output.hpp:3017-3021, writePowerSpectrum tools.hpp:370-452).
+// delta_N form (output.hpp:3017-3021, writePowerSpectrum tools.hpp:370-452),
+// and the spectra of phi and (GR) B of the last cycle as <prefix>_phi.dat and
+// <prefix>_B.dat (output.hpp:3176-3184, 3456-3461).
+// -zpk z adds one power-spectrum redshift (sim.z_pk): the same spectrum is
+// written inside the cycle as <prefix>000_deltaN.dat as main.cu:1080-1131 does
+// with EXACT_OUTPUT_REDSHIFTS (this driver is built with it): a plain spectrum
+// at the last cycle before z, then at the first cycle past z the spectrum
+// interpolated in redshift to z against that file.
 // -template builds the particle lattice as generateIC_basic does before the
 // displacement (loadHomogeneousTemplate, initialize with the reference's
 // arguments, initializeParticlePositions, updateRowBuffers; ic_basic.hpp:
@@ -49,6 +56,7 @@ struct Options {
     std::string pk, out, gadget, express, lightcone, spectra, tmpl;
     int numbins = 32;
     double lcvertex[3] = {0.5, 0.5, 0.5}, lcz = 0.0;
+    double zpk = -1.0;  // -1: no power-spectrum redshift inside the cycle
 };
 
 Options parse(int argc, char **argv)
@@ -75,6 +83,7 @@ Options parse(int argc, char **argv)
         else if (a == "-lightcone") o.lightcone = next();
         else if (a == "-spectra") o.spectra = next();
         else if (a == "-numbins") o.numbins = std::atoi(next());
+        else if (a == "-zpk") o.zpk = std::atof(next());
         else if (a == "-template") o.tmpl = next();
         else if (a == "-lcvertex") {
             if (std::sscanf(next(), "%lf,%lf,%lf", &o.lcvertex[0], &o.lcvertex[1], &o.lcvertex[2]) != 3)
@@ -87,6 +96,7 @@ Options parse(int argc, char **argv)
         return o;
     }
     if (o.pk.empty()) throw std::runtime_error("-pk <mPk file> is required");
+    if (o.zpk >= 0.0 && o.spectra.empty()) throw std::runtime_error("-zpk needs -spectra");
     if (o.zrelax < -1.0) o.zrelax = o.zin;  // parser.hpp:1468 default
     return o;
 }
@@ -192,8 +202,8 @@ int main(int argc, char **argv)
         part_simple_info info;
         info.mass = cosmo.Omega_m / (double)M;
         pcls.initialize(info, &lat, (uint64_t)M);
-        Field<Real> phi(lat), chi(lat), Bi(lat, 3), source(lat), Sij(lat, 6);
-        Field<Cplx> scalarFT(lat), SijFT(lat, 6), BiFT(lat, 3);
+        Field<Real> phi(lat), chi(lat), Bi(lat, 3), source(lat), Sij(lat, 6, symmetric);
+        Field<Cplx> scalarFT(lat), SijFT(lat, 6, symmetric), BiFT(lat, 3);
         double a, tau, dtau_old = 0.0, maxvel;
         {
             Lattice ic_lat(N, GEV_FP64);
@@ -291,6 +301,26 @@ int main(int argc, char **argv)
             IDbacklog.finalize();
         };
 
+        // writeSpectra's particle branch (output.hpp:2997-3022): the
+        // deconvolved CIC density spectrum as delta_N, z_target as sim.z_pk
+        auto write_deltaN = [&](const std::string &filename, double z_target) {
+            projection_init(&source);
+            scalarProjectionCIC_project(&pcls, &source);
+            plan_source.execute(FFT_FORWARD);
+            const int nb = opt.numbins;
+            std::vector<Real> kbin(nb), power(nb), kscatter(nb), pscatter(nb);
+            std::vector<int> occupation(nb);
+            extractPowerSpectrum(scalarFT, kbin.data(), power.data(), kscatter.data(), pscatter.data(),
+                                 occupation.data(), nb, true, KTYPE_LINEAR);
+            const Real np3 = (Real)M;
+            writePowerSpectrum(kbin.data(), power.data(), kscatter.data(), pscatter.data(), occupation.data(), nb,
+                               (Real)opt.L, np3 * np3 * (Real)(2. * M_PI * M_PI * cosmo.Omega_m * cosmo.Omega_m),
+                               filename.c_str(), "power spectrum of delta_N", a, z_target);
+        };
+        const int num_pk = opt.zpk >= 0.0 ? 1 : 0;
+        int pkcount = 0;
+        const std::string pkfile = opt.spectra + "000_deltaN.dat";  // "%s%s%03d_deltaN.dat" (output.hpp:3019)
+
         // The cycle of main.cu:608-1317 for one CDM species (no baryons, ncdm,
         // radiation, fluid or tensor evolution; vector method elliptic), call
         // for call in main.cu's order.  Ghost-layer folds and halo updates
@@ -366,6 +396,21 @@ int main(int argc, char **argv)
 
             write_lightcone(cycle);  // main.cu:1064-1070, outputs before the particle update
 
+            // power spectra (main.cu:1080-1103)
+            if (pkcount < num_pk && 1. / a < opt.zpk + 1.) {
+                std::printf(" writing power spectra at z = %f (cycle %d)\n", 1. / a - 1., cycle);
+                write_deltaN(pkfile, opt.zpk);
+                pkcount++;
+            }
+#ifdef EXACT_OUTPUT_REDSHIFTS
+            {  // main.cu:1106-1131: the spectrum before the crossing, for the interpolation
+                double tmp = a;
+                rungekutta4bg(tmp, fourpiG, cosmo, 0.5 * dtau);
+                rungekutta4bg(tmp, fourpiG, cosmo, 0.5 * dtau);
+                if (pkcount < num_pk && 1. / tmp < opt.zpk + 1.) write_deltaN(pkfile, opt.zpk);
+            }
+#endif
+
             // cdm particle update (main.cu:1225-1283)
             f_params[0] = a;
             f_params[1] = a * a * N;  // a * a * sim.numpts
@@ -394,21 +439,28 @@ int main(int argc, char **argv)
         }
         lat.sync();
 
-        if (!opt.spectra.empty()) {
-            // writeSpectra's particle branch (output.hpp:2997-3022) at the final time:
-            // the deconvolved CIC density spectrum as delta_N
-            projection_init(&source);
-            scalarProjectionCIC_project(&pcls, &source);
-            plan_source.execute(FFT_FORWARD);
+        if (!opt.spectra.empty()) {  // at the final time
+            write_deltaN(opt.spectra + "_deltaN.dat", -1.0);
+            // the metric spectra of writeSpectra (output.hpp:3176-3184, 3456-3461)
+            // for the fields of the last cycle
             const int nb = opt.numbins;
             std::vector<Real> kbin(nb), power(nb), kscatter(nb), pscatter(nb);
             std::vector<int> occupation(nb);
-            extractPowerSpectrum(scalarFT, kbin.data(), power.data(), kscatter.data(), pscatter.data(),
-                                 occupation.data(), nb, true, KTYPE_LINEAR);
             const Real np3 = (Real)M;
+            plan_phi.execute(FFT_FORWARD);
+            extractPowerSpectrum(scalarFT, kbin.data(), power.data(), kscatter.data(), pscatter.data(),
+                                 occupation.data(), nb, false, KTYPE_LINEAR);
             writePowerSpectrum(kbin.data(), power.data(), kscatter.data(), pscatter.data(), occupation.data(), nb,
-                               (Real)opt.L, np3 * np3 * (Real)(2. * M_PI * M_PI * cosmo.Omega_m * cosmo.Omega_m),
-                               (opt.spectra + "_deltaN.dat").c_str(), "power spectrum of delta_N", a);
+                               (Real)opt.L, np3 * np3 * (Real)(2. * M_PI * M_PI), (opt.spectra + "_phi.dat").c_str(),
+                               "power spectrum of phi", a);
+            if (gr_flag > 0) {
+                plan_Bi.execute(FFT_FORWARD);
+                extractPowerSpectrum(BiFT, kbin.data(), power.data(), kscatter.data(), pscatter.data(),
+                                     occupation.data(), nb, false, KTYPE_LINEAR);
+                writePowerSpectrum(kbin.data(), power.data(), kscatter.data(), pscatter.data(), occupation.data(), nb,
+                                   (Real)opt.L, (Real)(a * a * a * a * N * N * 2. * M_PI * M_PI),
+                                   (opt.spectra + "_B.dat").c_str(), "power spectrum of B", a);
+            }
         }
 
         if (!opt.out.empty()) {

@@ -34,6 +34,90 @@ def test_driver_rejects_bad_options(tmp_path):
     assert r.returncode == 1 and "header not recognized" in r.stderr
 
 
+_PK_WRITER = r"""
+#include <cstdio>
+#include <cstdlib>
+#include "gevolution_pm.hpp"
+using namespace gevpm;
+// argv: file a z_target, then numbins rows of k P sigk sigP count on stdin
+int main(int argc, char **argv)
+{
+    const int nb = 6;
+    Real kb[nb], pw[nb], ks[nb], ps[nb];
+    int oc[nb];
+    for (int i = 0; i < nb; i++)
+        if (std::scanf("%le %le %le %le %d", &kb[i], &pw[i], &ks[i], &ps[i], &oc[i]) != 5) return 2;
+    writePowerSpectrum(kb, pw, ks, ps, oc, nb, (Real)2.0, (Real)4.0, argv[1], "power spectrum of delta_N",
+                       std::atof(argv[2]), std::atof(argv[3]));
+    return 0;
+}
+"""
+
+
+def _table(path):
+    lines = open(path).read().splitlines()
+    return lines[:3], np.array([[float(x) for x in ln.split()] for ln in lines[3:]])
+
+
+def test_write_power_spectrum_exact_redshift(tmp_path):
+    """the façade's writePowerSpectrum built with EXACT_OUTPUT_REDSHIFTS
+    (tools.hpp:370-452), host code only: a spectrum taken past z_target is
+    interpolated against the file the previous call left, weight
+    (z_prev - z_target) / (z_prev - z) with z_prev as printed in that file's
+    header, and labelled z_target; no file -> written as is at its own
+    redshift; an unparsable header -> weight 1 (this call's values) at z_target;
+    before the crossing nothing is interpolated"""
+    import shutil
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "pkw.cpp"
+    src.write_text(_PK_WRITER)
+    exe = str(tmp_path / "pkw")
+    host = os.path.join(ROOT, "gevolution-2.0_amd", "host")
+    r = subprocess.run([gxx, "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-DEXACT_OUTPUT_REDSHIFTS",
+                        "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"), "-I" + host, "-o", exe,
+                        str(src)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rng = np.random.default_rng(3)
+    occ = np.array([3, 0, 7, 12, 0, 30])
+    k = np.arange(1, 7) * 0.5
+    P1, P2 = rng.uniform(1, 9, 6), rng.uniform(1, 9, 6)
+    sig = rng.uniform(0.1, 1, 6)
+
+    def write(path, a, zt, P):
+        rows = "\n".join("%.17g %.17g %.17g %.17g %d" % (k[i], P[i], 0.01 * k[i], sig[i], occ[i]) for i in range(6))
+        out = subprocess.run([exe, str(path), repr(a), repr(zt)], input=rows, capture_output=True, text=True,
+                             timeout=60)
+        assert out.returncode == 0, out.stderr
+        return out.stdout
+
+    sel = occ > 0
+    zt, z1, z2 = 9.5, 10.25, 8.75
+    f = tmp_path / "pk.dat"
+    write(f, 1 / (1 + z1), zt, P1)          # before the crossing: as is, at its own redshift
+    head, t1 = _table(f)
+    assert head[1] == "# redshift z=%f" % z1
+    assert np.allclose(t1[:, 1], P1[sel] / 4, rtol=1e-6) and np.array_equal(t1[:, 4], occ[sel])
+    write(f, 1 / (1 + z2), zt, P2)          # past it: interpolated to z_target
+    head, t2 = _table(f)
+    zprev = float("%f" % z1)
+    w = (zprev - zt) / (1 + zprev - (1 + z2))
+    assert head[1] == "# redshift z=%f" % zt
+    assert np.allclose(t2[:, 1], (1 - w) * t1[:, 1] + w * P2[sel] / 4, rtol=1e-6)
+    assert np.allclose(t2[:, 0], k[sel] / 2) and np.allclose(t2[:, 3], sig[sel] / 4 / np.sqrt(occ[sel]), rtol=1e-6)
+    g = tmp_path / "fresh.dat"              # no earlier file: nothing to interpolate
+    write(g, 1 / (1 + z2), zt, P2)
+    head, t3 = _table(g)
+    assert head[1] == "# redshift z=%f" % z2 and np.allclose(t3[:, 1], P2[sel] / 4, rtol=1e-6)
+    h = tmp_path / "bad.dat"                # unparsable header: weight 1, labelled z_target
+    h.write_text("# x\nnot a header\n")
+    out = write(h, 1 / (1 + z2), zt, P2)
+    assert "error parsing power spectrum file header" in out
+    head, t4 = _table(h)
+    assert head[1] == "# redshift z=%f" % zt and np.allclose(t4[:, 1], P2[sel] / 4, rtol=1e-6)
+
+
 def _read_out(path):
     with open(path, "rb") as f:
         n = int(np.frombuffer(f.read(8), np.int64)[0])
@@ -98,6 +182,54 @@ def _check_spectrum(path, gevpm, pid, pos, q, N, L, numbins, single):
     assert np.allclose(tab[:, 3], ps[sel] / rescalep / np.sqrt(occ[sel]), rtol=1e-3, atol=1e-3 * tab[:, 3].max())
 
 
+def _check_exact_spectrum(path, at_start, z_pk, N, L, single):
+    """-zpk (main.cu:1080-1131 with EXACT_OUTPUT_REDSHIFTS): the file holds the
+    density spectra of the last cycle start before z_pk (c0) and the first past
+    it, interpolated in redshift to z_pk with z_prev as printed in the header"""
+    zs = [1 / a - 1 for a, _ in at_start]
+    c0 = max(c for c in range(len(zs) - 1) if zs[c] >= z_pk > zs[c + 1])
+    lines = open(path).read().splitlines()
+    assert lines[0] == "# power spectrum of delta_N" and lines[1] == "# redshift z=%f" % z_pk
+    tab = np.array([[float(x) for x in ln.split()] for ln in lines[3:]])
+    rescalep = float(N ** 3) ** 2 * 2 * np.pi ** 2 * 0.31 ** 2
+    kb, p0, _, _, occ = at_start[c0][1]
+    p1, ps1 = at_start[c0 + 1][1][1], at_start[c0 + 1][1][3]
+    sel = occ > 0
+    zprev = float("%f" % zs[c0])
+    w = (zprev - z_pk) / (zprev - zs[c0 + 1])
+    assert 0 < w < 1
+    tol = 1e-4 if single else 1e-5
+    assert len(tab) == sel.sum() and np.array_equal(tab[:, 4].astype(np.int64), occ[sel])
+    assert np.allclose(tab[:, 0], kb[sel] / L, rtol=1e-5)
+    assert np.allclose(tab[:, 1], ((1 - w) * p0[sel] + w * p1[sel]) / rescalep, rtol=tol)
+    # sigma(Pk) is the later spectrum's, not interpolated
+    assert np.allclose(tab[:, 3], ps1[sel] / rescalep / np.sqrt(occ[sel]), rtol=1e-3, atol=1e-3 * tab[:, 3].max())
+
+
+def _check_metric_spectra(prefix, ctx, f, a, N, L, newton, single):
+    """the phi and (GR) B spectra the driver writes at the end (the façade's
+    extractPowerSpectrum of a scalar and of a three-component field, no
+    window) against the Python binding's spectra of the replay's fields"""
+    from gevpm.spectra import cross_spectrum, power_spectrum
+    cases = [("_phi.dat", f.phi, float(N ** 3) ** 2 * 2 * np.pi ** 2)]
+    if not newton:
+        cases.append(("_B.dat", f.Bi, a ** 4 * N ** 2 * 2 * np.pi ** 2))
+    for suffix, fld, rescalep in cases:
+        lines = open(prefix + suffix).read().splitlines()
+        tab = np.array([[float(x) for x in ln.split()] for ln in lines[3:]])
+        F = ctx.fft_r2c(fld)
+        if fld.shape[0] == 1:
+            res = power_spectrum(ctx, F, 20, deconvolve=False)
+        else:
+            res = cross_spectrum(ctx, F, F, 20, deconvolve=False)
+        kb, pw, _, _, occ = (t.numpy() for t in res)
+        sel = occ > 0
+        assert len(tab) == sel.sum() and np.array_equal(tab[:, 4].astype(np.int64), occ[sel]), suffix
+        assert np.allclose(tab[:, 0], kb[sel] / L, rtol=1e-5), suffix
+        tol = 1e-3 if single else 1e-5  # the driver's and gev_step's mixed fields differ at float rounding
+        assert np.allclose(tab[:, 1], pw[sel] / rescalep, rtol=tol, atol=tol * np.abs(tab[:, 1]).max()), suffix
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("newton,single", [(1, False), (0, False), (0, True)])
 def test_driver_matches_gev_step(tmp_path, newton, single):
@@ -107,6 +239,7 @@ def test_driver_matches_gev_step(tmp_path, newton, single):
     import gevpm
     import gevpm.cosmology as cosmo
     N, L, z_in, z_end, steps, seed = 16, 320.0, 100.0, 50.0, 4, 11
+    z_pk = 75.0  # between the starts of cycles 1 (z ~ 82) and 2 (z ~ 69)
     k, P = cosmo.bbks_table(z=z_in)
     mpk = str(tmp_path / "pk.txt")
     cosmo.write_mpk(mpk, k, P)
@@ -115,7 +248,7 @@ def test_driver_matches_gev_step(tmp_path, newton, single):
     exe = os.path.join(LIB, "main_pm_single" if single else "main_pm")
     cmd = [exe, "-n", str(N), "-L", str(L), "-zin", str(z_in), "-zend", str(z_end), "-steps", str(steps),
            "-seed", str(seed), "-pk", mpk, "-out", out, "-gadget", snap, "-express", restart,
-           "-spectra", str(tmp_path / "pk"), "-numbins", "20"] + (["-newton"] if newton else [])
+           "-spectra", str(tmp_path / "pk"), "-numbins", "20", "-zpk", repr(z_pk)] + (["-newton"] if newton else [])
     print("running", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -145,9 +278,14 @@ def test_driver_matches_gev_step(tmp_path, newton, single):
     else:
         p, f = p64, f64
     dtau = (gevpm.particle_horizon(1 / (1 + z_end), fpg, 0.31, 0.69) - st.tau) / steps
+    from gevpm.spectra import density_power
+    at_start = []  # (a, density spectrum) at each cycle start, where main.cu writes spectra
     for _ in range(steps):
         st.dtau = dtau
+        at_start.append((st.a, [t.cpu().numpy() for t in density_power(ctx if single else c64, p, 20)]))
         gevpm.step(p, f, st)
+    _check_exact_spectrum(str(tmp_path / "pk000_deltaN.dat"), at_start, z_pk, N, L, single)
+    _check_metric_spectra(str(tmp_path / "pk"), ctx if single else c64, f, st.a, N, L, newton, single)
     rp, rq, rid = (t.cpu().numpy() for t in p.store())
     o = np.argsort(rid)
     rp, rq = rp[:, o], rq[:, o]

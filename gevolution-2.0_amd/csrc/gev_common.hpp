@@ -63,6 +63,22 @@ __device__ __forceinline__ double wave_max(double v)
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
+// max over the wave of 32-bit unsigned values (also the max of non-negative
+// floats through their bit patterns), through DPP row shifts and broadcasts;
+// the result is uniform
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+    // row_shr:1,2,4,8 leave each row's maximum in its lane 15 (lanes shifted
+    // in from outside the row read 0); row_bcast:15 and row_bcast:31 carry
+    // rows 0-2 into lane 63
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xf, 0xf, true));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xf, 0xf, true));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
 __device__ __forceinline__ double wave_sum(double v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

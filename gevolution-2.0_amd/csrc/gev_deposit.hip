@@ -78,7 +78,7 @@ template <typename T, int KIND, int TX, int TY, int TZ>
 __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *out3, int64_t M,
                                                    const T *__restrict__ phi, double mass_d, double mass2_d,
                                                    double mass3_d, double a_d, int accumulate,
-                                                   const T *__restrict__ aux, const uint32_t *__restrict__ qb)
+                                                   const T *__restrict__ aux, const uint32_t *__restrict__ qb, int vec4)
 {
     constexpr int C = DepC<KIND>::C;
     constexpr int NS = TX * TY * TZ;                    // owned sites
@@ -89,10 +89,10 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     static_assert(NSEG <= PT_NT, "one run per thread in the scan");
     using AccT = typename DepAcc<T, KIND>::type;
     constexpr bool FIX = sizeof(AccT) == 4;
-    constexpr int NCB = (TX + 1) * (TY + 1) * (TZ + 1);  // cells of the halo box
-    __shared__ AccT acc[C][NS];
-    __shared__ int32_t ccount[FIX ? NCB : 1];
-    __shared__ float wmax[3][PT_NT / 64];
+    constexpr int NW = PT_NT / 64;
+    static_assert(!FIX || (TY * TZ <= 64 && NS % 4 == 0 && TX % 4 == 0), "fixed-point tile shape");
+    __shared__ __align__(16) AccT acc[C][NS];
+    __shared__ uint32_t wpm[NW];  // per wave: the largest |phi| of its share of the phi box (float bits)
     __shared__ T phs[NPH];
     __shared__ int32_t rbeg[NSEG + 1];
     __shared__ int32_t rst[NSEG];
@@ -103,35 +103,53 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     int x0, y0, z0;
     tile_origin<TX, TY, TZ>(N, x0, y0, z0);
     const bool use_phi = phi != nullptr;
+    const bool need_q_ = KIND != DEP_T00 || use_phi;
     // the particle updates' largest |q|^2 over the blocks of 8^3 cells that
     // the halo box touches (per dimension the block of the halo layer and
-    // those of the tile), loaded first so the loads overlap the staging below
-    float qqb = 0.0f;
-    if (FIX && qb && (KIND != DEP_T00 || use_phi)) {
+    // those of the tile), loaded by every wave for itself first, so the loads
+    // overlap the staging below (float bits: non-negative values order as
+    // their bit patterns)
+    uint32_t qbits = 0;
+    if (FIX && qb && need_q_) {
         const int nbq = N >> QB_LG;
         const int bx0 = x0 >> QB_LG, by0 = y0 >> QB_LG, bz0 = z0 >> QB_LG;
         const int nx = ((x0 + TX - 1) >> QB_LG) - bx0 + 2, ny = ((y0 + TY - 1) >> QB_LG) - by0 + 2,
                   nz = ((z0 + TZ - 1) >> QB_LG) - bz0 + 2;
-        for (int e = t; e < nx * ny * nz; e += PT_NT) {
+        for (int e = lane; e < nx * ny * nz; e += 64) {
             const int ix = e % nx, iy = (e / nx) % ny, iz = e / (nx * ny);
             const int bx = ix == 0 ? (((x0 - 1) & (int)msk) >> QB_LG) : bx0 + ix - 1;
             const int by = iy == 0 ? (((y0 - 1) & (int)msk) >> QB_LG) : by0 + iy - 1;
             const int bz = iz == 0 ? (((z0 - 1) & (int)msk) >> QB_LG) : bz0 + iz - 1;
-            qqb = fmaxf(qqb, __uint_as_float(qb[bx + nbq * (by + nbq * bz)]));
+            qbits = max(qbits, qb[bx + nbq * (by + nbq * bz)]);
         }
     }
 
-    for (int e = t; e < NS; e += PT_NT) {
+    if constexpr (FIX) {
+        int4 *a4 = reinterpret_cast<int4 *>(&acc[0][0]);
+        for (int e = t; e < C * NS / 4; e += PT_NT) a4[e] = make_int4(0, 0, 0, 0);
+    } else {
+        for (int e = t; e < NS; e += PT_NT) {
 #pragma unroll
-        for (int c = 0; c < C; c++) acc[c][e] = (AccT)0;
+            for (int c = 0; c < C; c++) acc[c][e] = (AccT)0;
+        }
     }
+    uint32_t pbits = 0;
     if (use_phi) {
         for (int e = t; e < NPH; e += PT_NT) {
             const int px = e % PX, py = (e / PX) % PY, pz = e / (PX * PY);
             const uint32_t g = ((uint32_t)(x0 - 1 + px) & msk) +
                                (uint32_t)N * (((uint32_t)(y0 - 1 + py) & msk) + (uint32_t)N * ((uint32_t)(z0 - 1 + pz) & msk));
-            phs[e] = phi[g];
+            const T ph = phi[g];
+            phs[e] = ph;
+            if constexpr (FIX) pbits = max(pbits, __float_as_uint(fabsf((float)ph)));
         }
+    }
+    if constexpr (FIX) {
+        // the wave's maxima (all lanes are active here); the scan's barriers
+        // below publish wpm
+        pbits = wave_max_u32(pbits);
+        if (FIX && qb && need_q_) qbits = wave_max_u32(qbits);
+        if (lane == 0) wpm[wv] = pbits;
     }
     // runs and their lengths; exclusive scan over NSEG <= PT_NT entries
     int32_t len = 0;
@@ -158,7 +176,6 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     __syncthreads();
     const int32_t total = rbeg[NSEG];
     const T a = (T)a_d, o = (T)1, ia2 = (T)(1.0 / (a_d * a_d));
-    const bool need_q_ = KIND != DEP_T00 || use_phi;
     // flattened particle index -> container slot
     auto locate = [&](int32_t i, int &run) -> int64_t {
         int lo = 0, hi = NSEG - 1;
@@ -175,60 +192,55 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     // S0, S1, S2: scales of the T00, Tij and T0i accumulators (1 in FP64)
     float S0 = 1.0f, S1 = 1.0f, S2 = 1.0f;
     if constexpr (FIX) {
-        __shared__ float wmax2[2][PT_NT / 64];
-        for (int e = t; e < NCB; e += PT_NT) {
-            const int cx = e % (TX + 1), cy = (e / (TX + 1)) % (TY + 1), cz = e / ((TX + 1) * (TY + 1));
-            const uint32_t g = ((uint32_t)(x0 - 1 + cx) & msk) +
-                               (uint32_t)N * (((uint32_t)(y0 - 1 + cy) & msk) + (uint32_t)N * ((uint32_t)(z0 - 1 + cz) & msk));
-            ccount[e] = v.start[g + 1] - v.start[g];
-        }
-        __syncthreads();
-        // site (sx,sy,sz) takes particles from the halo-box cells (sx+1-bx, ...)
-        float cmax = 0.0f, emax = 0.0f, pmax = 0.0f, tmax = 0.0f, vmax = 0.0f;
-        for (int e = t; e < NS; e += PT_NT) {
-            const int sx = e % TX, sy = (e / TX) % TY, sz = e / (TX * TY);
-            int c8 = 0;
+        // a site takes the particles of two cells in each of four halo rows,
+        // so the four rows' totals bound its particle count (run lengths of the
+        // scan; looser than the count of its eight cells by at most the row
+        // length, which costs a few of the 30 bits)
+        uint32_t cb = 0;
+        if (lane < TY * TZ) {
+            const int sy = lane % TY, sz = lane / TY;
 #pragma unroll
-            for (int b = 0; b < 8; b++)
-                c8 += ccount[(sx + 1 - ((b >> 2) & 1)) + (TX + 1) * ((sy + 1 - ((b >> 1) & 1)) + (TY + 1) * (sz + 1 - (b & 1)))];
-            cmax = fmaxf(cmax, (float)c8);
+            for (int d = 0; d < 4; d++) {
+                const int r = (sy + (d & 1)) + HY * (sz + (d >> 1));
+                cb += (uint32_t)(rbeg[2 * r + 2] - rbeg[2 * r]);
+            }
         }
-        if (use_phi)  // the phi box is staged (the scan's barriers ordered it)
-            for (int e = t; e < NPH; e += PT_NT) pmax = fmaxf(pmax, fabsf((float)phs[e]));
+        const float cmax = (float)wave_max_u32(cb);
+        float pmax = 0.0f;
+#pragma unroll
+        for (int j = 0; j < NW; j++) pmax = fmaxf(pmax, __uint_as_float(wpm[j]));
+        float emax = 0.0f, tmax = 0.0f, vmax = 0.0f;
+        const float a2 = (float)(a_d * a_d);
         if (need_q_ && qb) {
-            // bounds from this thread's share of the block maxima (the
-            // reduction below takes the tile's): a particle's e, q^2/e and
-            // |q_i| all increase with |q|^2
-            const float a2 = (float)(a_d * a_d);
-            emax = sqrtf(qqb + a2);
-            tmax = qqb / emax;
-            vmax = sqrtf(qqb);
+            // e, q^2/e and |q_i| all increase with |q|^2
+            const float qq = __uint_as_float(qbits);
+            emax = sqrtf(qq + a2);
+            tmax = qq / emax;
+            vmax = sqrtf(qq);
         } else if (need_q_) {
-            const float a2 = (float)(a_d * a_d);
+            // no valid block bounds: a pass over the tile's particles
+            __shared__ uint32_t wq[3][NW];
+            uint32_t eb = 0, tb = 0, vb = 0;
             for (int32_t i = t; i < total; i += PT_NT) {
                 int run;
                 const int64_t p = locate(i, run);
                 const float q0 = (float)v.q0[p], q1 = (float)v.q1[p], q2 = (float)v.q2[p];
                 const float qq = q0 * q0 + q1 * q1 + q2 * q2, ee = sqrtf(qq + a2);
-                emax = fmaxf(emax, ee);
-                tmax = fmaxf(tmax, qq / ee);
-                vmax = fmaxf(vmax, fmaxf(fabsf(q0), fmaxf(fabsf(q1), fabsf(q2))));
+                eb = max(eb, __float_as_uint(ee));
+                tb = max(tb, __float_as_uint(qq / ee));
+                vb = max(vb, __float_as_uint(fmaxf(fabsf(q0), fmaxf(fabsf(q1), fabsf(q2)))));
             }
-        }
-        for (int o_ = 32; o_ > 0; o_ >>= 1) {
-            cmax = fmaxf(cmax, __shfl_xor(cmax, o_, 64));
-            emax = fmaxf(emax, __shfl_xor(emax, o_, 64));
-            pmax = fmaxf(pmax, __shfl_xor(pmax, o_, 64));
-            tmax = fmaxf(tmax, __shfl_xor(tmax, o_, 64));
-            vmax = fmaxf(vmax, __shfl_xor(vmax, o_, 64));
-        }
-        if (lane == 0) {
-            wmax[0][wv] = cmax; wmax[1][wv] = emax; wmax[2][wv] = pmax; wmax2[0][wv] = tmax; wmax2[1][wv] = vmax;
-        }
-        __syncthreads();
-        for (int j = 0; j < PT_NT / 64; j++) {
-            cmax = fmaxf(cmax, wmax[0][j]); emax = fmaxf(emax, wmax[1][j]); pmax = fmaxf(pmax, wmax[2][j]);
-            tmax = fmaxf(tmax, wmax2[0][j]); vmax = fmaxf(vmax, wmax2[1][j]);
+            eb = wave_max_u32(eb);
+            tb = wave_max_u32(tb);
+            vb = wave_max_u32(vb);
+            if (lane == 0) { wq[0][wv] = eb; wq[1][wv] = tb; wq[2][wv] = vb; }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < NW; j++) {
+                emax = fmaxf(emax, __uint_as_float(wq[0][j]));
+                tmax = fmaxf(tmax, __uint_as_float(wq[1][j]));
+                vmax = fmaxf(vmax, __uint_as_float(wq[2][j]));
+            }
         }
         // largest |value| one particle adds to one site, per accumulator group:
         // T00's e + f phi + f2 phi^2 <= e (1 + 4|phi| + 9 phi^2) (f <= 4e, f2 <= 9e;
@@ -258,7 +270,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     // of one crowded cell contend in PT_NT/64 LDS atomic instructions at a
     // time instead of all in one; sparse tiles keep the lane-contiguous order,
     // whose loads coalesce fully.
-    constexpr int NCELL = (TX + 1) * (TY + 1) * (TZ + 1), NW = PT_NT / 64;
+    constexpr int NCELL = (TX + 1) * (TY + 1) * (TZ + 1);
     const bool spread = DEP_SPREAD_X4 > 0 && 4 * total >= DEP_SPREAD_X4 * NCELL;
     const int slot = spread ? lane * NW + wv : t;
     // In the FP32 form a crowded tile of a one-particle-per-pass kind instead
@@ -481,18 +493,44 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
         fac[1] = m1 / (double)S1;
         fac[2] = m2 / (double)S2;
     }
+    // component c's output array and the factor of its group
+    auto out_of = [&](int c) -> T * {
+        if (T00TIJ && c > 0) return (KIND == DEP_T00TIJL && c >= 7) ? out3 + (c - 7) * M : out2 + (c - 1) * M;
+        return out + c * M;
+    };
+    auto grp_of = [&](int c) {
+        return KIND == DEP_T00 ? 0 : KIND == DEP_TIJ ? 1 : KIND == DEP_T0I ? 2 : (c == 0 ? 0 : (c < 7 ? 1 : 2));
+    };
+    if (FIX && vec4) {
+        // four consecutive sites of one component per thread and pass: one
+        // 16-byte LDS read and one 16-byte store (outputs 16-byte aligned)
+        const float ff[3] = {(float)fac[0], (float)fac[1], (float)fac[2]};
+        constexpr int NV = NS / 4;
+        for (int w = t; w < C * NV; w += PT_NT) {
+            const int c = w / NV, s = (w - c * NV) * 4;
+            const int sx = s % TX, sy = (s / TX) % TY, sz = s / (TX * TY);
+            const uint32_t gs = (uint32_t)(x0 + sx) + (uint32_t)N * ((uint32_t)(y0 + sy) + (uint32_t)N * (uint32_t)(z0 + sz));
+            const int g = grp_of(c);
+            const float f = g == 0 ? ff[0] : (g == 1 ? ff[1] : ff[2]);
+            const int4 r = *reinterpret_cast<const int4 *>(&acc[c][s]);
+            float4 val = make_float4((float)r.x * f, (float)r.y * f, (float)r.z * f, (float)r.w * f);
+            float4 *dst = reinterpret_cast<float4 *>(out_of(c) + gs);
+            if (accumulate) {
+                const float4 o4 = *dst;
+                val.x += o4.x; val.y += o4.y; val.z += o4.z; val.w += o4.w;
+            }
+            *dst = val;
+        }
+        return;
+    }
     for (int s = t; s < NS; s += PT_NT) {
         const int sx = s % TX, sy = (s / TX) % TY, sz = s / (TX * TY);
         const uint32_t gs = (uint32_t)(x0 + sx) + (uint32_t)N * ((uint32_t)(y0 + sy) + (uint32_t)N * (uint32_t)(z0 + sz));
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            T *oc = out + c * M;
-            double ms = mass_d;
-            if (T00TIJ && c > 0 && c < 7) { oc = out2 + (c - 1) * M; ms = mass2_d; }
-            if (KIND == DEP_T00TIJL && c >= 7) { oc = out3 + (c - 7) * M; ms = mass3_d; }
-            const int grp = KIND == DEP_T00 ? 0 : KIND == DEP_TIJ ? 1 : KIND == DEP_T0I ? 2
-                                                                 : (c == 0 ? 0 : (c < 7 ? 1 : 2));
-            T val = FIX ? (T)((double)acc[c][s] * fac[grp]) : (T)(acc[c][s] * ms);
+            T *oc = out_of(c);
+            const double ms = (T00TIJ && c > 0) ? ((KIND == DEP_T00TIJL && c >= 7) ? mass3_d : mass2_d) : mass_d;
+            T val = FIX ? (T)((double)acc[c][s] * fac[grp_of(c)]) : (T)(acc[c][s] * ms);
             if (accumulate) val += oc[gs];
             oc[gs] = val;
         }
@@ -510,8 +548,10 @@ static int launch_dep(gev_pcls *p, T *out, T *out2, const T *phi, double mass, d
     // of the fixed-point tiles while they are valid (not for the adjoint's
     // vector deposit, whose values are not momenta)
     const uint32_t *qb = (sizeof(T) == 4 && KIND != DEP_VEC3 && p->qbound_valid) ? p->qbound : nullptr;
+    // 16-byte stores of four sites when every output is 16-byte aligned
+    const int vec4 = (((uintptr_t)out | (uintptr_t)out2 | (uintptr_t)out3) & 15) == 0;
     hipLaunchKernelGGL((k_deposit<T, KIND, TX, TY, TZ>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), out, out2,
-                       out3, c->M, phi, mass, mass2, mass3, a, acc, aux, qb);
+                       out3, c->M, phi, mass, mass2, mass3, a, acc, aux, qb, vec4);
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }

@@ -136,11 +136,15 @@ template <> __device__ __forceinline__ int32_t move_by<float>(const PV<float> &v
 template <int TX, int TY, int TZ>
 __device__ __forceinline__ void tile_origin(int N, int &x0, int &y0, int &z0)
 {
-    const int ntx = N / TX, nty = N / TY;
-    int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-    x0 = (int)(b % ntx) * TX;
-    y0 = (int)((b / ntx) % nty) * TY;
-    z0 = (int)(b / ((int64_t)ntx * nty)) * TZ;
+    // N and the tile edges are powers of two (N <= 1024): shifts and masks in
+    // 32 bits, the same tile order as xcd_remap
+    const uint32_t nb = gridDim.x, b0 = blockIdx.x;
+    const uint32_t per = nb >> 3;
+    const uint32_t b = (nb < 16 || b0 >= (per << 3)) ? b0 : (b0 & 7u) * per + (b0 >> 3);
+    const uint32_t lx = (uint32_t)__builtin_ctz((unsigned)(N / TX)), ly = (uint32_t)__builtin_ctz((unsigned)(N / TY));
+    x0 = (int)(b & ((1u << lx) - 1u)) * TX;
+    y0 = (int)((b >> lx) & ((1u << ly) - 1u)) * TY;
+    z0 = (int)(b >> (lx + ly)) * TZ;
 }
 
 // ---------------------------------------------------------- kick / drift --

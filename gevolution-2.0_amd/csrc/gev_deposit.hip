@@ -18,8 +18,8 @@
 // projections do.  FP32 tiles sum in 32-bit fixed point (integer LDS
 // atomics, which this part executes several times faster than FP32 or FP64
 // ones, and whose sums do not depend on the order): each tile bounds its
-// per-site sums by (largest particle count of the 8 cells around one of its
-// sites) x (largest per-particle value, from |q| and |phi|, separately for
+// per-site sums by (largest particle total of the 4 halo rows around one of
+// its sites) x (largest per-particle value, from |q| and |phi|, separately for
 // the T00, Tij and T0i components, whose sizes differ by the ratio of
 // kinetic to rest energy) and scales so that bound is 2^30.  |q| is bounded
 // by the largest |q|^2 per 8^3-cell block that the last particle update
@@ -77,7 +77,7 @@ template <> struct DepAcc<float, 4> { using type = double; };  // DEP_VEC3
 template <typename T, int KIND, int TX, int TY, int TZ>
 __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *out3, int64_t M,
                                                    const T *__restrict__ phi, double mass_d, double mass2_d,
-                                                   double mass3_d, double a_d, int accumulate,
+                                                   double mass3_d, double a_d, double ia2_d, int accumulate,
                                                    const T *__restrict__ aux, const uint32_t *__restrict__ qb, int vec4)
 {
     constexpr int C = DepC<KIND>::C;
@@ -135,13 +135,22 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     }
     uint32_t pbits = 0;
     if (use_phi) {
-        for (int e = t; e < NPH; e += PT_NT) {
+        // all of a thread's loads issued before the first LDS write
+        constexpr int NPI = (NPH + PT_NT - 1) / PT_NT;
+        T ph[NPI];
+#pragma unroll
+        for (int k = 0; k < NPI; k++) {
+            const int e = t + k * PT_NT;
             const int px = e % PX, py = (e / PX) % PY, pz = e / (PX * PY);
             const uint32_t g = ((uint32_t)(x0 - 1 + px) & msk) +
                                (uint32_t)N * (((uint32_t)(y0 - 1 + py) & msk) + (uint32_t)N * ((uint32_t)(z0 - 1 + pz) & msk));
-            const T ph = phi[g];
-            phs[e] = ph;
-            if constexpr (FIX) pbits = max(pbits, __float_as_uint(fabsf((float)ph)));
+            ph[k] = e < NPH ? phi[g] : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < NPI; k++) {
+            const int e = t + k * PT_NT;
+            if (e < NPH) phs[e] = ph[k];
+            if constexpr (FIX) pbits = max(pbits, __float_as_uint(fabsf((float)ph[k])));
         }
     }
     if constexpr (FIX) {
@@ -175,7 +184,7 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
     if (t == NSEG - 1) rbeg[NSEG] = off + incl;
     __syncthreads();
     const int32_t total = rbeg[NSEG];
-    const T a = (T)a_d, o = (T)1, ia2 = (T)(1.0 / (a_d * a_d));
+    const T a = (T)a_d, o = (T)1, ia2 = (T)ia2_d;  // ia2_d = 1 / a^2
     // flattened particle index -> container slot
     auto locate = [&](int32_t i, int &run) -> int64_t {
         int lo = 0, hi = NSEG - 1;
@@ -507,7 +516,9 @@ __global__ __launch_bounds__(PT_NT) void k_deposit(PV<T> v, T *out, T *out2, T *
         const float ff[3] = {(float)fac[0], (float)fac[1], (float)fac[2]};
         constexpr int NV = NS / 4;
         for (int w = t; w < C * NV; w += PT_NT) {
-            const int c = w / NV, s = (w - c * NV) * 4;
+            // NV a multiple of the wave size: the component is the same across the wave
+            const int c = NV % 64 == 0 ? __builtin_amdgcn_readfirstlane(w / NV) : w / NV;
+            const int s = (w - c * NV) * 4;
             const int sx = s % TX, sy = (s / TX) % TY, sz = s / (TX * TY);
             const uint32_t gs = (uint32_t)(x0 + sx) + (uint32_t)N * ((uint32_t)(y0 + sy) + (uint32_t)N * (uint32_t)(z0 + sz));
             const int g = grp_of(c);
@@ -551,7 +562,7 @@ static int launch_dep(gev_pcls *p, T *out, T *out2, const T *phi, double mass, d
     // 16-byte stores of four sites when every output is 16-byte aligned
     const int vec4 = (((uintptr_t)out | (uintptr_t)out2 | (uintptr_t)out3) & 15) == 0;
     hipLaunchKernelGGL((k_deposit<T, KIND, TX, TY, TZ>), dim3(nb), dim3(PT_NT), 0, c->stream, make_pv<T>(p), out, out2,
-                       out3, c->M, phi, mass, mass2, mass3, a, acc, aux, qb, vec4);
+                       out3, c->M, phi, mass, mass2, mass3, a, 1.0 / (a * a), acc, aux, qb, vec4);
     GEV_HIP(hipGetLastError());
     return GEV_OK;
 }

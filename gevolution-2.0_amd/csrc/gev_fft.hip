@@ -743,7 +743,52 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
         __syncthreads();  // twiddles loaded; previous transform done with lds
         reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, -1, ld, st);
     };
-    if constexpr (OP == ZOP_VECTOR) {
+    // ZOP_SCALAR3: the k tables in LDS (FP64), read per mode by the
+    // combination and the operator
+    constexpr bool TAB = OP == ZOP_SCALAR3;
+    __shared__ double gkd[TAB ? L : 1];
+    __shared__ cx<double> ksd[TAB ? L : 1];
+    if constexpr (TAB)
+        for (int m = threadIdx.x; m < L; m += NT) {
+            gkd[m] = zp.gk2[m];
+            ksd[m] = zp.ks[m];
+        }
+    if constexpr (OP == ZOP_SCALAR3) {
+        // chi from the three kz groups (formed by the y pass):
+        //   sum_c w_c S_c = FFT_z(A) + g2 FFT_z(B) + k2 FFT_z(C)
+        // each group read by its transform's first stage straight from
+        // memory, so only the FP64 sums stay live between the transforms
+        auto fft = [&](auto kind) {
+            constexpr int K = decltype(kind)::value;
+            const cx<T> *src = in + K * NK + off;
+            auto ld = [&](int line, int row) { return src[(int64_t)row * stride + line]; };
+            auto st = [&](int ln, int row, cx<T> v, int slot) {
+                lines[slot] = ln;
+                poss[slot] = row;
+                const cx<double> d = mk((double)v.x, (double)v.y);
+                if constexpr (K == 0) {
+                    acc[slot] = d;
+                } else if constexpr (K == 1) {
+                    acc[slot] = acc[slot] + d * gkd[row];
+                } else {
+                    // the last group completes the mode: the operator here,
+                    // so each slot's FP64 sum dies at once (eight of them
+                    // held through an operator loop doubled the registers)
+                    const cx<double> sum = acc[slot] + ksd[row] * d;
+                    const int kx = ct * TX + ln;
+                    cx<double> r = mk(0.0, 0.0);
+                    if (kx <= H && !(kx == 0 && ky == 0 && row == 0))
+                        r = sum * kop_scalar_norm(gkd[kx] + gy + gkd[row], L);
+                    vin[0][slot] = mk((T)r.x, (T)r.y);
+                }
+            };
+            __syncthreads();  // tables loaded; previous transform done with lds
+            reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, -1, ld, st);
+        };
+        fft(std::integral_constant<int, 0>{});
+        fft(std::integral_constant<int, 1>{});
+        fft(std::integral_constant<int, 2>{});
+    } else if constexpr (OP == ZOP_VECTOR) {
         // all 24 first-stage operands (3 components x the 8 elements reg_fft's
         // first stage reads: line t % TX, rows t / TX + r L/8) are loaded
         // before the first transform
@@ -793,17 +838,11 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const int64_t e = (int64_t)(jt + r * NB) * stride;
-            if constexpr (OP == ZOP_SCALAR3) {
-                A[r] = src[e];
-                B[r] = src[NK + e];
-                C[r] = src[2 * NK + e];
-            } else {
-                const cx<T> s0 = src[e], s1 = src[NK + e], s2 = src[2 * NK + e], s3 = src[3 * NK + e],
-                            s4 = src[4 * NK + e], s5 = src[5 * NK + e];
-                A[r] = s0 * wa0 + wa1 * s1 + s3 * wa3 + s5 * wa5;
-                B[r] = s0 + s3 - s5 * (T)2;
-                C[r] = wc2 * s2 + wc4 * s4;
-            }
+            const cx<T> s0 = src[e], s1 = src[NK + e], s2 = src[2 * NK + e], s3 = src[3 * NK + e],
+                        s4 = src[4 * NK + e], s5 = src[5 * NK + e];
+            A[r] = s0 * wa0 + wa1 * s1 + s3 * wa3 + s5 * wa5;
+            B[r] = s0 + s3 - s5 * (T)2;
+            C[r] = wc2 * s2 + wc4 * s4;
         }
         auto fft = [&](const cx<T> *pre, auto kind) {
             constexpr int K = decltype(kind)::value;
@@ -829,7 +868,7 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
         const int kx = ct * TX + lines[s], kz = poss[s];
         const bool pad = kx > H;
         const bool zero_mode = (kx == 0 && ky == 0 && kz == 0);
-        const double kk = pad ? 1.0 : zp.gk2[kx] + gy + zp.gk2[kz];
+        const double kk = (pad || TAB) ? 1.0 : zp.gk2[kx] + gy + zp.gk2[kz];
         if constexpr (OP == ZOP_POISSON) {
             cx<double> r = mk(0.0, 0.0);
             if (!pad) {
@@ -838,7 +877,9 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
                 if (!(zero_mode && zp.modif == 0.0)) r = kop_poisson(sv, kk, zp.coeff, zp.modif);
             }
             vin[0][s] = mk((T)r.x, (T)r.y);
-        } else if constexpr (OP == ZOP_SCALAR || OP == ZOP_SCALAR3) {
+        } else if constexpr (OP == ZOP_SCALAR3) {
+            // applied with the last forward transform
+        } else if constexpr (OP == ZOP_SCALAR) {
             cx<double> r = mk(0.0, 0.0);
             if (!pad && !zero_mode) r = acc[s] * kop_scalar_norm(kk, L);
             vin[0][s] = mk((T)r.x, (T)r.y);
@@ -876,10 +917,22 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
 // the vector and scalar operators keep 3 x 8 values per thread in registers:
 // at most 512 threads per workgroup there; Poisson takes 512 too up to L = 512
 // (three workgroups per CU instead of one, 4 % faster at L = 512)
+// threads per workgroup of the scalar-projection and vector z solves: the
+// scalar one (150+ registers, one 512-thread workgroup per CU) runs three
+// 256-thread workgroups per CU (0.90 against 1.02 ms at 512^3, FP32); the
+// vector one measured slower that way (1.75 against 1.34 ms)
+#ifndef ZS3_THREADS
+#define ZS3_THREADS 256
+#endif
+#ifndef ZS2_THREADS
+#define ZS2_THREADS 512
+#endif
 template <typename T, int L, int OP> constexpr int zs_tx()
 {
     constexpr bool narrow = OP != ZOP_POISSON || L <= 512;
-    return (narrow && col_tx<T, L>() * L / 8 > 512) ? 4096 / L : col_tx<T, L>();
+    constexpr int tx = (narrow && col_tx<T, L>() * L / 8 > 512) ? 4096 / L : col_tx<T, L>();
+    constexpr int cap = OP == ZOP_SCALAR3 ? ZS3_THREADS : (OP == ZOP_VECTOR ? ZS2_THREADS : 1 << 30);
+    return (tx * L / 8 > cap && cap * 8 / L >= 1) ? cap * 8 / L : tx;
 }
 
 template <typename T, int N, int OP>

@@ -694,7 +694,16 @@ static void launch_col_comb(cx<T> *data, int P, int64_t NK, const cx<T> *tw, con
 // FP32 vector projection from L = 128 up: 4 waves per SIMD (two workgroups
 // per CU) at the price of a spill of up to 80 bytes, 4.5 -> 4.1 ms for the
 // 512^3 B solve; the scalar projection and FP64 measured no better that way
-template <typename T, int L, int OP> constexpr int zs_waves() { return (OP == 2 && sizeof(T) == 4 && L >= 128) ? 4 : 1; }
+// ZS2_DIRECT: the vector solve reads each component in its transform's first
+// stage and applies the projection per mode with the last one (1.31 against
+// 1.34 ms at 512^3, spill 56 against 80 bytes); 0 keeps the all-operands-first form
+#ifndef ZS2_DIRECT
+#define ZS2_DIRECT 1
+#endif
+#ifndef ZS2_WAVES
+#define ZS2_WAVES 4
+#endif
+template <typename T, int L, int OP> constexpr int zs_waves() { return (OP == 2 && sizeof(T) == 4 && L >= 128) ? ZS2_WAVES : 1; }
 
 template <typename T, int L, int TX, int OP>
 __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(const cx<T> *in, cx<T> *out, int P, int64_t NK,
@@ -788,6 +797,39 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
         fft(std::integral_constant<int, 0>{});
         fft(std::integral_constant<int, 1>{});
         fft(std::integral_constant<int, 2>{});
+    } else if constexpr (OP == ZOP_VECTOR && ZS2_DIRECT) {
+        // each component read by its transform's first stage straight from
+        // memory; the projection applied per mode when the last component
+        // arrives, so no operator loop holds eight modes' FP64 values
+        auto fwd = [&](auto held) {
+            constexpr int CH = decltype(held)::value;
+            const cx<T> *src = in + CH * NK + off;
+            auto ld = [&](int line, int row) { return src[(int64_t)row * stride + line]; };
+            auto st = [&](int ln, int row, cx<T> v, int slot) {
+                lines[slot] = ln;
+                poss[slot] = row;
+                if constexpr (CH < 2) {
+                    vin[CH][slot] = v;
+                } else {
+                    const int kx = ct * TX + ln;
+                    cx<double> bo[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+                    if (kx <= H && !(kx == 0 && ky == 0 && row == 0)) {
+                        const cx<double> sv[3] = {mk((double)vin[0][slot].x, (double)vin[0][slot].y),
+                                                  mk((double)vin[1][slot].x, (double)vin[1][slot].y),
+                                                  mk((double)v.x, (double)v.y)};
+                        kop_vector(sv, zp.gk2[kx] + gy + zp.gk2[row], zp.ks[kx], kyv, zp.ks[row], zp.coeff, zp.modif,
+                                   bo);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 3; j++) vin[j][slot] = mk((T)bo[j].x, (T)bo[j].y);
+                }
+            };
+            __syncthreads();  // twiddles loaded; previous transform done with lds
+            reg_fft<T, L, TX, true, 1, false, 1>(lds, twl, -1, ld, st);
+        };
+        fwd(std::integral_constant<int, 0>{});
+        fwd(std::integral_constant<int, 1>{});
+        fwd(std::integral_constant<int, 2>{});
     } else if constexpr (OP == ZOP_VECTOR) {
         // all 24 first-stage operands (3 components x the 8 elements reg_fft's
         // first stage reads: line t % TX, rows t / TX + r L/8) are loaded
@@ -883,7 +925,7 @@ __global__ __launch_bounds__(TX * L / 8, (zs_waves<T, L, OP>())) void k_zsolve(c
             cx<double> r = mk(0.0, 0.0);
             if (!pad && !zero_mode) r = acc[s] * kop_scalar_norm(kk, L);
             vin[0][s] = mk((T)r.x, (T)r.y);
-        } else {
+        } else if constexpr (!ZS2_DIRECT) {
             cx<double> bo[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
             if (!pad && !zero_mode) {
                 const cx<double> sv[3] = {mk((double)vin[0][s].x, (double)vin[0][s].y),

@@ -29,6 +29,10 @@
 #include "gev_kops.hpp"
 #include "gev_pcommon.hpp"
 
+#ifndef ADJ_CHUNK_X4
+#define ADJ_CHUNK_X4 1  // four times the mean particles per tile cell from which the reverse kick/drift walks per-lane chunks
+#endif
+
 namespace gev {
 
 using D6 = Dual<double, 6>;
@@ -301,11 +305,22 @@ __global__ __launch_bounds__(NT) void k_adj_gr_kick_drift(PV<T> v, const T *__re
     tile_rows_load<TX, TY, NROW>(v.start, N, x0, y0, z0, lane, rb0, incl);
     incl = wave_row_scan(incl, lane);
     const int total = __shfl(incl, NROW - 1, 64);
-    const int niter = (total + NT - 1) / NT;  // the same for every lane of the workgroup
+    // A crowded tile gives every lane its own contiguous chunk of Q particles,
+    // so the lanes of one instruction sit Q particles apart, mostly in
+    // different cells, and their field-adjoint atomics rarely meet on one LDS
+    // address; other tiles keep consecutive particles in consecutive lanes
+    // (coalesced loads)
+    constexpr int NW = NT / 64;
+    const bool chunk = ADJ_CHUNK_X4 > 0 && 4 * total >= ADJ_CHUNK_X4 * (TX * TY * TZ);
+    const int Q = (total + 63) / 64;
+    // the same for every lane of the workgroup
+    const int niter = chunk ? (Q + NW - 1) / NW : (total + NT - 1) / NT;
     __syncthreads();
     const int64_t n = v.n;
     for (int it = 0; it < niter; it++) {
-        const int64_t p = tile_locate<NROW>(rb0, incl, tid + it * NT);
+        const int sq = it * NW + (tid >> 6);  // position inside the lane's chunk
+        const int f = chunk ? (sq < Q ? lane * Q + sq : total) : tid + it * NT;
+        const int64_t p = tile_locate<NROW>(rb0, incl, f);
         if (p < 0) continue;
         const int64_t id = v.id[p];
         int x, y, z;

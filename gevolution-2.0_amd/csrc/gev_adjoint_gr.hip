@@ -154,12 +154,29 @@ __device__ __forceinline__ void adj_gr_particle(const T rt[3], const T qt[3], co
         nq[j] = aq;
     }
 
-    // ---- field adjoints of the kick (pre-kick q), lambda = lqt
+    // ---- field adjoints of the kick (pre-kick q, lambda = lqt) and of the
+    // drift (kicked q', lambda = LX), summed per field over the stencil
+    // offsets in registers first: 76 distinct sites take one atomic each
+    // instead of 152 scattered adds
+    const R o = (R)1;
     const R K = s.dtau_k * Nd;
     const R v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
     const R e2 = v2 + s.a_k * s.a_k, e = sqrt(e2);
+    const R qq[3] = {qk[0].v, qk[1].v, qk[2].v};
+    const R w2 = qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2];
+    const R f2 = w2 + s.a_d * s.a_d, ie = (R)1 / sqrt(f2);
+    const R qL = qq[0] * LX[0] + qq[1] * LX[1] + qq[2] * LX[2];
+    // st[dx + 1][dy + 1][dz + 1]: one field's adjoint at stencil offset (dx, dy, dz)
+    R st[4][4][4];
+    auto clear = [&]() {
+#pragma unroll
+        for (int i = 0; i < 64; i++) st[i >> 4][(i >> 2) & 3][i & 3] = (R)0;
+    };
+    auto add = [&](int dx, int dy, int dz, R val) { st[dx + 1][dy + 1][dz + 1] += val; };
+    auto put = [&](int f, int dx, int dy, int dz) { sc(f, dx, dy, dz, st[dx + 1][dy + 1][dz + 1]); };
+    // phi: g = Phibar grad2(phi) - grad1(chi) + pgradB / (bcoeff e); q' = q - K e g
     {
-        // g = Phibar grad2(phi) - grad1(chi) + pgradB / (bcoeff e); q' = q - K e g
+        clear();
         const R Phi = cic<R>(Fp, r);
         const R ph = Phi / e2;
         const R Phibar = 4.0 * ph * v2 + (1.0 + s.a_k * s.a_k * ph) * (v2 + e2) / e2;
@@ -168,104 +185,115 @@ __device__ __forceinline__ void adj_gr_particle(const T rt[3], const T qt[3], co
         grad2<R>(Fp, r, G2);
         const R lG2[3] = {-K * e * Phibar * lqt[0], -K * e * Phibar * lqt[1], -K * e * Phibar * lqt[2]};
         const R lPhi = -K * e * dPhibar * (G2[0] * lqt[0] + G2[1] * lqt[1] + G2[2] * lqt[2]);
+        const R lph = s.nf_d >= 1 ? (R)(s.dtau_d * qL * (3.0 - w2 / f2) * ie) : (R)0;
 #pragma unroll
         for (int b = 0; b < 8; b++) {
             const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
             const R w = cw(b, r);
-            sc(0, bx, by, bz, w * lPhi);
-            const R hw = 0.5 * w;
-            sc(0, bx + 1, by, bz, hw * lG2[0]);
-            sc(0, bx - 1, by, bz, -hw * lG2[0]);
-            sc(0, bx, by + 1, bz, hw * lG2[1]);
-            sc(0, bx, by - 1, bz, -hw * lG2[1]);
-            sc(0, bx, by, bz + 1, hw * lG2[2]);
-            sc(0, bx, by, bz - 1, -hw * lG2[2]);
+            add(bx, by, bz, w * (lPhi + lph));
+            const R hw = (R)0.5 * w;
+            add(bx + 1, by, bz, hw * lG2[0]);
+            add(bx - 1, by, bz, -hw * lG2[0]);
+            add(bx, by + 1, bz, hw * lG2[1]);
+            add(bx, by - 1, bz, -hw * lG2[1]);
+            add(bx, by, bz + 1, hw * lG2[2]);
+            add(bx, by, bz - 1, -hw * lG2[2]);
         }
-        if (s.nf_k >= 2) {
-            const R lG1[3] = {K * e * lqt[0], K * e * lqt[1], K * e * lqt[2]};
 #pragma unroll
-            for (int b = 0; b < 8; b++) {
-                R gw[3];
-                cw_grad(b, r, gw);
-                sc(1, (b >> 2) & 1, (b >> 1) & 1, b & 1, gw[0] * lG1[0] + gw[1] * lG1[1] + gw[2] * lG1[2]);
-            }
+        for (int b = 0; b < 8; b++) {
+            const int u = (b >> 2) & 1, v_ = (b >> 1) & 1, w_ = b & 1;
+            put(0, u, v_, w_);                              // the cube
         }
-        if (s.nf_k >= 3) {
-            // pgradB (gevolution.hpp:968-994) is linear in B: its transpose, term by term
-            const R c = -K / s.bc_k;
-            const R l0 = c * lqt[0], l1 = c * lqt[1], l2 = c * lqt[2];
-                        const R o = 1.0;
-            // pg[0]
-            R t = l0 * q[1];
-            sc(3, 1, 0, 0, t * (o - r[2])); sc(3, 0, 0, 0, -t * (o - r[2]));
-            sc(3, 1, 0, 1, t * r[2]); sc(3, 0, 0, 1, -t * r[2]);
-            t = l0 * q[2];
-            sc(4, 1, 0, 0, t * (o - r[1])); sc(4, 0, 0, 0, -t * (o - r[1]));
-            sc(4, 1, 1, 0, t * r[1]); sc(4, 0, 1, 0, -t * r[1]);
-            t = l0 * q[0];
 #pragma unroll
-            for (int by = 0; by < 2; by++)
-#pragma unroll
-                for (int bz = 0; bz < 2; bz++) {
-                    const R w = (by ? r[1] : o - r[1]) * (bz ? r[2] : o - r[2]) * t;
-                    sc(2, -1, by, bz, w * (r[0] - o)); sc(2, 0, by, bz, w * (o - 2.0 * r[0])); sc(2, 1, by, bz, w * r[0]);
-                }
-            // pg[1]
-            t = l1 * q[2];
-            sc(4, 0, 1, 0, t * (o - r[0])); sc(4, 0, 0, 0, -t * (o - r[0]));
-            sc(4, 1, 1, 0, t * r[0]); sc(4, 1, 0, 0, -t * r[0]);
-            t = l1 * q[0];
-            sc(2, 0, 1, 0, t * (o - r[2])); sc(2, 0, 0, 0, -t * (o - r[2]));
-            sc(2, 0, 1, 1, t * r[2]); sc(2, 0, 0, 1, -t * r[2]);
-            t = l1 * q[1];
-#pragma unroll
-            for (int bx = 0; bx < 2; bx++)
-#pragma unroll
-                for (int bz = 0; bz < 2; bz++) {
-                    const R w = (bx ? r[0] : o - r[0]) * (bz ? r[2] : o - r[2]) * t;
-                    sc(3, bx, -1, bz, w * (r[1] - o)); sc(3, bx, 0, bz, w * (o - 2.0 * r[1])); sc(3, bx, 1, bz, w * r[1]);
-                }
-            // pg[2]
-            t = l2 * q[0];
-            sc(2, 0, 0, 1, t * (o - r[1])); sc(2, 0, 0, 0, -t * (o - r[1]));
-            sc(2, 0, 1, 1, t * r[1]); sc(2, 0, 1, 0, -t * r[1]);
-            t = l2 * q[1];
-            sc(3, 0, 0, 1, t * (o - r[0])); sc(3, 0, 0, 0, -t * (o - r[0]));
-            sc(3, 1, 0, 1, t * r[0]); sc(3, 1, 0, 0, -t * r[0]);
-            t = l2 * q[2];
-#pragma unroll
-            for (int bx = 0; bx < 2; bx++)
-#pragma unroll
-                for (int by = 0; by < 2; by++) {
-                    const R w = (bx ? r[0] : o - r[0]) * (by ? r[1] : o - r[1]) * t;
-                    sc(4, bx, by, -1, w * (r[2] - o)); sc(4, bx, by, 0, w * (o - 2.0 * r[2])); sc(4, bx, by, 1, w * r[2]);
-                }
+        for (int b = 0; b < 4; b++) {
+            const int u = (b >> 1) & 1, v_ = b & 1;
+            put(0, -1, u, v_); put(0, 2, u, v_);            // x faces
+            put(0, u, -1, v_); put(0, u, 2, v_);            // y faces
+            put(0, u, v_, -1); put(0, u, v_, 2);            // z faces
         }
     }
-    // ---- field adjoints of the drift (kicked q'), lambda = LX
-    {
-        const R qq[3] = {qk[0].v, qk[1].v, qk[2].v};
-        const R w2 = qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2];
-        const R f2 = w2 + s.a_d * s.a_d, ie = 1.0 / sqrt(f2);
-        const R qL = qq[0] * LX[0] + qq[1] * LX[1] + qq[2] * LX[2];
-        const R lph = s.dtau_d * qL * (3.0 - w2 / f2) * ie, lch = -s.dtau_d * qL * ie;
+    // chi: the kick's -grad1(chi) and the drift's -chi
+    if (s.nf_k >= 2 || s.nf_d >= 2) {
+        clear();
+        const R lG1[3] = {K * e * lqt[0], K * e * lqt[1], K * e * lqt[2]};
+        const R lch = (R)(-s.dtau_d * qL * ie);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            const R w = cw(b, r);
             const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
-            if (s.nf_d >= 1) sc(0, bx, by, bz, w * lph);
-            if (s.nf_d >= 2) sc(1, bx, by, bz, w * lch);
+            R val = (R)0;
+            if (s.nf_k >= 2) {
+                R gw[3];
+                cw_grad(b, r, gw);
+                val += gw[0] * lG1[0] + gw[1] * lG1[1] + gw[2] * lG1[2];
+            }
+            if (s.nf_d >= 2) val += cw(b, r) * lch;
+            add(bx, by, bz, val);
         }
-        if (s.nf_d >= 3) {
-            const R c = s.dtau_d / s.bc_d, o = 1.0;
-            const R b0 = c * LX[0], b1 = c * LX[1], b2 = c * LX[2];
-                        sc(2, 0, 0, 0, b0 * (o - r[1]) * (o - r[2])); sc(2, 0, 1, 0, b0 * r[1] * (o - r[2]));
-            sc(2, 0, 0, 1, b0 * (o - r[1]) * r[2]);       sc(2, 0, 1, 1, b0 * r[1] * r[2]);
-            sc(3, 0, 0, 0, b1 * (o - r[0]) * (o - r[2])); sc(3, 1, 0, 0, b1 * r[0] * (o - r[2]));
-            sc(3, 0, 0, 1, b1 * (o - r[0]) * r[2]);       sc(3, 1, 0, 1, b1 * r[0] * r[2]);
-            sc(4, 0, 0, 0, b2 * (o - r[0]) * (o - r[1])); sc(4, 1, 0, 0, b2 * r[0] * (o - r[1]));
-            sc(4, 0, 1, 0, b2 * (o - r[0]) * r[1]);       sc(4, 1, 1, 0, b2 * r[0] * r[1]);
+#pragma unroll
+        for (int b = 0; b < 8; b++) put(1, (b >> 2) & 1, (b >> 1) & 1, b & 1);
+    }
+    // B: pgradB (gevolution.hpp:968-994) is linear in B, its transpose term by
+    // term; the drift's staggered B interpolation (:1213-1226)
+    if (s.nf_k >= 3 || s.nf_d >= 3) {
+        const R c = s.nf_k >= 3 ? -K / s.bc_k : (R)0;
+        const R l0 = c * lqt[0], l1 = c * lqt[1], l2 = c * lqt[2];
+        const R cd = s.nf_d >= 3 ? (R)(s.dtau_d / s.bc_d) : (R)0;
+        const R b0 = cd * LX[0], b1 = cd * LX[1], b2 = cd * LX[2];
+        const R s0 = o - r[0], s1 = o - r[1], s2 = o - r[2];
+        // B0: x stencil of pg[0], the y and z links of pg[1] and pg[2], the drift
+        clear();
+        {
+            const R t = l0 * q[0];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int by = (b >> 1) & 1, bz = b & 1;
+                const R w = (by ? r[1] : s1) * (bz ? r[2] : s2) * t;
+                add(-1, by, bz, w * (r[0] - o)); add(0, by, bz, w * (o - (R)2 * r[0])); add(1, by, bz, w * r[0]);
+            }
+            const R t1 = l1 * q[0], t2 = l2 * q[0];
+            add(0, 1, 0, t1 * s2 - t2 * r[1] + b0 * r[1] * s2);
+            add(0, 0, 0, -t1 * s2 - t2 * s1 + b0 * s1 * s2);
+            add(0, 1, 1, t1 * r[2] + t2 * r[1] + b0 * r[1] * r[2]);
+            add(0, 0, 1, -t1 * r[2] + t2 * s1 + b0 * s1 * r[2]);
         }
+#pragma unroll
+        for (int b = 0; b < 12; b++) put(2, b / 4 - 1, (b >> 1) & 1, b & 1);
+        // B1: y stencil of pg[1], the x and z links of pg[0] and pg[2], the drift
+        clear();
+        {
+            const R t = l1 * q[1];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int bx = (b >> 1) & 1, bz = b & 1;
+                const R w = (bx ? r[0] : s0) * (bz ? r[2] : s2) * t;
+                add(bx, -1, bz, w * (r[1] - o)); add(bx, 0, bz, w * (o - (R)2 * r[1])); add(bx, 1, bz, w * r[1]);
+            }
+            const R t0 = l0 * q[1], t2 = l2 * q[1];
+            add(1, 0, 0, t0 * s2 - t2 * r[0] + b1 * r[0] * s2);
+            add(0, 0, 0, -t0 * s2 - t2 * s0 + b1 * s0 * s2);
+            add(1, 0, 1, t0 * r[2] + t2 * r[0] + b1 * r[0] * r[2]);
+            add(0, 0, 1, -t0 * r[2] + t2 * s0 + b1 * s0 * r[2]);
+        }
+#pragma unroll
+        for (int b = 0; b < 12; b++) put(3, (b >> 1) & 1, b / 4 - 1, b & 1);
+        // B2: z stencil of pg[2], the x and y links of pg[0] and pg[1], the drift
+        clear();
+        {
+            const R t = l2 * q[2];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int bx = (b >> 1) & 1, by = b & 1;
+                const R w = (bx ? r[0] : s0) * (by ? r[1] : s1) * t;
+                add(bx, by, -1, w * (r[2] - o)); add(bx, by, 0, w * (o - (R)2 * r[2])); add(bx, by, 1, w * r[2]);
+            }
+            const R t0 = l0 * q[2], t1 = l1 * q[2];
+            add(1, 0, 0, t0 * s1 - t1 * r[0] + b2 * r[0] * s1);
+            add(0, 0, 0, -t0 * s1 - t1 * s0 + b2 * s0 * s1);
+            add(1, 1, 0, t0 * r[1] + t1 * r[0] + b2 * r[0] * r[1]);
+            add(0, 1, 0, -t0 * r[1] + t1 * s0 + b2 * s0 * r[1]);
+        }
+#pragma unroll
+        for (int b = 0; b < 12; b++) put(4, (b >> 1) & 1, b & 1, b / 4 - 1);
     }
 }
 

@@ -11,8 +11,9 @@
 //     cell order).
 // The order of particles inside a cell follows the counter atomics, so it can
 // change from run to run; IDs, cell keys and cell_start do not.  The CIC
-// deposits live in gev_deposit.hip (LDS atomics: their sums are not
-// bit-reproducible from run to run) and the kick/drift in gev_pupdate.hip.
+// deposits live in gev_deposit.hip (LDS atomics: the FP64 sums are not
+// bit-reproducible from run to run, the FP32 form's 32-bit fixed-point sums
+// are) and the kick/drift in gev_pupdate.hip.
 #include "gev_common.hpp"
 #include "gev_kernels.hpp"
 #include "gev_pcommon.hpp"

@@ -310,15 +310,20 @@ class PMRunGR:
             k0 = starts[si]
             k1 = starts[si + 1] if si + 1 < len(starts) else self.steps
             states = {k0: self._ckpts[k0]}
+            held = -1
             if k1 - k0 > 1:
                 self._restore(self._ckpts[k0])
                 for k in range(k0, k1 - 1):
                     self._cycle(k)
-                    states[k + 1] = self._save()
+                    if k + 1 < k1 - 1:
+                        states[k + 1] = self._save()
+                # the container and phi, chi now hold the start state of cycle k1 - 1
+                held = k1 - 1
             if self.mem_probe:
                 self.mem_probe()
             for k in range(k1 - 1, k0 - 1, -1):
-                self._restore(states[k])
+                if k != held:  # the reverse cycle leaves the particles and fields as they are
+                    self._restore(states[k])
                 lphi, lchi = self._reverse_cycle(k, self.phi, self.chi, lx, lq, lphi, lchi)
             del states
         lpsi = c.real(1)

@@ -231,13 +231,20 @@ class PMRunGR:
         c.solve_vector_T0i_links(B, phi, k_["bcoeff"], 0.0, work=self.work3)
         return phi, chi, B
 
-    def _cycle(self, k):
+    def _cycle(self, k, keep=False):
+        """cycle k; with keep, returns the fields the reverse cycle reads
+        (phi, chi, B, T00, links) so that it need not recompute them"""
         s, p = self.sched[k], self.pcls
         phi, chi, B = self._fields(k, self.phi, self.chi)
+        kept = None
+        if keep:  # hand the deposit buffers over and deposit into fresh ones next time
+            kept = (phi, chi, B, self.T00, self.L)
+            self.T00, self.L = torch.empty_like(self.T00), torch.empty_like(self.L)
         N = self.N
         p.kick_drift(1, s["dtau_k"], s["dtau"], phi, chi, B, 3, 3, s["a"], s["a_d"], s["a"] ** 2 * N,
                      s["a_d"] ** 2 * N)
         self.phi, self.chi = phi, chi
+        return kept
 
     def _ic(self, psi):
         p = self.pcls
@@ -274,18 +281,24 @@ class PMRunGR:
                  for t in self._ckpts.values())
         return nb + (self._final.numel() if self._final is not None else 0)
 
-    def _reverse_cycle(self, k, phi_old, chi_old, lx, lq, lphi, lchi):
+    def _reverse_cycle(self, k, phi_old, chi_old, lx, lq, lphi, lchi, kept=None):
         """the adjoint of cycle k from its start state (in the container, with
         phi_old, chi_old); lphi, lchi: adjoints of the cycle's phi and chi
-        (consumed).  Returns the adjoints of phi_old and chi_old."""
+        (consumed); kept: the cycle's fields from the replay (phi, chi, B,
+        T00, links), else they are recomputed.  Returns the adjoints of
+        phi_old and chi_old."""
         s, c, p, k_ = self.sched[k], self.ctx, self.pcls, self._coef(k)
         N = self.N
-        phi, chi, B = self._fields(k, phi_old, chi_old)
+        if kept is None:
+            phi, chi, B = self._fields(k, phi_old, chi_old)
+            T00, L = self.T00, self.L
+        else:
+            phi, chi, B, T00, L = kept
         lB = c.real(3)
         p.adj_gr_kick_drift(phi, chi, B, lx, lq, lphi, lchi, lB, 3, 3, s["a"], s["a_d"], s["dtau_k"], s["dtau"])
         # B = V (L (1 + phi_a + phi_b)), V self-adjoint
         c.solve_vector(lB, k_["bcoeff"], 0.0, work=self.work3)
-        c.adj_link_factor(phi, self.L, lB, lphi)           # lB now holds lL
+        c.adj_link_factor(phi, L, lB, lphi)                # lB now holds lL
         # chi = C Sij, Sij = stencil(phi) + ccoeff Tij
         lS = c.adj_solve_chi(lchi, work1=self.work1, work6=self.work6)
         c.adj_source_Tij(phi, lS, lphi)
@@ -293,7 +306,7 @@ class PMRunGR:
         # phi = P src, P self-adjoint
         lsrc = c.solve_poisson(lphi, k_["pcoeff"], k_["modif"], work=self.work1)
         lphi_old, lchi_old, lT00 = c.real(1), c.real(1), c.real(1)
-        c.adj_source_T00(phi_old, self.T00, lsrc, lT00, lphi_old, lchi_old, self.om, k_["c1"], k_["c2"], k_["c3"])
+        c.adj_source_T00(phi_old, T00, lsrc, lT00, lphi_old, lchi_old, self.om, k_["c1"], k_["c2"], k_["c3"])
         p.adj_gr_deposit(phi_old, lT00, lTij, lB, lx, lq, lphi_old, s["a"])
         return lphi_old, lchi_old
 
@@ -310,11 +323,12 @@ class PMRunGR:
             k0 = starts[si]
             k1 = starts[si + 1] if si + 1 < len(starts) else self.steps
             states = {k0: self._ckpts[k0]}
+            kept = {}  # the fields the replay formed, by cycle
             held = -1
             if k1 - k0 > 1:
                 self._restore(self._ckpts[k0])
                 for k in range(k0, k1 - 1):
-                    self._cycle(k)
+                    kept[k] = self._cycle(k, keep=True)
                     if k + 1 < k1 - 1:
                         states[k + 1] = self._save()
                 # the container and phi, chi now hold the start state of cycle k1 - 1
@@ -324,8 +338,8 @@ class PMRunGR:
             for k in range(k1 - 1, k0 - 1, -1):
                 if k != held:  # the reverse cycle leaves the particles and fields as they are
                     self._restore(states[k])
-                lphi, lchi = self._reverse_cycle(k, self.phi, self.chi, lx, lq, lphi, lchi)
-            del states
+                lphi, lchi = self._reverse_cycle(k, self.phi, self.chi, lx, lq, lphi, lchi, kept.pop(k, None))
+            del states, kept
         lpsi = c.real(1)
         c.adj_ic_lattice(lx, lq, self.cx, self.cq, lpsi)
         return lpsi

@@ -118,6 +118,10 @@ class Context:
     def real(self, ncomp=1):
         return torch.zeros((ncomp, self.N, self.N, self.N), dtype=self.rdtype, device=self.device)
 
+    def real_empty(self, ncomp=1):
+        """an uninitialised real field, for outputs that a call overwrites whole"""
+        return torch.empty((ncomp, self.N, self.N, self.N), dtype=self.rdtype, device=self.device)
+
     def fourier(self, ncomp=1):
         return torch.zeros((ncomp, self.N, self.N, self.P), dtype=self.cdtype, device=self.device)
 
@@ -138,7 +142,7 @@ class Context:
     def fft_c2r(self, F, out=None, preserve=True):
         nc = self.ncomp_fourier(F, "fft_c2r")
         src = F.clone() if preserve else F
-        out = self.real(nc) if out is None else out
+        out = self.real_empty(nc) if out is None else out
         pf, po = self.k(src, nc, "fft_c2r(in)"), self.r(out, nc, "fft_c2r(out)")
         self.bind_stream()
         check(lib.gev_fft_c2r(self._h, pf, po, nc), "gev_fft_c2r")
@@ -217,7 +221,7 @@ class Context:
 
     def adj_solve_chi(self, lchi, out=None, work1=None, work6=None):
         """the adjoint of solve_chi: lS (6) = c2r(conj(weights) r2c(lchi) / (2 k^4 N))"""
-        out = self.real(6) if out is None else out
+        out = self.real_empty(6) if out is None else out
         work1 = self.fourier(1) if work1 is None else work1
         work6 = self.fourier(6) if work6 is None else work6
         self.fft_r2c(lchi, out=work1)
@@ -255,7 +259,7 @@ class Context:
 
     def solve_chi(self, Sij, out=None, work6=None, work1=None):
         """c2r(projectFTscalar(r2c(Sij)))"""
-        out = self.real(1) if out is None else out
+        out = self.real_empty(1) if out is None else out
         work6 = self.fourier(6) if work6 is None else work6
         work1 = self.fourier(1) if work1 is None else work1
         self.bind_stream()
@@ -266,7 +270,7 @@ class Context:
 
     def solve_chi_Tij(self, phi, Tij, coeff, out=None, work6=None, work1=None):
         """c2r(projectFTscalar(r2c(prepareFTsource(phi, Tij, coeff)))); Tij holds Sij afterwards for N < 128"""
-        out = self.real(1) if out is None else out
+        out = self.real_empty(1) if out is None else out
         work6 = self.fourier(6) if work6 is None else work6
         work1 = self.fourier(1) if work1 is None else work1
         self.bind_stream()

@@ -251,15 +251,15 @@ class PMRunGR:
         p.init_lattice()
         p.initialize_q(psi, -self.cq)
         p.displace(psi, self.cx)
-        self.phi.zero_()
-        self.chi.zero_()
+        self.phi, self.chi = self.ctx.real(1), self.ctx.real(1)  # fresh: saved states may share the old ones
 
     def _save(self):
         return (self.pcls.save(), self.phi.clone(), self.chi.clone())
 
     def _restore(self, st):
+        # phi and chi are never written in place once saved: no copies needed
         self.pcls.restore(st[0])
-        self.phi, self.chi = st[1].clone(), st[2].clone()
+        self.phi, self.chi = st[1], st[2]
 
     def forward(self, psi):
         psi = psi.reshape(1, self.N, self.N, self.N).to(self.ctx.rdtype).contiguous()
@@ -305,7 +305,8 @@ class PMRunGR:
         lTij = lS.mul_(k_["ccoeff"])
         # phi = P src, P self-adjoint
         lsrc = c.solve_poisson(lphi, k_["pcoeff"], k_["modif"], work=self.work1)
-        lphi_old, lchi_old, lT00 = c.real(1), c.real(1), c.real(1)
+        lphi_old, lchi_old = c.real(1), c.real(1)
+        lT00 = torch.empty_like(lphi_old)  # overwritten by adj_source_T00
         c.adj_source_T00(phi_old, T00, lsrc, lT00, lphi_old, lchi_old, self.om, k_["c1"], k_["c2"], k_["c3"])
         p.adj_gr_deposit(phi_old, lT00, lTij, lB, lx, lq, lphi_old, s["a"])
         return lphi_old, lchi_old

@@ -14,10 +14,11 @@
 //
 // P, C and V are FFT-diagonal: P and V are real symmetric (self-adjoint) as
 // real-space maps, and C's adjoint is the same transform pair with the
-// conjugate per-mode weights (k_adj_project_ft_scalar).  The per-particle
-// Jacobians come from evaluating the forward functor arithmetic on dual
-// numbers (gev_dual.hpp) seeded in the particle's cell offsets r and momenta
-// q; the field adjoints are scattered with global atomics (their order
+// conjugate per-mode weights (k_adj_project_ft_scalar).  The reverse of the
+// kick + drift is written out term by term (adj_gr_particle); the deposits'
+// per-particle Jacobians come from evaluating the forward functor arithmetic
+// on dual numbers (gev_dual.hpp) seeded in the particle's cell offsets r and
+// momenta q.  The field adjoints are scattered with atomics (their order
 // follows atomic arbitration).  Per-particle adjoints lx, lq are [3][n]
 // arrays indexed by particle ID, as in the Newtonian adjoint; positions in box
 // units (d/dx = N d/dr).
@@ -62,41 +63,57 @@ template <typename T> struct FAt {
     __device__ __forceinline__ T operator()(int dx, int dy, int dz) const { return f[site(N, x + dx, y + dy, z + dz)]; }
 };
 
-// update_pos rate (gevolution.hpp:1162-1232) over a generic real type: the
-// displacement per unit dtau, v + b / bcoeff
+// corner values c[4 dx + 2 dy + dz] of a cube as a field accessor (grad1 of
+// it is the r-gradient of their trilinear interpolation)
+template <typename R> struct CubeAt {
+    const R *c;
+    __device__ __forceinline__ R operator()(int dx, int dy, int dz) const { return c[4 * dx + 2 * dy + dz]; }
+};
+
+// pgradB of update_q (gevolution.hpp:968-994) is linear in the momenta,
+// pg_l = sum_m P[l][m] q_m; the matrix P at offsets r and, for the reverse,
+// its r-derivatives: dPd[l][j] = dP[l][l]/dr_j (the second-order stencils of
+// B_l along l times the trilinear weights across) and the three mixed
+// differences M[l] = d^2 B_l / dr_a dr_b over the cube, a, b the other two
+// axes (dP[0][1]/dr_2 = dP[2][1]/dr_0 = M[1], and so on)
 template <typename R, class FA>
-__device__ __forceinline__ void drift_rate(const R r[3], const R q[3], const FA &Fp, const FA &Fc, const FA &B0,
-                                           const FA &B1, const FA &B2, int nfield, double a, double bcoeff, R out[3])
+__device__ __forceinline__ void pgrad_matrix(const FA &B0, const FA &B1, const FA &B2, const R r[3], R P[3][3],
+                                             R dPd[3][3], R M[3])
 {
-    using B = typename DualBase<R>::type;  // the real type under R
-    R v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-    const R e2 = v2 + (B)(a * a);
-    R ph = (R)0, ch = (R)0;
-    if (nfield >= 1) ph = cic<R>(Fp, r);
-    if (nfield >= 2) ch = cic<R>(Fc, r);
-    const R ie = frsqrt(e2);
-    v2 = ((R)1 + ((R)3 - v2 * ie * ie) * ph - ch) * ie;
+    const R o = (R)1, s0 = o - r[0], s1 = o - r[1], s2 = o - r[2];
+    // staggered links across each axis (the drift's B interpolation uses the same ones)
+    P[0][1] = s2 * (B1(1, 0, 0) - B1(0, 0, 0)) + r[2] * (B1(1, 0, 1) - B1(0, 0, 1));
+    P[0][2] = s1 * (B2(1, 0, 0) - B2(0, 0, 0)) + r[1] * (B2(1, 1, 0) - B2(0, 1, 0));
+    P[1][0] = s2 * (B0(0, 1, 0) - B0(0, 0, 0)) + r[2] * (B0(0, 1, 1) - B0(0, 0, 1));
+    P[1][2] = s0 * (B2(0, 1, 0) - B2(0, 0, 0)) + r[0] * (B2(1, 1, 0) - B2(1, 0, 0));
+    P[2][0] = s1 * (B0(0, 0, 1) - B0(0, 0, 0)) + r[1] * (B0(0, 1, 1) - B0(0, 1, 0));
+    P[2][1] = s0 * (B1(0, 0, 1) - B1(0, 0, 0)) + r[0] * (B1(1, 0, 1) - B1(1, 0, 0));
+    M[0] = B0(0, 1, 1) - B0(0, 0, 1) - B0(0, 1, 0) + B0(0, 0, 0);
+    M[1] = B1(1, 0, 1) - B1(0, 0, 1) - B1(1, 0, 0) + B1(0, 0, 0);
+    M[2] = B2(1, 1, 0) - B2(0, 1, 0) - B2(1, 0, 0) + B2(0, 0, 0);
+    // the diagonal: B_l's three-point stencil along l at the four lines across
+    // (u, v the offsets along the two other axes in increasing axis order)
+    auto diag = [&](int l, auto at) {
+        const R rl = r[l];
+        const int a = l == 0 ? 1 : 0, b = l == 2 ? 1 : 2;
+        const R ra = r[a], rb = r[b], sa = o - ra, sb = o - rb;
+        R X[2][2], L[2][2];
 #pragma unroll
-    for (int l = 0; l < 3; l++) out[l] = q[l] * v2;
-    if (nfield >= 3) {
-        const R o = (R)1;
-        R b[3];
-        b[0] = B0(0, 0, 0) * (o - r[1]) * (o - r[2]);
-        b[1] = B1(0, 0, 0) * (o - r[0]) * (o - r[2]);
-        b[2] = B2(0, 0, 0) * (o - r[0]) * (o - r[1]);
-        b[1] += B1(1, 0, 0) * r[0] * (o - r[2]);
-        b[2] += B2(1, 0, 0) * r[0] * (o - r[1]);
-        b[0] += B0(0, 1, 0) * r[1] * (o - r[2]);
-        b[2] += B2(0, 1, 0) * (o - r[0]) * r[1];
-        b[0] += B0(0, 0, 1) * (o - r[1]) * r[2];
-        b[1] += B1(0, 0, 1) * (o - r[0]) * r[2];
-        b[1] += B1(1, 0, 1) * r[0] * r[2];
-        b[0] += B0(0, 1, 1) * r[1] * r[2];
-        b[2] += B2(1, 1, 0) * r[0] * r[1];
-        const B ibc = (B)(1.0 / bcoeff);
+        for (int u = 0; u < 2; u++)
 #pragma unroll
-        for (int l = 0; l < 3; l++) out[l] += b[l] * ibc;
-    }
+            for (int v = 0; v < 2; v++) {
+                const R m = at(-1, u, v), c = at(0, u, v), p = at(1, u, v);
+                X[u][v] = (rl - o) * m + (o - (R)2 * rl) * c + rl * p;
+                L[u][v] = m - (R)2 * c + p;
+            }
+        P[l][l] = sa * sb * X[0][0] + ra * sb * X[1][0] + sa * rb * X[0][1] + ra * rb * X[1][1];
+        dPd[l][l] = sa * sb * L[0][0] + ra * sb * L[1][0] + sa * rb * L[0][1] + ra * rb * L[1][1];
+        dPd[l][a] = sb * (X[1][0] - X[0][0]) + rb * (X[1][1] - X[0][1]);
+        dPd[l][b] = sa * (X[0][1] - X[0][0]) + ra * (X[1][1] - X[1][0]);
+    };
+    diag(0, [&](int t, int u, int v) { return B0(t, u, v); });
+    diag(1, [&](int t, int u, int v) { return B1(u, t, v); });
+    diag(2, [&](int t, int u, int v) { return B2(u, v, t); });
 }
 
 struct AdjKD {
@@ -110,62 +127,129 @@ struct AdjKD {
 // sc(f, dx, dy, dz, value) with f = 0 phi, 1 chi, 2..4 B_x..B_z and the
 // site's offset from the particle's cell.  Fields are read through the
 // accessors Fp, Fc, FB0..FB2 (offsets from the particle's cell).
+//
+// The forward (update_q, gevolution.hpp:888-1011, then the update_pos rate,
+// :1162-1232):
+//   e = sqrt(|q|^2 + a_k^2),  Phibar = 4 Phi v2 / e^2 + (1 + a_k^2 Phi / e^2)(v2 + e^2) / e^2
+//   q' = q - K (e Phibar G2 - e G1 + P q / bcoeff_k),  K = dtau_k N
+//   x' = x + dtau_d (q' fac + b / bcoeff_d),  fac = (1 + (3 - w2 / f^2) Phi - Chi) / f
+// with v2 = |q|^2, w2 = |q'|^2, f^2 = w2 + a_d^2, Phi, Chi the CIC values,
+// G2 = grad2(phi), G1 = grad1(chi) (the r-gradient of Chi), P the pgradB
+// matrix and b the staggered B interpolation, whose r-gradient is P's
+// off-diagonal (d(mu.b)/dr_j = sum_{l != j} mu_l P[j][l]).  Its reverse is
+// written out term by term: one pass over the stencil per particle in place
+// of six forward tangent directions.
 template <typename T, class FA, class SC>
 __device__ __forceinline__ void adj_gr_particle(const T rt[3], const T qt[3], const T LX[3], const T LQ[3], const FA &Fp,
                                                 const FA &Fc, const FA &FB0, const FA &FB1, const FA &FB2, int N,
                                                 const AdjKD &s, T nx[3], T nq[3], SC &sc)
 {
-    // the per-particle Jacobians and adjoints in the container's precision
-    // (FP32 for the mixed form: half the registers of FP64 dual numbers)
+    // in the container's precision (FP32 for the mixed form)
     using R = T;
-    using DT = Dual<R, 6>;
     const R r[3] = {rt[0], rt[1], rt[2]};
     const R q[3] = {qt[0], qt[1], qt[2]};
     const R Nd = (R)N;
-    // kick on duals seeded in (r, q)
-    DT rD[3], qk[3];
+    const R o = (R)1;
+    const int nfm = s.nf_k > s.nf_d ? s.nf_k : s.nf_d;
+    // ---- the kick, keeping what its reverse needs
+    const R K = (R)(s.dtau_k * N);
+    const R a2k = (R)(s.a_k * s.a_k);
+    const R v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const R e2 = v2 + a2k, ie2 = o / e2, e = sqrt(e2);
+    const R Phi = cic<R>(Fp, r);
+    R gP[3], G2[3], G1[3] = {(R)0, (R)0, (R)0};
+    grad1<R>(Fp, r, gP);  // dPhi/dr
+    grad2<R>(Fp, r, G2);
+    if (nfm >= 2) grad1<R>(Fc, r, G1);
+    const R Chi = s.nf_d >= 2 ? cic<R>(Fc, r) : (R)0;
+    R P[3][3] = {}, dPd[3][3] = {}, Mx[3] = {};
+    if (nfm >= 3) pgrad_matrix<R>(FB0, FB1, FB2, r, P, dPd, Mx);
+    const R p1 = Phi * ie2;
+    const R Phibar = (R)4 * p1 * v2 + (o + a2k * p1) * (v2 + e2) * ie2;
+    const R dPhibar = ((R)4 * v2 + a2k * (v2 + e2) * ie2) * ie2;  // dPhibar/dPhi
+    const R dPbv = a2k * ie2 * ie2 * (((R)4 * Phi + o) - (R)2 * Phi * v2 * ie2);  // dPhibar/dv2 (e^2 = v2 + a_k^2)
+    const R k2 = s.nf_k >= 2 ? o : (R)0;
+    const R ibk = s.nf_k >= 3 ? (R)(1.0 / s.bc_k) : (R)0;
+    R qq[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) { rD[j] = DT::seed(r[j], j); qk[j] = DT::seed(q[j], 3 + j); }
-    phys_update_q<DT>(rD, qk, Fp, Fc, FB0, FB1, FB2, s.nf_k, DT(s.a_k), DT(s.dtau_k), DT(s.bc_k), DT(Nd));
-    // drift rate on duals seeded in (r, q')
-    DT qp[3], rate[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) qp[j] = DT::seed(qk[j].v, 3 + j);
-    drift_rate<DT>(rD, qp, Fp, Fc, FB0, FB1, FB2, s.nf_d, s.a_d, s.bc_d, rate);
-
-    // adjoint of the kicked momentum: its own plus the drift's use of it
+    for (int l = 0; l < 3; l++)
+        qq[l] = q[l] - K * (e * (Phibar * G2[l] - k2 * G1[l]) + ibk * (P[l][0] * q[0] + P[l][1] * q[1] + P[l][2] * q[2]));
+    // ---- the drift's reverse at (r, q'), mu = dtau_d LX
+    const R a2d = (R)(s.a_d * s.a_d);
+    const R w2 = qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2];
+    const R f2 = w2 + a2d, ie = o / sqrt(f2), ie_2 = ie * ie, ie3 = ie * ie_2;
+    const R ph = s.nf_d >= 1 ? Phi : (R)0;
+    const R dtd = (R)s.dtau_d;
+    const R mu[3] = {dtd * LX[0], dtd * LX[1], dtd * LX[2]};
+    const R mq = mu[0] * qq[0] + mu[1] * qq[1] + mu[2] * qq[2];
+    const R qL = qq[0] * LX[0] + qq[1] * LX[1] + qq[2] * LX[2];
+    const R t3 = (R)3 - w2 * ie_2;
+    const R fac = (o + t3 * ph - Chi) * ie;
+    // dfac/dw2
+    const R dfac = -(R)0.5 * ie3 * (o + (R)3 * ph - Chi) - ph * ie3 + (R)1.5 * ph * w2 * ie3 * ie_2;
     R lqt[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        R acc = LQ[j];
+    for (int j = 0; j < 3; j++) lqt[j] = LQ[j] + mu[j] * fac + (R)2 * mq * dfac * qq[j];
+    const R ibd = s.nf_d >= 3 ? (R)(1.0 / s.bc_d) : (R)0;
+    const R cph = s.nf_d >= 1 ? mq * ie * t3 : (R)0, cch = s.nf_d >= 2 ? mq * ie : (R)0;
+    R ar[3];
+    ar[0] = cph * gP[0] - cch * G1[0] + ibd * (mu[1] * P[0][1] + mu[2] * P[0][2]);
+    ar[1] = cph * gP[1] - cch * G1[1] + ibd * (mu[0] * P[1][0] + mu[2] * P[1][2]);
+    ar[2] = cph * gP[2] - cch * G1[2] + ibd * (mu[0] * P[2][0] + mu[1] * P[2][1]);
+    // ---- the kick's reverse, lambda = lqt
+    const R lg2 = lqt[0] * G2[0] + lqt[1] * G2[1] + lqt[2] * G2[2];
+    const R lg1 = k2 * (lqt[0] * G1[0] + lqt[1] * G1[1] + lqt[2] * G1[2]);
+    // momenta: d(e)/dq_j = q_j / e, dPhibar/dq_j = 2 q_j dPhibar/dv2, P linear
+    const R cq = (Phibar * lg2 - lg1) / e + (R)2 * e * lg2 * dPbv;
 #pragma unroll
-        for (int l = 0; l < 3; l++) acc += s.dtau_d * LX[l] * rate[l].d[3 + j];
-        lqt[j] = acc;
+    for (int j = 0; j < 3; j++)
+        nq[j] = lqt[j] - K * (q[j] * cq + ibk * (lqt[0] * P[0][j] + lqt[1] * P[1][j] + lqt[2] * P[2][j]));
+    // offsets: lambda.G2 is the trilinear interpolation of the corner values
+    // c_b = lambda . (phi(b + e_l) - phi(b - e_l)) / 2
+    R cb[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+        cb[b] = (R)0.5 * (lqt[0] * (Fp(bx + 1, by, bz) - Fp(bx - 1, by, bz)) +
+                          lqt[1] * (Fp(bx, by + 1, bz) - Fp(bx, by - 1, bz)) +
+                          lqt[2] * (Fp(bx, by, bz + 1) - Fp(bx, by, bz - 1)));
+    }
+    R dlg2[3];
+    grad1<R>(CubeAt<R>{cb}, r, dlg2);
+    R kr[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) kr[j] = e * (lg2 * dPhibar * gP[j] + Phibar * dlg2[j]);
+    if (s.nf_k >= 2) {
+        // lambda.G1 = sum_l lambda_l dChi/dr_l: its r-gradient takes the mixed
+        // second differences of chi over the cube
+        const R s0 = o - r[0], s1 = o - r[1], s2 = o - r[2];
+        const R H01 = s2 * (Fc(1, 1, 0) - Fc(1, 0, 0) - Fc(0, 1, 0) + Fc(0, 0, 0)) +
+                      r[2] * (Fc(1, 1, 1) - Fc(1, 0, 1) - Fc(0, 1, 1) + Fc(0, 0, 1));
+        const R H02 = s1 * (Fc(1, 0, 1) - Fc(1, 0, 0) - Fc(0, 0, 1) + Fc(0, 0, 0)) +
+                      r[1] * (Fc(1, 1, 1) - Fc(1, 1, 0) - Fc(0, 1, 1) + Fc(0, 1, 0));
+        const R H12 = s0 * (Fc(0, 1, 1) - Fc(0, 1, 0) - Fc(0, 0, 1) + Fc(0, 0, 0)) +
+                      r[0] * (Fc(1, 1, 1) - Fc(1, 1, 0) - Fc(1, 0, 1) + Fc(1, 0, 0));
+        kr[0] -= e * (lqt[1] * H01 + lqt[2] * H02);
+        kr[1] -= e * (lqt[0] * H01 + lqt[2] * H12);
+        kr[2] -= e * (lqt[0] * H02 + lqt[1] * H12);
+    }
+    if (s.nf_k >= 3) {
+        // lambda . P q: the diagonal's own r-derivatives and the mixed ones
+        const R l0 = lqt[0], l1 = lqt[1], l2 = lqt[2];
+        kr[0] += ibk * (l0 * q[0] * dPd[0][0] + l1 * (q[1] * dPd[1][0] + q[2] * Mx[2]) +
+                        l2 * (q[1] * Mx[1] + q[2] * dPd[2][0]));
+        kr[1] += ibk * (l0 * (q[0] * dPd[0][1] + q[2] * Mx[2]) + l1 * q[1] * dPd[1][1] +
+                        l2 * (q[0] * Mx[0] + q[2] * dPd[2][1]));
+        kr[2] += ibk * (l0 * (q[0] * dPd[0][2] + q[1] * Mx[1]) + l1 * (q[0] * Mx[0] + q[1] * dPd[1][2]) +
+                        l2 * q[2] * dPd[2][2]);
     }
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        R ar = 0.0, aq = 0.0;
-#pragma unroll
-        for (int l = 0; l < 3; l++) {
-            ar += s.dtau_d * LX[l] * rate[l].d[j] + lqt[l] * qk[l].d[j];
-            aq += lqt[l] * qk[l].d[3 + j];
-        }
-        nx[j] = LX[j] + Nd * ar;
-        nq[j] = aq;
-    }
+    for (int j = 0; j < 3; j++) nx[j] = LX[j] + Nd * (ar[j] - K * kr[j]);
 
     // ---- field adjoints of the kick (pre-kick q, lambda = lqt) and of the
     // drift (kicked q', lambda = LX), summed per field over the stencil
     // offsets in registers first: 76 distinct sites take one atomic each
     // instead of 152 scattered adds
-    const R o = (R)1;
-    const R K = s.dtau_k * Nd;
-    const R v2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-    const R e2 = v2 + s.a_k * s.a_k, e = sqrt(e2);
-    const R qq[3] = {qk[0].v, qk[1].v, qk[2].v};
-    const R w2 = qq[0] * qq[0] + qq[1] * qq[1] + qq[2] * qq[2];
-    const R f2 = w2 + s.a_d * s.a_d, ie = (R)1 / sqrt(f2);
-    const R qL = qq[0] * LX[0] + qq[1] * LX[1] + qq[2] * LX[2];
     // st[dx + 1][dy + 1][dz + 1]: one field's adjoint at stencil offset (dx, dy, dz)
     R st[4][4][4];
     auto clear = [&]() {
@@ -177,15 +261,9 @@ __device__ __forceinline__ void adj_gr_particle(const T rt[3], const T qt[3], co
     // phi: g = Phibar grad2(phi) - grad1(chi) + pgradB / (bcoeff e); q' = q - K e g
     {
         clear();
-        const R Phi = cic<R>(Fp, r);
-        const R ph = Phi / e2;
-        const R Phibar = 4.0 * ph * v2 + (1.0 + s.a_k * s.a_k * ph) * (v2 + e2) / e2;
-        const R dPhibar = (4.0 * v2 + s.a_k * s.a_k * (v2 + e2) / e2) / e2;
-        R G2[3];
-        grad2<R>(Fp, r, G2);
         const R lG2[3] = {-K * e * Phibar * lqt[0], -K * e * Phibar * lqt[1], -K * e * Phibar * lqt[2]};
-        const R lPhi = -K * e * dPhibar * (G2[0] * lqt[0] + G2[1] * lqt[1] + G2[2] * lqt[2]);
-        const R lph = s.nf_d >= 1 ? (R)(s.dtau_d * qL * (3.0 - w2 / f2) * ie) : (R)0;
+        const R lPhi = -K * e * dPhibar * lg2;
+        const R lph = s.nf_d >= 1 ? dtd * qL * t3 * ie : (R)0;
 #pragma unroll
         for (int b = 0; b < 8; b++) {
             const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;

@@ -14,18 +14,16 @@
 //
 // P, C and V are FFT-diagonal: P and V are real symmetric (self-adjoint) as
 // real-space maps, and C's adjoint is the same transform pair with the
-// conjugate per-mode weights (k_adj_project_ft_scalar).  The reverse of the
-// kick + drift is written out term by term (adj_gr_particle); the deposits'
-// per-particle Jacobians come from evaluating the forward functor arithmetic
-// on dual numbers (gev_dual.hpp) seeded in the particle's cell offsets r and
-// momenta q.  The field adjoints are scattered with atomics (their order
-// follows atomic arbitration).  Per-particle adjoints lx, lq are [3][n]
+// conjugate per-mode weights (k_adj_project_ft_scalar).  The per-particle
+// reverses of the kick + drift (adj_gr_particle) and of the deposits
+// (adj_gr_deposit_particle) are written out term by term, one pass over the
+// stencil per particle.  The field adjoints are scattered with atomics (their
+// order follows atomic arbitration).  Per-particle adjoints lx, lq are [3][n]
 // arrays indexed by particle ID, as in the Newtonian adjoint; positions in box
 // units (d/dx = N d/dr).
 #include <type_traits>
 
 #include "gev_common.hpp"
-#include "gev_dual.hpp"
 #include "gev_kernels.hpp"
 #include "gev_kops.hpp"
 #include "gev_pcommon.hpp"
@@ -35,8 +33,6 @@
 #endif
 
 namespace gev {
-
-using D6 = Dual<double, 6>;
 
 template <typename T> __device__ __forceinline__ void gadd(T *p, double v) { atomicAdd(p, (T)v); }
 
@@ -469,79 +465,126 @@ __global__ __launch_bounds__(NT) void k_adj_gr_kick_drift(PV<T> v, const T *__re
 // f = 0 lT00, 1..6 lTij, 7..9 lL (the field adjoints), 10 phi_old.  Out:
 // gx, gq the adjoint contributions to the particle (d/dx in box units), and
 // sc(b, value) phi_old's adjoint at corner b (when has_phi).
+//
+// The particle's share of <adjoints, deposits> is one scalar
+//   S = sum_b W_b (e + f phi_b + f2 phi_b^2) lT00_b               (gevolution.hpp:1729-1854)
+//     + (1 + cf pint) mij / e (sum_d A_d q_d^2 + q0 q1 X01 + q0 q2 X02 + q1 q2 X12)   (:2582-2965)
+//     + m0i (q0 Lx + q1 Ly + q2 Lz)                                (:2031-2201, no link factor)
+// with W_b the CIC weights, e = sqrt(q^2 + a^2), f = 3 e + q^2 / e,
+// f2 = (f^2 / 2 + q^2 / (1 + q^2 / a^2)) / e, cf = 4 + a^2 / e^2, pint the
+// CIC of phi, A_d the CIC of lT_dd, X the edge interpolations of the
+// off-diagonal lT and L the face interpolations of the link adjoints; its
+// r- and q-gradients are written out term by term.
 template <typename T, class FA, class SC>
 __device__ __forceinline__ void adj_gr_deposit_particle(const T rt[3], const T qt[3], const FA &F, bool has_phi,
                                                         bool has_links, int N, double m00, double mij, double m0i,
                                                         double a_d, T gx[3], T gq[3], SC &sc)
 {
-    using R = T;  // the container's precision (FP32 duals for the mixed form)
-    using DT = Dual<R, 6>;
-    const R a = (R)a_d;
-    DT rD[3], qD[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) { rD[j] = DT::seed(rt[j], j); qD[j] = DT::seed(qt[j], 3 + j); }
-    const DT o((R)1);
-    DT W[8];
-#pragma unroll
-    for (int b = 0; b < 8; b++)
-        W[b] = (((b >> 2) & 1) ? rD[0] : o - rD[0]) * (((b >> 1) & 1) ? rD[1] : o - rD[1]) * ((b & 1) ? rD[2] : o - rD[2]);
-    R ph[8];
-#pragma unroll
-    for (int b = 0; b < 8; b++) ph[b] = has_phi ? F(10, (b >> 2) & 1, (b >> 1) & 1, b & 1) : (R)0;
-    // T00 (gevolution.hpp:1729-1854)
-    const DT q2 = qD[0] * qD[0] + qD[1] * qD[1] + qD[2] * qD[2];
-    const DT e = fsqrt(q2 + a * a), ie = frcp(e);
-    const DT f = (R)3 * e + q2 * ie;
-    const DT f2 = ((R)0.5 * f * f + q2 * frcp(o + q2 * (R)(1.0 / (a_d * a_d)))) * ie;
-    DT Ssum((R)0);
-    R lt[8];
+    using R = T;  // the container's precision
+    const R o = (R)1;
+    const R r[3] = {rt[0], rt[1], rt[2]};
+    const R q[3] = {qt[0], qt[1], qt[2]};
+    const R s0 = o - r[0], s1 = o - r[1], s2 = o - r[2];
+    const R a2 = (R)(a_d * a_d);
+    R W[8], ph[8], lt[8];
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-        lt[b] = (R)m00 * F(0, (b >> 2) & 1, (b >> 1) & 1, b & 1);
-        Ssum += W[b] * (e + f * ph[b] + f2 * (ph[b] * ph[b])) * lt[b];
+        const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+        W[b] = cw(b, r);
+        ph[b] = has_phi ? F(10, bx, by, bz) : (R)0;
+        lt[b] = (R)m00 * F(0, bx, by, bz);
     }
-    // Tij (gevolution.hpp:2582-2965, no CIC_PROJECT_TIJ) without its factor fT
-    const DT e2 = q2 + a * a, iet = frsqrt(e2);
-    DT pint((R)0);
+    // T00
+    const R q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const R e = fsqrt(q2 + a2), ie = frcp(e), ie2 = ie * ie, ie3 = ie2 * ie;
+    const R f = (R)3 * e + q2 * ie;
+    const R iden = frcp(o + q2 * (R)(1.0 / (a_d * a_d))), h = q2 * iden;
+    const R hf = (R)0.5 * f * f + h;
+    const R f2 = hf * ie;
+    R c0[8], A0 = (R)0, A1 = (R)0, A2 = (R)0;
 #pragma unroll
-    for (int b = 0; b < 8; b++) pint += W[b] * ph[b];
-    const DT cf = (R)4 + (a * a) * frcp(e2);
-    const DT fT = o + cf * pint;
-    DT Sij((R)0);
-    const int dc[3] = {0, 3, 5};
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        DT acc((R)0);
-#pragma unroll
-        for (int b = 0; b < 8; b++) acc += W[b] * F(1 + dc[d], (b >> 2) & 1, (b >> 1) & 1, b & 1);
-        Sij += acc * qD[d] * qD[d] * iet;
+    for (int b = 0; b < 8; b++) {
+        c0[b] = lt[b] * (e + f * ph[b] + f2 * (ph[b] * ph[b]));
+        const R wl = W[b] * lt[b];
+        A0 += wl;
+        A1 += wl * ph[b];
+        A2 += wl * ph[b] * ph[b];
     }
-    Sij += qD[0] * qD[1] * iet * ((o - rD[2]) * F(2, 0, 0, 0) + rD[2] * F(2, 0, 0, 1));
-    Sij += qD[0] * qD[2] * iet * ((o - rD[1]) * F(3, 0, 0, 0) + rD[1] * F(3, 0, 1, 0));
-    Sij += qD[1] * qD[2] * iet * ((o - rD[0]) * F(5, 0, 0, 0) + rD[0] * F(5, 1, 0, 0));
-    Sij = Sij * (R)mij;
-    Ssum += fT * Sij;
-    // T0i links (gevolution.hpp:2031-2201) without the link factor
-    if (has_links) {
-        DT Sl = (o - rD[1]) * (o - rD[2]) * qD[0] * F(7, 0, 0, 0) + (o - rD[0]) * (o - rD[2]) * qD[1] * F(8, 0, 0, 0) +
-                (o - rD[0]) * (o - rD[1]) * qD[2] * F(9, 0, 0, 0);
-        Sl += rD[0] * (o - rD[2]) * qD[1] * F(8, 1, 0, 0) + rD[0] * (o - rD[1]) * qD[2] * F(9, 1, 0, 0);
-        Sl += rD[1] * (o - rD[2]) * qD[0] * F(7, 0, 1, 0) + rD[1] * (o - rD[0]) * qD[2] * F(9, 0, 1, 0);
-        Sl += rD[2] * (o - rD[1]) * qD[0] * F(7, 0, 0, 1) + rD[2] * (o - rD[0]) * qD[1] * F(8, 0, 0, 1);
-        Sl += rD[1] * rD[2] * qD[0] * F(7, 0, 1, 1) + rD[0] * rD[2] * qD[1] * F(8, 1, 0, 1) +
-              rD[0] * rD[1] * qD[2] * F(9, 1, 1, 0);
-        Ssum += Sl * (R)m0i;
+    R gr[3];
+    grad1<R>(CubeAt<R>{c0}, r, gr);
+    // d/d(q^2) of e, f, f2
+    const R df = (R)2.5 * ie - (R)0.5 * q2 * ie3;
+    const R df2 = (f * df + iden * iden) * ie - (R)0.5 * hf * ie3;
+    const R dS0 = (R)0.5 * ie * A0 + df * A1 + df2 * A2;
+    // Tij without and with its factor fT = 1 + cf pint
+    R pint = (R)0, cd[8], Ad[3] = {(R)0, (R)0, (R)0};
+    const int dc[3] = {1, 4, 6};
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const int bx = (b >> 2) & 1, by = (b >> 1) & 1, bz = b & 1;
+        pint += W[b] * ph[b];
+        R c = (R)0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const R v = F(dc[d], bx, by, bz);
+            Ad[d] += W[b] * v;
+            c += q[d] * q[d] * v;
+        }
+        cd[b] = c;
     }
+    R gpint[3], gd[3];
+    grad1<R>(CubeAt<R>{ph}, r, gpint);
+    grad1<R>(CubeAt<R>{cd}, r, gd);
+    const R X01 = s2 * F(2, 0, 0, 0) + r[2] * F(2, 0, 0, 1);
+    const R X02 = s1 * F(3, 0, 0, 0) + r[1] * F(3, 0, 1, 0);
+    const R X12 = s0 * F(5, 0, 0, 0) + r[0] * F(5, 1, 0, 0);
+    gd[0] += q[1] * q[2] * (F(5, 1, 0, 0) - F(5, 0, 0, 0));
+    gd[1] += q[0] * q[2] * (F(3, 0, 1, 0) - F(3, 0, 0, 0));
+    gd[2] += q[0] * q[1] * (F(2, 0, 0, 1) - F(2, 0, 0, 0));
+    const R Q = Ad[0] * q[0] * q[0] + Ad[1] * q[1] * q[1] + Ad[2] * q[2] * q[2] + q[0] * q[1] * X01 + q[0] * q[2] * X02 +
+                q[1] * q[2] * X12;
+    const R mi = (R)mij;
+    const R Sij = mi * ie * Q;
+    const R cf = (R)4 + a2 * ie2, fT = o + cf * pint;
+    const R dQ[3] = {(R)2 * Ad[0] * q[0] + q[1] * X01 + q[2] * X02, (R)2 * Ad[1] * q[1] + q[0] * X01 + q[2] * X12,
+                     (R)2 * Ad[2] * q[2] + q[0] * X02 + q[1] * X12};
+    // the terms along q: T00's through q^2, cf's (dcf/dq_j = -2 a^2 q_j / e^4), 1 / e's
+    const R cq = (R)2 * dS0 - (R)2 * Sij * pint * a2 * ie2 * ie2 - fT * mi * ie3 * Q;
+    const R fm = fT * mi * ie;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        gx[j] = (R)N * Ssum.d[j];
-        gq[j] = Ssum.d[3 + j];
+        gr[j] += Sij * cf * gpint[j] + fm * gd[j];
+        gq[j] = cq * q[j] + fm * dQ[j];
     }
+    // T0i links: linear in q over the face interpolations of the link adjoints
+    if (has_links) {
+        const R m = (R)m0i;
+        const R Lx = s1 * s2 * F(7, 0, 0, 0) + r[1] * s2 * F(7, 0, 1, 0) + s1 * r[2] * F(7, 0, 0, 1) +
+                     r[1] * r[2] * F(7, 0, 1, 1);
+        const R Ly = s0 * s2 * F(8, 0, 0, 0) + r[0] * s2 * F(8, 1, 0, 0) + s0 * r[2] * F(8, 0, 0, 1) +
+                     r[0] * r[2] * F(8, 1, 0, 1);
+        const R Lz = s0 * s1 * F(9, 0, 0, 0) + r[0] * s1 * F(9, 1, 0, 0) + s0 * r[1] * F(9, 0, 1, 0) +
+                     r[0] * r[1] * F(9, 1, 1, 0);
+        gq[0] += m * Lx;
+        gq[1] += m * Ly;
+        gq[2] += m * Lz;
+        const R dLy0 = s2 * (F(8, 1, 0, 0) - F(8, 0, 0, 0)) + r[2] * (F(8, 1, 0, 1) - F(8, 0, 0, 1));
+        const R dLz0 = s1 * (F(9, 1, 0, 0) - F(9, 0, 0, 0)) + r[1] * (F(9, 1, 1, 0) - F(9, 0, 1, 0));
+        const R dLx1 = s2 * (F(7, 0, 1, 0) - F(7, 0, 0, 0)) + r[2] * (F(7, 0, 1, 1) - F(7, 0, 0, 1));
+        const R dLz1 = s0 * (F(9, 0, 1, 0) - F(9, 0, 0, 0)) + r[0] * (F(9, 1, 1, 0) - F(9, 1, 0, 0));
+        const R dLx2 = s1 * (F(7, 0, 0, 1) - F(7, 0, 0, 0)) + r[1] * (F(7, 0, 1, 1) - F(7, 0, 1, 0));
+        const R dLy2 = s0 * (F(8, 0, 0, 1) - F(8, 0, 0, 0)) + r[0] * (F(8, 1, 0, 1) - F(8, 1, 0, 0));
+        gr[0] += m * (q[1] * dLy0 + q[2] * dLz0);
+        gr[1] += m * (q[0] * dLx1 + q[2] * dLz1);
+        gr[2] += m * (q[0] * dLx2 + q[1] * dLy2);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) gx[j] = (R)N * gr[j];
     if (has_phi) {
         // d/dphi_b: T00's e + f phi + f2 phi^2 per corner, Tij's fT through the CIC of phi
-        const R tt = Sij.v * cf.v;
+        const R tt = Sij * cf;
 #pragma unroll
-        for (int b = 0; b < 8; b++) sc(b, W[b].v * (lt[b] * (f.v + (R)2 * f2.v * ph[b]) + tt));
+        for (int b = 0; b < 8; b++) sc(b, W[b] * (lt[b] * (f + (R)2 * f2 * ph[b]) + tt));
     }
 }
 

@@ -28,6 +28,14 @@
 #include "gev_kops.hpp"
 #include "gev_pcommon.hpp"
 
+// FP32 tiles of the reverse kick/drift and deposit: 1 the wide ones
+// (16 x 8 x 8), 0 the 8 x 8 x 4 ones (more workgroups per CU, more halo)
+#ifndef ADJ_KD_WIDE
+#define ADJ_KD_WIDE 1
+#endif
+#ifndef ADJ_DEP_WIDE
+#define ADJ_DEP_WIDE 1
+#endif
 #ifndef ADJ_CHUNK_X4
 #define ADJ_CHUNK_X4 1  // four times the mean particles per tile cell from which the reverse kick/drift walks per-lane chunks
 #endif
@@ -807,7 +815,7 @@ int adj_gr_kick_drift_impl(gev_pcls *p, const T *phi, const T *chi, const T *Bi,
     if (p->n > 0) {
         bool done = false;
         if constexpr (sizeof(T) == 4) {
-            if (N >= 16) {
+            if (ADJ_KD_WIDE && N >= 16) {
                 launch(std::integral_constant<int, 16>{}, std::integral_constant<int, 8>{},
                        std::integral_constant<int, 8>{}, std::integral_constant<int, 512>{});
                 done = true;
@@ -843,7 +851,7 @@ int adj_gr_deposit_impl(gev_pcls *p, const T *phi, const T *lT00, const T *lTij,
     if (p->n > 0) {
         bool done = false;
         if constexpr (sizeof(T) == 4) {  // FP64 fields take twice the LDS: the smaller tile
-            if (N >= 16) {
+            if (ADJ_DEP_WIDE && N >= 16) {
                 launch(std::integral_constant<int, 16>{}, I8{}, I8{}, std::integral_constant<int, 256>{});
                 done = true;
             }

@@ -442,58 +442,116 @@ int gev_evolve_ft_vector(gev_ctx *c, const void *S, void *B, double a2dtau)
                          evolve_ft_vector_impl<float>(c, (const cx<float> *)S, (cx<float> *)B, a2dtau));
 }
 
-// ---- container checkpoints (device memory of gev_pcls_state_bytes bytes)
-static int64_t pcls_state_bytes(const gev_pcls *p)
+// ---- container checkpoints (device memory of gev_pcls_state_bytes bytes):
+// a 256-byte header holding the count, then positions, momenta, IDs, cells
+// and cell starts, each at a 256-byte aligned offset, copied by one kernel
+// with 16-byte accesses (a save is one launch and no host synchronisation)
+namespace gev {
+struct CopySegs {
+    char *dst[9];
+    const char *src[9];
+    int64_t bytes[9];  // multiples of 4
+    int nseg;
+    int64_t count;     // written to *count_dst when that is set (a save)
+    int64_t *count_dst;
+};
+
+__global__ __launch_bounds__(256) void k_copy_segs(CopySegs L)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (int64_t)gridDim.x * blockDim.x;
+    if (L.count_dst && t == 0) *L.count_dst = L.count;
+    for (int g = 0; g < L.nseg; g++) {
+        const int64_t nb = L.bytes[g], nv = nb >> 4;
+        const uint4 *src = (const uint4 *)L.src[g];
+        uint4 *dst = (uint4 *)L.dst[g];
+        int64_t i = t;
+        for (; i + 3 * st < nv; i += 4 * st) {
+            const uint4 a = src[i], b = src[i + st], c = src[i + 2 * st], d = src[i + 3 * st];
+            dst[i] = a;
+            dst[i + st] = b;
+            dst[i + 2 * st] = c;
+            dst[i + 3 * st] = d;
+        }
+        for (; i < nv; i += st) dst[i] = src[i];
+        const int64_t tail = (nb - (nv << 4)) >> 2;  // up to three 4-byte words
+        if (t < tail) ((uint32_t *)(L.dst[g] + (nv << 4)))[t] = ((const uint32_t *)(L.src[g] + (nv << 4)))[t];
+    }
+}
+}  // namespace gev
+
+static int64_t ckpt_round(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+// the container arrays and their offsets in a checkpoint of n particles
+static int pcls_state_segs(gev_pcls *p, int64_t n, char *blob, bool save, gev::CopySegs &L)
 {
     const int64_t es = (p->ctx->prec == GEV_FP64) ? 8 : 4;
-    return 8 + 6 * p->n * es + 12 * p->n + 4 * (p->ctx->M + 1);
+    char *arr[9] = {(char *)p->pos[0], (char *)p->pos[1], (char *)p->pos[2], (char *)p->q[0], (char *)p->q[1],
+                    (char *)p->q[2], (char *)p->id, (char *)p->cell, (char *)p->cell_start};
+    const int64_t nb[9] = {n * es, n * es, n * es, n * es, n * es, n * es, 8 * n, 4 * n, 4 * (p->ctx->M + 1)};
+    int64_t off = 256;
+    L.nseg = 9;
+    for (int g = 0; g < 9; g++) {
+        L.dst[g] = save ? blob + off : arr[g];
+        L.src[g] = save ? arr[g] : blob + off;
+        L.bytes[g] = nb[g];
+        if (((uintptr_t)L.dst[g] | (uintptr_t)L.src[g]) & 15) return GEV_ERR_ARG;
+        off += ckpt_round(nb[g]);
+    }
+    return GEV_OK;
+}
+
+static int64_t pcls_state_bytes(const gev_pcls *p, int64_t n)
+{
+    const int64_t es = (p->ctx->prec == GEV_FP64) ? 8 : 4;
+    return 256 + 6 * ckpt_round(n * es) + ckpt_round(8 * n) + ckpt_round(4 * n) + ckpt_round(4 * (p->ctx->M + 1));
+}
+
+static int launch_copy_segs(gev_pcls *p, const gev::CopySegs &L)
+{
+    int64_t most = 0;
+    for (int g = 0; g < L.nseg; g++) most = L.bytes[g] > most ? L.bytes[g] : most;
+    // four 16-byte vectors per thread and pass, at most eight workgroups per CU
+    int64_t nbk = (most / 16 + 4 * 256 - 1) / (4 * 256);
+    nbk = nbk < 1 ? 1 : (nbk > 2048 ? 2048 : nbk);
+    hipLaunchKernelGGL(gev::k_copy_segs, dim3((unsigned)nbk), dim3(256), 0, p->ctx->stream, L);
+    GEV_HIP(hipGetLastError());
+    return GEV_OK;
 }
 
 int gev_pcls_state_bytes(gev_pcls *p, int64_t *bytes)
 {
     CHECK(p && bytes, "gev_pcls_state_bytes: NULL argument");
-    *bytes = pcls_state_bytes(p);
+    *bytes = pcls_state_bytes(p, p->n);
     return GEV_OK;
 }
 
-int gev_pcls_save(gev_pcls *p, void *dst)
+int gev_pcls_save(gev_pcls *p, void *dst, int64_t bytes)
 {
     CHECK(p && dst, "gev_pcls_save: NULL argument");
     NEED_SORTED(p);
-    const int64_t n = p->n, es = (p->ctx->prec == GEV_FP64) ? 8 : 4;
-    hipStream_t s = p->ctx->stream;
-    char *d = (char *)dst;
-    GEV_HIP(hipMemcpyAsync(d, &p->n, 8, hipMemcpyHostToDevice, s));
-    d += 8;
-    for (int k = 0; k < 3; k++, d += n * es) GEV_HIP(hipMemcpyAsync(d, p->pos[k], n * es, hipMemcpyDeviceToDevice, s));
-    for (int k = 0; k < 3; k++, d += n * es) GEV_HIP(hipMemcpyAsync(d, p->q[k], n * es, hipMemcpyDeviceToDevice, s));
-    GEV_HIP(hipMemcpyAsync(d, p->id, 8 * n, hipMemcpyDeviceToDevice, s));
-    d += 8 * n;
-    GEV_HIP(hipMemcpyAsync(d, p->cell, 4 * n, hipMemcpyDeviceToDevice, s));
-    d += 4 * n;
-    GEV_HIP(hipMemcpyAsync(d, p->cell_start, 4 * (p->ctx->M + 1), hipMemcpyDeviceToDevice, s));
-    GEV_HIP(hipStreamSynchronize(s));  // the count went from host memory
-    return GEV_OK;
+    CHECK(bytes >= pcls_state_bytes(p, p->n), "gev_pcls_save: buffer smaller than gev_pcls_state_bytes");
+    gev::CopySegs L{};
+    CHECK(pcls_state_segs(p, p->n, (char *)dst, true, L) == GEV_OK, "gev_pcls_save: buffer not 16-byte aligned");
+    L.count = p->n;
+    L.count_dst = (int64_t *)dst;
+    return launch_copy_segs(p, L);
 }
 
-int gev_pcls_restore(gev_pcls *p, const void *src)
+int gev_pcls_restore(gev_pcls *p, const void *src, int64_t bytes)
 {
     CHECK(p && src, "gev_pcls_restore: NULL argument");
+    CHECK(bytes >= 256, "gev_pcls_restore: buffer smaller than a checkpoint header");
     p->qbound_valid = false;
     hipStream_t s = p->ctx->stream;
     int64_t n = 0;
     GEV_HIP(hipMemcpyAsync(&n, src, 8, hipMemcpyDeviceToHost, s));
     GEV_HIP(hipStreamSynchronize(s));
     CHECK(n >= 0 && n <= p->cap, "gev_pcls_restore: saved state does not fit this container");
-    const int64_t es = (p->ctx->prec == GEV_FP64) ? 8 : 4;
-    const char *d = (const char *)src + 8;
-    for (int k = 0; k < 3; k++, d += n * es) GEV_HIP(hipMemcpyAsync(p->pos[k], d, n * es, hipMemcpyDeviceToDevice, s));
-    for (int k = 0; k < 3; k++, d += n * es) GEV_HIP(hipMemcpyAsync(p->q[k], d, n * es, hipMemcpyDeviceToDevice, s));
-    GEV_HIP(hipMemcpyAsync(p->id, d, 8 * n, hipMemcpyDeviceToDevice, s));
-    d += 8 * n;
-    GEV_HIP(hipMemcpyAsync(p->cell, d, 4 * n, hipMemcpyDeviceToDevice, s));
-    d += 4 * n;
-    GEV_HIP(hipMemcpyAsync(p->cell_start, d, 4 * (p->ctx->M + 1), hipMemcpyDeviceToDevice, s));
+    CHECK(bytes >= pcls_state_bytes(p, n), "gev_pcls_restore: buffer smaller than the saved state");
+    gev::CopySegs L{};
+    CHECK(pcls_state_segs(p, n, (char *)src, false, L) == GEV_OK, "gev_pcls_restore: buffer not 16-byte aligned");
+    const int rc = launch_copy_segs(p, L);
+    if (rc) return rc;
     p->n = n;
     p->sorted = true;
     return GEV_OK;

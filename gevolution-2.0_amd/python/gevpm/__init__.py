@@ -596,13 +596,14 @@ class Particles:
         buf = torch.empty(nb, dtype=torch.uint8, device=self.ctx.device) if buf is None else buf
         pb = _checked(buf, "save(buf)", torch.uint8, nb, self.ctx.device, at_least=True)
         self.ctx.bind_stream()
-        check(lib.gev_pcls_save(self._h, pb), "gev_pcls_save")
+        check(lib.gev_pcls_save(self._h, pb, buf.numel()), "gev_pcls_save")
         return buf
 
     def restore(self, buf):
-        pb = _checked(buf, "restore(buf)", torch.uint8, self.state_bytes(), self.ctx.device, at_least=True)
+        """the state a save wrote into buf (the library checks its count and size)"""
+        pb = _checked(buf, "restore(buf)", torch.uint8, 8, self.ctx.device, at_least=True)
         self.ctx.bind_stream()
-        check(lib.gev_pcls_restore(self._h, pb), "gev_pcls_restore")
+        check(lib.gev_pcls_restore(self._h, pb, buf.numel()), "gev_pcls_restore")
 
     # ------------------------------------------------ adjoint (Newtonian)
     def adj_deposit(self, lq, u):

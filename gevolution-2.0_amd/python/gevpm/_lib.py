@@ -99,8 +99,8 @@ SIGNATURES = {
     "gev_kick_drift": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _dp, _i]),
     "gev_pcls_rebin_counted": (_i, [_vp]),
     "gev_pcls_state_bytes": (_i, [_vp, C.POINTER(C.c_int64)]),
-    "gev_pcls_save": (_i, [_vp, _vp]),
-    "gev_pcls_restore": (_i, [_vp, _vp]),
+    "gev_pcls_save": (_i, [_vp, _vp, C.c_int64]),
+    "gev_pcls_restore": (_i, [_vp, _vp, C.c_int64]),
     "gev_pcls_save_gadget2": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header), _i, _d, _d, _vp]),
     "gev_pcls_load_gadget2": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),
     "gev_pcls_save_express": (_i, [_vp, C.c_char_p, C.POINTER(gev_gadget2_header)]),

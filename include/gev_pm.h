@@ -206,10 +206,13 @@ int gev_initialize_q(gev_pcls *p, const void *pot, double coeff, double *max_ret
 
 /* ---- container checkpoints (no reference counterpart: the state that a
  *      hibernation file holds for the particles, kept in device memory for
- *      the adjoint's recompute).  dst/src: device buffers of *bytes bytes. */
+ *      the adjoint's recompute).  dst/src: 16-byte aligned device buffers of
+ *      `bytes` bytes, at least gev_pcls_state_bytes of the container saved
+ *      (a restore checks the saved count against the container's capacity
+ *      and the buffer's size). */
 int gev_pcls_state_bytes(gev_pcls *p, int64_t *bytes);
-int gev_pcls_save(gev_pcls *p, void *dst);
-int gev_pcls_restore(gev_pcls *p, const void *src);
+int gev_pcls_save(gev_pcls *p, void *dst, int64_t bytes);
+int gev_pcls_restore(gev_pcls *p, const void *src, int64_t bytes);
 
 /* ---- Gadget-2 particle snapshots, single file, particle type 1
  *      (perfParticles_gevolution::saveGadget2 / loadGadget2,

@@ -834,7 +834,7 @@ int adj_gr_kick_drift_impl(gev_pcls *p, const T *phi, const T *chi, const T *Bi,
 
 template <typename T>
 int adj_gr_deposit_impl(gev_pcls *p, const T *phi, const T *lT00, const T *lTij, const T *lL, T *lx, T *lq, T *lphi,
-                        double a)
+                        double a, double tij_scale)
 {
     gev_ctx *c = p->ctx;
     const double dx = 1.0 / (double)c->N, m = p->mass / (dx * dx * dx);
@@ -844,7 +844,7 @@ int adj_gr_deposit_impl(gev_pcls *p, const T *phi, const T *lT00, const T *lTij,
         constexpr int NT = decltype(nt)::value;
         const unsigned nb = (unsigned)((int64_t)(N / TX) * (N / TY) * (N / TZ));
         hipLaunchKernelGGL((k_adj_gr_deposit<T, TX, TY, TZ, NT>), dim3(nb), dim3(NT), 0, c->stream, make_pv<T>(p), phi,
-                           lT00, lTij, lL, lx, lq, lphi, c->M, phi ? m / a : m, m / a, m, a);
+                           lT00, lTij, lL, lx, lq, lphi, c->M, phi ? m / a : m, tij_scale * m / a, m, a);
     };
     using I4 = std::integral_constant<int, 4>;
     using I8 = std::integral_constant<int, 8>;
@@ -903,7 +903,7 @@ template <typename T> int adj_project_ft_scalar_impl(gev_ctx *c, const cx<T> *lc
     template int adj_gr_kick_drift_impl<T>(gev_pcls *, const T *, const T *, const T *, T *, T *, T *, T *, T *, int, \
                                            int, double, double, double, double);                                 \
     template int adj_gr_deposit_impl<T>(gev_pcls *, const T *, const T *, const T *, const T *, T *, T *, T *,      \
-                                        double);                                                                   \
+                                        double, double);                                                           \
     template int adj_source_T00_impl<T>(gev_ctx *, const T *, const T *, const T *, T *, T *, T *, double, double, \
                                         double, double);                                                           \
     template int adj_source_Tij_phi_impl<T>(gev_ctx *, const T *, const T *, T *);                                 \

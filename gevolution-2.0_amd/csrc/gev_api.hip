@@ -613,16 +613,17 @@ int gev_adj_gr_kick_drift(gev_pcls *p, const void *phi, const void *chi, const v
 }
 
 int gev_adj_gr_deposit(gev_pcls *p, const void *phi, const void *lT00, const void *lTij, const void *lL, void *lx,
-                       void *lq, void *lphi, double a)
+                       void *lq, void *lphi, double a, double tij_scale)
 {
     CHECK(p && lT00 && lTij && lx && lq && (!phi || lphi), "gev_adj_gr_deposit: NULL argument");
     NEED_SORTED(p);
     return PREC_DISPATCH(p->ctx,
                          adj_gr_deposit_impl<double>(p, (const double *)phi, (const double *)lT00,
                                                      (const double *)lTij, (const double *)lL, (double *)lx,
-                                                     (double *)lq, (double *)lphi, a),
+                                                     (double *)lq, (double *)lphi, a, tij_scale),
                          adj_gr_deposit_impl<float>(p, (const float *)phi, (const float *)lT00, (const float *)lTij,
-                                                    (const float *)lL, (float *)lx, (float *)lq, (float *)lphi, a));
+                                                    (const float *)lL, (float *)lx, (float *)lq, (float *)lphi, a,
+                                                    tij_scale));
 }
 
 int gev_adj_source_T00(gev_ctx *c, const void *phi, const void *T00, const void *lsrc, void *lT00, void *lphi,

@@ -58,7 +58,7 @@ int adj_gr_kick_drift_impl(gev_pcls *p, const T *phi, const T *chi, const T *Bi,
                            T *lB, int nf_k, int nf_d, double a_k, double a_d, double dtau_k, double dtau_d);
 template <typename T>
 int adj_gr_deposit_impl(gev_pcls *p, const T *phi, const T *lT00, const T *lTij, const T *lL, T *lx, T *lq, T *lphi,
-                        double a);
+                        double a, double tij_scale);
 template <typename T>
 int adj_source_T00_impl(gev_ctx *c, const T *phi, const T *T00, const T *lsrc, T *lT00, T *lphi, T *lchi, double bg,
                         double c1, double c2, double c3);

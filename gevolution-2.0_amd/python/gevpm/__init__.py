@@ -624,14 +624,15 @@ class Particles:
                                         self.f(lchi, 1, nm + "(lchi)"), self.f(lB, 3, nm + "(lB)"), int(nf_kick),
                                         int(nf_drift), a_kick, a_drift, dtau_kick, dtau_drift), "gev_adj_gr_kick_drift")
 
-    def adj_gr_deposit(self, phi, lT00, lTij, lL, lx, lq, lphi, a):
-        """reverse of project_T00_Tij_T0i_links (coeff 1): lx, lq += , lphi += """
+    def adj_gr_deposit(self, phi, lT00, lTij, lL, lx, lq, lphi, a, tij_scale=1.0):
+        """reverse of project_T00_Tij_T0i_links (coeff 1), the Tij adjoint being
+        tij_scale lTij: lx, lq += , lphi += """
         self.ctx.bind_stream()
         nm = "adj_gr_deposit"
         check(lib.gev_adj_gr_deposit(self._h, self.f(phi, 1, nm + "(phi)", True), self.f(lT00, 1, nm + "(lT00)"),
                                      self.f(lTij, 6, nm + "(lTij)"), self.f(lL, 3, nm + "(lL)", True),
                                      self.per_particle(lx, nm + "(lx)"), self.per_particle(lq, nm + "(lq)"),
-                                     self.f(lphi, 1, nm + "(lphi)", True), a), "gev_adj_gr_deposit")
+                                     self.f(lphi, 1, nm + "(lphi)", True), a, tij_scale), "gev_adj_gr_deposit")
 
     def adj_gather(self, lx, lq, phi=None, lrho=None, K=0.0, mN=0.0, C_prev=0.0):
         self.ctx.bind_stream()

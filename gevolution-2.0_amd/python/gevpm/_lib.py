@@ -125,7 +125,7 @@ SIGNATURES = {
     "gev_adj_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "gev_adj_ic_lattice": (_i, [_vp, _vp, _vp, _d, _d, _vp]),
     "gev_adj_gr_kick_drift": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d]),
-    "gev_adj_gr_deposit": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d]),
+    "gev_adj_gr_deposit": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d]),
     "gev_adj_source_T00": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d]),
     "gev_adj_source_Tij": (_i, [_vp, _vp, _vp, _vp]),
     "gev_adj_link_factor": (_i, [_vp, _vp, _vp, _vp, _vp]),

@@ -301,14 +301,13 @@ class PMRunGR:
         c.adj_link_factor(phi, L, lB, lphi)                # lB now holds lL
         # chi = C Sij, Sij = stencil(phi) + ccoeff Tij
         lS = c.adj_solve_chi(lchi, work1=self.work1, work6=self.work6)
-        c.adj_source_Tij(phi, lS, lphi)
-        lTij = lS.mul_(k_["ccoeff"])
+        c.adj_source_Tij(phi, lS, lphi)  # Tij's adjoint is ccoeff lS: the deposit's reverse scales it
         # phi = P src, P self-adjoint
         lsrc = c.solve_poisson(lphi, k_["pcoeff"], k_["modif"], work=self.work1)
         lphi_old, lchi_old = c.real(1), c.real(1)
         lT00 = torch.empty_like(lphi_old)  # overwritten by adj_source_T00
         c.adj_source_T00(phi_old, T00, lsrc, lT00, lphi_old, lchi_old, self.om, k_["c1"], k_["c2"], k_["c3"])
-        p.adj_gr_deposit(phi_old, lT00, lTij, lB, lx, lq, lphi_old, s["a"])
+        p.adj_gr_deposit(phi_old, lT00, lS, lB, lx, lq, lphi_old, s["a"], k_["ccoeff"])
         return lphi_old, lchi_old
 
     def backward(self, g_rho):

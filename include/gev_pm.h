@@ -373,8 +373,9 @@ int gev_adj_ic_lattice(gev_ctx *ctx, const void *lx, const void *lq, double cx, 
  *      replaced by the adjoints before the kick; lphi, lchi, lB (3) += the
  *      fields' adjoints
  *   gev_adj_gr_deposit: the T00 + Tij + T0i-link deposits (gev_project_T00_Tij_
- *      T0i_links with coeff 1) given their adjoints lT00, lTij (6), lL (3, may
- *      be NULL): lx, lq += ; lphi += phi's adjoint (phi may be NULL: no phi)
+ *      T0i_links with coeff 1) given their adjoints lT00, tij_scale lTij (6),
+ *      lL (3, may be NULL): lx, lq += ; lphi += phi's adjoint (phi may be
+ *      NULL: no phi)
  *   gev_adj_source_T00: prepareFTsource T00 form (bg, c1, c2, c3 as in
  *      gev_prepare_source_T00) given lsrc: lT00 = (overwritten), lphi +=,
  *      lchi +=
@@ -389,7 +390,7 @@ int gev_adj_gr_kick_drift(gev_pcls *p, const void *phi, const void *chi, const v
                           void *lphi, void *lchi, void *lB, int nfield_kick, int nfield_drift, double a_kick,
                           double a_drift, double dtau_kick, double dtau_drift);
 int gev_adj_gr_deposit(gev_pcls *p, const void *phi, const void *lT00, const void *lTij, const void *lL, void *lx,
-                       void *lq, void *lphi, double a);
+                       void *lq, void *lphi, double a, double tij_scale);
 int gev_adj_source_T00(gev_ctx *ctx, const void *phi, const void *T00, const void *lsrc, void *lT00, void *lphi,
                        void *lchi, double bg, double c1, double c2, double c3);
 int gev_adj_source_Tij(gev_ctx *ctx, const void *phi, const void *lS, void *lphi);

@@ -105,7 +105,7 @@ def test_adjoint_deposit_matches_differences(gp):
 
     p.load(pos, q, pid)
     lx, lq, lphi = ctx.real(1).new_zeros(3, n), ctx.real(1).new_zeros(3, n), ctx.real(1)
-    p.adj_gr_deposit(phi, lT00, lTij, lL, lx, lq, lphi, a)
+    p.adj_gr_deposit(phi, lT00, lTij * 0.25, lL, lx, lq, lphi, a, tij_scale=4.0)  # Tij's adjoint 4 (lTij / 4)
     ad = dot(lx, dev_t(d["x"])) + dot(lq, dev_t(d["q"])) + dot(lphi, dev_t(d["phi"]))
     check_fd(J, ad, 1e-6, 1e-7, "T00+Tij+T0i-link deposits")
 

@@ -246,3 +246,39 @@ def test_gr_run_mixed_gradient_against_fp64(gp):
     rel = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
     print("GR mixed vs FP64 gradient rel L2", rel, flush=True)
     assert rel < 1e-3
+
+
+def test_mixed_reverse_kernels_against_fp64(gp):
+    """the FP32 reverse kick/drift and reverse deposit against their FP64 forms
+    on the same particles, fields and adjoints (16^3 with crowded cells, so
+    the chunked lane order and the wide tiles are both exercised): relative
+    L2 of every output within 2e-6 (FP32 rounding gives about 1e-7)"""
+    N, n = 16, 6000
+    rng = np.random.default_rng(7)
+    c = rng.integers(0, N, size=(3, n))
+    c[:, :1500] = rng.integers(5, 7, size=(3, 1500))  # crowded cells
+    pos = (c + rng.uniform(0.05, 0.95, size=(3, n))) / N
+    q = rng.normal(0, 0.02, (3, n))
+    pid = rng.permutation(n).astype(np.int64)
+    fl = dict(phi=rng.normal(0, 1e-3, (1, N, N, N)), chi=rng.normal(0, 1e-3, (1, N, N, N)),
+              B=rng.normal(0, 1e-5, (3, N, N, N)), lT00=rng.normal(size=(1, N, N, N)),
+              lTij=rng.normal(size=(6, N, N, N)), lL=rng.normal(size=(3, N, N, N)))
+    lx0, lq0 = rng.normal(size=(3, n)), rng.normal(size=(3, n))
+    out = {}
+    for prec in (gp.GEV_FP64, gp.GEV_MIXED):
+        ctx = gp.Context(N, prec)
+        dt = ctx.rdtype
+        p = gp.Particles(ctx, n, 0.31 / N ** 3)
+        p.load(pos, q, pid)
+        f = {k: dev_t(v, dt) for k, v in fl.items()}
+        lx, lq = dev_t(lx0, dt), dev_t(lq0, dt)
+        lphi, lchi, lB = ctx.real(1), ctx.real(1), ctx.real(3)
+        p.adj_gr_kick_drift(f["phi"], f["chi"], f["B"], lx, lq, lphi, lchi, lB, 3, 3, 0.1, 0.102, 0.015, 0.02)
+        gx, gq, gphi = ctx.real(1).new_zeros(3, n), ctx.real(1).new_zeros(3, n), ctx.real(1)
+        p.adj_gr_deposit(f["phi"], f["lT00"], f["lTij"], f["lL"], gx, gq, gphi, 0.1, tij_scale=0.7)
+        out[prec] = dict(lx=lx, lq=lq, lphi=lphi, lchi=lchi, lB=lB, gx=gx, gq=gq, gphi=gphi)
+    for k in out[gp.GEV_FP64]:
+        a, b = out[gp.GEV_FP64][k].double(), out[gp.GEV_MIXED][k].double()
+        rel = float(torch.linalg.norm(a - b) / torch.linalg.norm(a))
+        print(f"{k}: mixed against FP64 rel L2 {rel:.2e}", flush=True)
+        assert rel <= 2e-6, k

@@ -514,14 +514,17 @@ class Particles:
         check(lib.gev_update_pos_newton(self._h, dtau, a), "gev_update_pos_newton")
 
     def kick_drift(self, gr, dtau_kick, dtau_drift, phi, chi, Bi, nf_kick, nf_drift, a_kick, a_drift, bc_kick,
-                   bc_drift):
+                   bc_drift, want_max=True):
+        """-> the kick's maximum functor value (reading it back waits for the
+        stream), or None with want_max=False (no wait)"""
         r = C.c_double()
         self.ctx.bind_stream()
         nm = "kick_drift"
         check(lib.gev_kick_drift(self._h, int(gr), dtau_kick, dtau_drift, self.f(phi, 1, nm + "(phi)"),
                                  self.f(chi, 1, nm + "(chi)", True), self.f(Bi, 3, nm + "(Bi)", True), nf_kick,
-                                 nf_drift, a_kick, a_drift, bc_kick, bc_drift, C.byref(r), 1), "gev_kick_drift")
-        return r.value
+                                 nf_drift, a_kick, a_drift, bc_kick, bc_drift, C.byref(r) if want_max else None, 1),
+              "gev_kick_drift")
+        return r.value if want_max else None
 
     def displace(self, xi, coeff=1.0):
         r = C.c_double()

@@ -70,7 +70,7 @@ class PMRun:
         s, p, c = self.sched[n], self.pcls, self.ctx
         p.project_mass(self.rho)
         c.solve_poisson(self.rho, self.fpg / s["a"], 0.0, out=self.phi, work=self.work)
-        p.kick_drift(0, s["dtau_k"], s["dtau"], self.phi, None, None, 1, 0, s["a"], s["a_d"], 1.0, 1.0)
+        p.kick_drift(0, s["dtau_k"], s["dtau"], self.phi, None, None, 1, 0, s["a"], s["a_d"], 1.0, 1.0, want_max=False)
 
     def forward(self, psi):
         psi = psi.reshape(1, self.N, self.N, self.N).to(self.ctx.rdtype).contiguous()
@@ -242,7 +242,7 @@ class PMRunGR:
             self.T00, self.L = torch.empty_like(self.T00), torch.empty_like(self.L)
         N = self.N
         p.kick_drift(1, s["dtau_k"], s["dtau"], phi, chi, B, 3, 3, s["a"], s["a_d"], s["a"] ** 2 * N,
-                     s["a_d"] ** 2 * N)
+                     s["a_d"] ** 2 * N, want_max=False)
         self.phi, self.chi = phi, chi
         return kept
 

@@ -188,7 +188,8 @@ int gev_update_pos_newton(gev_pcls *p, double dtau, double a);               /* 
  * kicked momentum and (dtau_drift, a_drift, bcoeff_drift, nfield_drift), and
  * counts the re-binning ranks.  rebin=1 finishes the re-binning; rebin=0
  * leaves it to gev_pcls_rebin_counted (no other call may touch the container
- * in between). */
+ * in between).  *max_ret gets the kick's maximum functor value, which waits
+ * for the stream; max_ret = NULL skips it (no wait). */
 int gev_kick_drift(gev_pcls *p, int gr, double dtau_kick, double dtau_drift, const void *phi, const void *chi,
                    const void *Bi, int nfield_kick, int nfield_drift, double a_kick, double a_drift,
                    double bcoeff_kick, double bcoeff_drift, double *max_ret, int rebin);
